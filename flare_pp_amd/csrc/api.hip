@@ -1,0 +1,1511 @@
+// api.hip -- the C ABI of include/flare_hip.h: handles, host-side setup,
+// coefficient-file I/O and the orchestration of the HIP kernels and rocBLAS
+// contractions.
+#include <rocblas/rocblas.h>
+
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "flare_hip.h"
+#include "kernels.h"
+#include "mfma.h"
+#include "util.h"
+
+namespace flare {
+thread_local std::string g_err;
+void set_error(const std::string &m) { g_err = m; }
+}  // namespace flare
+
+using flare::DevBuf;
+using flare::Error;
+
+#define BLAS_CHECK(expr)                                                     \
+  do {                                                                       \
+    rocblas_status _s = (expr);                                              \
+    if (_s != rocblas_status_success)                                        \
+      throw ::flare::Error{FLARE_EHIP, std::string(#expr) + ": " +           \
+                                           rocblas_status_to_string(_s)};    \
+  } while (0)
+
+#define REQUIRE(cond, msg) FLARE_REQUIRE(cond, msg)
+
+template <class Fn>
+static int guarded(Fn &&fn) {
+  try {
+    fn();
+    return FLARE_OK;
+  } catch (const Error &e) {
+    flare::set_error(e.msg);
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    flare::set_error("host allocation failed");
+    return FLARE_ENOMEM;
+  } catch (const std::exception &e) {
+    flare::set_error(e.what());
+    return FLARE_EINVAL;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// context
+
+struct flare_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  rocblas_handle blas = nullptr;
+  int blocking = 1;
+  // stage timing with HIP events on the context stream (when enabled)
+  int profiling = 0;
+  struct Stage {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Stage> pending;
+  std::vector<std::string> stage_names;
+  std::map<std::string, std::pair<double, int64_t>> stage_ms;
+  void mark_begin(const std::string &name) {
+    if (!profiling) return;
+    Stage st{name, nullptr, nullptr};
+    FLARE_HIP_CHECK(hipEventCreate(&st.a));
+    FLARE_HIP_CHECK(hipEventCreate(&st.b));
+    FLARE_HIP_CHECK(hipEventRecord(st.a, stream));
+    pending.push_back(st);
+  }
+  void mark_end() {
+    if (!profiling || pending.empty()) return;
+    FLARE_HIP_CHECK(hipEventRecord(pending.back().b, stream));
+  }
+  void collect() {
+    if (pending.empty()) return;
+    FLARE_HIP_CHECK(hipStreamSynchronize(stream));
+    for (auto &st : pending) {
+      float ms = 0;
+      FLARE_HIP_CHECK(hipEventElapsedTime(&ms, st.a, st.b));
+      auto it = stage_ms.find(st.name);
+      if (it == stage_ms.end()) {
+        stage_names.push_back(st.name);
+        stage_ms[st.name] = {0.0, 0};
+      }
+      stage_ms[st.name].first += ms;
+      stage_ms[st.name].second += 1;
+      (void)hipEventDestroy(st.a);
+      (void)hipEventDestroy(st.b);
+    }
+    pending.clear();
+  }
+  DevBuf ynorm_d, ynorm_f;
+  std::map<std::string, std::unique_ptr<DevBuf>> ws;
+  std::map<int64_t, std::unique_ptr<DevBuf>> b2idx;
+
+  void *buf(const std::string &key, size_t bytes) {
+    auto &p = ws[key];
+    if (!p) p.reset(new DevBuf);
+    p->ensure(bytes);
+    return p->ptr;
+  }
+  template <typename T>
+  T *buf_t(const std::string &key, size_t count) {
+    return reinterpret_cast<T *>(buf(key, std::max<size_t>(count, 1) * sizeof(T)));
+  }
+  void use() { FLARE_HIP_CHECK(hipSetDevice(device)); }
+  void sync() { FLARE_HIP_CHECK(hipStreamSynchronize(stream)); }
+  // end of an entry point: wait for results unless running non-blocking
+  void finish() {
+    collect();
+    if (blocking) sync();
+  }
+  const int *b2_index(int S, int N, int L) {
+    int64_t key = ((int64_t)S << 32) | (N << 16) | L;
+    auto &p = b2idx[key];
+    if (!p) {
+      int nr = S * N;
+      std::vector<int> code;
+      for (int c1 = 0; c1 < nr; c1++)
+        for (int c2 = c1; c2 < nr; c2++)
+          for (int l = 0; l <= L; l++) code.push_back((c1 << 20) | (c2 << 8) | l);
+      p.reset(new DevBuf);
+      p->ensure(code.size() * sizeof(int));
+      FLARE_HIP_CHECK(hipMemcpy(p->ptr, code.data(), code.size() * sizeof(int),
+                                hipMemcpyHostToDevice));
+    }
+    return p->as<int>();
+  }
+};
+
+static void gemm_rm(flare_ctx *c, bool ta, bool tb, int M, int N, int K,
+                    double alpha, const double *A, int lda, const double *B,
+                    int ldb, double beta, double *C, int ldc) {
+  if (M == 0 || N == 0) return;
+  REQUIRE(K > 0, "empty inner dimension");
+  BLAS_CHECK(rocblas_dgemm(c->blas, tb ? rocblas_operation_transpose
+                                       : rocblas_operation_none,
+                           ta ? rocblas_operation_transpose
+                              : rocblas_operation_none,
+                           N, M, K, &alpha, B, ldb, A, lda, &beta, C, ldc));
+}
+
+static void sgemm_rm(flare_ctx *c, bool ta, bool tb, int M, int N, int K,
+                     float alpha, const float *A, int lda, const float *B,
+                     int ldb, float beta, float *C, int ldc) {
+  if (M == 0 || N == 0) return;
+  REQUIRE(K > 0, "empty inner dimension");
+  BLAS_CHECK(rocblas_sgemm(c->blas, tb ? rocblas_operation_transpose
+                                       : rocblas_operation_none,
+                           ta ? rocblas_operation_transpose
+                              : rocblas_operation_none,
+                           N, M, K, &alpha, B, ldb, A, lda, &beta, C, ldc));
+}
+
+// ---------------------------------------------------------------------------
+// cell matrices, restating Eigen's dynamic-size 3x3 arithmetic
+// (structure.cpp:12-24, 120-133): products summed left to right without
+// FMA; inverses through partial-pivot LU with reciprocal back substitution.
+
+namespace {
+
+void mm3(const double *a, const double *b, double *c) {
+  double t[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      volatile double s = a[i * 3] * b[j];
+      volatile double p1 = a[i * 3 + 1] * b[3 + j];
+      s = s + p1;
+      volatile double p2 = a[i * 3 + 2] * b[6 + j];
+      s = s + p2;
+      t[i * 3 + j] = s;
+    }
+  std::memcpy(c, t, sizeof(t));
+}
+
+int lu3(double lu[3][3], int tr[3]) {
+  int nt = 0;
+  for (int k = 0; k < 3; k++) {
+    int p = k;
+    double best = std::fabs(lu[k][k]);
+    for (int i = k + 1; i < 3; i++)
+      if (std::fabs(lu[i][k]) > best) {
+        best = std::fabs(lu[i][k]);
+        p = i;
+      }
+    tr[k] = p;
+    if (best != 0) {
+      if (p != k) {
+        for (int j = 0; j < 3; j++) std::swap(lu[k][j], lu[p][j]);
+        nt++;
+      }
+      for (int i = k + 1; i < 3; i++) lu[i][k] = lu[i][k] / lu[k][k];
+    }
+    for (int i = k + 1; i < 3; i++)
+      for (int j = k + 1; j < 3; j++) {
+        volatile double pr = lu[i][k] * lu[k][j];
+        lu[i][j] = lu[i][j] - pr;
+      }
+  }
+  return nt;
+}
+
+void inv3(const double *a, double *out) {
+  double lu[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) lu[i][j] = a[i * 3 + j];
+  int tr[3];
+  lu3(lu, tr);
+  double x[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
+  for (int k = 0; k < 3; k++)
+    if (tr[k] != k)
+      for (int j = 0; j < 3; j++) std::swap(x[k][j], x[tr[k]][j]);
+  for (int j = 0; j < 3; j++)
+    for (int i = 0; i < 3; i++) {
+      double b = x[i][j];
+      for (int r = i + 1; r < 3; r++) {
+        volatile double pr = b * lu[r][i];
+        x[r][j] = x[r][j] - pr;
+      }
+    }
+  for (int j = 0; j < 3; j++)
+    for (int i = 2; i >= 0; i--) {
+      double ainv = 1.0 / lu[i][i];
+      double b = x[i][j] * ainv;
+      x[i][j] = b;
+      for (int r = 0; r < i; r++) {
+        volatile double pr = b * lu[r][i];
+        x[r][j] = x[r][j] - pr;
+      }
+    }
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) out[i * 3 + j] = x[i][j];
+}
+
+double volume3(const double *cell) {
+  double lu[3][3];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) lu[i][j] = cell[i * 3 + j];
+  int tr[3];
+  int nt = lu3(lu, tr);
+  volatile double prod = lu[0][0] * lu[1][1];
+  prod = prod * lu[2][2];
+  return std::fabs((nt % 2 ? -1.0 : 1.0) * prod);
+}
+
+// structure.cpp:135-165
+double single_sweep(const double *cell) {
+  const double *v1 = cell, *v2 = cell + 3, *v3 = cell + 6;
+  auto dot = [](const double *a, const double *b) {
+    volatile double s = a[0] * b[0];
+    volatile double p = a[1] * b[1];
+    s = s + p;
+    p = a[2] * b[2];
+    s = s + p;
+    return (double)s;
+  };
+  double ab = dot(v1, v2), ac = dot(v1, v3), bc = dot(v2, v3);
+  double a = std::sqrt(dot(v1, v1)), b = std::sqrt(dot(v2, v2)),
+         c = std::sqrt(dot(v3, v3));
+  double m[6] = {a * std::sqrt(1 - std::pow(ab / (a * b), 2)),
+                 b * std::sqrt(1 - std::pow(ab / (a * b), 2)),
+                 a * std::sqrt(1 - std::pow(ac / (a * c), 2)),
+                 c * std::sqrt(1 - std::pow(ac / (a * c), 2)),
+                 b * std::sqrt(1 - std::pow(bc / (b * c), 2)),
+                 c * std::sqrt(1 - std::pow(bc / (b * c), 2))};
+  double s = m[0];
+  for (int i = 0; i < 6; i++)
+    if (m[i] < s) s = m[i];
+  return s;
+}
+
+flare::WrapHost make_wrap(const double *cell, double cutoff) {
+  flare::WrapHost w;
+  std::memcpy(w.cell, cell, sizeof(w.cell));
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) w.ct[j * 3 + i] = cell[i * 3 + j];
+  inv3(w.ct, w.cti);
+  mm3(cell, w.ct, w.cd);
+  inv3(w.cd, w.cdi);
+  w.volume = volume3(cell);
+  w.single_sweep = single_sweep(cell);
+  w.cutoff = cutoff;
+  double sw = std::ceil(cutoff / w.single_sweep);
+  REQUIRE(std::isfinite(sw) && sw >= 1 && sw < 64,
+          "cutoff / cell combination needs too many periodic images");
+  w.sweep = (int)sw;
+  auto cross = [](const double *u, const double *v) {
+    double c0 = u[1] * v[2] - u[2] * v[1];
+    double c1 = u[2] * v[0] - u[0] * v[2];
+    double c2 = u[0] * v[1] - u[1] * v[0];
+    return std::sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+  };
+  const double *a0 = cell, *a1 = cell + 3, *a2 = cell + 6;
+  double width[3] = {w.volume / cross(a1, a2), w.volume / cross(a2, a0),
+                     w.volume / cross(a0, a1)};
+  for (int k = 0; k < 3; k++) {
+    double b = std::floor(width[k] / (cutoff * (1.0 + 1e-9)));
+    if (!(b >= 1)) b = 1;
+    if (b > 256) b = 256;
+    w.nb[k] = (int)b;
+  }
+  return w;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// neighbor list
+
+struct flare_nlist {
+  flare_ctx *ctx = nullptr;
+  flare::NListBuffers b;
+  flare::WrapHost wh;
+  int n = 0;
+  int species_count[flare::kMaxSpecies] = {0};
+  int n_species_present = 0;
+  bool built = false;
+
+  flare::NListDev dev() const {
+    flare::NListDev d;
+    d.n = n;
+    d.P = b.P;
+    d.cum = b.cum.as<int>();
+    d.idx = b.idx.as<int>();
+    d.nspec = b.nspec.as<int>();
+    d.rel = b.rel.as<double>();
+    d.rev = b.rev.as<int>();
+    d.center = b.center.as<int>();
+    return d;
+  }
+  void row_offsets(int S, int *off) const {
+    off[0] = 0;
+    for (int s = 0; s < S; s++) off[s + 1] = off[s] + species_count[s];
+  }
+};
+
+// ---------------------------------------------------------------------------
+// B2 parameters
+
+static flare::B2Dev make_b2(const flare_b2_params *p, int pair_pred) {
+  REQUIRE(p, "null B2 parameters");
+  REQUIRE(p->n_species >= 1 && p->n_species <= flare::kMaxSpecies,
+          "n_species must be in [1, 8]");
+  REQUIRE(p->n_max >= 1 && p->n_max <= flare::kMaxN, "n_max must be in [1, 16]");
+  REQUIRE(p->l_max >= 0 && p->l_max <= flare::kMaxL, "l_max must be in [0, 10]");
+  REQUIRE(p->radial_basis == FLARE_RADIAL_CHEBYSHEV,
+          "only the chebyshev radial basis is supported");
+  REQUIRE(p->cutoff_function == FLARE_CUTOFF_QUADRATIC ||
+              p->cutoff_function == FLARE_CUTOFF_COSINE,
+          "cutoff function must be quadratic or cosine");
+  REQUIRE(p->cutoffs, "null cutoff matrix");
+  flare::B2Dev d;
+  d.S = p->n_species;
+  d.N = p->n_max;
+  d.L = p->l_max;
+  d.H = (d.L + 1) * (d.L + 1);
+  d.NH = d.N * d.H;
+  d.W = d.S * d.NH;
+  d.nr = d.S * d.N;
+  d.F = flare::b2_n_descriptors(d.S, d.N, d.L);
+  d.cutoff_kind = p->cutoff_function;
+  d.pair_pred = pair_pred;
+  d.r1 = p->radial_r1;
+  for (int q = 0; q < flare::kMaxSpecies * flare::kMaxSpecies; q++) d.cut[q] = 0;
+  for (int q = 0; q < d.S * d.S; q++) {
+    REQUIRE(p->cutoffs[q] > d.r1, "pair cutoffs must exceed radial_hyps[0]");
+    d.cut[q] = p->cutoffs[q];
+  }
+  return d;
+}
+
+static void check_species(const flare_nlist *nl, int S) {
+  REQUIRE(nl && nl->built, "neighbor list not built");
+  for (int s = S; s < flare::kMaxSpecies; s++)
+    REQUIRE(nl->species_count[s] == 0,
+            "structure holds species codes beyond n_species");
+}
+
+// Shared forward: A (atom order), B (species-grouped rows), |B|^2.
+struct Forward {
+  void *A = nullptr, *B = nullptr;
+  double *norm2 = nullptr;
+};
+
+static Forward run_forward(flare_ctx *c, const flare::B2Dev &p,
+                           const flare_nlist *nl, const int *d_species,
+                           int prec, const std::string &tag,
+                           void *B_out = nullptr) {
+  Forward f;
+  const size_t ts = prec == flare::kPrecF64 ? sizeof(double) : sizeof(float);
+  const int n = nl->n;
+  f.A = c->buf(tag + ".A", std::max<size_t>(1, (size_t)n * p.W) * ts);
+  f.B = B_out ? B_out : c->buf(tag + ".B", std::max<size_t>(1, (size_t)n * p.F) * ts);
+  f.norm2 = c->buf_t<double>(tag + ".norm2", n);
+  c->mark_begin("forward");
+  flare::launch_b2_forward(c->stream, p, nl->dev(), d_species,
+                           nl->b.row_of.as<int>(), c->ynorm_d.as<double>(),
+                           c->ynorm_f.as<float>(), c->b2_index(p.S, p.N, p.L),
+                           prec, f.A, f.B, f.norm2);
+  c->mark_end();
+  return f;
+}
+
+// Forces / virial assembly shared by the mapped and GP paths.
+static void run_forces(flare_ctx *c, const flare::B2Dev &p,
+                       const flare_nlist *nl, const int *d_species, int prec,
+                       int mode, const Forward &fw, const void *w,
+                       const double *E_row, double *forces, double *energy,
+                       double *virial6, double *stress6,
+                       const std::string &tag) {
+  const int n = nl->n;
+  const int64_t P = nl->b.P;
+  const double *dA = nullptr;
+  if (mode == 0) {
+    double *d = c->buf_t<double>(tag + ".dA", (size_t)std::max<int64_t>(P, 1) * 3 * p.NH);
+    c->mark_begin("pair_dA");
+    flare::launch_pair_dA(c->stream, p, nl->dev(), d_species,
+                          c->ynorm_d.as<double>(), d);
+    c->mark_end();
+    dA = d;
+  }
+  double *pf = c->buf_t<double>(tag + ".pf", (size_t)std::max<int64_t>(P, 1) * 3);
+  double *vir = c->buf_t<double>(tag + ".vir", (size_t)n * 9);
+  c->mark_begin(mode == 0 ? "forces_plain" : "forces_adjoint");
+  flare::launch_forces(c->stream, prec, mode, p, nl->dev(), d_species,
+                       nl->b.row_of.as<int>(), c->ynorm_d.as<double>(),
+                       c->ynorm_f.as<float>(), c->b2_index(p.S, p.N, p.L), fw.A,
+                       w, dA, pf, vir);
+  c->mark_end();
+  c->mark_begin("gather_reduce");
+  if (forces) flare::launch_gather(c->stream, nl->dev(), pf, forces);
+  double *sums = c->buf_t<double>(tag + ".sums", 10);
+  if (n > 0) {
+    flare::launch_colsum(c->stream, n, 1, E_row, sums);
+    flare::launch_colsum(c->stream, n, 9, vir, sums + 1);
+  } else {
+    FLARE_HIP_CHECK(hipMemsetAsync(sums, 0, 10 * sizeof(double), c->stream));
+  }
+  flare::launch_finish(c->stream, sums, 1.0 / nl->wh.volume, energy, virial6,
+                       stress6);
+  c->mark_end();
+}
+
+// ---------------------------------------------------------------------------
+// coefficient files (pair_flare.cpp:263-416; sparse_gp.cpp:1142-1206)
+
+struct flare_pair {
+  flare_ctx *ctx = nullptr;
+  int power = 2, S = 1, N = 1, L = 0, F = 0, cutoff_kind = 0, legacy = 0;
+  int64_t beta_size = 0;
+  double cutoff = 0;
+  std::string radial = "chebyshev", cutoff_name = "quadratic";
+  std::vector<double> cutoffs;
+  DevBuf beta, beta_f32;
+  bool have_last = false;
+  int last_prec = 0, last_n = 0;
+  int last_off[flare::kMaxSpecies + 1] = {0};
+};
+
+namespace {
+
+bool parse_int_line(const std::string &s, int &v) {
+  const char *p = s.c_str();
+  while (*p == ' ' || *p == '\t') p++;
+  char *end = nullptr;
+  errno = 0;
+  long x = std::strtol(p, &end, 10);
+  if (end == p || errno) return false;
+  while (*end == ' ' || *end == '\t' || *end == '\r' || *end == '\n') end++;
+  if (*end != 0) return false;
+  v = (int)x;
+  return true;
+}
+
+std::string first_word(const std::string &s) {
+  std::istringstream is(s);
+  std::string w;
+  is >> w;
+  return w;
+}
+
+void setup_pair(flare_pair *pr, const std::vector<double> &packed) {
+  REQUIRE(pr->power == 1 || pr->power == 2, "Power should be 1 or 2.");
+  REQUIRE(pr->S >= 1 && pr->S <= flare::kMaxSpecies, "n_species must be in [1, 8]");
+  REQUIRE(pr->N >= 1 && pr->N <= flare::kMaxN, "n_max must be in [1, 16]");
+  REQUIRE(pr->L >= 0 && pr->L <= flare::kMaxL, "l_max must be in [0, 10]");
+  pr->F = flare::b2_n_descriptors(pr->S, pr->N, pr->L);
+  int64_t check = pr->power == 1 ? (int64_t)pr->F : (int64_t)pr->F * (pr->F + 1) / 2;
+  if (check != pr->beta_size)
+    throw Error{FLARE_EFORMAT, "Beta size doesn't match the number of descriptors."};
+  REQUIRE((int64_t)packed.size() == pr->S * pr->beta_size, "beta value count mismatch");
+  pr->cutoff = -1;
+  for (double v : pr->cutoffs) pr->cutoff = std::max(pr->cutoff, v);
+  flare_ctx *c = pr->ctx;
+  size_t per = pr->power == 1 ? (size_t)pr->F : (size_t)pr->F * pr->F;
+  pr->beta.ensure(pr->S * per * sizeof(double));
+  pr->beta_f32.ensure(pr->S * per * sizeof(float));
+  double *tmp = c->buf_t<double>("pair.packed", packed.size());
+  FLARE_HIP_CHECK(hipMemcpyAsync(tmp, packed.data(), packed.size() * sizeof(double),
+                                 hipMemcpyHostToDevice, c->stream));
+  for (int s = 0; s < pr->S; s++)
+    flare::launch_unpack_beta(c->stream, pr->F, pr->power, tmp + s * pr->beta_size,
+                              pr->beta.as<double>() + s * per);
+  flare::launch_to_f32(c->stream, (int64_t)(pr->S * per), pr->beta.as<double>(),
+                       pr->beta_f32.as<float>());
+  c->sync();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// sparse GP
+
+struct flare_sgp {
+  flare_ctx *ctx = nullptr;
+  int S = 1, F = 0, M = 0;
+  double sigma = 1, sig2 = 1, power = 2, jitter = 1e-8;
+  std::vector<int> m_by_type, m_off;
+  DevBuf P, Phat, pnorm, pnorm2, alpha, Kuu, Lcm, Linv_cm;
+  bool have_alpha = false;
+};
+
+// ===========================================================================
+extern "C" {
+
+const char *flare_version(void) { return "flare_hip 0.1.0 (gfx950)"; }
+
+const char *flare_last_error(void) { return flare::g_err.c_str(); }
+
+int flare_ctx_create(int device, flare_ctx **out) {
+  return guarded([&] {
+    REQUIRE(out, "null output handle");
+    *out = nullptr;
+    std::unique_ptr<flare_ctx> c(new flare_ctx);
+    c->device = device;
+    c->use();
+    BLAS_CHECK(rocblas_create_handle(&c->blas));
+    BLAS_CHECK(rocblas_set_stream(c->blas, c->stream));
+    double tab[flare::kNormTableSize];
+    float tabf[flare::kNormTableSize];
+    flare::ylm_norm_table(tab);
+    for (int q = 0; q < flare::kNormTableSize; q++) tabf[q] = (float)tab[q];
+    c->ynorm_d.ensure(sizeof(tab));
+    c->ynorm_f.ensure(sizeof(tabf));
+    FLARE_HIP_CHECK(hipMemcpy(c->ynorm_d.ptr, tab, sizeof(tab), hipMemcpyHostToDevice));
+    FLARE_HIP_CHECK(hipMemcpy(c->ynorm_f.ptr, tabf, sizeof(tabf), hipMemcpyHostToDevice));
+    *out = c.release();
+  });
+}
+
+int flare_ctx_destroy(flare_ctx *ctx) {
+  return guarded([&] {
+    if (!ctx) return;
+    ctx->use();
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->blas) rocblas_destroy_handle(ctx->blas);
+    delete ctx;
+  });
+}
+
+int flare_ctx_set_stream(flare_ctx *ctx, void *stream) {
+  return guarded([&] {
+    REQUIRE(ctx, "null context");
+    ctx->stream = (hipStream_t)stream;
+    BLAS_CHECK(rocblas_set_stream(ctx->blas, ctx->stream));
+  });
+}
+
+int flare_ctx_set_blocking(flare_ctx *ctx, int blocking) {
+  return guarded([&] {
+    REQUIRE(ctx, "null context");
+    ctx->blocking = blocking ? 1 : 0;
+  });
+}
+
+int flare_ctx_profile(flare_ctx *ctx, int enable) {
+  return guarded([&] {
+    REQUIRE(ctx, "null context");
+    ctx->use();
+    ctx->collect();
+    ctx->profiling = enable ? 1 : 0;
+    ctx->stage_ms.clear();
+    ctx->stage_names.clear();
+  });
+}
+
+int flare_ctx_profile_get(flare_ctx *ctx, int index, char *name, int name_len,
+                          double *total_ms, int64_t *count) {
+  return guarded([&] {
+    REQUIRE(ctx && name && total_ms && count && name_len > 0, "null argument");
+    ctx->use();
+    ctx->collect();
+    REQUIRE(index >= 0 && index < (int)ctx->stage_names.size(), "no such stage");
+    const std::string &k = ctx->stage_names[index];
+    std::snprintf(name, name_len, "%s", k.c_str());
+    *total_ms = ctx->stage_ms[k].first;
+    *count = ctx->stage_ms[k].second;
+  });
+}
+
+int flare_ctx_synchronize(flare_ctx *ctx) {
+  return guarded([&] {
+    REQUIRE(ctx, "null context");
+    ctx->use();
+    ctx->sync();
+  });
+}
+
+// ---- neighbor list ---------------------------------------------------------
+int flare_nlist_create(flare_ctx *ctx, flare_nlist **out) {
+  return guarded([&] {
+    REQUIRE(ctx && out, "null argument");
+    flare_nlist *nl = new flare_nlist;
+    nl->ctx = ctx;
+    *out = nl;
+  });
+}
+
+int flare_nlist_destroy(flare_nlist *nl) {
+  return guarded([&] {
+    if (!nl) return;
+    nl->ctx->use();
+    (void)hipStreamSynchronize(nl->ctx->stream);
+    delete nl;
+  });
+}
+
+int flare_nlist_build(flare_ctx *ctx, flare_nlist *nl, int n_atoms,
+                      const double *cell, const double *d_positions,
+                      const int *d_species, double cutoff) {
+  return guarded([&] {
+    REQUIRE(ctx && nl && cell, "null argument");
+    REQUIRE(n_atoms >= 0, "negative atom count");
+    REQUIRE(n_atoms == 0 || (d_positions && d_species), "null positions/species");
+    REQUIRE(cutoff > 0 && std::isfinite(cutoff), "cutoff must be positive");
+    for (int q = 0; q < 9; q++) REQUIRE(std::isfinite(cell[q]), "non-finite cell");
+    ctx->use();
+    nl->built = false;
+    nl->wh = make_wrap(cell, cutoff);
+    REQUIRE(nl->wh.volume > 0, "singular cell");
+    nl->n = n_atoms;
+    ctx->mark_begin("nlist");
+    flare::nlist_build_device(ctx->stream, nl->b, n_atoms, d_positions, d_species,
+                              nl->wh);
+    ctx->mark_end();
+    nl->n_species_present = flare::species_rows_device(
+        ctx->stream, nl->b, n_atoms, d_species, nl->species_count);
+    nl->built = true;
+  });
+}
+
+int flare_nlist_get_view(const flare_nlist *nl, flare_nlist_view *v) {
+  return guarded([&] {
+    REQUIRE(nl && v, "null argument");
+    REQUIRE(nl->built, "neighbor list not built");
+    v->n_atoms = nl->n;
+    v->n_pairs = nl->b.P;
+    v->sweep = nl->wh.sweep;
+    v->volume = nl->wh.volume;
+    v->single_sweep_cutoff = nl->wh.single_sweep;
+    v->cutoff = nl->wh.cutoff;
+    for (int s = 0; s < flare::kMaxSpecies; s++) v->species_count[s] = nl->species_count[s];
+    v->d_neighbor_count = nl->b.count.as<int>();
+    v->d_cumulative_neighbor_count = nl->b.cum.as<int>();
+    v->d_structure_indices = nl->b.idx.as<int>();
+    v->d_neighbor_species = nl->b.nspec.as<int>();
+    v->d_relative_positions = nl->b.rel.as<double>();
+    v->d_wrapped_positions = nl->b.wrapped.as<double>();
+    v->d_reverse_pair = nl->b.rev.as<int>();
+    v->d_atom_indices = nl->b.atom_of_row.as<int>();
+  });
+}
+
+// ---- B2 -------------------------------------------------------------------
+int flare_b2_n_descriptors(const flare_b2_params *p) {
+  if (!p) return -1;
+  return flare::b2_n_descriptors(p->n_species, p->n_max, p->l_max);
+}
+
+int flare_b2_compute(flare_ctx *ctx, const flare_b2_params *params,
+                     const flare_nlist *nl, const int *d_species,
+                     double *d_descriptors, double *d_norms,
+                     double *d_single_bond) {
+  return guarded([&] {
+    REQUIRE(ctx, "null context");
+    flare::B2Dev p = make_b2(params, 0);
+    check_species(nl, p.S);
+    ctx->use();
+    Forward fw = run_forward(ctx, p, nl, d_species, flare::kPrecF64, "b2",
+                             d_descriptors);
+    if (d_norms) flare::launch_sqrt(ctx->stream, nl->n, fw.norm2, d_norms);
+    if (d_single_bond && nl->n)
+      FLARE_HIP_CHECK(hipMemcpyAsync(d_single_bond, fw.A,
+                                     (size_t)nl->n * p.W * sizeof(double),
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+    ctx->sync();
+  });
+}
+
+// host bookkeeping of the species-grouped B2 pair list
+struct B2Pairs {
+  std::vector<int> count, atom_of_row;
+  std::vector<int64_t> q0;  // per row
+  int64_t total = 0;
+  int64_t by_type[flare::kMaxSpecies] = {0};
+};
+
+static B2Pairs b2_pairs(flare_ctx *c, const flare::B2Dev &p,
+                        const flare_nlist *nl, const int *d_species) {
+  B2Pairs r;
+  const int n = nl->n;
+  int *cnt = c->buf_t<int>("b2.count", n);
+  flare::launch_b2_filter_count(c->stream, p, nl->dev(), d_species, cnt);
+  r.count.resize(n);
+  r.atom_of_row.resize(n);
+  if (n) {
+    FLARE_HIP_CHECK(hipMemcpyAsync(r.count.data(), cnt, n * sizeof(int),
+                                   hipMemcpyDeviceToHost, c->stream));
+    FLARE_HIP_CHECK(hipMemcpyAsync(r.atom_of_row.data(), nl->b.atom_of_row.ptr,
+                                   n * sizeof(int), hipMemcpyDeviceToHost,
+                                   c->stream));
+  }
+  c->sync();
+  r.q0.resize(n);
+  int off[flare::kMaxSpecies + 1];
+  nl->row_offsets(p.S, off);
+  int64_t q = 0;
+  for (int s = 0; s < p.S; s++) {
+    for (int row = off[s]; row < off[s + 1]; row++) {
+      r.q0[row] = q;
+      q += r.count[r.atom_of_row[row]];
+      r.by_type[s] += r.count[r.atom_of_row[row]];
+    }
+  }
+  r.total = q;
+  return r;
+}
+
+int flare_b2_count_pairs(flare_ctx *ctx, const flare_b2_params *params,
+                         const flare_nlist *nl, const int *d_species,
+                         int64_t *n_pairs, int64_t *n_pairs_by_type) {
+  return guarded([&] {
+    REQUIRE(ctx && n_pairs, "null argument");
+    flare::B2Dev p = make_b2(params, 0);
+    check_species(nl, p.S);
+    ctx->use();
+    B2Pairs r = b2_pairs(ctx, p, nl, d_species);
+    *n_pairs = r.total;
+    if (n_pairs_by_type)
+      for (int s = 0; s < p.S; s++) n_pairs_by_type[s] = r.by_type[s];
+  });
+}
+
+int flare_b2_compute_dervs(flare_ctx *ctx, const flare_b2_params *params,
+                           const flare_nlist *nl, const int *d_species,
+                           double *d_force_dervs, double *d_force_dots,
+                           double *d_neighbor_coordinates,
+                           int *d_neighbor_counts,
+                           int *d_cumulative_neighbor_counts,
+                           int *d_neighbor_indices) {
+  return guarded([&] {
+    REQUIRE(ctx, "null context");
+    flare::B2Dev p = make_b2(params, 0);
+    check_species(nl, p.S);
+    ctx->use();
+    const int n = nl->n;
+    Forward fw = run_forward(ctx, p, nl, d_species, flare::kPrecF64, "b2d");
+    B2Pairs r = b2_pairs(ctx, p, nl, d_species);
+    std::vector<int> ncount(n), ncum(n);
+    int off[flare::kMaxSpecies + 1];
+    nl->row_offsets(p.S, off);
+    for (int s = 0; s < p.S; s++) {
+      int64_t start = off[s] < off[s + 1] ? r.q0[off[s]] : 0;
+      for (int row = off[s]; row < off[s + 1]; row++) {
+        ncount[row] = r.count[r.atom_of_row[row]];
+        ncum[row] = (int)(r.q0[row] - start);
+      }
+    }
+    if (n && d_neighbor_counts)
+      FLARE_HIP_CHECK(hipMemcpyAsync(d_neighbor_counts, ncount.data(), n * sizeof(int),
+                                     hipMemcpyHostToDevice, ctx->stream));
+    if (n && d_cumulative_neighbor_counts)
+      FLARE_HIP_CHECK(hipMemcpyAsync(d_cumulative_neighbor_counts, ncum.data(),
+                                     n * sizeof(int), hipMemcpyHostToDevice,
+                                     ctx->stream));
+    const int64_t Q = r.total;
+    if (Q > 0) {
+      int64_t *q0 = ctx->buf_t<int64_t>("b2d.q0", n);
+      int64_t *fpair = ctx->buf_t<int64_t>("b2d.fpair", Q);
+      FLARE_HIP_CHECK(hipMemcpyAsync(q0, r.q0.data(), n * sizeof(int64_t),
+                                     hipMemcpyHostToDevice, ctx->stream));
+      flare::launch_b2_fill_fpair(ctx->stream, p, nl->dev(), d_species,
+                                  nl->b.row_of.as<int>(), q0, fpair);
+      double *dB = d_force_dervs ? d_force_dervs
+                                 : ctx->buf_t<double>("b2d.dB", (size_t)Q * 3 * p.F);
+      double *fd = d_force_dots ? d_force_dots : ctx->buf_t<double>("b2d.fd", Q * 3);
+      double *co = d_neighbor_coordinates ? d_neighbor_coordinates
+                                          : ctx->buf_t<double>("b2d.co", Q * 3);
+      int *ni = d_neighbor_indices ? d_neighbor_indices : ctx->buf_t<int>("b2d.ni", Q);
+      flare::launch_b2_dervs(ctx->stream, p, nl->dev(), d_species,
+                             nl->b.row_of.as<int>(), ctx->ynorm_d.as<double>(),
+                             ctx->b2_index(p.S, p.N, p.L), Q, fpair,
+                             (const double *)fw.A, (const double *)fw.B, dB, fd,
+                             co, ni);
+    }
+    ctx->sync();
+  });
+}
+
+// ---- mapped pair style -----------------------------------------------------
+// PairFLARE::read_file (pair_flare.cpp:263-402) on the host: header lines,
+// then whitespace-separated values (grab, pair_flare.cpp:404-416).
+static void parse_coefficients(const char *path, flare_pair &pr,
+                               std::vector<double> &packed) {
+  std::ifstream f(path);
+  if (!f) throw Error{FLARE_EIO, std::string("Cannot open potential file ") + path};
+  std::string line;
+  auto next = [&](const char *what) {
+    if (!std::getline(f, line))
+      throw Error{FLARE_EFORMAT, std::string("unexpected end of file before ") + what};
+  };
+  next("header");  // date and contributor
+  next("power");
+  int power = 0;
+  if (parse_int_line(line, power)) {
+    pr.power = power;
+    next("radial basis");
+    pr.radial = first_word(line);
+  } else {  // legacy layout: no power line, power 2
+    pr.legacy = 1;
+    pr.power = 2;
+    pr.radial = first_word(line);
+  }
+  next("descriptor settings");
+  {
+    std::istringstream is(line);
+    long long bs = 0;
+    if (!(is >> pr.S >> pr.N >> pr.L >> bs))
+      throw Error{FLARE_EFORMAT, "malformed 'n_species n_max l_max beta_size' line"};
+    pr.beta_size = bs;
+  }
+  next("cutoff function");
+  pr.cutoff_name = first_word(line);
+  if (pr.radial != "chebyshev")
+    throw Error{FLARE_EINVAL, "unsupported radial basis '" + pr.radial + "'"};
+  if (pr.cutoff_name == "quadratic")
+    pr.cutoff_kind = FLARE_CUTOFF_QUADRATIC;
+  else if (pr.cutoff_name == "cosine")
+    pr.cutoff_kind = FLARE_CUTOFF_COSINE;
+  else
+    throw Error{FLARE_EINVAL, "unsupported cutoff function '" + pr.cutoff_name + "'"};
+  if (pr.power != 1 && pr.power != 2) throw Error{FLARE_EINVAL, "Power should be 1 or 2."};
+  REQUIRE(pr.S >= 1 && pr.S <= flare::kMaxSpecies, "n_species must be in [1, 8]");
+  REQUIRE(pr.N >= 1 && pr.N <= flare::kMaxN, "n_max must be in [1, 16]");
+  REQUIRE(pr.L >= 0 && pr.L <= flare::kMaxL, "l_max must be in [0, 10]");
+  pr.F = flare::b2_n_descriptors(pr.S, pr.N, pr.L);
+  int64_t check = pr.power == 1 ? (int64_t)pr.F : (int64_t)pr.F * (pr.F + 1) / 2;
+  if (check != pr.beta_size)
+    throw Error{FLARE_EFORMAT, "Beta size doesn't match the number of descriptors."};
+  const int n_cut = pr.legacy ? 1 : pr.S * pr.S;
+  std::vector<double> vals;
+  const int64_t need = n_cut + pr.S * pr.beta_size;
+  vals.reserve(need);
+  std::string tok;
+  while ((int64_t)vals.size() < need && (f >> tok)) {
+    char *end = nullptr;
+    double v = std::strtod(tok.c_str(), &end);
+    if (end == tok.c_str() || *end != 0)
+      throw Error{FLARE_EFORMAT, "non-numeric token '" + tok + "'"};
+    vals.push_back(v);
+  }
+  if ((int64_t)vals.size() < need)
+    throw Error{FLARE_EFORMAT, "file ends after " + std::to_string(vals.size()) +
+                                   " of " + std::to_string(need) + " values"};
+  pr.cutoffs.assign(pr.S * pr.S, vals[0]);
+  if (!pr.legacy)
+    for (int q = 0; q < pr.S * pr.S; q++) pr.cutoffs[q] = vals[q];
+  pr.cutoff = -1;
+  for (double v : pr.cutoffs) pr.cutoff = std::max(pr.cutoff, v);
+  packed.assign(vals.begin() + n_cut, vals.end());
+}
+
+static void fill_info(const flare_pair &pr, flare_pair_info *o) {
+  std::memset(o, 0, sizeof(*o));
+  o->power = pr.power;
+  o->n_species = pr.S;
+  o->n_max = pr.N;
+  o->l_max = pr.L;
+  o->n_descriptors = pr.F;
+  o->beta_size = pr.beta_size;
+  o->cutoff_function = pr.cutoff_kind;
+  o->legacy = pr.legacy;
+  o->cutoff = pr.cutoff;
+  std::snprintf(o->radial_basis, sizeof(o->radial_basis), "%s", pr.radial.c_str());
+  std::snprintf(o->cutoff_name, sizeof(o->cutoff_name), "%s", pr.cutoff_name.c_str());
+}
+
+int flare_coeff_read(const char *path, flare_pair_info *info, double *cutoffs,
+                     double *beta_packed) {
+  return guarded([&] {
+    REQUIRE(path && info, "null argument");
+    flare_pair pr;
+    std::vector<double> packed;
+    parse_coefficients(path, pr, packed);
+    fill_info(pr, info);
+    if (cutoffs) std::copy(pr.cutoffs.begin(), pr.cutoffs.end(), cutoffs);
+    if (beta_packed) std::copy(packed.begin(), packed.end(), beta_packed);
+  });
+}
+
+int flare_pair_load(flare_ctx *ctx, const char *path, flare_pair **out) {
+  return guarded([&] {
+    REQUIRE(ctx && path && out, "null argument");
+    *out = nullptr;
+    ctx->use();
+    std::unique_ptr<flare_pair> pr(new flare_pair);
+    pr->ctx = ctx;
+    std::vector<double> packed;
+    parse_coefficients(path, *pr, packed);
+    setup_pair(pr.get(), packed);
+    *out = pr.release();
+  });
+}
+
+int flare_pair_create(flare_ctx *ctx, int power, int n_species, int n_max,
+                      int l_max, int cutoff_function, const double *cutoffs,
+                      const double *beta_packed, flare_pair **out) {
+  return guarded([&] {
+    REQUIRE(ctx && cutoffs && beta_packed && out, "null argument");
+    *out = nullptr;
+    ctx->use();
+    std::unique_ptr<flare_pair> pr(new flare_pair);
+    pr->ctx = ctx;
+    pr->power = power;
+    pr->S = n_species;
+    pr->N = n_max;
+    pr->L = l_max;
+    REQUIRE(cutoff_function == FLARE_CUTOFF_QUADRATIC ||
+                cutoff_function == FLARE_CUTOFF_COSINE,
+            "cutoff function must be quadratic or cosine");
+    pr->cutoff_kind = cutoff_function;
+    pr->cutoff_name = cutoff_function == FLARE_CUTOFF_QUADRATIC ? "quadratic" : "cosine";
+    REQUIRE(n_species >= 1 && n_species <= flare::kMaxSpecies, "n_species must be in [1, 8]");
+    int F = flare::b2_n_descriptors(n_species, n_max, l_max);
+    pr->beta_size = power == 1 ? (int64_t)F : (int64_t)F * (F + 1) / 2;
+    pr->cutoffs.assign(cutoffs, cutoffs + n_species * n_species);
+    std::vector<double> packed(beta_packed, beta_packed + n_species * pr->beta_size);
+    setup_pair(pr.get(), packed);
+    *out = pr.release();
+  });
+}
+
+int flare_pair_destroy(flare_pair *pr) {
+  return guarded([&] {
+    if (!pr) return;
+    pr->ctx->use();
+    (void)hipStreamSynchronize(pr->ctx->stream);
+    delete pr;
+  });
+}
+
+int flare_pair_get_info(const flare_pair *pr, flare_pair_info *o) {
+  return guarded([&] {
+    REQUIRE(pr && o, "null argument");
+    fill_info(*pr, o);
+  });
+}
+
+int flare_pair_get_cutoffs(const flare_pair *pr, double *cutoffs) {
+  return guarded([&] {
+    REQUIRE(pr && cutoffs, "null argument");
+    std::copy(pr->cutoffs.begin(), pr->cutoffs.end(), cutoffs);
+  });
+}
+
+int flare_pair_get_beta(flare_ctx *ctx, const flare_pair *pr, double *beta) {
+  return guarded([&] {
+    REQUIRE(ctx && pr && beta, "null argument");
+    ctx->use();
+    size_t per = pr->power == 1 ? (size_t)pr->F : (size_t)pr->F * pr->F;
+    FLARE_HIP_CHECK(hipMemcpy(beta, pr->beta.ptr, pr->S * per * sizeof(double),
+                              hipMemcpyDeviceToHost));
+  });
+}
+
+int flare_pair_compute(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
+                       const int *d_species, int form, const double *d_beta,
+                       double *d_energy, double *d_forces, double *d_virial,
+                       double *d_eatom) {
+  return guarded([&] {
+    REQUIRE(ctx && pr, "null argument");
+    REQUIRE(form >= FLARE_FORM_PLAIN_FP64 && form <= FLARE_FORM_ADJOINT_MIXED,
+            "unknown form");
+    flare_b2_params bp;
+    bp.n_species = pr->S;
+    bp.n_max = pr->N;
+    bp.l_max = pr->L;
+    bp.radial_basis = FLARE_RADIAL_CHEBYSHEV;
+    bp.cutoff_function = pr->cutoff_kind;
+    bp.radial_r1 = 0.0;  // radial_hyps = {0, cutoff} (pair_flare.cpp:346-349)
+    bp.cutoffs = pr->cutoffs.data();
+    flare::B2Dev p = make_b2(&bp, 1);
+    check_species(nl, p.S);
+    ctx->use();
+    const int n = nl->n;
+    const int F = p.F;
+    const int prec = (form == FLARE_FORM_PLAIN_FP64 || form == FLARE_FORM_ADJOINT_FP64)
+                         ? flare::kPrecF64
+                         : flare::kPrecF32;
+    const int mode = form == FLARE_FORM_PLAIN_FP64 ? 0 : 1;
+    const size_t per = pr->power == 1 ? (size_t)F : (size_t)F * F;
+    const size_t ts = prec == flare::kPrecF64 ? sizeof(double) : sizeof(float);
+    Forward fw = run_forward(ctx, p, nl, d_species, prec, "pair");
+    int off[flare::kMaxSpecies + 1];
+    nl->row_offsets(p.S, off);
+    // beta in the working precision
+    const void *beta = pr->beta.ptr;
+    if (prec == flare::kPrecF64) {
+      if (d_beta) beta = d_beta;
+    } else {
+      beta = pr->beta_f32.ptr;
+      if (d_beta) {
+        float *bf = ctx->buf_t<float>("pair.beta_override", p.S * per);
+        flare::launch_to_f32(ctx->stream, (int64_t)(p.S * per), d_beta, bf);
+        beta = bf;
+      }
+    }
+    // GEMM input rows: B itself in FP64 (as pair_flare), B/|B| otherwise
+    const void *X = fw.B;
+    int normalized = 0;
+    if (prec == flare::kPrecF32) {
+      float *xn = ctx->buf_t<float>("pair.X", (size_t)n * F);
+      flare::launch_normalize_rows_f32(ctx->stream, n, F, (const float *)fw.B,
+                                       fw.norm2, xn);
+      X = xn;
+      normalized = 1;
+    }
+    void *Z = nullptr;
+    ctx->mark_begin("gemm");
+    if (pr->power == 2) {
+      Z = ctx->buf("pair.Z", std::max<size_t>(1, (size_t)n * F) * ts);
+      for (int s = 0; s < p.S; s++) {
+        int rows = off[s + 1] - off[s];
+        if (!rows) continue;
+        if (prec == flare::kPrecF64) {
+          gemm_rm(ctx, false, false, rows, F, F, 1.0,
+                  (const double *)X + (size_t)off[s] * F, F,
+                  (const double *)beta + s * per, F, 0.0,
+                  (double *)Z + (size_t)off[s] * F, F);
+        } else if (form == FLARE_FORM_ADJOINT_MIXED) {
+          flare::gemm_bf16x3(ctx->stream, rows, F, F,
+                             (const float *)X + (size_t)off[s] * F,
+                             (const float *)beta + s * per,
+                             (float *)Z + (size_t)off[s] * F,
+                             ctx->buf("mfma.ws", flare::gemm_bf16x3_workspace(rows, F, F)));
+        } else {
+          sgemm_rm(ctx, false, false, rows, F, F, 1.0f,
+                   (const float *)X + (size_t)off[s] * F, F,
+                   (const float *)beta + s * per, F, 0.0f,
+                   (float *)Z + (size_t)off[s] * F, F);
+        }
+      }
+    }
+    ctx->mark_end();
+    void *w = ctx->buf("pair.w", std::max<size_t>(1, (size_t)n * F) * ts);
+    double *E_row = ctx->buf_t<double>("pair.E", n);
+    ctx->mark_begin("energy_w");
+    flare::launch_mapped_w(ctx->stream, prec, n, F, pr->power, normalized, X, Z,
+                           fw.norm2, nl->b.species_of_row.as<int>(), beta, w,
+                           E_row);
+    ctx->mark_end();
+    run_forces(ctx, p, nl, d_species, prec, mode, fw, w, E_row, d_forces,
+               d_energy, d_virial, nullptr, "pair");
+    if (d_eatom)
+      flare::launch_rows_to_atoms(ctx->stream, n, nl->b.atom_of_row.as<int>(),
+                                  E_row, d_eatom);
+    pr->have_last = true;
+    pr->last_prec = prec;
+    pr->last_n = n;
+    for (int s = 0; s <= p.S; s++) pr->last_off[s] = off[s];
+    ctx->finish();
+  });
+}
+
+int flare_pair_beta_grad(flare_ctx *ctx, flare_pair *pr, double *d_grad) {
+  return guarded([&] {
+    REQUIRE(ctx && pr && d_grad, "null argument");
+    REQUIRE(pr->have_last, "no force evaluation to differentiate");
+    ctx->use();
+    const int n = pr->last_n, F = pr->F;
+    const int *off = pr->last_off;
+    const double *norm2 = ctx->ws["pair.norm2"]->as<double>();
+    double *dhat = ctx->buf_t<double>("pair.dhat", (size_t)n * F);
+    if (pr->last_prec == flare::kPrecF64)
+      flare::launch_normalize_rows_f64(ctx->stream, n, F,
+                                       ctx->ws["pair.B"]->as<double>(), norm2, dhat);
+    else
+      flare::launch_to_f64(ctx->stream, (int64_t)n * F,
+                           ctx->ws["pair.X"]->as<float>(), dhat);
+    // empty environments (|B|^2 < 1e-8) carry no energy
+    flare::launch_mask_rows(ctx->stream, n, F, norm2, 1e-8, dhat);
+    for (int s = 0; s < pr->S; s++) {
+      int rows = off[s + 1] - off[s];
+      if (pr->power == 2) {
+        double *g = d_grad + (size_t)s * F * F;
+        if (!rows) {
+          FLARE_HIP_CHECK(hipMemsetAsync(g, 0, (size_t)F * F * sizeof(double), ctx->stream));
+          continue;
+        }
+        gemm_rm(ctx, true, false, F, F, rows, 1.0, dhat + (size_t)off[s] * F, F,
+                dhat + (size_t)off[s] * F, F, 0.0, g, F);
+      } else {
+        double *g = d_grad + (size_t)s * F;
+        if (!rows) {
+          FLARE_HIP_CHECK(hipMemsetAsync(g, 0, (size_t)F * sizeof(double), ctx->stream));
+          continue;
+        }
+        flare::launch_colsum(ctx->stream, rows, F, dhat + (size_t)off[s] * F, g);
+      }
+    }
+    ctx->sync();
+  });
+}
+
+int flare_write_coefficients(const char *path, const char *contributor,
+                             int power, int n_species, int n_max, int l_max,
+                             const char *radial_basis,
+                             const char *cutoff_function,
+                             const double *cutoffs, int64_t beta_size,
+                             const double *beta_packed) {
+  return guarded([&] {
+    REQUIRE(path && cutoffs && beta_packed, "null argument");
+    FILE *f = std::fopen(path, "w");
+    if (!f) throw Error{FLARE_EIO, std::string("cannot write ") + path};
+    time_t now = std::time(nullptr);
+    std::string t(std::ctime(&now));
+    if (!t.empty() && t.back() == '\n') t.pop_back();
+    std::fprintf(f, "DATE: %s CONTRIBUTOR: %s\n", t.c_str(),
+                 contributor ? contributor : "");
+    std::fprintf(f, "%d\n", power);
+    std::fprintf(f, "%s\n", radial_basis ? radial_basis : "chebyshev");
+    std::fprintf(f, "%d %d %d %lld\n", n_species, n_max, l_max, (long long)beta_size);
+    std::fprintf(f, "%s\n", cutoff_function ? cutoff_function : "quadratic");
+    for (int q = 0; q < n_species * n_species; q++) std::fprintf(f, "%.2f ", cutoffs[q]);
+    std::fprintf(f, "\n");
+    // five values a line, two spaces before positives, one before the rest;
+    // the line counter runs on across species (sparse_gp.cpp:1173-1203)
+    int count = 0;
+    for (int s = 0; s < n_species; s++) {
+      if (count != 0) std::fprintf(f, "\n");
+      for (int64_t j = 0; j < beta_size; j++) {
+        double v = beta_packed[(int64_t)s * beta_size + j];
+        std::fprintf(f, v > 0 ? "  %.16e" : " %.16e", v);
+        if (++count == 5) {
+          count = 0;
+          std::fprintf(f, "\n");
+        }
+      }
+    }
+    if (std::fclose(f) != 0) throw Error{FLARE_EIO, std::string("cannot write ") + path};
+  });
+}
+
+// ---- sparse GP ---------------------------------------------------------------
+static void sgp_factor(flare_sgp *gp) {
+  flare_ctx *c = gp->ctx;
+  const int M = gp->M, F = gp->F;
+  int *info = c->buf_t<int>("sgp.info", 1);
+  FLARE_HIP_CHECK(hipMemsetAsync(info, 0, sizeof(int), c->stream));
+  gp->Kuu.ensure((size_t)std::max(M, 1) * M * sizeof(double));
+  gp->Lcm.ensure((size_t)std::max(M, 1) * M * sizeof(double));
+  gp->Linv_cm.ensure((size_t)std::max(M, 1) * M * sizeof(double));
+  double *K = gp->Kuu.as<double>(), *Lc = gp->Lcm.as<double>(),
+         *Li = gp->Linv_cm.as<double>();
+  FLARE_HIP_CHECK(hipMemsetAsync(K, 0, (size_t)M * M * sizeof(double), c->stream));
+  FLARE_HIP_CHECK(hipMemsetAsync(Li, 0, (size_t)M * M * sizeof(double), c->stream));
+  for (int s = 0; s < gp->S; s++) {
+    int Ms = gp->m_by_type[s], o = gp->m_off[s];
+    if (!Ms) continue;
+    const double *Ps = gp->P.as<double>() + (size_t)o * F;
+    double *Ks = K + (size_t)o * M + o;
+    gemm_rm(c, false, true, Ms, Ms, F, 1.0, Ps, F, Ps, F, 0.0, Ks, M);
+    flare::launch_kuu_block(c->stream, Ms, M, Ks, gp->pnorm.as<double>() + o,
+                            gp->sig2, gp->power);
+  }
+  FLARE_HIP_CHECK(hipMemcpyAsync(Lc, K, (size_t)M * M * sizeof(double),
+                                 hipMemcpyDeviceToDevice, c->stream));
+  flare::launch_add_diag(c->stream, M, Lc, M, gp->jitter);
+  for (int s = 0; s < gp->S; s++) {
+    int Ms = gp->m_by_type[s], o = gp->m_off[s];
+    if (!Ms) continue;
+    double *A = Lc + (size_t)o * M + o;
+    const int nb = 64;
+    for (int j = 0; j < Ms; j += nb) {
+      int jb = std::min(nb, Ms - j);
+      double *A11 = A + (size_t)j * M + j;
+      flare::launch_chol_panel(c->stream, jb, A11, M, info);
+      int m2 = Ms - j - jb;
+      if (m2 > 0) {
+        double one = 1.0, mone = -1.0;
+        double *A21 = A + (size_t)j * M + (j + jb);
+        double *A22 = A + (size_t)(j + jb) * M + (j + jb);
+        BLAS_CHECK(rocblas_dtrsm(c->blas, rocblas_side_right, rocblas_fill_lower,
+                                 rocblas_operation_transpose,
+                                 rocblas_diagonal_non_unit, m2, jb, &one, A11, M,
+                                 A21, M));
+        BLAS_CHECK(rocblas_dsyrk(c->blas, rocblas_fill_lower,
+                                 rocblas_operation_none, m2, jb, &mone, A21, M,
+                                 &one, A22, M));
+      }
+    }
+  }
+  flare::launch_zero_upper(c->stream, M, Lc, M);
+  for (int s = 0; s < gp->S; s++) {
+    int Ms = gp->m_by_type[s], o = gp->m_off[s];
+    if (!Ms) continue;
+    BLAS_CHECK(rocblas_dtrtri(c->blas, rocblas_fill_lower,
+                              rocblas_diagonal_non_unit, Ms,
+                              Lc + (size_t)o * M + o, M, Li + (size_t)o * M + o, M));
+  }
+  int hinfo = 0;
+  FLARE_HIP_CHECK(hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost,
+                                 c->stream));
+  c->sync();
+  REQUIRE(hinfo == 0, "K_uu + jitter is not positive definite");
+}
+
+int flare_sgp_create(flare_ctx *ctx, int n_species, int n_descriptors,
+                     const int *n_sparse_by_type, const double *d_sparse,
+                     double sigma, double power, double jitter,
+                     flare_sgp **out) {
+  return guarded([&] {
+    REQUIRE(ctx && n_sparse_by_type && out, "null argument");
+    REQUIRE(n_species >= 1 && n_species <= flare::kMaxSpecies, "n_species must be in [1, 8]");
+    REQUIRE(n_descriptors > 0, "n_descriptors must be positive");
+    REQUIRE(power > 0, "power must be positive");
+    *out = nullptr;
+    ctx->use();
+    std::unique_ptr<flare_sgp> gp(new flare_sgp);
+    gp->ctx = ctx;
+    gp->S = n_species;
+    gp->F = n_descriptors;
+    gp->sigma = sigma;
+    gp->sig2 = sigma * sigma;
+    gp->power = power;
+    gp->jitter = jitter;
+    gp->m_by_type.assign(n_sparse_by_type, n_sparse_by_type + n_species);
+    gp->m_off.assign(n_species + 1, 0);
+    for (int s = 0; s < n_species; s++) {
+      REQUIRE(gp->m_by_type[s] >= 0, "negative sparse count");
+      gp->m_off[s + 1] = gp->m_off[s] + gp->m_by_type[s];
+    }
+    gp->M = gp->m_off[n_species];
+    const int M = gp->M, F = gp->F;
+    REQUIRE(M == 0 || d_sparse, "null sparse descriptors");
+    gp->P.ensure((size_t)std::max(M, 1) * F * sizeof(double));
+    gp->Phat.ensure((size_t)std::max(M, 1) * F * sizeof(double));
+    gp->pnorm.ensure((size_t)std::max(M, 1) * sizeof(double));
+    gp->pnorm2.ensure((size_t)std::max(M, 1) * sizeof(double));
+    gp->alpha.ensure((size_t)std::max(M, 1) * sizeof(double));
+    FLARE_HIP_CHECK(hipMemsetAsync(gp->alpha.ptr, 0, std::max(M, 1) * sizeof(double),
+                                   ctx->stream));
+    if (M) {
+      FLARE_HIP_CHECK(hipMemcpyAsync(gp->P.ptr, d_sparse, (size_t)M * F * sizeof(double),
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+      flare::launch_row_norms(ctx->stream, M, F, gp->P.as<double>(),
+                              gp->pnorm.as<double>(), gp->Phat.as<double>());
+      flare::launch_square(ctx->stream, M, gp->pnorm.as<double>(), gp->pnorm2.as<double>());
+    }
+    sgp_factor(gp.get());
+    *out = gp.release();
+  });
+}
+
+int flare_sgp_destroy(flare_sgp *gp) {
+  return guarded([&] {
+    if (!gp) return;
+    gp->ctx->use();
+    (void)hipStreamSynchronize(gp->ctx->stream);
+    delete gp;
+  });
+}
+
+int flare_sgp_set_alpha(flare_ctx *ctx, flare_sgp *gp, const double *d_alpha) {
+  return guarded([&] {
+    REQUIRE(ctx && gp, "null argument");
+    REQUIRE(gp->M == 0 || d_alpha, "null alpha");
+    ctx->use();
+    if (gp->M)
+      FLARE_HIP_CHECK(hipMemcpyAsync(gp->alpha.ptr, d_alpha, gp->M * sizeof(double),
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+    ctx->sync();
+    gp->have_alpha = true;
+  });
+}
+
+int flare_sgp_get_L_inverse(flare_ctx *ctx, const flare_sgp *gp, double *d_out) {
+  return guarded([&] {
+    REQUIRE(ctx && gp && d_out, "null argument");
+    ctx->use();
+    flare::launch_transpose(ctx->stream, gp->M, gp->Linv_cm.as<double>(), d_out);
+    ctx->sync();
+  });
+}
+
+int flare_sgp_get_Kuu(flare_ctx *ctx, const flare_sgp *gp, double *d_out) {
+  return guarded([&] {
+    REQUIRE(ctx && gp && d_out, "null argument");
+    ctx->use();
+    if (gp->M)
+      FLARE_HIP_CHECK(hipMemcpyAsync(d_out, gp->Kuu.ptr,
+                                     (size_t)gp->M * gp->M * sizeof(double),
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+    ctx->sync();
+  });
+}
+
+struct GPState {
+  Forward fw;
+  double *kval = nullptr;  // per species blocks rows_s x M_s
+  std::vector<size_t> koff;
+};
+
+static GPState gp_mean(flare_ctx *ctx, flare_sgp *gp, const flare_b2_params *params,
+                       const flare_nlist *nl, const int *d_species, int form,
+                       double *d_mean_efs, bool want_k) {
+  REQUIRE(form == FLARE_FORM_PLAIN_FP64 || form == FLARE_FORM_ADJOINT_FP64,
+          "the sparse-GP path runs the FP64 forms");
+  flare::B2Dev p = make_b2(params, 0);
+  check_species(nl, p.S);
+  REQUIRE(p.S == gp->S && p.F == gp->F, "descriptor settings differ from the sparse set");
+  REQUIRE(gp->have_alpha, "alpha not set");
+  const int n = nl->n, F = p.F;
+  GPState st;
+  st.fw = run_forward(ctx, p, nl, d_species, flare::kPrecF64, "gp");
+  const double *B = (const double *)st.fw.B;
+  int off[flare::kMaxSpecies + 1];
+  nl->row_offsets(p.S, off);
+  size_t ktot = 0;
+  st.koff.assign(p.S + 1, 0);
+  for (int s = 0; s < p.S; s++) {
+    st.koff[s] = ktot;
+    ktot += (size_t)(off[s + 1] - off[s]) * gp->m_by_type[s];
+  }
+  st.koff[p.S] = ktot;
+  double *Kraw = ctx->buf_t<double>("gp.K", ktot);
+  st.kval = ctx->buf_t<double>("gp.kval", ktot);
+  double *cm = ctx->buf_t<double>("gp.C", ktot);
+  double *e_row = ctx->buf_t<double>("gp.e", n);
+  double *G = ctx->buf_t<double>("gp.G", (size_t)n * F);
+  for (int s = 0; s < p.S; s++) {
+    int rows = off[s + 1] - off[s], Ms = gp->m_by_type[s], o = gp->m_off[s];
+    if (!rows) continue;
+    double *Gs = G + (size_t)off[s] * F;
+    if (!Ms) {
+      FLARE_HIP_CHECK(hipMemsetAsync(Gs, 0, (size_t)rows * F * sizeof(double), ctx->stream));
+      FLARE_HIP_CHECK(hipMemsetAsync(e_row + off[s], 0, rows * sizeof(double), ctx->stream));
+      continue;
+    }
+    double *Ks = Kraw + st.koff[s];
+    gemm_rm(ctx, false, true, rows, Ms, F, 1.0, B + (size_t)off[s] * F, F,
+            gp->P.as<double>() + (size_t)o * F, F, 0.0, Ks, Ms);
+    flare::launch_gp_epilogue(ctx->stream, rows, Ms, 0, Ks, st.fw.norm2 + off[s],
+                              gp->pnorm.as<double>() + o, gp->alpha.as<double>() + o,
+                              gp->sig2, gp->power, st.kval + st.koff[s],
+                              cm + st.koff[s], e_row + off[s]);
+    gemm_rm(ctx, false, false, rows, F, Ms, 1.0, cm + st.koff[s], Ms,
+            gp->Phat.as<double>() + (size_t)o * F, F, 0.0, Gs, F);
+  }
+  double *w = ctx->buf_t<double>("gp.w", (size_t)n * F);
+  double *E_row = ctx->buf_t<double>("gp.E", n);
+  flare::launch_gp_w(ctx->stream, n, F, G, e_row, B, st.fw.norm2, gp->sig2,
+                     gp->power, w, E_row);
+  const int mode = form == FLARE_FORM_PLAIN_FP64 ? 0 : 1;
+  run_forces(ctx, p, nl, d_species, flare::kPrecF64, mode, st.fw, w, E_row,
+             d_mean_efs + 1, d_mean_efs, nullptr, d_mean_efs + 1 + 3 * (size_t)n,
+             "gp");
+  (void)want_k;
+  return st;
+}
+
+int flare_sgp_predict_mean(flare_ctx *ctx, flare_sgp *gp,
+                           const flare_b2_params *p, const flare_nlist *nl,
+                           const int *d_species, int form, double *d_mean_efs) {
+  return guarded([&] {
+    REQUIRE(ctx && gp && d_mean_efs, "null argument");
+    ctx->use();
+    gp_mean(ctx, gp, p, nl, d_species, form, d_mean_efs, false);
+    ctx->sync();
+  });
+}
+
+int flare_sgp_predict_local_uncertainties(flare_ctx *ctx, flare_sgp *gp,
+                                          const flare_b2_params *params,
+                                          const flare_nlist *nl,
+                                          const int *d_species, int form,
+                                          double *d_mean_efs,
+                                          double *d_variances) {
+  return guarded([&] {
+    REQUIRE(ctx && gp && d_mean_efs && d_variances, "null argument");
+    ctx->use();
+    GPState st = gp_mean(ctx, gp, params, nl, d_species, form, d_mean_efs, true);
+    const int n = nl->n, F = gp->F, M = gp->M;
+    int off[flare::kMaxSpecies + 1];
+    nl->row_offsets(gp->S, off);
+    double *kself = ctx->buf_t<double>("gp.kself", n);
+    flare::launch_kself(ctx->stream, n, F, (const double *)st.fw.B, st.fw.norm2,
+                        gp->sig2, gp->power, kself);
+    double *Q = ctx->buf_t<double>("gp.Q", st.koff[gp->S]);
+    for (int s = 0; s < gp->S; s++) {
+      int rows = off[s + 1] - off[s], Ms = gp->m_by_type[s], o = gp->m_off[s];
+      if (!rows) continue;
+      if (Ms)
+        gemm_rm(ctx, false, false, rows, Ms, Ms, 1.0, st.kval + st.koff[s], Ms,
+                gp->Linv_cm.as<double>() + (size_t)o * M + o, M, 0.0,
+                Q + st.koff[s], Ms);
+      flare::launch_row_sqsum_sub(ctx->stream, rows, Ms, Q + st.koff[s],
+                                  kself + off[s], d_variances + off[s]);
+    }
+    ctx->sync();
+  });
+}
+
+int flare_envs_envs(flare_ctx *ctx, int n_species, int n_descriptors,
+                    const int *n1_by_type, const double *d1,
+                    const int *n2_by_type, const double *d2, double sigma,
+                    double power, double *d_out) {
+  return guarded([&] {
+    REQUIRE(ctx && n1_by_type && n2_by_type && d_out, "null argument");
+    REQUIRE(n_species >= 1 && n_species <= flare::kMaxSpecies, "n_species must be in [1, 8]");
+    ctx->use();
+    const int F = n_descriptors;
+    int n1 = 0, n2 = 0;
+    std::vector<int> o1(n_species + 1, 0), o2(n_species + 1, 0);
+    for (int s = 0; s < n_species; s++) {
+      o1[s + 1] = o1[s] + n1_by_type[s];
+      o2[s + 1] = o2[s] + n2_by_type[s];
+    }
+    n1 = o1[n_species];
+    n2 = o2[n_species];
+    if (!n1 || !n2) return;
+    double *nrm1 = ctx->buf_t<double>("ee.n1", n1);
+    double *nrm2 = ctx->buf_t<double>("ee.n2", n2);
+    double *sq1 = ctx->buf_t<double>("ee.s1", n1);
+    flare::launch_row_norms(ctx->stream, n1, F, d1, nrm1, nullptr);
+    flare::launch_row_norms(ctx->stream, n2, F, d2, nrm2, nullptr);
+    flare::launch_square(ctx->stream, n1, nrm1, sq1);
+    FLARE_HIP_CHECK(hipMemsetAsync(d_out, 0, (size_t)n1 * n2 * sizeof(double), ctx->stream));
+    double *tmp = ctx->buf_t<double>("ee.tmp", (size_t)n1 * n2);
+    for (int s = 0; s < n_species; s++) {
+      int r = n1_by_type[s], m = n2_by_type[s];
+      if (!r || !m) continue;
+      double *blk = tmp;  // r x m contiguous
+      gemm_rm(ctx, false, true, r, m, F, 1.0, d1 + (size_t)o1[s] * F, F,
+              d2 + (size_t)o2[s] * F, F, 0.0, blk, m);
+      double *kv = ctx->buf_t<double>("ee.kv", (size_t)r * m);
+      flare::launch_gp_epilogue(ctx->stream, r, m, 0, blk, sq1 + o1[s], nrm2 + o2[s],
+                                nullptr, sigma * sigma, power, kv, nullptr, nullptr);
+      flare::launch_copy_block(ctx->stream, r, m, kv, m, d_out + (size_t)o1[s] * n2 + o2[s], n2);
+    }
+    ctx->sync();
+  });
+}
+
+int flare_sgp_mapping_coefficients(flare_ctx *ctx, const flare_sgp *gp,
+                                   double *d_out) {
+  return guarded([&] {
+    REQUIRE(ctx && gp && d_out, "null argument");
+    REQUIRE(gp->power == 1.0 || gp->power == 2.0,
+            "mapping coefficients exist for power 1 and 2 only");
+    ctx->use();
+    const int F = gp->F;
+    const int64_t bs = gp->power == 1.0 ? (int64_t)F : (int64_t)F * (F + 1) / 2;
+    double *full = ctx->buf_t<double>("map.full", (size_t)F * F);
+    for (int s = 0; s < gp->S; s++) {
+      int Ms = gp->m_by_type[s], o = gp->m_off[s];
+      double *dst = d_out + (size_t)s * bs;
+      if (!Ms) {
+        FLARE_HIP_CHECK(hipMemsetAsync(dst, 0, bs * sizeof(double), ctx->stream));
+        continue;
+      }
+      const double *Ph = gp->Phat.as<double>() + (size_t)o * F;
+      double *Y = ctx->buf_t<double>("map.Y", (size_t)Ms * F);
+      flare::launch_scale_rows(ctx->stream, Ms, F, Ph, gp->alpha.as<double>() + o, Y);
+      if (gp->power == 2.0) {
+        gemm_rm(ctx, true, false, F, F, Ms, gp->sig2, Ph, F, Y, F, 0.0, full, F);
+        flare::launch_pack_beta(ctx->stream, F, 2, full, dst);
+      } else {
+        flare::launch_colsum(ctx->stream, Ms, F, Y, dst);
+        flare::launch_scale(ctx->stream, F, gp->sig2, dst);
+      }
+    }
+    ctx->sync();
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,334 @@
+// descriptors.hip -- single bond and B2 power spectrum.
+//
+//  b2_forward_kernel   per atom: Chebyshev x cutoff and Y_lm per pair,
+//                      A[c, lm] = sum_pairs g_n Y_lm (b2.cpp:315-398,
+//                      lammps_descriptor.cpp:7-107), then
+//                      B[p(c1,c2) (L+1) + l] = sum_m A[c1,lm] A[c2,lm]
+//                      (b2.cpp:203-241) and |B|^2.  One wavefront per atom;
+//                      per-pair values are staged in LDS, each lane owns a
+//                      set of (n, lm) outputs.  FP32 instantiation uses
+//                      compensated (Kahan) sums.
+//  pair_dA_kernel      plain form: materialises dA/dr per pair in the
+//                      compact layout [pair][xyz][n][lm] (the neighbor's
+//                      species block only; b2.cpp:381-391).
+//  b2_dervs_kernel     DescriptorValues force derivatives dB/dr and
+//                      B . dB/dr per pair (b2.cpp:219-240).
+#include "common.h"
+#include "kernels.h"
+
+namespace flare {
+
+template <typename T>
+struct KahanAcc {
+  T s = 0, c = 0;
+  __device__ inline void add(T v) {
+    T y = v - c;
+    T t = s + y;
+    c = (t - s) - y;
+    s = t;
+  }
+};
+
+// One wavefront (64 threads) per atom.  Dynamic LDS layout (T):
+//   A[W], Acomp[W], g[64*N], Y[64*H]   and int spec[64]
+template <typename T, bool KAHAN>
+__global__ void __launch_bounds__(64)
+b2_forward_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
+                  const int *__restrict__ row_of, const T *__restrict__ ynorm,
+                  const int *__restrict__ b2_idx, T *__restrict__ A_out,
+                  T *__restrict__ B_out, double *__restrict__ norm2_out) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int i = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int N = p.N, H = p.H, NH = p.NH, W = p.W, F = p.F, S = p.S;
+  T *A = reinterpret_cast<T *>(smem);
+  T *Ac = A + W;
+  T *gb = Ac + W;
+  T *yb = gb + kWave * N;
+  int *spec = reinterpret_cast<int *>(yb + kWave * H);
+
+  for (int o = lane; o < W; o += kWave) {
+    A[o] = 0;
+    Ac[o] = 0;
+  }
+  const int si = species[i];
+  const int b = nl.cum[i], e = nl.cum[i + 1];
+  for (int c0 = b; c0 < e; c0 += kWave) {
+    const int cnt = min(kWave, e - c0);
+    __syncthreads();
+    if (lane < cnt) {
+      const int q = c0 + lane;
+      const double *rv = nl.rel + 4 * (int64_t)q;
+      const int sj = nl.nspec[q];
+      int sflag = -1;
+      if (pair_inside(p, si, sj, rv)) {
+        sflag = sj;
+        T r = (T)rv[0];
+        T rinv = T(1) / r;
+        T ux = (T)rv[1] * rinv, uy = (T)rv[2] * rinv, uz = (T)rv[3] * rinv;
+        T g[kMaxN];
+        radial_basis<T>(r, (T)p.cut[si * S + sj], (T)p.r1, N, p.cutoff_kind, g,
+                        (T *)nullptr);
+        for (int n = 0; n < N; n++) gb[lane * N + n] = g[n];
+        T *yrow = yb + lane * H;
+        ylm_visit<T, false>(ux, uy, uz, rinv, p.L, ynorm,
+                            [&](int idx, T y, T, T, T) { yrow[idx] = y; });
+      }
+      spec[lane] = sflag;
+    }
+    __syncthreads();
+    for (int o = lane; o < NH; o += kWave) {
+      const int n = o / H, lm = o - (o / H) * H;
+      for (int s = 0; s < S; s++) {
+        if (KAHAN) {
+          KahanAcc<T> acc;
+          for (int k = 0; k < cnt; k++)
+            if (spec[k] == s) acc.add(gb[k * N + n] * yb[k * H + lm]);
+          // merge into the running (sum, compensation) pair
+          KahanAcc<T> run;
+          run.s = A[s * NH + o];
+          run.c = Ac[s * NH + o];
+          run.add(acc.s);
+          run.add(-acc.c);
+          A[s * NH + o] = run.s;
+          Ac[s * NH + o] = run.c;
+        } else {
+          T acc = 0;
+          for (int k = 0; k < cnt; k++)
+            if (spec[k] == s) acc += gb[k * N + n] * yb[k * H + lm];
+          A[s * NH + o] += acc;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  T *Ai = A_out + (int64_t)i * W;
+  for (int o = lane; o < W; o += kWave) Ai[o] = A[o];
+  // B2 contraction; b2_idx[f] = (c1 << 20) | (c2 << 8) | l
+  const int row = row_of[i];
+  T *Bi = B_out + (int64_t)row * F;
+  double nrm = 0;
+  for (int f = lane; f < F; f += kWave) {
+    const int code = b2_idx[f];
+    const int c1 = code >> 20, c2 = (code >> 8) & 0xfff, l = code & 0xff;
+    const T *a1 = A + c1 * H + l * l;
+    const T *a2 = A + c2 * H + l * l;
+    T s = 0;
+    for (int m = 0; m < 2 * l + 1; m++) s += a1[m] * a2[m];
+    Bi[f] = s;
+    nrm += (double)s * (double)s;
+  }
+  nrm = wave_sum(nrm);
+  if (lane == 0) norm2_out[row] = nrm;
+}
+
+// Plain form: dA/dr for `pb` (<= 64) pairs per workgroup of 256 threads.
+// out[(p*3 + c)*NH + n*H + lm]; zero for pairs outside the species cutoff.
+__global__ void __launch_bounds__(256)
+pair_dA_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
+               const double *__restrict__ ynorm, int pb,
+               double *__restrict__ out) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = p.N, H = p.H, NH = p.NH;
+  double *g = reinterpret_cast<double *>(smem);  // pb*N
+  double *dg = g + pb * N;                       // pb*N
+  double *Y = dg + pb * N;                       // pb*H
+  double *Yg = Y + pb * H;                       // pb*3*H
+  double *uhat = Yg + pb * 3 * H;                // pb*3
+  int *ins = reinterpret_cast<int *>(uhat + pb * 3);
+  const int64_t p0 = (int64_t)blockIdx.x * pb;
+  const int t = threadIdx.x;
+  if (t < pb) {
+    const int64_t q = p0 + t;
+    int inside = 0;
+    if (q < nl.P) {
+      const double *rv = nl.rel + 4 * q;
+      const int si = species[nl.center[q]], sj = nl.nspec[q];
+      if (pair_inside(p, si, sj, rv)) {
+        inside = 1;
+        double r = rv[0], rinv = 1.0 / r;
+        double ux = rv[1] * rinv, uy = rv[2] * rinv, uz = rv[3] * rinv;
+        double gl[kMaxN], dgl[kMaxN];
+        radial_basis<double>(r, p.cut[si * p.S + sj], p.r1, N, p.cutoff_kind,
+                             gl, dgl);
+        for (int n = 0; n < N; n++) {
+          g[t * N + n] = gl[n];
+          dg[t * N + n] = dgl[n];
+        }
+        double *yr = Y + t * H;
+        double *ygr = Yg + t * 3 * H;
+        ylm_visit<double, true>(ux, uy, uz, rinv, p.L, ynorm,
+                                [&](int idx, double y, double yx, double yy,
+                                    double yz) {
+                                  yr[idx] = y;
+                                  ygr[idx] = yx;
+                                  ygr[H + idx] = yy;
+                                  ygr[2 * H + idx] = yz;
+                                });
+        uhat[3 * t] = ux;
+        uhat[3 * t + 1] = uy;
+        uhat[3 * t + 2] = uz;
+      }
+    }
+    ins[t] = inside;
+  }
+  __syncthreads();
+  const int per = 3 * NH;
+  const int64_t npair = min((int64_t)pb, nl.P - p0);
+  const int64_t total = npair * per;
+  double *o = out + p0 * per;
+  for (int64_t v = t; v < total; v += blockDim.x) {
+    const int k = (int)(v / per);
+    const int rem = (int)(v - (int64_t)k * per);
+    const int c = rem / NH;
+    const int nl_ = rem - c * NH;
+    const int n = nl_ / H, lm = nl_ - (nl_ / H) * H;
+    double val = 0;
+    if (ins[k])
+      val = dg[k * N + n] * uhat[3 * k + c] * Y[k * H + lm] +
+            g[k * N + n] * Yg[k * 3 * H + c * H + lm];
+    o[v] = val;
+  }
+}
+
+// DescriptorValues force derivatives.  One wavefront per filtered pair q
+// (pairs of the B2 list in species-grouped order, fpair[q] = index in the
+// Structure list).  Writes dB[(3q + c)*F + f], fdots[3q + c] = B . dB,
+// coords[3q + c] and nidx[q].
+__global__ void __launch_bounds__(64)
+b2_dervs_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
+                const int *__restrict__ row_of, const double *__restrict__ ynorm,
+                const int *__restrict__ b2_idx, int64_t nq,
+                const int64_t *__restrict__ fpair,
+                const double *__restrict__ A_in, const double *__restrict__ B_in,
+                double *__restrict__ dB, double *__restrict__ fdots,
+                double *__restrict__ coords, int *__restrict__ nidx) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = p.N, H = p.H, NH = p.NH, W = p.W, F = p.F;
+  double *A = reinterpret_cast<double *>(smem);  // W
+  double *dA = A + W;                            // 3*NH
+  double *g = dA + 3 * NH;                       // N
+  double *dg = g + N;                            // N
+  double *Y = dg + N;                            // H
+  double *Yg = Y + H;                            // 3H
+  double *uh = Yg + 3 * H;                       // 3
+  const int64_t q = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int64_t ps = fpair[q];
+  const int i = nl.center[ps];
+  const int si = species[i], sj = nl.nspec[ps];
+  const double *rv = nl.rel + 4 * ps;
+  for (int o = lane; o < W; o += kWave) A[o] = A_in[(int64_t)i * W + o];
+  if (lane == 0) {
+    double r = rv[0], rinv = 1.0 / r;
+    double ux = rv[1] * rinv, uy = rv[2] * rinv, uz = rv[3] * rinv;
+    double gl[kMaxN], dgl[kMaxN];
+    radial_basis<double>(r, p.cut[si * p.S + sj], p.r1, N, p.cutoff_kind, gl,
+                         dgl);
+    for (int n = 0; n < N; n++) {
+      g[n] = gl[n];
+      dg[n] = dgl[n];
+    }
+    ylm_visit<double, true>(ux, uy, uz, rinv, p.L, ynorm,
+                            [&](int idx, double y, double yx, double yy,
+                                double yz) {
+                              Y[idx] = y;
+                              Yg[idx] = yx;
+                              Yg[H + idx] = yy;
+                              Yg[2 * H + idx] = yz;
+                            });
+    uh[0] = ux;
+    uh[1] = uy;
+    uh[2] = uz;
+    coords[3 * q + 0] = rv[1];
+    coords[3 * q + 1] = rv[2];
+    coords[3 * q + 2] = rv[3];
+    nidx[q] = nl.idx[ps];
+  }
+  __syncthreads();
+  for (int v = lane; v < 3 * NH; v += kWave) {
+    int c = v / NH, rem = v - c * NH;
+    int n = rem / H, lm = rem - (rem / H) * H;
+    dA[v] = dg[n] * uh[c] * Y[lm] + g[n] * Yg[c * H + lm];
+  }
+  __syncthreads();
+  const double *Bi = B_in + (int64_t)row_of[i] * F;
+  double fd[3] = {0, 0, 0};
+  for (int f = lane; f < F; f += kWave) {
+    const int code = b2_idx[f];
+    const int c1 = code >> 20, c2 = (code >> 8) & 0xfff, l = code & 0xff;
+    const bool in1 = (c1 / N) == sj, in2 = (c2 / N) == sj;
+    const int n1 = c1 - (c1 / N) * N, n2 = c2 - (c2 / N) * N;
+    for (int c = 0; c < 3; c++) {
+      double s = 0;
+      for (int m = 0; m < 2 * l + 1; m++) {
+        int lm = l * l + m;
+        double d2 = in2 ? dA[c * NH + n2 * H + lm] : 0.0;
+        double d1 = in1 ? dA[c * NH + n1 * H + lm] : 0.0;
+        s += A[c1 * H + lm] * d2 + d1 * A[c2 * H + lm];
+      }
+      dB[(3 * q + c) * F + f] = s;
+      fd[c] += s * Bi[f];
+    }
+  }
+  for (int c = 0; c < 3; c++) {
+    double v = wave_sum(fd[c]);
+    if (lane == 0) fdots[3 * q + c] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+
+void launch_b2_forward(hipStream_t st, const B2Dev &p, const NListDev &nl,
+                       const int *species, const int *row_of,
+                       const double *ynorm, const float *ynorm_f,
+                       const int *b2_idx, int prec,
+                       void *A_out, void *B_out, double *norm2_out) {
+  if (nl.n == 0) return;
+  size_t tsz = prec == kPrecF64 ? sizeof(double) : sizeof(float);
+  size_t lds = tsz * (2 * (size_t)p.W + kWave * (size_t)(p.N + p.H)) +
+               kWave * sizeof(int);
+  FLARE_REQUIRE(lds <= 64 * 1024,
+                "descriptor settings too large for the single-bond kernel");
+  if (prec == kPrecF64)
+    hipLaunchKernelGGL((b2_forward_kernel<double, false>), dim3(nl.n),
+                       dim3(kWave), lds, st, p, nl, species, row_of, ynorm,
+                       b2_idx, (double *)A_out, (double *)B_out, norm2_out);
+  else
+    hipLaunchKernelGGL((b2_forward_kernel<float, true>), dim3(nl.n),
+                       dim3(kWave), lds, st, p, nl, species, row_of, ynorm_f,
+                       b2_idx, (float *)A_out, (float *)B_out, norm2_out);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_pair_dA(hipStream_t st, const B2Dev &p, const NListDev &nl,
+                    const int *species, const double *ynorm, double *out) {
+  if (nl.P == 0) return;
+  // pairs per workgroup: as many as fit 64 KB of LDS, at most 64
+  const size_t per_pair =
+      sizeof(double) * (2 * (size_t)p.N + 4 * (size_t)p.H + 3) + sizeof(int);
+  int pb = (int)std::min<size_t>(kWave, (64 * 1024) / per_pair);
+  FLARE_REQUIRE(pb >= 1, "descriptor settings too large for the plain form");
+  size_t lds = per_pair * pb;
+  int64_t blocks = (nl.P + pb - 1) / pb;
+  hipLaunchKernelGGL(pair_dA_kernel, dim3((unsigned)blocks), dim3(256), lds,
+                     st, p, nl, species, ynorm, pb, out);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_b2_dervs(hipStream_t st, const B2Dev &p, const NListDev &nl,
+                     const int *species, const int *row_of,
+                     const double *ynorm, const int *b2_idx, int64_t nq,
+                     const int64_t *fpair, const double *A, const double *B,
+                     double *dB, double *fdots, double *coords, int *nidx) {
+  if (nq == 0) return;
+  size_t lds = sizeof(double) *
+               ((size_t)p.W + 3 * (size_t)p.NH + 2 * (size_t)p.N + 4 * (size_t)p.H + 3);
+  hipLaunchKernelGGL(b2_dervs_kernel, dim3((unsigned)nq), dim3(kWave), lds, st,
+                     p, nl, species, row_of, ynorm, b2_idx, nq, fpair, A, B, dB,
+                     fdots, coords, nidx);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace flare

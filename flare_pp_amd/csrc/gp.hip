@@ -1,0 +1,311 @@
+// gp.hip -- NormalizedDotProduct sparse-GP pieces.
+//
+//  gp_epilogue   k(i,u) = sig2 (B_i.P_u / (|B_i||P_u|))^xi on the raw
+//                GEMM output, zero when either norm < 1e-8
+//                (normalized_dot_product.cpp:27-87, 115-210); also the
+//                adjoint weights c = alpha t^(xi-1) and e = sum alpha t^xi.
+//  gp_w          dE_i/dB_i = sig2 xi / |B| (sum_u c_u P_hat_u - e B_hat),
+//                the chain rule of envs_struc's force kernel
+//                (normalized_dot_product.cpp:176-190) applied to alpha.
+//  kuu / chol    K_uu + jitter I (sparse_gp.cpp:581-582), blocked Cholesky
+//                panel; the triangular solves/updates go to rocBLAS.
+//  pack/unpack   mapping coefficient file convention: packed upper
+//                triangle, off-diagonals doubled (normalized_dot_product.cpp
+//                :736-753, pair_flare.cpp:378-395).
+#include "common.h"
+#include "kernels.h"
+
+namespace flare {
+
+__device__ inline double powi(double t, double power) {
+  if (power == 2.0) return t * t;
+  if (power == 1.0) return t;
+  return pow(t, power);
+}
+
+__global__ void __launch_bounds__(64)
+gp_epilogue_kernel(int rows, int M, int normalized, const double *__restrict__ K,
+                   const double *__restrict__ norm2_rows,
+                   const double *__restrict__ pnorm,
+                   const double *__restrict__ alpha, double sig2, double power,
+                   double *__restrict__ kval, double *__restrict__ cmat,
+                   double *__restrict__ e_row) {
+  const int r = blockIdx.x;
+  const int lane = threadIdx.x;
+  const double nb = sqrt(norm2_rows[r]);
+  const bool empty = nb < 1e-8;
+  double e = 0;
+  for (int u = lane; u < M; u += kWave) {
+    const double pn = pnorm[u];
+    double kv = 0, cv = 0;
+    if (!empty && !(pn < 1e-8)) {
+      const double dot = K[(int64_t)r * M + u];
+      const double t = normalized ? dot : dot / (nb * pn);
+      const double tp = powi(t, power - 1.0);
+      kv = sig2 * powi(t, power);
+      cv = alpha ? alpha[u] * tp : 0.0;
+      if (alpha) e += alpha[u] * powi(t, power);
+    }
+    if (kval) kval[(int64_t)r * M + u] = kv;
+    if (cmat) cmat[(int64_t)r * M + u] = cv;
+  }
+  e = wave_sum(e);
+  if (lane == 0 && e_row) e_row[r] = e;
+}
+
+__global__ void __launch_bounds__(64)
+gp_w_kernel(int rows, int F, const double *__restrict__ G,
+            const double *__restrict__ e_row, const double *__restrict__ B,
+            const double *__restrict__ norm2, double sig2, double power,
+            double *__restrict__ w, double *__restrict__ E_row) {
+  const int r = blockIdx.x;
+  const int lane = threadIdx.x;
+  const double nb = sqrt(norm2[r]);
+  const bool empty = nb < 1e-8;
+  const double e = e_row[r];
+  const double s = empty ? 0.0 : sig2 * power / nb;
+  for (int f = lane; f < F; f += kWave) {
+    double v = 0;
+    if (!empty)
+      v = s * (G[(int64_t)r * F + f] - e * B[(int64_t)r * F + f] / nb);
+    w[(int64_t)r * F + f] = v;
+  }
+  if (lane == 0) E_row[r] = empty ? 0.0 : sig2 * e;
+}
+
+__global__ void __launch_bounds__(64)
+row_sqsum_sub_kernel(int rows, int M, const double *__restrict__ Q,
+                     const double *__restrict__ kself,
+                     double *__restrict__ var) {
+  const int r = blockIdx.x;
+  const int lane = threadIdx.x;
+  double s = 0;
+  for (int u = lane; u < M; u += kWave) {
+    double q = Q[(int64_t)r * M + u];
+    s += q * q;
+  }
+  s = wave_sum(s);
+  if (lane == 0) var[r] = kself[r] - s;
+}
+
+__global__ void __launch_bounds__(64)
+kself_kernel(int rows, int F, const double *__restrict__ B,
+             const double *__restrict__ norm2, double sig2, double power,
+             double *__restrict__ kself) {
+  const int r = blockIdx.x;
+  const int lane = threadIdx.x;
+  const double nb = sqrt(norm2[r]);
+  double d = 0;
+  for (int f = lane; f < F; f += kWave) {
+    double b = B[(int64_t)r * F + f];
+    d += b * b;
+  }
+  d = wave_sum(d);
+  if (lane == 0) kself[r] = (nb < 1e-8) ? 0.0 : sig2 * powi(d / (nb * nb), power);
+}
+
+__global__ void __launch_bounds__(64)
+row_norms_kernel(int rows, int F, const double *__restrict__ X,
+                 double *__restrict__ norm, double *__restrict__ xhat) {
+  const int r = blockIdx.x;
+  const int lane = threadIdx.x;
+  double s = 0;
+  for (int f = lane; f < F; f += kWave) {
+    double x = X[(int64_t)r * F + f];
+    s += x * x;
+  }
+  s = wave_sum(s);
+  const double nrm = sqrt(s);
+  if (lane == 0 && norm) norm[r] = nrm;
+  if (xhat)
+    for (int f = lane; f < F; f += kWave)
+      xhat[(int64_t)r * F + f] = nrm < 1e-8 ? 0.0 : X[(int64_t)r * F + f] / nrm;
+}
+
+// Cholesky of one nb x nb diagonal block (column-major, lower), nb <= 64.
+__global__ void __launch_bounds__(256)
+chol_panel_kernel(int nb, double *__restrict__ A, int ld, int *info) {
+  __shared__ double a[64][65];
+  const int t = threadIdx.x;
+  for (int q = t; q < nb * nb; q += blockDim.x) {
+    int i = q % nb, j = q / nb;
+    a[i][j] = A[(int64_t)j * ld + i];
+  }
+  __syncthreads();
+  for (int j = 0; j < nb; j++) {
+    if (t == 0) {
+      double d = a[j][j];
+      if (!(d > 0)) {
+        atomicExch(info, 1);
+        d = 1.0;
+      }
+      a[j][j] = sqrt(d);
+    }
+    __syncthreads();
+    for (int i = j + 1 + t; i < nb; i += blockDim.x) a[i][j] /= a[j][j];
+    __syncthreads();
+    const int m = nb - j - 1;
+    for (int q = t; q < m * m; q += blockDim.x) {
+      int i = j + 1 + q / m, k = j + 1 + q % m;
+      if (k <= i) a[i][k] -= a[i][j] * a[k][j];
+    }
+    __syncthreads();
+  }
+  for (int q = t; q < nb * nb; q += blockDim.x) {
+    int i = q % nb, j = q / nb;
+    A[(int64_t)j * ld + i] = (i >= j) ? a[i][j] : 0.0;
+  }
+}
+
+__global__ void zero_upper_kernel(int M, double *__restrict__ A, int ld) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)M * M) return;
+  int j = (int)(q / M), i = (int)(q - (int64_t)j * M);  // column-major (i, j)
+  if (i < j) A[(int64_t)j * ld + i] = 0.0;
+}
+
+__global__ void scale_rows_kernel(int rows, int F, const double *__restrict__ X,
+                                  const double *__restrict__ s,
+                                  double *__restrict__ out) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)rows * F) return;
+  out[q] = X[q] * s[q / F];
+}
+
+// full (F x F symmetric) -> packed file order (one thread per row i)
+__global__ void pack_beta_kernel(int F, const double *__restrict__ full,
+                                 double *__restrict__ packed) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= F) return;
+  int64_t base = (int64_t)i * F - (int64_t)i * (i - 1) / 2;
+  for (int j = i; j < F; j++) {
+    double v = full[(int64_t)i * F + j];
+    packed[base + (j - i)] = (i == j) ? v : 2 * v;
+  }
+}
+
+__global__ void unpack_beta_kernel(int F, const double *__restrict__ packed,
+                                   double *__restrict__ full) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= F) return;
+  int64_t base = (int64_t)i * F - (int64_t)i * (i - 1) / 2;
+  for (int j = i; j < F; j++) {
+    double v = packed[base + (j - i)];
+    if (i == j) {
+      full[(int64_t)i * F + j] = v;
+    } else {
+      full[(int64_t)i * F + j] = v / 2;
+      full[(int64_t)j * F + i] = v / 2;
+    }
+  }
+}
+
+__global__ void to_f32_kernel(int64_t n, const double *__restrict__ x,
+                              float *__restrict__ y) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) y[q] = (float)x[q];
+}
+
+// ---------------------------------------------------------------------------
+static inline unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+void launch_gp_epilogue(hipStream_t st, int rows, int M, int normalized,
+                        const double *K, const double *norm2_rows,
+                        const double *pnorm, const double *alpha, double sig2,
+                        double power, double *kval, double *cmat,
+                        double *e_row) {
+  if (rows == 0) return;
+  hipLaunchKernelGGL(gp_epilogue_kernel, dim3(rows), dim3(kWave), 0, st, rows,
+                     M, normalized, K, norm2_rows, pnorm, alpha, sig2, power,
+                     kval, cmat, e_row);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_gp_w(hipStream_t st, int rows, int F, const double *G,
+                 const double *e_row, const double *B, const double *norm2,
+                 double sig2, double power, double *w, double *E_row) {
+  if (rows == 0) return;
+  hipLaunchKernelGGL(gp_w_kernel, dim3(rows), dim3(kWave), 0, st, rows, F, G,
+                     e_row, B, norm2, sig2, power, w, E_row);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_row_sqsum_sub(hipStream_t st, int rows, int M, const double *Q,
+                          const double *kself, double *var) {
+  if (rows == 0) return;
+  hipLaunchKernelGGL(row_sqsum_sub_kernel, dim3(rows), dim3(kWave), 0, st,
+                     rows, M, Q, kself, var);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_kself(hipStream_t st, int rows, int F, const double *B,
+                  const double *norm2, double sig2, double power,
+                  double *kself) {
+  if (rows == 0) return;
+  hipLaunchKernelGGL(kself_kernel, dim3(rows), dim3(kWave), 0, st, rows, F, B,
+                     norm2, sig2, power, kself);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_row_norms(hipStream_t st, int rows, int F, const double *X,
+                      double *norm, double *xhat) {
+  if (rows == 0) return;
+  hipLaunchKernelGGL(row_norms_kernel, dim3(rows), dim3(kWave), 0, st, rows, F,
+                     X, norm, xhat);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_chol_panel(hipStream_t st, int nb, double *A, int ld, int *info) {
+  FLARE_REQUIRE(nb > 0 && nb <= 64, "panel width must be in [1, 64]");
+  hipLaunchKernelGGL(chol_panel_kernel, dim3(1), dim3(256), 0, st, nb, A, ld,
+                     info);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_zero_upper(hipStream_t st, int M, double *A, int ld) {
+  if (M == 0) return;
+  hipLaunchKernelGGL(zero_upper_kernel, dim3(nblk((int64_t)M * M, 256)),
+                     dim3(256), 0, st, M, A, ld);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_scale_rows(hipStream_t st, int rows, int F, const double *X,
+                       const double *s, double *out) {
+  if (rows == 0) return;
+  hipLaunchKernelGGL(scale_rows_kernel, dim3(nblk((int64_t)rows * F, 256)),
+                     dim3(256), 0, st, rows, F, X, s, out);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_pack_beta(hipStream_t st, int F, int power, const double *full,
+                      double *packed) {
+  if (power == 1) {
+    FLARE_HIP_CHECK(hipMemcpyAsync(packed, full, sizeof(double) * F,
+                                   hipMemcpyDeviceToDevice, st));
+    return;
+  }
+  hipLaunchKernelGGL(pack_beta_kernel, dim3(nblk(F, 64)), dim3(64), 0, st, F,
+                     full, packed);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_unpack_beta(hipStream_t st, int F, int power, const double *packed,
+                        double *full) {
+  if (power == 1) {
+    FLARE_HIP_CHECK(hipMemcpyAsync(full, packed, sizeof(double) * F,
+                                   hipMemcpyDeviceToDevice, st));
+    return;
+  }
+  hipLaunchKernelGGL(unpack_beta_kernel, dim3(nblk(F, 64)), dim3(64), 0, st, F,
+                     packed, full);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_to_f32(hipStream_t st, int64_t n, const double *x, float *y) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(to_f32_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, n, x,
+                     y);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace flare

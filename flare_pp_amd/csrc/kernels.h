@@ -1,0 +1,143 @@
+// kernels.h -- device buffers and the host-side launchers of every kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "common.h"
+
+namespace flare {
+
+enum Prec { kPrecF64 = 0, kPrecF32 = 1 };
+
+// Growable device allocation owned by a handle.
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t cap = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  void ensure(size_t bytes) {
+    if (bytes <= cap && ptr) return;
+    if (ptr) {
+      FLARE_HIP_CHECK(hipFree(ptr));
+      ptr = nullptr;
+      cap = 0;
+    }
+    size_t b = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&ptr, b);
+    if (e != hipSuccess) {
+      ptr = nullptr;
+      throw Error{2, std::string("hipMalloc of ") + std::to_string(b) +
+                         " bytes failed: " + hipGetErrorString(e)};
+    }
+    cap = b;
+  }
+  template <typename T>
+  T *as() const {
+    return reinterpret_cast<T *>(ptr);
+  }
+  ~DevBuf() {
+    if (ptr) (void)hipFree(ptr);
+  }
+};
+
+// Cell matrices of the wrap (structure.cpp:120-133) and the cell-list grid.
+struct WrapHost {
+  double ct[9], cdi[9], cd[9], cti[9], cell[9];
+  int nb[3];
+  int sweep;
+  double cutoff, volume, single_sweep;
+};
+
+struct NListBuffers {
+  DevBuf wrapped, bin_of, bin_count, bin_start, bin_fill, bin_atoms, count,
+      cum, keys_tmp, rel_tmp, keys, rel, idx, nspec, center, rev;
+  DevBuf row_of, atom_of_row, species_of_row, species_count;
+  int P = 0;
+};
+
+// neighbors.hip
+void nlist_build_device(hipStream_t st, NListBuffers &b, int n,
+                        const double *d_pos, const int *d_species,
+                        const WrapHost &wh);
+// species-grouped row map: row_of[i], atom_of_row[row], species_of_row[row]
+// and per-species counts (kMaxSpecies ints, copied to host_counts).
+// Returns the number of species actually present (max species + 1).
+int species_rows_device(hipStream_t st, NListBuffers &b, int n,
+                        const int *d_species, int *host_counts);
+
+// descriptors.hip
+void launch_b2_forward(hipStream_t st, const B2Dev &p, const NListDev &nl,
+                       const int *species, const int *row_of,
+                       const double *ynorm, const float *ynorm_f,
+                       const int *b2_idx, int prec, void *A_out, void *B_out,
+                       double *norm2_out);
+void launch_pair_dA(hipStream_t st, const B2Dev &p, const NListDev &nl,
+                    const int *species, const double *ynorm, double *out);
+void launch_b2_dervs(hipStream_t st, const B2Dev &p, const NListDev &nl,
+                     const int *species, const int *row_of,
+                     const double *ynorm, const int *b2_idx, int64_t nq,
+                     const int64_t *fpair, const double *A, const double *B,
+                     double *dB, double *fdots, double *coords, int *nidx);
+
+// forces.hip
+// Mapped energy per row and w = dE/dB (lammps_descriptor.cpp:250-304).
+// X = GEMM input rows (B itself, or B/|B| when normalized), Z = beta X.
+// power 1: beta1 holds one F-vector per species, Z unused.
+void launch_mapped_w(hipStream_t st, int prec, int n_rows, int F, int power,
+                     int normalized, const void *X, const void *Z,
+                     const double *norm2, const int *species_of_row,
+                     const void *beta1, void *w_out, double *e_row);
+// Per-atom u = dE/dA, pair forces f_ik and per-atom virial rows (9 values,
+// V[a][b] = -sum_k r_a f_b).  mode 0: plain (reads dA from `dA`),
+// mode 1: adjoint (recomputes the pair basis).
+void launch_forces(hipStream_t st, int prec, int mode, const B2Dev &p,
+                   const NListDev &nl, const int *species, const int *row_of,
+                   const double *ynorm, const float *ynorm_f, const int *b2_idx,
+                   const void *A, const void *w, const double *dA,
+                   double *pair_f, double *vir_atom);
+// F_a = sum_k (f_ak - f_rev(ak)) into forces (stride 3)
+void launch_gather(hipStream_t st, const NListDev &nl, const double *pair_f,
+                   double *forces);
+// column sums of an (n x k) row-major matrix into out[k], deterministic
+void launch_colsum(hipStream_t st, int n, int k, const double *x, double *out);
+// final assembly of energy, virial (LAMMPS order) and GP stress
+void launch_finish(hipStream_t st, const double *sums /* e, V9 */,
+                   double inv_volume, double *energy, double *virial6,
+                   double *stress6);
+
+// gp.hip
+// GP kernel epilogue on raw dot products K = X P^T (rows x M), per row:
+//   t = K/(|B||P|), k = sig2 t^xi (stored in kval), c = alpha t^(xi-1),
+//   e = sum alpha t^xi; empty rows/columns give zeros.
+void launch_gp_epilogue(hipStream_t st, int rows, int M, int normalized,
+                        const double *K, const double *norm2_rows,
+                        const double *pnorm, const double *alpha, double sig2,
+                        double power, double *kval, double *cmat,
+                        double *e_row);
+// w = sig2 xi / |B| (G - e B / |B|) (Appendix-A chain rule)
+void launch_gp_w(hipStream_t st, int rows, int F, const double *G,
+                 const double *e_row, const double *B, const double *norm2,
+                 double sig2, double power, double *w, double *E_row);
+void launch_row_sqsum_sub(hipStream_t st, int rows, int M, const double *Q,
+                          const double *kself, double *var);
+void launch_kself(hipStream_t st, int rows, int F, const double *B,
+                  const double *norm2, double sig2, double power,
+                  double *kself);
+void launch_row_norms(hipStream_t st, int rows, int F, const double *X,
+                      double *norm, double *xhat);
+// panel Cholesky of an nb x nb diagonal block (column-major, leading dim ld)
+void launch_chol_panel(hipStream_t st, int nb, double *A, int ld, int *info);
+void launch_zero_upper(hipStream_t st, int M, double *A, int ld);
+void launch_scale_rows(hipStream_t st, int rows, int F, const double *X,
+                       const double *s, double *out);
+void launch_pack_beta(hipStream_t st, int F, int power, const double *full,
+                      double *packed);
+void launch_unpack_beta(hipStream_t st, int F, int power, const double *packed,
+                        double *full);
+void launch_to_f32(hipStream_t st, int64_t n, const double *x, float *y);
+
+}  // namespace flare

@@ -1,0 +1,490 @@
+// neighbors.hip -- Structure neighbor list on the GPU.
+//
+// Produces the list of Structure::compute_neighbors (structure.cpp:52-118)
+// with the same set and the same order: for atom i, neighbors sorted by
+// (j, s1, s2, s3) with s in [-sweep, sweep]^3, predicate 0 < |r| < cutoff on
+// r = (w_j - w_i) + s1 a1 + s2 a2 + s3 a3 (structure.cpp:68-77), evaluated
+// with the reference's operation order and no FMA contraction so that the
+// relative vectors are bit-identical.
+//
+// Build: wrap + bin (fractional cell list) -> per-atom candidate count ->
+// scan -> fill -> per-atom rank sort by key -> reverse-pair index.  One
+// wavefront per atom enumerates the candidates of the 27 (or, along thin
+// axes, all) neighboring bins cooperatively.
+#include "common.h"
+#include "kernels.h"
+
+namespace flare {
+
+// 3x3 matrices used by the wrap (structure.cpp:120-133), row-major.
+struct WrapMats {
+  double ct[9], cdi[9], cd[9], cti[9], cell[9];
+  int nb[3];
+  int sweep;
+  double cutoff;
+};
+
+__device__ inline void rowmul3(const double *v, const double *b, double *o) {
+#pragma clang fp contract(off)
+  for (int j = 0; j < 3; j++) {
+    double s = v[0] * b[j];
+    s = s + v[1] * b[3 + j];
+    s = s + v[2] * b[6 + j];
+    o[j] = s;
+  }
+}
+
+__global__ void wrap_bin_kernel(int n, const double *__restrict__ pos,
+                                WrapMats m, double *__restrict__ wrapped,
+                                int *__restrict__ bin_of) {
+#pragma clang fp contract(off)
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+  double t[3], rel[3], rw[3], t2[3], w[3];
+  rowmul3(p, m.ct, t);
+  rowmul3(t, m.cdi, rel);
+  for (int c = 0; c < 3; c++) rw[c] = rel[c] - floor(rel[c]);
+  rowmul3(rw, m.cd, t2);
+  rowmul3(t2, m.cti, w);
+  for (int c = 0; c < 3; c++) wrapped[3 * i + c] = w[c];
+  int b[3];
+  for (int c = 0; c < 3; c++) {
+    int v = (int)floor(rw[c] * m.nb[c]);
+    b[c] = v < 0 ? 0 : (v >= m.nb[c] ? m.nb[c] - 1 : v);
+  }
+  bin_of[i] = (b[0] * m.nb[1] + b[1]) * m.nb[2] + b[2];
+}
+
+__global__ void bin_count_kernel(int n, const int *__restrict__ bin_of,
+                                 int *__restrict__ bin_count) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&bin_count[bin_of[i] + 1], 1);
+}
+
+__global__ void bin_fill_kernel(int n, const int *__restrict__ bin_of,
+                                const int *__restrict__ bin_start,
+                                int *__restrict__ fillp,
+                                int *__restrict__ bin_atoms) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int b = bin_of[i];
+  int slot = atomicAdd(&fillp[b], 1);
+  bin_atoms[bin_start[b] + slot] = i;
+}
+
+// Exclusive scan of n ints into out[0..n] (out[n] = total), one workgroup.
+__global__ void __launch_bounds__(1024)
+scan_kernel(int n, const int *__restrict__ in, int *__restrict__ out) {
+  __shared__ int part[1024];
+  int tid = threadIdx.x;
+  int per = (n + blockDim.x - 1) / blockDim.x;
+  int lo = tid * per, hi = min(n, lo + per);
+  int s = 0;
+  for (int k = lo; k < hi; k++) s += in[k];
+  part[tid] = s;
+  __syncthreads();
+  for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+    int v = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  int run = part[tid] - s;  // exclusive prefix of this thread's chunk
+  for (int k = lo; k < hi; k++) {
+    out[k] = run;
+    run += in[k];
+  }
+  if (tid == blockDim.x - 1) out[n] = part[tid];
+}
+
+namespace {
+
+// One per-axis candidate: (target bin along axis, image shift)
+__device__ inline void axis_entry(int nb, int home, int sw, int e, int &bin,
+                                  int &shift) {
+  if (nb >= 3) {
+    int t = home + e - 1;
+    shift = 0;
+    if (t < 0) {
+      t += nb;
+      shift = -1;
+    } else if (t >= nb) {
+      t -= nb;
+      shift = 1;
+    }
+    bin = t;
+  } else {
+    int ns = 2 * sw + 1;
+    bin = e / ns;
+    shift = e % ns - sw;
+  }
+}
+
+__device__ inline int axis_len(int nb, int sw) {
+  return nb >= 3 ? 3 : nb * (2 * sw + 1);
+}
+
+__device__ inline int64_t pair_key(int j, int s1, int s2, int s3, int sw) {
+  int64_t u = 2 * sw + 1;
+  return (((int64_t)j * u + (s1 + sw)) * u + (s2 + sw)) * u + (s3 + sw);
+}
+
+// Reference predicate and relative vector (structure.cpp:68-77)
+__device__ inline bool image_test(const double *wi, const double *wj,
+                                  const double *cell, int s1, int s2, int s3,
+                                  double cutoff, double *o) {
+#pragma clang fp contract(off)
+  double im[3];
+  for (int c = 0; c < 3; c++) {
+    double d = wj[c] - wi[c];
+    d = d + (double)s1 * cell[c];
+    d = d + (double)s2 * cell[3 + c];
+    d = d + (double)s3 * cell[6 + c];
+    im[c] = d;
+  }
+  double dist = sqrt(im[0] * im[0] + im[1] * im[1] + im[2] * im[2]);
+  o[0] = dist;
+  o[1] = im[0];
+  o[2] = im[1];
+  o[3] = im[2];
+  return (dist < cutoff) && (dist != 0);
+}
+
+}  // namespace
+
+// MODE 0: count candidates per atom.  MODE 1: write them (unsorted) into
+// the atom's CSR segment.  One wavefront per atom.
+template <int MODE>
+__global__ void __launch_bounds__(256)
+nbr_enum_kernel(int n, WrapMats m, const double *__restrict__ wrapped,
+                const int *__restrict__ bin_of,
+                const int *__restrict__ bin_start,
+                const int *__restrict__ bin_atoms, int *__restrict__ count,
+                const int *__restrict__ cum, int64_t *__restrict__ keys,
+                double *__restrict__ rel) {
+  __shared__ int prefix_sh[4][kWave];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
+  const int i = blockIdx.x * 4 + wv;
+  if (i >= n) return;  // whole wave exits together
+  int *prefix = prefix_sh[wv];
+  const int sw = m.sweep;
+  const int hb = bin_of[i];
+  const int hx = hb / (m.nb[1] * m.nb[2]);
+  const int hy = (hb / m.nb[2]) % m.nb[1];
+  const int hz = hb % m.nb[2];
+  const int lx = axis_len(m.nb[0], sw), ly = axis_len(m.nb[1], sw),
+            lz = axis_len(m.nb[2], sw);
+  const int n_combo = lx * ly * lz;
+  const double wi[3] = {wrapped[3 * i], wrapped[3 * i + 1], wrapped[3 * i + 2]};
+  const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
+  int found = 0;
+  int64_t base = (MODE == 1) ? (int64_t)cum[i] : 0;
+
+  for (int c0 = 0; c0 < n_combo; c0 += kWave) {
+    int c = c0 + lane;
+    int cb = 0, s1 = 0, s2 = 0, s3 = 0, csize = 0;
+    if (c < n_combo) {
+      int ex = c / (ly * lz), ey = (c / lz) % ly, ez = c % lz;
+      int bx, by, bz;
+      axis_entry(m.nb[0], hx, sw, ex, bx, s1);
+      axis_entry(m.nb[1], hy, sw, ey, by, s2);
+      axis_entry(m.nb[2], hz, sw, ez, bz, s3);
+      cb = (bx * m.nb[1] + by) * m.nb[2] + bz;
+      csize = bin_start[cb + 1] - bin_start[cb];
+    }
+    // inclusive scan of combo sizes across the wave
+    int incl = csize;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      int v = __shfl_up(incl, off, kWave);
+      if (lane >= off) incl += v;
+    }
+    prefix[lane] = incl;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int total = __shfl(incl, kWave - 1, kWave);
+    for (int t0 = 0; t0 < total; t0 += kWave) {
+      int t = t0 + lane;
+      bool pass = false;
+      double o[4];
+      int64_t key = 0;
+      if (t < total) {
+        // first combo whose inclusive prefix exceeds t
+        int lo = 0, hi = kWave - 1;
+        while (lo < hi) {
+          int mid = (lo + hi) >> 1;
+          if (prefix[mid] > t)
+            hi = mid;
+          else
+            lo = mid + 1;
+        }
+        int cc = lo;
+        int before = cc ? prefix[cc - 1] : 0;
+        int cidx = c0 + cc;
+        int ex = cidx / (ly * lz), ey = (cidx / lz) % ly, ez = cidx % lz;
+        int bx, by, bz, t1, t2, t3;
+        axis_entry(m.nb[0], hx, sw, ex, bx, t1);
+        axis_entry(m.nb[1], hy, sw, ey, by, t2);
+        axis_entry(m.nb[2], hz, sw, ez, bz, t3);
+        int b = (bx * m.nb[1] + by) * m.nb[2] + bz;
+        int j = bin_atoms[bin_start[b] + (t - before)];
+        const double wj[3] = {wrapped[3 * j], wrapped[3 * j + 1],
+                              wrapped[3 * j + 2]};
+        pass = image_test(wi, wj, m.cell, t1, t2, t3, m.cutoff, o);
+        key = pair_key(j, t1, t2, t3, sw);
+      }
+      uint64_t mask = __ballot(pass);
+      if (MODE == 1 && pass) {
+        int64_t slot = base + found + __popcll(mask & lt_mask);
+        if (slot >= (int64_t)cum[i + 1]) continue;  // count/fill disagree
+        keys[slot] = key;
+        rel[4 * slot + 0] = o[0];
+        rel[4 * slot + 1] = o[1];
+        rel[4 * slot + 2] = o[2];
+        rel[4 * slot + 3] = o[3];
+      }
+      found += __popcll(mask);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (MODE == 0 && lane == 0) count[i] = found;
+}
+
+// Rank sort of each atom's segment by key; writes the final CSR arrays.
+__global__ void __launch_bounds__(256)
+nbr_sort_kernel(int n, int sweep, const int *__restrict__ cum,
+                const int64_t *__restrict__ keys_in,
+                const double *__restrict__ rel_in,
+                const int *__restrict__ species,
+                int64_t *__restrict__ keys_out, double *__restrict__ rel_out,
+                int *__restrict__ idx_out, int *__restrict__ nspec_out,
+                int *__restrict__ center_out) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int i = blockIdx.x * 4 + threadIdx.x / kWave;
+  if (i >= n) return;
+  const int b = cum[i], e = cum[i + 1];
+  const int64_t u = 2 * sweep + 1;
+  const int64_t u3 = u * u * u;
+  for (int q = b + lane; q < e; q += kWave) {
+    int64_t k = keys_in[q];
+    int rank = 0;
+    for (int r = b; r < e; r++) rank += (keys_in[r] < k);
+    int dst = b + rank;
+    keys_out[dst] = k;
+    int j = (int)(k / u3);
+    idx_out[dst] = j;
+    nspec_out[dst] = species[j];
+    center_out[dst] = i;
+    for (int c = 0; c < 4; c++) rel_out[4 * dst + c] = rel_in[4 * q + c];
+  }
+}
+
+// Reverse pair: (i -> j, s) <-> (j -> i, -s); both exist since the
+// predicate is symmetric bitwise (fl(a-b) = -fl(b-a)).
+__global__ void nbr_reverse_kernel(int64_t P, int sweep,
+                                   const int *__restrict__ cum,
+                                   const int64_t *__restrict__ keys,
+                                   const int *__restrict__ idx,
+                                   const int *__restrict__ center,
+                                   int *__restrict__ rev) {
+  int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const int64_t u = 2 * sweep + 1;
+  int64_t k = keys[p];
+  int s3 = (int)(k % u) - sweep;
+  int s2 = (int)((k / u) % u) - sweep;
+  int s1 = (int)((k / (u * u)) % u) - sweep;
+  int j = idx[p], i = center[p];
+  int64_t want = pair_key(i, -s1, -s2, -s3, sweep);
+  int lo = cum[j], hi = cum[j + 1];
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (keys[mid] < want)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  rev[p] = (lo < cum[j + 1] && keys[lo] == want) ? lo : -1;
+}
+
+// Species-grouped row map (b2.cpp:104-176 regroup: atoms of species 0 in
+// original order, then species 1, ...).  One workgroup; each thread owns a
+// contiguous chunk of atoms.
+__global__ void __launch_bounds__(1024)
+species_rows_kernel(int n, const int *__restrict__ species,
+                    int *__restrict__ row_of, int *__restrict__ atom_of_row,
+                    int *__restrict__ species_of_row, int *__restrict__ counts) {
+  __shared__ int part[kMaxSpecies][1025];
+  __shared__ int bad;
+  const int t = threadIdx.x;
+  const int nt = blockDim.x;
+  if (t == 0) bad = 0;
+  const int per = (n + nt - 1) / nt;
+  const int lo = t * per, hi = min(n, lo + per);
+  int loc[kMaxSpecies];
+  for (int s = 0; s < kMaxSpecies; s++) loc[s] = 0;
+  __syncthreads();
+  for (int k = lo; k < hi; k++) {
+    int s = species[k];
+    if (s < 0 || s >= kMaxSpecies)
+      bad = 1;
+    else
+      loc[s]++;
+  }
+  for (int s = 0; s < kMaxSpecies; s++) part[s][t + 1] = loc[s];
+  if (t == 0)
+    for (int s = 0; s < kMaxSpecies; s++) part[s][0] = 0;
+  __syncthreads();
+  if (t < kMaxSpecies) {  // serial exclusive scan per species (nt values)
+    int run = 0;
+    for (int q = 0; q <= nt; q++) {
+      int v = part[t][q];
+      part[t][q] = run;
+      run += v;
+    }
+    part[t][nt] = run;  // total of species t
+  }
+  __syncthreads();
+  int off[kMaxSpecies];
+  int acc = 0;
+  for (int s = 0; s < kMaxSpecies; s++) {
+    off[s] = acc;
+    acc += part[s][nt];
+  }
+  if (!bad) {
+    int run[kMaxSpecies];
+    for (int s = 0; s < kMaxSpecies; s++) run[s] = off[s] + part[s][t + 1];
+    for (int k = lo; k < hi; k++) {
+      int s = species[k];
+      int row = run[s]++;
+      row_of[k] = row;
+      atom_of_row[row] = k;
+      species_of_row[row] = s;
+    }
+  }
+  if (t == 0) {
+    for (int s = 0; s < kMaxSpecies; s++) counts[s] = part[s][nt];
+    counts[kMaxSpecies] = bad;
+  }
+}
+
+int species_rows_device(hipStream_t st, NListBuffers &b, int n,
+                        const int *d_species, int *host_counts) {
+  b.row_of.ensure((size_t)std::max(n, 1) * sizeof(int));
+  b.atom_of_row.ensure((size_t)std::max(n, 1) * sizeof(int));
+  b.species_of_row.ensure((size_t)std::max(n, 1) * sizeof(int));
+  b.species_count.ensure((kMaxSpecies + 1) * sizeof(int));
+  hipLaunchKernelGGL(species_rows_kernel, dim3(1), dim3(1024), 0, st, n,
+                     d_species, b.row_of.as<int>(), b.atom_of_row.as<int>(),
+                     b.species_of_row.as<int>(), b.species_count.as<int>());
+  FLARE_HIP_CHECK(hipGetLastError());
+  int h[kMaxSpecies + 1];
+  FLARE_HIP_CHECK(hipMemcpyAsync(h, b.species_count.ptr, sizeof(h),
+                                 hipMemcpyDeviceToHost, st));
+  FLARE_HIP_CHECK(hipStreamSynchronize(st));
+  FLARE_REQUIRE(h[kMaxSpecies] == 0,
+                "species codes must lie in [0, " + std::to_string(kMaxSpecies) + ")");
+  int ns = 0;
+  for (int s = 0; s < kMaxSpecies; s++) {
+    host_counts[s] = h[s];
+    if (h[s] > 0) ns = s + 1;
+  }
+  return ns;
+}
+
+// ---------------------------------------------------------------------------
+// Host driver
+
+void nlist_build_device(hipStream_t st, NListBuffers &b, int n,
+                        const double *d_pos, const int *d_species,
+                        const WrapHost &wh) {
+  WrapMats m;
+  for (int q = 0; q < 9; q++) {
+    m.ct[q] = wh.ct[q];
+    m.cdi[q] = wh.cdi[q];
+    m.cd[q] = wh.cd[q];
+    m.cti[q] = wh.cti[q];
+    m.cell[q] = wh.cell[q];
+  }
+  for (int k = 0; k < 3; k++) m.nb[k] = wh.nb[k];
+  m.sweep = wh.sweep;
+  m.cutoff = wh.cutoff;
+  const int nbins = wh.nb[0] * wh.nb[1] * wh.nb[2];
+
+  b.wrapped.ensure((size_t)n * 3 * sizeof(double));
+  b.bin_of.ensure((size_t)n * sizeof(int));
+  b.bin_count.ensure((size_t)(nbins + 1) * sizeof(int));
+  b.bin_start.ensure((size_t)(nbins + 1) * sizeof(int));
+  b.bin_fill.ensure((size_t)(nbins + 1) * sizeof(int));
+  b.bin_atoms.ensure((size_t)std::max(n, 1) * sizeof(int));
+  b.count.ensure((size_t)std::max(n, 1) * sizeof(int));
+  b.cum.ensure((size_t)(n + 1) * sizeof(int));
+
+  const int tb = 256;
+  const int g1 = (n + tb - 1) / tb;
+  if (n > 0) {
+    hipLaunchKernelGGL(wrap_bin_kernel, dim3(g1), dim3(tb), 0, st, n, d_pos, m,
+                       b.wrapped.as<double>(), b.bin_of.as<int>());
+  }
+  FLARE_HIP_CHECK(hipMemsetAsync(b.bin_count.ptr, 0, (nbins + 1) * sizeof(int), st));
+  FLARE_HIP_CHECK(hipMemsetAsync(b.bin_fill.ptr, 0, (nbins + 1) * sizeof(int), st));
+  if (n > 0)
+    hipLaunchKernelGGL(bin_count_kernel, dim3(g1), dim3(tb), 0, st, n,
+                       b.bin_of.as<int>(), b.bin_count.as<int>());
+  // bin_count holds counts shifted by one; scan the first nbins entries of
+  // bin_count[1..] into bin_start[0..nbins]
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, nbins,
+                     b.bin_count.as<int>() + 1, b.bin_start.as<int>());
+  if (n > 0)
+    hipLaunchKernelGGL(bin_fill_kernel, dim3(g1), dim3(tb), 0, st, n,
+                       b.bin_of.as<int>(), b.bin_start.as<int>(),
+                       b.bin_fill.as<int>(), b.bin_atoms.as<int>());
+  const int gw = (n + 3) / 4;
+  if (n > 0)
+    hipLaunchKernelGGL(nbr_enum_kernel<0>, dim3(gw), dim3(256), 0, st, n, m,
+                       b.wrapped.as<double>(), b.bin_of.as<int>(),
+                       b.bin_start.as<int>(), b.bin_atoms.as<int>(),
+                       b.count.as<int>(), (const int *)nullptr,
+                       (int64_t *)nullptr, (double *)nullptr);
+  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, n,
+                     b.count.as<int>(), b.cum.as<int>());
+  int P = 0;
+  FLARE_HIP_CHECK(hipMemcpyAsync(&P, b.cum.as<int>() + n, sizeof(int),
+                                 hipMemcpyDeviceToHost, st));
+  FLARE_HIP_CHECK(hipStreamSynchronize(st));
+  b.P = P;
+  size_t Pa = std::max(P, 1);
+  b.keys_tmp.ensure(Pa * sizeof(int64_t));
+  b.rel_tmp.ensure(Pa * 4 * sizeof(double));
+  b.keys.ensure(Pa * sizeof(int64_t));
+  b.rel.ensure(Pa * 4 * sizeof(double));
+  b.idx.ensure(Pa * sizeof(int));
+  b.nspec.ensure(Pa * sizeof(int));
+  b.center.ensure(Pa * sizeof(int));
+  b.rev.ensure(Pa * sizeof(int));
+  if (n > 0) {
+    hipLaunchKernelGGL(nbr_enum_kernel<1>, dim3(gw), dim3(256), 0, st, n, m,
+                       b.wrapped.as<double>(), b.bin_of.as<int>(),
+                       b.bin_start.as<int>(), b.bin_atoms.as<int>(),
+                       (int *)nullptr, b.cum.as<int>(), b.keys_tmp.as<int64_t>(),
+                       b.rel_tmp.as<double>());
+    hipLaunchKernelGGL(nbr_sort_kernel, dim3(gw), dim3(256), 0, st, n,
+                       wh.sweep, b.cum.as<int>(), b.keys_tmp.as<int64_t>(),
+                       b.rel_tmp.as<double>(), d_species, b.keys.as<int64_t>(),
+                       b.rel.as<double>(), b.idx.as<int>(), b.nspec.as<int>(),
+                       b.center.as<int>());
+  }
+  if (P > 0) {
+    int gp = (P + tb - 1) / tb;
+    hipLaunchKernelGGL(nbr_reverse_kernel, dim3(gp), dim3(tb), 0, st,
+                       (int64_t)P, wh.sweep, b.cum.as<int>(),
+                       b.keys.as<int64_t>(), b.idx.as<int>(),
+                       b.center.as<int>(), b.rev.as<int>());
+  }
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace flare

@@ -1,0 +1,124 @@
+"""GPU sparse-GP prediction against the oracle (sparse_gp.cpp:93-133,
+576-628, 682-702; normalized_dot_product.cpp:27-210, 643-759)."""
+import numpy as np
+import pytest
+
+from flare_pp_amd import (B2, NormalizedDotProduct, PairFLARE, SparseGP,
+                          Structure, efs_to_results, local_stds, synth)
+from helpers import grouped_sparse, small_two_species
+
+pytestmark = pytest.mark.gpu
+
+
+def nrel(a, b):
+    return np.abs(np.asarray(a) - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def build(cell, species, pos, settings, rc, sigma, power, atoms, alpha):
+    b2 = B2("chebyshev", "quadratic", [0, rc], [], settings)
+    s = Structure(cell, species, pos, rc, [b2])
+    gp = SparseGP([NormalizedDotProduct(sigma, power)], 0.1, 0.1, 0.1)
+    gp.add_specific_environments(s, atoms)
+    gp.set_alpha(alpha)
+    return s, gp, b2
+
+
+def oracle_side(orc, cell, species, pos, settings, rc, atoms):
+    nl = orc.NeighborList(cell, species, pos, rc)
+    S = settings[0]
+    o = orc.B2Result(nl, species, *settings, np.full((S, S), rc), dervs=True)
+    g = o.grouped()
+    sel = [[k for k, a in enumerate(g["atom_indices"][off:off + cnt]) if a in atoms]
+           for off, cnt in zip(np.concatenate([[0], np.cumsum(g["n_by_type"])[:-1]]),
+                               g["n_by_type"])]
+    P, Pn, m = grouped_sparse(g, sel)
+    return o, g, P, Pn, m
+
+
+@pytest.mark.parametrize("power", [1, 2])
+def test_config1_mean_and_variance(orc, cfg1, power):
+    c = cfg1
+    s, gp, b2 = build(c["cell"], c["species"], c["positions"], c["settings"], 5.0,
+                      c["sigma"], power, c["sparse_atoms"], c["alpha"])
+    o, g, P, Pn, m = oracle_side(orc, c["cell"], c["species"], c["positions"],
+                                 c["settings"], 5.0, c["sparse_atoms"])
+    V = orc.volume(c["cell"])
+    want = orc.predict_mean(o, m, P, Pn, c["sigma"], power, V, c["alpha"])
+    n = s.noa
+    for form in ("plain", "adjoint_fp64"):
+        print(f"form {form} power {power}", flush=True)
+        efs = gp.predict_mean(s, form=form)
+        assert abs(efs[0] - want[0]) < 1e-10 * max(1, abs(want[0]))
+        assert nrel(efs[1:1 + 3 * n], want[1:1 + 3 * n]) < 1e-10
+        assert nrel(efs[1 + 3 * n:], want[1 + 3 * n:]) < 1e-10
+    # variance: compared relative to sigma^2, the scale the cancellation
+    # sigma^2 - |L^-1 k|^2 is measured on
+    Li_o = orc.cholesky_inverse(orc.envs_envs(1, m, P, Pn, m, P, Pn, c["sigma"],
+                                              power), 1e-8)
+    var_o = orc.cluster_uncertainties(o, m, P, Pn, c["sigma"], power, Li_o)
+    _, (var,) = gp.predict_local_uncertainties(s)
+    assert np.abs(var - var_o).max() < 1e-6 * c["sigma"] ** 2
+    Kuu = gp.Kuu
+    K_o = orc.envs_envs(1, m, P, Pn, m, P, Pn, c["sigma"], power)
+    assert nrel(Kuu, K_o) < 1e-12
+    Li = gp.L_inv
+    Lm = np.linalg.inv(Li)
+    assert nrel(Lm @ Lm.T, K_o + 1e-8 * np.eye(len(K_o))) < 1e-10
+    # ASE-side conventions (sparse_gp_calculator.py:56-109)
+    e, f, st = efs_to_results(efs, n)
+    assert e == efs[0] and f.shape == (n, 3) and st[0] == -efs[1 + 3 * n]
+    stds = local_stds(s.descriptors[0].atom_indices, var, c["sigma"])
+    assert stds.shape == (n, 3)
+
+
+def test_two_species_mean_variance_mapping(orc, tmp_path):
+    cell, species, pos = small_two_species(seed=12)
+    settings, rc, sigma = [2, 4, 3], 5.0, 1.5
+    atoms = [0, 2, 3, 5, 8, 11, 13, 20, 27]
+    alpha = np.random.default_rng(3).normal(size=len(atoms))
+    s, gp, b2 = build(cell, species, pos, settings, rc, sigma, 2, atoms, alpha)
+    o, g, P, Pn, m = oracle_side(orc, cell, species, pos, settings, rc, atoms)
+    V = orc.volume(cell)
+    alpha_grouped = np.concatenate([alpha[[atoms.index(a) for a in blk if a in atoms]]
+                                    for blk in s.descriptors[0].atom_indices])
+    gp.set_alpha(alpha_grouped)
+    want = orc.predict_mean(o, m, P, Pn, sigma, 2, V, alpha_grouped)
+    efs = gp.predict_mean(s, form="adjoint_fp64")
+    assert nrel(efs, want) < 1e-10
+    beta = gp.mapping_coefficients()
+    beta_o = orc.mapping_coefficients(2, m, P, Pn, sigma, 2, alpha_grouped)
+    assert nrel(beta, beta_o) < 1e-12
+    # GP -> coefficient file -> mapped pair style reproduces the GP mean
+    path = tmp_path / "lmp.flare"
+    gp.write_mapping_coefficients(str(path), "test", 0)
+    pair = PairFLARE(str(path))
+    e, f, v = pair.compute(cell, species, pos, form="plain")
+    n = len(species)
+    assert abs(e - efs[0]) < 1e-10 * max(1, abs(e))
+    assert nrel(f.ravel(), efs[1:1 + 3 * n]) < 1e-10
+    st = efs[1 + 3 * n:]
+    assert nrel(v, np.array([st[0], st[3], st[5], st[1], st[2], st[4]]) * V) < 1e-9
+
+
+def test_envs_envs(orc, cfg1):
+    c = cfg1
+    s, gp, b2 = build(c["cell"], c["species"], c["positions"], c["settings"], 5.0,
+                      2.0, 2, c["sparse_atoms"], c["alpha"])
+    k = NormalizedDotProduct(2.0, 2)
+    from flare_pp_amd import ClusterDescriptor
+    envs = ClusterDescriptor(s.descriptors[0])
+    K = k.envs_envs(envs, gp.sparse_descriptors[0])
+    o, g, P, Pn, m = oracle_side(orc, c["cell"], c["species"], c["positions"],
+                                 c["settings"], 5.0, c["sparse_atoms"])
+    want = orc.envs_envs(1, g["n_by_type"], g["descriptors"], g["norms"], m, P, Pn,
+                         2.0, 2)
+    assert nrel(K, want) < 1e-12
+
+
+def test_config3_mean(orc):
+    c = synth.config3()
+    s, gp, b2 = build(c["cell"], c["species"], c["positions"], c["settings"], 5.0,
+                      c["sigma"], c["power"], c["sparse_atoms"], c["alpha"])
+    e_plain = gp.predict_mean(s, form="plain").copy()
+    e_adj = gp.predict_mean(s, form="adjoint_fp64")
+    assert nrel(e_adj, e_plain) < 1e-11

@@ -1,0 +1,67 @@
+"""GPU neighbor list and species grouping against the oracle: the integer
+arrays bit-exact, the relative vectors and wrapped positions bitwise
+(structure.cpp:27-43, 52-133)."""
+import numpy as np
+import pytest
+
+from flare_pp_amd import Structure, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def compare(orc, cell, species, pos, cutoff):
+    print(f"case n={len(species)} cutoff={cutoff}", flush=True)
+    s = Structure(cell, species, pos, cutoff)
+    o = orc.NeighborList(cell, species, pos, cutoff, "cells")
+    assert s.n_neighbors == o.n_pairs
+    np.testing.assert_array_equal(s.neighbor_count, o.count)
+    np.testing.assert_array_equal(s.cumulative_neighbor_count, o.cum)
+    np.testing.assert_array_equal(s.structure_indices, o.idx)
+    np.testing.assert_array_equal(s.neighbor_species, o.nspec)
+    np.testing.assert_array_equal(s.relative_positions, o.rel)
+    np.testing.assert_array_equal(s.wrapped_positions, o.wrapped)
+    assert s.sweep == o.sweep
+    assert s.volume == orc.volume(cell)
+    return s, o
+
+
+def test_config1(orc, cfg1):
+    c = cfg1
+    compare(orc, c["cell"], c["species"], c["positions"], 5.0)
+
+
+@pytest.mark.parametrize("cutoff", [3.0, 5.5])
+def test_triclinic_small_cell(orc, cutoff):
+    rng = np.random.default_rng(11)
+    cell = np.array([[3.1, 0.0, 0.0], [0.9, 2.7, 0.0], [0.4, -0.6, 4.2]])
+    pos = rng.random((7, 3)) @ cell + rng.normal(0, 0.3, (7, 3))
+    species = rng.integers(0, 2, 7)
+    compare(orc, cell, species, pos, cutoff)
+
+
+def test_isolated_and_empty(orc):
+    cell = np.eye(3) * 30.0
+    pos = np.array([[1.0, 1.0, 1.0], [15.0, 15.0, 15.0], [17.0, 15.5, 15.0]])
+    s, _ = compare(orc, cell, [0, 1, 0], pos, 5.0)
+    assert s.neighbor_count[0] == 0
+
+
+def test_config2_full(orc):
+    c = synth.config2()
+    s, o = compare(orc, c["cell"], c["species"], c["positions"], 5.0)
+    # reverse pairs: (i -> j, s) and (j -> i, -s) have opposite vectors
+    from flare_pp_amd.structure import d2h
+    rev = d2h(s._view.d_reverse_pair, s.n_neighbors, np.int32)
+    assert rev.min() >= 0
+    np.testing.assert_array_equal(o.rel[rev, 1:], -o.rel[:, 1:])
+    np.testing.assert_array_equal(rev[rev], np.arange(len(rev)))
+
+
+def test_species_grouping(orc):
+    c = synth.config4()
+    s = Structure(c["cell"], c["species"], c["positions"], 4.5)
+    rows = s._atom_of_row()
+    want = np.concatenate([np.flatnonzero(np.asarray(c["species"]) == k)
+                           for k in range(3)])
+    np.testing.assert_array_equal(rows, want)
+    assert list(s.species_count[:3]) == [22118, 16589, 16589]
