@@ -394,23 +394,33 @@ static void check_species(const flare_nlist *nl, int S) {
 struct Forward {
   void *A = nullptr, *B = nullptr;
   double *norm2 = nullptr;
+  void *xh = nullptr, *xl = nullptr;
+  int kp = 0;
 };
 
+// out_mode 0 writes B rows, 1 writes B/|B|; with bf16 true the latter is
+// also split into BF16 hi/lo rows (K padded) for the matrix-core GEMM.
 static Forward run_forward(flare_ctx *c, const flare::B2Dev &p,
                            const flare_nlist *nl, const int *d_species,
                            int prec, const std::string &tag,
-                           void *B_out = nullptr) {
+                           void *B_out = nullptr, int out_mode = 0,
+                           bool bf16 = false) {
   Forward f;
   const size_t ts = prec == flare::kPrecF64 ? sizeof(double) : sizeof(float);
   const int n = nl->n;
   f.A = c->buf(tag + ".A", std::max<size_t>(1, (size_t)n * p.W) * ts);
   f.B = B_out ? B_out : c->buf(tag + ".B", std::max<size_t>(1, (size_t)n * p.F) * ts);
   f.norm2 = c->buf_t<double>(tag + ".norm2", n);
+  f.kp = flare::bf16_padded_k(p.F);
+  if (bf16) {
+    f.xh = c->buf(tag + ".xh", std::max<size_t>(1, (size_t)n * f.kp) * 2);
+    f.xl = c->buf(tag + ".xl", std::max<size_t>(1, (size_t)n * f.kp) * 2);
+  }
   c->mark_begin("forward");
   flare::launch_b2_forward(c->stream, p, nl->dev(), d_species,
                            nl->b.row_of.as<int>(), c->ynorm_d.as<double>(),
                            c->ynorm_f.as<float>(), c->b2_index(p.S, p.N, p.L),
-                           prec, f.A, f.B, f.norm2);
+                           prec, out_mode, f.kp, f.xh, f.xl, f.A, f.B, f.norm2);
   c->mark_end();
   return f;
 }
@@ -418,10 +428,9 @@ static Forward run_forward(flare_ctx *c, const flare::B2Dev &p,
 // Forces / virial assembly shared by the mapped and GP paths.
 static void run_forces(flare_ctx *c, const flare::B2Dev &p,
                        const flare_nlist *nl, const int *d_species, int prec,
-                       int mode, const Forward &fw, const void *w,
-                       const double *E_row, double *forces, double *energy,
-                       double *virial6, double *stress6,
-                       const std::string &tag) {
+                       int mode, const Forward &fw, const flare::WInput &wi,
+                       double *forces, double *energy, double *virial6,
+                       double *stress6, const std::string &tag) {
   const int n = nl->n;
   const int64_t P = nl->b.P;
   const double *dA = nullptr;
@@ -438,15 +447,14 @@ static void run_forces(flare_ctx *c, const flare::B2Dev &p,
   c->mark_begin(mode == 0 ? "forces_plain" : "forces_adjoint");
   flare::launch_forces(c->stream, prec, mode, p, nl->dev(), d_species,
                        nl->b.row_of.as<int>(), c->ynorm_d.as<double>(),
-                       c->ynorm_f.as<float>(), c->b2_index(p.S, p.N, p.L), fw.A,
-                       w, dA, pf, vir);
+                       c->ynorm_f.as<float>(), fw.A, wi, dA, pf, vir);
   c->mark_end();
   c->mark_begin("gather_reduce");
   if (forces) flare::launch_gather(c->stream, nl->dev(), pf, forces);
   double *sums = c->buf_t<double>(tag + ".sums", 10);
   if (n > 0) {
-    flare::launch_colsum(c->stream, n, 1, E_row, sums);
-    flare::launch_colsum(c->stream, n, 9, vir, sums + 1);
+    double *part = c->buf_t<double>(tag + ".part", 128 * 10);
+    flare::launch_energy_virial_sums(c->stream, n, wi.E_row, vir, part, sums);
   } else {
     FLARE_HIP_CHECK(hipMemsetAsync(sums, 0, 10 * sizeof(double), c->stream));
   }
@@ -465,7 +473,7 @@ struct flare_pair {
   double cutoff = 0;
   std::string radial = "chebyshev", cutoff_name = "quadratic";
   std::vector<double> cutoffs;
-  DevBuf beta, beta_f32;
+  DevBuf beta, beta_f32, beta_h, beta_l;  // FP64, FP32, BF16 hi/lo (K padded)
   bool have_last = false;
   int last_prec = 0, last_n = 0;
   int last_off[flare::kMaxSpecies + 1] = {0};
@@ -517,6 +525,13 @@ void setup_pair(flare_pair *pr, const std::vector<double> &packed) {
                               pr->beta.as<double>() + s * per);
   flare::launch_to_f32(c->stream, (int64_t)(pr->S * per), pr->beta.as<double>(),
                        pr->beta_f32.as<float>());
+  if (pr->power == 2) {
+    const int kp = flare::bf16_padded_k(pr->F);
+    pr->beta_h.ensure((size_t)pr->S * pr->F * kp * 2);
+    pr->beta_l.ensure((size_t)pr->S * pr->F * kp * 2);
+    flare::split_bf16(c->stream, pr->S * pr->F, pr->F, kp, pr->beta.as<double>(),
+                      pr->beta_h.ptr, pr->beta_l.ptr);
+  }
   c->sync();
 }
 
@@ -1024,11 +1039,17 @@ int flare_pair_compute(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
     const int mode = form == FLARE_FORM_PLAIN_FP64 ? 0 : 1;
     const size_t per = pr->power == 1 ? (size_t)F : (size_t)F * F;
     const size_t ts = prec == flare::kPrecF64 ? sizeof(double) : sizeof(float);
-    Forward fw = run_forward(ctx, p, nl, d_species, prec, "pair");
+    const bool mixed = form == FLARE_FORM_ADJOINT_MIXED && pr->power == 2;
+    // GEMM rows: B itself in FP64 (as pair_flare), B/|B| in the FP32 forms
+    const int normalized = prec == flare::kPrecF32 ? 1 : 0;
+    Forward fw = run_forward(ctx, p, nl, d_species, prec, "pair", nullptr,
+                             normalized, mixed);
     int off[flare::kMaxSpecies + 1];
     nl->row_offsets(p.S, off);
     // beta in the working precision
     const void *beta = pr->beta.ptr;
+    const void *bh = pr->beta_h.ptr, *bl = pr->beta_l.ptr;
+    const int kp = fw.kp;
     if (prec == flare::kPrecF64) {
       if (d_beta) beta = d_beta;
     } else {
@@ -1037,18 +1058,16 @@ int flare_pair_compute(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
         float *bf = ctx->buf_t<float>("pair.beta_override", p.S * per);
         flare::launch_to_f32(ctx->stream, (int64_t)(p.S * per), d_beta, bf);
         beta = bf;
+        if (mixed) {
+          void *h = ctx->buf("pair.beta_oh", (size_t)p.S * F * kp * 2);
+          void *l = ctx->buf("pair.beta_ol", (size_t)p.S * F * kp * 2);
+          flare::split_bf16(ctx->stream, p.S * F, F, kp, d_beta, h, l);
+          bh = h;
+          bl = l;
+        }
       }
     }
-    // GEMM input rows: B itself in FP64 (as pair_flare), B/|B| otherwise
     const void *X = fw.B;
-    int normalized = 0;
-    if (prec == flare::kPrecF32) {
-      float *xn = ctx->buf_t<float>("pair.X", (size_t)n * F);
-      flare::launch_normalize_rows_f32(ctx->stream, n, F, (const float *)fw.B,
-                                       fw.norm2, xn);
-      X = xn;
-      normalized = 1;
-    }
     void *Z = nullptr;
     ctx->mark_begin("gemm");
     if (pr->power == 2) {
@@ -1061,12 +1080,13 @@ int flare_pair_compute(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
                   (const double *)X + (size_t)off[s] * F, F,
                   (const double *)beta + s * per, F, 0.0,
                   (double *)Z + (size_t)off[s] * F, F);
-        } else if (form == FLARE_FORM_ADJOINT_MIXED) {
-          flare::gemm_bf16x3(ctx->stream, rows, F, F,
-                             (const float *)X + (size_t)off[s] * F,
-                             (const float *)beta + s * per,
-                             (float *)Z + (size_t)off[s] * F,
-                             ctx->buf("mfma.ws", flare::gemm_bf16x3_workspace(rows, F, F)));
+        } else if (mixed) {
+          flare::gemm_bf16x3(ctx->stream, rows, F, kp,
+                             (const __bf16 *)fw.xh + (size_t)off[s] * kp,
+                             (const __bf16 *)fw.xl + (size_t)off[s] * kp,
+                             (const __bf16 *)bh + (size_t)s * F * kp,
+                             (const __bf16 *)bl + (size_t)s * F * kp,
+                             (float *)Z + (size_t)off[s] * F, F);
         } else {
           sgemm_rm(ctx, false, false, rows, F, F, 1.0f,
                    (const float *)X + (size_t)off[s] * F, F,
@@ -1076,15 +1096,11 @@ int flare_pair_compute(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
       }
     }
     ctx->mark_end();
-    void *w = ctx->buf("pair.w", std::max<size_t>(1, (size_t)n * F) * ts);
     double *E_row = ctx->buf_t<double>("pair.E", n);
-    ctx->mark_begin("energy_w");
-    flare::launch_mapped_w(ctx->stream, prec, n, F, pr->power, normalized, X, Z,
-                           fw.norm2, nl->b.species_of_row.as<int>(), beta, w,
-                           E_row);
-    ctx->mark_end();
-    run_forces(ctx, p, nl, d_species, prec, mode, fw, w, E_row, d_forces,
-               d_energy, d_virial, nullptr, "pair");
+    flare::WInput wi{pr->power == 2 ? 1 : 2, normalized, X, Z,
+                     pr->power == 1 ? beta : nullptr, nullptr, fw.norm2, E_row};
+    run_forces(ctx, p, nl, d_species, prec, mode, fw, wi, d_forces, d_energy,
+               d_virial, nullptr, "pair");
     if (d_eatom)
       flare::launch_rows_to_atoms(ctx->stream, n, nl->b.atom_of_row.as<int>(),
                                   E_row, d_eatom);
@@ -1108,9 +1124,9 @@ int flare_pair_beta_grad(flare_ctx *ctx, flare_pair *pr, double *d_grad) {
     if (pr->last_prec == flare::kPrecF64)
       flare::launch_normalize_rows_f64(ctx->stream, n, F,
                                        ctx->ws["pair.B"]->as<double>(), norm2, dhat);
-    else
+    else  // the FP32 forms keep the normalised rows B/|B|
       flare::launch_to_f64(ctx->stream, (int64_t)n * F,
-                           ctx->ws["pair.X"]->as<float>(), dhat);
+                           ctx->ws["pair.B"]->as<float>(), dhat);
     // empty environments (|B|^2 < 1e-8) carry no energy
     flare::launch_mask_rows(ctx->stream, n, F, norm2, 1e-8, dhat);
     for (int s = 0; s < pr->S; s++) {
@@ -1386,7 +1402,8 @@ static GPState gp_mean(flare_ctx *ctx, flare_sgp *gp, const flare_b2_params *par
   flare::launch_gp_w(ctx->stream, n, F, G, e_row, B, st.fw.norm2, gp->sig2,
                      gp->power, w, E_row);
   const int mode = form == FLARE_FORM_PLAIN_FP64 ? 0 : 1;
-  run_forces(ctx, p, nl, d_species, flare::kPrecF64, mode, st.fw, w, E_row,
+  flare::WInput wi{0, 0, nullptr, nullptr, nullptr, w, st.fw.norm2, E_row};
+  run_forces(ctx, p, nl, d_species, flare::kPrecF64, mode, st.fw, wi,
              d_mean_efs + 1, d_mean_efs, nullptr, d_mean_efs + 1 + 3 * (size_t)n,
              "gp");
   (void)want_k;

@@ -18,6 +18,58 @@
 
 namespace flare {
 
+// Output stage shared by the forward kernels: B2 contraction from the
+// single-bond values in LDS (b2.cpp:203-241), |B|^2 in FP64, and the rows
+// handed to the contraction with beta / the sparse set:
+//   out.mode 0: B itself;   1: d = B/|B| (zero when |B|^2 < 1e-8, the
+//   empty-environment threshold of pair_flare.cpp:135), plus optionally its
+//   BF16 hi + lo split into rows of out.kp (zero padded).
+struct FwdOut {
+  int mode;
+  int kp;
+  __bf16 *xh, *xl;
+};
+
+template <typename T>
+__device__ inline void b2_contract_store(const B2Dev &p, const T *Al, T *Bs,
+                                         const int *__restrict__ b2_idx,
+                                         int row, const FwdOut &o,
+                                         T *__restrict__ B_out,
+                                         double *__restrict__ norm2_out) {
+  const int lane = threadIdx.x;
+  const int H = p.H, F = p.F;
+  double nrm = 0;
+  for (int f = lane; f < F; f += kWave) {
+    const int code = b2_idx[f];
+    const int c1 = code >> 20, c2 = (code >> 8) & 0xfff, l = code & 0xff;
+    const T *a1 = Al + c1 * H + l * l;
+    const T *a2 = Al + c2 * H + l * l;
+    T sacc = 0;
+    for (int m = 0; m < 2 * l + 1; m++) sacc += a1[m] * a2[m];
+    Bs[f] = sacc;
+    nrm += (double)sacc * (double)sacc;
+  }
+  nrm = wave_sum(nrm);
+  if (lane == 0) norm2_out[row] = nrm;
+  __syncthreads();
+  T *Bi = B_out + (int64_t)row * F;
+  if (o.mode == 0) {
+    for (int f = lane; f < F; f += kWave) Bi[f] = Bs[f];
+    return;
+  }
+  const T sc = nrm < 1e-8 ? T(0) : (T)(1.0 / sqrt(nrm));
+  for (int f = lane; f < F; f += kWave) Bi[f] = Bs[f] * sc;
+  if (o.xh) {
+    __bf16 *xh = o.xh + (int64_t)row * o.kp, *xl = o.xl + (int64_t)row * o.kp;
+    for (int f = lane; f < o.kp; f += kWave) {
+      float x = f < F ? (float)(Bs[f] * sc) : 0.0f;
+      __bf16 h = (__bf16)x;
+      xh[f] = h;
+      xl[f] = (__bf16)(x - (float)h);
+    }
+  }
+}
+
 template <typename T>
 struct KahanAcc {
   T s = 0, c = 0;
@@ -35,8 +87,9 @@ template <typename T, bool KAHAN>
 __global__ void __launch_bounds__(64)
 b2_forward_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
                   const int *__restrict__ row_of, const T *__restrict__ ynorm,
-                  const int *__restrict__ b2_idx, T *__restrict__ A_out,
-                  T *__restrict__ B_out, double *__restrict__ norm2_out) {
+                  const int *__restrict__ b2_idx, FwdOut out,
+                  T *__restrict__ A_out, T *__restrict__ B_out,
+                  double *__restrict__ norm2_out) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int i = blockIdx.x;
   const int lane = threadIdx.x;
@@ -46,6 +99,7 @@ b2_forward_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
   T *gb = Ac + W;
   T *yb = gb + kWave * N;
   int *spec = reinterpret_cast<int *>(yb + kWave * H);
+  T *Bs = reinterpret_cast<T *>(spec + kWave);  // F
 
   for (int o = lane; o < W; o += kWave) {
     A[o] = 0;
@@ -104,22 +158,187 @@ b2_forward_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
   __syncthreads();
   T *Ai = A_out + (int64_t)i * W;
   for (int o = lane; o < W; o += kWave) Ai[o] = A[o];
-  // B2 contraction; b2_idx[f] = (c1 << 20) | (c2 << 8) | l
-  const int row = row_of[i];
-  T *Bi = B_out + (int64_t)row * F;
-  double nrm = 0;
-  for (int f = lane; f < F; f += kWave) {
-    const int code = b2_idx[f];
-    const int c1 = code >> 20, c2 = (code >> 8) & 0xfff, l = code & 0xff;
-    const T *a1 = A + c1 * H + l * l;
-    const T *a2 = A + c2 * H + l * l;
-    T s = 0;
-    for (int m = 0; m < 2 * l + 1; m++) s += a1[m] * a2[m];
-    Bi[f] = s;
-    nrm += (double)s * (double)s;
+  // b2_idx[f] = (c1 << 20) | (c2 << 8) | l
+  b2_contract_store<T>(p, A, Bs, b2_idx, row_of[i], out, B_out, norm2_out);
+}
+
+// ---------------------------------------------------------------------------
+// Matrix-core single bond.  Per atom the accumulation is a small product
+//   A (nr x H) = G^T (nr x K) . Y (K x H),  K = pairs,
+// G[k][s*N + n] = g_n(r_k) when the neighbor's species is s, else 0.
+// One wavefront per atom; 64 pairs are staged per chunk (lane = pair) and
+// consumed by v_mfma_*_16x16x4 in K steps of 4 pairs.  Operand layouts
+// (64 lanes): A[i][k] lane = i + 16 k; B[k][j] lane = j + 16 k; the
+// result D[i][j] sits in lane j + 16 (i / 4), element i % 4 for the FP32
+// instruction and in lane j + 16 (i % 4), element i / 4 for the FP64 one
+// (both measured on gfx950).
+// FP32: each K step is accumulated from zero and Kahan-added into the
+// running (sum, compensation) pair; FP64 accumulates in the matrix core.
+
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+typedef __attribute__((ext_vector_type(4))) double f64x4;
+
+template <typename T>
+struct Mma;
+template <>
+struct Mma<float> {
+  typedef f32x4 V;
+  static __device__ inline V run(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
-  nrm = wave_sum(nrm);
-  if (lane == 0) norm2_out[row] = nrm;
+  // row of result element j held by a lane of row group g = lane / 16
+  static __device__ inline int row(int g, int j) { return 4 * g + j; }
+};
+template <>
+struct Mma<double> {
+  typedef f64x4 V;
+  static __device__ inline V run(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int g, int j) { return g + 4 * j; }
+};
+
+template <typename T, int MT, int NT, bool KAHAN, int NMAX>
+__global__ void __launch_bounds__(64)
+b2_forward_mfma_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
+                       const int *__restrict__ row_of,
+                       const T *__restrict__ ynorm, const int *__restrict__ b2_idx,
+                       FwdOut out, T *__restrict__ A_out, T *__restrict__ B_out,
+                       double *__restrict__ norm2_out) {
+  typedef typename Mma<T>::V V;
+  constexpr int KP = MT * 16 + 1;  // padded row of G
+  constexpr int YP = NT * 16 + 1;  // padded row of Y
+  extern __shared__ __align__(16) unsigned char smem[];
+  T *Gs = reinterpret_cast<T *>(smem);
+  T *Ys = Gs + kWave * KP;
+  T *Al = Ys + kWave * YP;  // nr x H
+  T *Bs = Al + p.nr * p.H;  // F
+  const int i = blockIdx.x;
+  const int lane = threadIdx.x;
+  const int N = p.N, H = p.H, W = p.W, F = p.F, S = p.S, nr = p.nr;
+  V acc[MT][NT], cmp[MT][NT];
+#pragma unroll
+  for (int a = 0; a < MT; a++)
+#pragma unroll
+    for (int b = 0; b < NT; b++) {
+      acc[a][b] = V{0, 0, 0, 0};
+      cmp[a][b] = V{0, 0, 0, 0};
+    }
+  const int si = species[i];
+  const int beg = nl.cum[i], end = nl.cum[i + 1];
+  const int col = lane & 15, krow = lane >> 4;
+  for (int c0 = beg; c0 < end; c0 += kWave) {
+    const int cnt = min(kWave, end - c0);
+    __syncthreads();
+    {
+      T *gr = Gs + lane * KP;
+      T *yr = Ys + lane * YP;
+      for (int c = 0; c < MT * 16; c++) gr[c] = 0;
+      for (int c = 0; c < NT * 16; c++) yr[c] = 0;
+      if (lane < cnt) {
+        const int q = c0 + lane;
+        const double *rv = nl.rel + 4 * (int64_t)q;
+        const int sj = nl.nspec[q];
+        if (pair_inside(p, si, sj, rv)) {
+          const T r = (T)rv[0];
+          const T rinv = T(1) / r;
+          T g[NMAX];
+          radial_basis<T, NMAX>(r, (T)p.cut[si * S + sj], (T)p.r1, N,
+                                p.cutoff_kind, g, (T *)nullptr);
+#pragma unroll
+          for (int n = 0; n < NMAX; n++)
+            if (n < N) gr[sj * N + n] = g[n];
+          ylm_visit<T, false>((T)rv[1] * rinv, (T)rv[2] * rinv,
+                              (T)rv[3] * rinv, rinv, p.L, ynorm,
+                              [&](int idx, T y, T, T, T) { yr[idx] = y; });
+        }
+      }
+    }
+    __syncthreads();
+    const int steps = (cnt + 3) >> 2;
+    for (int st = 0; st < steps; st++) {
+      const int kk = 4 * st + krow;
+      T a[MT], b[NT];
+#pragma unroll
+      for (int m = 0; m < MT; m++) a[m] = Gs[kk * KP + m * 16 + col];
+#pragma unroll
+      for (int n = 0; n < NT; n++) b[n] = Ys[kk * YP + n * 16 + col];
+#pragma unroll
+      for (int m = 0; m < MT; m++)
+#pragma unroll
+        for (int n = 0; n < NT; n++) {
+          if (KAHAN) {
+            V part = Mma<T>::run(a[m], b[n], V{0, 0, 0, 0});
+            V y = part - cmp[m][n];
+            V t = acc[m][n] + y;
+            cmp[m][n] = (t - acc[m][n]) - y;
+            acc[m][n] = t;
+          } else {
+            acc[m][n] = Mma<T>::run(a[m], b[n], acc[m][n]);
+          }
+        }
+    }
+  }
+  __syncthreads();
+  T *Ai = A_out + (int64_t)i * W;
+#pragma unroll
+  for (int m = 0; m < MT; m++)
+#pragma unroll
+    for (int n = 0; n < NT; n++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int c = m * 16 + Mma<T>::row(krow, j);
+        const int lm = n * 16 + col;
+        if (c < nr && lm < H) {
+          T v = KAHAN ? acc[m][n][j] - cmp[m][n][j] : acc[m][n][j];
+          Al[c * H + lm] = v;
+          Ai[c * H + lm] = v;
+        }
+      }
+  __syncthreads();
+  b2_contract_store<T>(p, Al, Bs, b2_idx, row_of[i], out, B_out, norm2_out);
+}
+
+template <typename T, bool KAHAN, int MT, int NT>
+static void launch_mfma_t(hipStream_t st, const B2Dev &p, const NListDev &nl,
+                          const int *species, const int *row_of, const T *ynorm,
+                          const int *b2_idx, const FwdOut &out, void *A_out,
+                          void *B_out, double *norm2_out) {
+  size_t lds = sizeof(T) * ((size_t)kWave * (MT * 16 + 1 + NT * 16 + 1) +
+                            (size_t)p.nr * p.H + (size_t)p.F);
+  FLARE_REQUIRE(lds <= 64 * 1024, "descriptor settings too large");
+  if (p.N <= 8)
+    hipLaunchKernelGGL((b2_forward_mfma_kernel<T, MT, NT, KAHAN, 8>), dim3(nl.n),
+                       dim3(kWave), lds, st, p, nl, species, row_of, ynorm,
+                       b2_idx, out, (T *)A_out, (T *)B_out, norm2_out);
+  else
+    hipLaunchKernelGGL((b2_forward_mfma_kernel<T, MT, NT, KAHAN, kMaxN>),
+                       dim3(nl.n), dim3(kWave), lds, st, p, nl, species, row_of,
+                       ynorm, b2_idx, out, (T *)A_out, (T *)B_out, norm2_out);
+}
+
+template <typename T, bool KAHAN>
+static bool launch_mfma(hipStream_t st, const B2Dev &p, const NListDev &nl,
+                        const int *species, const int *row_of, const T *ynorm,
+                        const int *b2_idx, const FwdOut &out, void *A_out,
+                        void *B_out, double *norm2_out) {
+  const int MT = (p.nr + 15) / 16, NT = (p.H + 15) / 16;
+#define FLARE_MFMA_CASE(m, n)                                                 \
+  if (MT == m && NT == n) {                                                   \
+    launch_mfma_t<T, KAHAN, m, n>(st, p, nl, species, row_of, ynorm, b2_idx,  \
+                                  out, A_out, B_out, norm2_out);              \
+    return true;                                                              \
+  }
+  FLARE_MFMA_CASE(1, 1)
+  FLARE_MFMA_CASE(1, 2)
+  FLARE_MFMA_CASE(1, 3)
+  FLARE_MFMA_CASE(1, 4)
+  FLARE_MFMA_CASE(2, 1)
+  FLARE_MFMA_CASE(2, 2)
+  FLARE_MFMA_CASE(2, 3)
+  FLARE_MFMA_CASE(2, 4)
+#undef FLARE_MFMA_CASE
+  return false;
 }
 
 // Plain form: dA/dr for `pb` (<= 64) pairs per workgroup of 256 threads.
@@ -283,22 +502,32 @@ b2_dervs_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
 void launch_b2_forward(hipStream_t st, const B2Dev &p, const NListDev &nl,
                        const int *species, const int *row_of,
                        const double *ynorm, const float *ynorm_f,
-                       const int *b2_idx, int prec,
-                       void *A_out, void *B_out, double *norm2_out) {
+                       const int *b2_idx, int prec, int out_mode, int kp,
+                       void *xh, void *xl, void *A_out, void *B_out,
+                       double *norm2_out) {
   if (nl.n == 0) return;
-  size_t tsz = prec == kPrecF64 ? sizeof(double) : sizeof(float);
-  size_t lds = tsz * (2 * (size_t)p.W + kWave * (size_t)(p.N + p.H)) +
-               kWave * sizeof(int);
-  FLARE_REQUIRE(lds <= 64 * 1024,
-                "descriptor settings too large for the single-bond kernel");
-  if (prec == kPrecF64)
-    hipLaunchKernelGGL((b2_forward_kernel<double, false>), dim3(nl.n),
-                       dim3(kWave), lds, st, p, nl, species, row_of, ynorm,
-                       b2_idx, (double *)A_out, (double *)B_out, norm2_out);
-  else
-    hipLaunchKernelGGL((b2_forward_kernel<float, true>), dim3(nl.n),
-                       dim3(kWave), lds, st, p, nl, species, row_of, ynorm_f,
-                       b2_idx, (float *)A_out, (float *)B_out, norm2_out);
+  FwdOut out{out_mode, kp, (__bf16 *)xh, (__bf16 *)xl};
+  // matrix-core path when the channel and harmonic counts fit its tiles
+  bool done = prec == kPrecF64
+                  ? launch_mfma<double, false>(st, p, nl, species, row_of, ynorm,
+                                               b2_idx, out, A_out, B_out, norm2_out)
+                  : launch_mfma<float, true>(st, p, nl, species, row_of, ynorm_f,
+                                             b2_idx, out, A_out, B_out, norm2_out);
+  if (!done) {
+    size_t tsz = prec == kPrecF64 ? sizeof(double) : sizeof(float);
+    size_t lds = tsz * (2 * (size_t)p.W + kWave * (size_t)(p.N + p.H) + p.F) +
+                 kWave * sizeof(int);
+    FLARE_REQUIRE(lds <= 64 * 1024,
+                  "descriptor settings too large for the single-bond kernel");
+    if (prec == kPrecF64)
+      hipLaunchKernelGGL((b2_forward_kernel<double, false>), dim3(nl.n),
+                         dim3(kWave), lds, st, p, nl, species, row_of, ynorm,
+                         b2_idx, out, (double *)A_out, (double *)B_out, norm2_out);
+    else
+      hipLaunchKernelGGL((b2_forward_kernel<float, true>), dim3(nl.n),
+                         dim3(kWave), lds, st, p, nl, species, row_of, ynorm_f,
+                         b2_idx, out, (float *)A_out, (float *)B_out, norm2_out);
+  }
   FLARE_HIP_CHECK(hipGetLastError());
 }
 

@@ -1,12 +1,10 @@
 // forces.hip -- energy, u = dE/dA and pair forces.
 //
-//  mapped_w_kernel   E_i and w = dE_i/dB_i of the mapped potential
-//                    (lammps_descriptor.cpp:250-304): power 2
-//                    E = B.beta B/|B|^2, w = 2(beta B - E B)/|B|^2;
-//                    power 1 E = B.beta/|B|, w = beta/|B| - E B/|B|^2.
-//                    Environments with |B|^2 < 1e-8 give zero
-//                    (pair_flare.cpp:135).
-//  force_kernel      per atom: u[c,lm] = sum_c' w[p(c,c'),l] A[c',lm] (1+d_cc')
+//  force_kernel      per atom: E_i and w = dE_i/dB_i of the mapped
+//                    potential (lammps_descriptor.cpp:250-304; power 2
+//                    E = B.beta B/|B|^2, w = 2(beta B - E B)/|B|^2, power 1
+//                    E = B.beta/|B|, w = beta/|B| - E B/|B|^2) unless w is
+//                    given, then u[c,lm] = sum_c' w[p(c,c'),l] A[c',lm] (1+d_cc')
 //                    (lammps_descriptor.cpp:272-303), then for every pair
 //                    f_ik = sum_{n,lm} u[s_k N + n, lm] dA[n,lm]/dr_ik.
 //                    plain:   dA/dr read from the materialised tensor
@@ -24,46 +22,12 @@
 namespace flare {
 
 template <typename T>
-__global__ void __launch_bounds__(64)
-mapped_w_kernel(int n_rows, int F, int power, int normalized,
-                const T *__restrict__ X, const T *__restrict__ Z,
-                const double *__restrict__ norm2,
-                const int *__restrict__ species_of_row,
-                const T *__restrict__ beta1, T *__restrict__ w,
-                double *__restrict__ e_row) {
-  const int r = blockIdx.x;
-  const int lane = threadIdx.x;
-  const double n2 = norm2[r];
-  const T *x = X + (int64_t)r * F;
-  T *wr = w + (int64_t)r * F;
-  if (n2 < 1e-8) {
-    for (int f = lane; f < F; f += kWave) wr[f] = 0;
-    if (lane == 0) e_row[r] = 0;
-    return;
-  }
-  const double bn = sqrt(n2);
-  if (power == 2) {
-    const T *z = Z + (int64_t)r * F;
-    double d = 0;
-    for (int f = lane; f < F; f += kWave) d += (double)x[f] * (double)z[f];
-    d = wave_sum(d);
-    const double E = normalized ? d : d / n2;
-    const double sc = normalized ? 1.0 / bn : 1.0 / n2;
-    for (int f = lane; f < F; f += kWave)
-      wr[f] = (T)(2.0 * ((double)z[f] - E * (double)x[f]) * sc);
-    if (lane == 0) e_row[r] = E;
-  } else {
-    const T *b = beta1 + (int64_t)species_of_row[r] * F;
-    double d = 0;
-    for (int f = lane; f < F; f += kWave) d += (double)x[f] * (double)b[f];
-    d = wave_sum(d);
-    const double E = normalized ? d : d / bn;
-    const double dsc = normalized ? 1.0 : 1.0 / bn;  // d_hat = x * dsc
-    for (int f = lane; f < F; f += kWave)
-      wr[f] = (T)(((double)b[f] - E * (double)x[f] * dsc) / bn);
-    if (lane == 0) e_row[r] = E;
-  }
-}
+struct WSource {
+  int normalized;
+  const T *X, *Z, *beta1, *w;
+  const double *norm2;
+  double *E_row;
+};
 
 namespace {
 
@@ -107,12 +71,19 @@ __device__ inline void adjoint_pair(const B2Dev &p, T r, T x, T y, T z,
 
 }  // namespace
 
-// One wavefront per atom.  Dynamic LDS (T): A[W], w[F], u[W].
-template <typename T, int MODE, int NMAX>
+// One wavefront per atom.  Dynamic LDS (T): A[W], w[F], u[W], x[F].
+// WM selects where w = dE_i/dB_i comes from:
+//   0  precomputed (sparse-GP path);
+//   1  mapped power 2 from the GEMM rows X and Z = beta X: E = X.Z/|X|^2,
+//      w = 2 (Z - E X) / (|B| |X|)   (X = B, or B/|B| when normalized);
+//   2  mapped power 1 from X and beta_s: E = X.beta/|X|,
+//      w = (beta - E B/|B|) / |B|    (lammps_descriptor.cpp:262-270).
+// Environments with |B|^2 < 1e-8 carry no energy (pair_flare.cpp:135).
+template <typename T, int MODE, int NMAX, int WM>
 __global__ void __launch_bounds__(64)
 force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
              const int *__restrict__ row_of, const T *__restrict__ ynorm,
-             const T *__restrict__ A_in, const T *__restrict__ w_in,
+             const T *__restrict__ A_in, WSource<T> src,
              const double *__restrict__ dA, double *__restrict__ pair_f,
              double *__restrict__ vir_atom) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -123,9 +94,39 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
   T *w = A + W;
   T *u = w + F;
   const T *Ai = A_in + (int64_t)i * W;
-  const T *wi = w_in + (int64_t)row_of[i] * F;
+  const int row = row_of[i];
   for (int o = lane; o < W; o += kWave) A[o] = Ai[o];
-  for (int f = lane; f < F; f += kWave) w[f] = wi[f];
+  if (WM == 0) {
+    const T *wi = src.w + (int64_t)row * F;
+    for (int f = lane; f < F; f += kWave) w[f] = wi[f];
+  } else {
+    const double n2 = src.norm2[row];
+    const T *x = src.X + (int64_t)row * F;
+    if (n2 < 1e-8) {
+      for (int f = lane; f < F; f += kWave) w[f] = 0;
+      if (lane == 0) src.E_row[row] = 0;
+    } else {
+      const double bn = sqrt(n2);
+      const T *z = WM == 1 ? src.Z + (int64_t)row * F
+                           : src.beta1 + (int64_t)species[i] * F;
+      double d = 0;
+      for (int f = lane; f < F; f += kWave) d += (double)x[f] * (double)z[f];
+      d = wave_sum(d);
+      double E, sx, sz;
+      if (WM == 1) {
+        E = src.normalized ? d : d / n2;
+        sz = src.normalized ? 2.0 / bn : 2.0 / n2;
+        sx = -E * sz;
+      } else {
+        E = src.normalized ? d : d / bn;
+        sz = 1.0 / bn;
+        sx = -E / bn * (src.normalized ? 1.0 : 1.0 / bn);
+      }
+      for (int f = lane; f < F; f += kWave)
+        w[f] = (T)(sz * (double)z[f] + sx * (double)x[f]);
+      if (lane == 0) src.E_row[row] = E;
+    }
+  }
   __syncthreads();
   for (int o = lane; o < W; o += kWave) {
     const int c = o / H, lm = o - (o / H) * H;
@@ -197,27 +198,73 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
     for (int q = 0; q < 9; q++) vir_atom[9 * (int64_t)i + q] = v[q];
 }
 
-__global__ void gather_kernel(int n, const int *__restrict__ cum,
-                              const int *__restrict__ rev,
-                              const double *__restrict__ pf,
-                              double *__restrict__ forces) {
-  int a = blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= n) return;
+// 16 lanes per atom, four atoms per wavefront
+__global__ void __launch_bounds__(256)
+gather_kernel(int n, const int *__restrict__ cum, const int *__restrict__ rev,
+              const double *__restrict__ pf, double *__restrict__ forces) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int a = t >> 4, sub = t & 15;
   double f0 = 0, f1 = 0, f2 = 0;
-  for (int k = cum[a]; k < cum[a + 1]; k++) {
-    int r = rev[k];
-    f0 += pf[3 * (int64_t)k];
-    f1 += pf[3 * (int64_t)k + 1];
-    f2 += pf[3 * (int64_t)k + 2];
-    if (r >= 0) {
-      f0 -= pf[3 * (int64_t)r];
-      f1 -= pf[3 * (int64_t)r + 1];
-      f2 -= pf[3 * (int64_t)r + 2];
+  if (a < n) {
+    for (int k = cum[a] + sub; k < cum[a + 1]; k += 16) {
+      const int r = rev[k];
+      f0 += pf[3 * (int64_t)k];
+      f1 += pf[3 * (int64_t)k + 1];
+      f2 += pf[3 * (int64_t)k + 2];
+      if (r >= 0) {
+        f0 -= pf[3 * (int64_t)r];
+        f1 -= pf[3 * (int64_t)r + 1];
+        f2 -= pf[3 * (int64_t)r + 2];
+      }
     }
   }
-  forces[3 * (int64_t)a] = f0;
-  forces[3 * (int64_t)a + 1] = f1;
-  forces[3 * (int64_t)a + 2] = f2;
+#pragma unroll
+  for (int off = 8; off > 0; off >>= 1) {
+    f0 += __shfl_xor(f0, off, 16);
+    f1 += __shfl_xor(f1, off, 16);
+    f2 += __shfl_xor(f2, off, 16);
+  }
+  if (a < n && sub == 0) {
+    forces[3 * (int64_t)a] = f0;
+    forces[3 * (int64_t)a + 1] = f1;
+    forces[3 * (int64_t)a + 2] = f2;
+  }
+}
+
+// Deterministic two-stage sum of the per-row energies and per-atom virial
+// rows: stage 1 gives kPartials partial sums over fixed contiguous ranges.
+constexpr int kPartials = 128;
+
+__global__ void __launch_bounds__(256)
+partial_sums_kernel(int n, const double *__restrict__ e_row,
+                    const double *__restrict__ vir, double *__restrict__ part) {
+  __shared__ double sh[256];
+  const int t = threadIdx.x;
+  const int per = (n + gridDim.x - 1) / gridDim.x;
+  const int lo = blockIdx.x * per, hi = min(n, lo + per);
+  for (int q = 0; q < 10; q++) {
+    double s = 0;
+    for (int r = lo + t; r < hi; r += blockDim.x)
+      s += q == 0 ? e_row[r] : vir[(int64_t)r * 9 + (q - 1)];
+    sh[t] = s;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+      if (t < off) sh[t] += sh[t + off];
+      __syncthreads();
+    }
+    if (t == 0) part[blockIdx.x * 10 + q] = sh[0];
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(64)
+partial_final_kernel(int nparts, const double *__restrict__ part,
+                     double *__restrict__ sums) {
+  const int q = threadIdx.x;
+  if (q >= 10) return;
+  double s = 0;
+  for (int b = 0; b < nparts; b++) s += part[b * 10 + q];
+  sums[q] = s;
 }
 
 // one workgroup per column; fixed partition => deterministic
@@ -265,59 +312,57 @@ __global__ void finish_kernel(const double *__restrict__ sums,
 }
 
 // ---------------------------------------------------------------------------
-void launch_mapped_w(hipStream_t st, int prec, int n_rows, int F, int power,
-                     int normalized, const void *X, const void *Z,
-                     const double *norm2, const int *species_of_row,
-                     const void *beta1, void *w_out, double *e_row) {
-  if (n_rows == 0) return;
-  if (prec == kPrecF64)
-    hipLaunchKernelGGL(mapped_w_kernel<double>, dim3(n_rows), dim3(kWave), 0,
-                       st, n_rows, F, power, normalized, (const double *)X,
-                       (const double *)Z, norm2, species_of_row,
-                       (const double *)beta1, (double *)w_out, e_row);
-  else
-    hipLaunchKernelGGL(mapped_w_kernel<float>, dim3(n_rows), dim3(kWave), 0,
-                       st, n_rows, F, power, normalized, (const float *)X,
-                       (const float *)Z, norm2, species_of_row,
-                       (const float *)beta1, (float *)w_out, e_row);
-  FLARE_HIP_CHECK(hipGetLastError());
-}
-
-template <typename T, int MODE>
+template <typename T, int MODE, int WM>
 static void launch_forces_t(hipStream_t st, const B2Dev &p, const NListDev &nl,
                             const int *species, const int *row_of,
-                            const T *ynorm, const void *A, const void *w,
-                            const double *dA, double *pair_f,
-                            double *vir_atom) {
+                            const T *ynorm, const void *A, const WSource<T> &src,
+                            const double *dA, double *pair_f, double *vir_atom) {
   size_t lds = sizeof(T) * (2 * (size_t)p.W + (size_t)p.F);
   FLARE_REQUIRE(lds <= 64 * 1024, "descriptor settings too large for the force kernel");
   if (p.N <= 8)
-    hipLaunchKernelGGL((force_kernel<T, MODE, 8>), dim3(nl.n), dim3(kWave),
-                       lds, st, p, nl, species, row_of, ynorm, (const T *)A,
-                       (const T *)w, dA, pair_f, vir_atom);
+    hipLaunchKernelGGL((force_kernel<T, MODE, 8, WM>), dim3(nl.n), dim3(kWave),
+                       lds, st, p, nl, species, row_of, ynorm, (const T *)A, src,
+                       dA, pair_f, vir_atom);
   else
-    hipLaunchKernelGGL((force_kernel<T, MODE, kMaxN>), dim3(nl.n),
+    hipLaunchKernelGGL((force_kernel<T, MODE, kMaxN, WM>), dim3(nl.n),
                        dim3(kWave), lds, st, p, nl, species, row_of, ynorm,
-                       (const T *)A, (const T *)w, dA, pair_f, vir_atom);
+                       (const T *)A, src, dA, pair_f, vir_atom);
+}
+
+template <typename T, int MODE>
+static void launch_forces_w(hipStream_t st, const B2Dev &p, const NListDev &nl,
+                            const int *species, const int *row_of,
+                            const T *ynorm, const void *A, const WInput &wi,
+                            const double *dA, double *pair_f, double *vir_atom) {
+  WSource<T> src{wi.normalized, (const T *)wi.X, (const T *)wi.Z,
+                 (const T *)wi.beta1, (const T *)wi.w, wi.norm2, wi.E_row};
+  if (wi.mode == 0)
+    launch_forces_t<T, MODE, 0>(st, p, nl, species, row_of, ynorm, A, src, dA,
+                                pair_f, vir_atom);
+  else if (wi.mode == 1)
+    launch_forces_t<T, MODE, 1>(st, p, nl, species, row_of, ynorm, A, src, dA,
+                                pair_f, vir_atom);
+  else
+    launch_forces_t<T, MODE, 2>(st, p, nl, species, row_of, ynorm, A, src, dA,
+                                pair_f, vir_atom);
 }
 
 void launch_forces(hipStream_t st, int prec, int mode, const B2Dev &p,
                    const NListDev &nl, const int *species, const int *row_of,
-                   const double *ynorm, const float *ynorm_f, const int *b2_idx,
-                   const void *A, const void *w, const double *dA,
-                   double *pair_f, double *vir_atom) {
-  (void)b2_idx;
+                   const double *ynorm, const float *ynorm_f, const void *A,
+                   const WInput &wi, const double *dA, double *pair_f,
+                   double *vir_atom) {
   if (nl.n == 0) return;
   if (prec == kPrecF64) {
     if (mode == 0)
-      launch_forces_t<double, 0>(st, p, nl, species, row_of, ynorm, A, w, dA,
+      launch_forces_w<double, 0>(st, p, nl, species, row_of, ynorm, A, wi, dA,
                                  pair_f, vir_atom);
     else
-      launch_forces_t<double, 1>(st, p, nl, species, row_of, ynorm, A, w, dA,
+      launch_forces_w<double, 1>(st, p, nl, species, row_of, ynorm, A, wi, dA,
                                  pair_f, vir_atom);
   } else {
     FLARE_REQUIRE(mode == 1, "the plain (materialised) form runs in FP64 only");
-    launch_forces_t<float, 1>(st, p, nl, species, row_of, ynorm_f, A, w, dA,
+    launch_forces_w<float, 1>(st, p, nl, species, row_of, ynorm_f, A, wi, dA,
                               pair_f, vir_atom);
   }
   FLARE_HIP_CHECK(hipGetLastError());
@@ -326,8 +371,17 @@ void launch_forces(hipStream_t st, int prec, int mode, const B2Dev &p,
 void launch_gather(hipStream_t st, const NListDev &nl, const double *pair_f,
                    double *forces) {
   if (nl.n == 0) return;
-  hipLaunchKernelGGL(gather_kernel, dim3((nl.n + 255) / 256), dim3(256), 0, st,
-                     nl.n, nl.cum, nl.rev, pair_f, forces);
+  hipLaunchKernelGGL(gather_kernel, dim3((nl.n * 16 + 255) / 256), dim3(256), 0,
+                     st, nl.n, nl.cum, nl.rev, pair_f, forces);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_energy_virial_sums(hipStream_t st, int n, const double *e_row,
+                               const double *vir, double *part, double *sums) {
+  hipLaunchKernelGGL(partial_sums_kernel, dim3(kPartials), dim3(256), 0, st, n,
+                     e_row, vir, part);
+  hipLaunchKernelGGL(partial_final_kernel, dim3(1), dim3(64), 0, st, kPartials,
+                     part, sums);
   FLARE_HIP_CHECK(hipGetLastError());
 }
 

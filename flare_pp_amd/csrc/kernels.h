@@ -55,7 +55,8 @@ struct WrapHost {
 struct NListBuffers {
   DevBuf wrapped, bin_of, bin_count, bin_start, bin_fill, bin_atoms, count,
       cum, keys_tmp, rel_tmp, keys, rel, idx, nspec, center, rev;
-  DevBuf row_of, atom_of_row, species_of_row, species_count;
+  DevBuf row_of, atom_of_row, species_of_row, species_count, species_blk,
+      scan_tmp;
   int P = 0;
 };
 
@@ -70,10 +71,14 @@ int species_rows_device(hipStream_t st, NListBuffers &b, int n,
                         const int *d_species, int *host_counts);
 
 // descriptors.hip
+// A (atom order), GEMM rows (species-grouped: B when out_mode 0, B/|B| when
+// 1, plus the BF16 hi/lo split of the latter into rows of kp when xh/xl are
+// non-null) and |B|^2.
 void launch_b2_forward(hipStream_t st, const B2Dev &p, const NListDev &nl,
                        const int *species, const int *row_of,
                        const double *ynorm, const float *ynorm_f,
-                       const int *b2_idx, int prec, void *A_out, void *B_out,
+                       const int *b2_idx, int prec, int out_mode, int kp,
+                       void *xh, void *xl, void *A_out, void *B_out,
                        double *norm2_out);
 void launch_pair_dA(hipStream_t st, const B2Dev &p, const NListDev &nl,
                     const int *species, const double *ynorm, double *out);
@@ -84,24 +89,31 @@ void launch_b2_dervs(hipStream_t st, const B2Dev &p, const NListDev &nl,
                      double *dB, double *fdots, double *coords, int *nidx);
 
 // forces.hip
-// Mapped energy per row and w = dE/dB (lammps_descriptor.cpp:250-304).
-// X = GEMM input rows (B itself, or B/|B| when normalized), Z = beta X.
-// power 1: beta1 holds one F-vector per species, Z unused.
-void launch_mapped_w(hipStream_t st, int prec, int n_rows, int F, int power,
-                     int normalized, const void *X, const void *Z,
-                     const double *norm2, const int *species_of_row,
-                     const void *beta1, void *w_out, double *e_row);
+// Source of w = dE_i/dB_i for the force kernel.  mode 0: precomputed w
+// rows; 1: mapped power 2 from GEMM rows X and Z = beta X; 2: mapped power 1
+// from X and beta1 (one F-vector per species).  Modes 1 and 2 also write the
+// per-row energies.  X is B, or B/|B| when `normalized`.
+struct WInput {
+  int mode, normalized;
+  const void *X, *Z, *beta1, *w;
+  const double *norm2;
+  double *E_row;
+};
 // Per-atom u = dE/dA, pair forces f_ik and per-atom virial rows (9 values,
 // V[a][b] = -sum_k r_a f_b).  mode 0: plain (reads dA from `dA`),
 // mode 1: adjoint (recomputes the pair basis).
 void launch_forces(hipStream_t st, int prec, int mode, const B2Dev &p,
                    const NListDev &nl, const int *species, const int *row_of,
-                   const double *ynorm, const float *ynorm_f, const int *b2_idx,
-                   const void *A, const void *w, const double *dA,
-                   double *pair_f, double *vir_atom);
+                   const double *ynorm, const float *ynorm_f, const void *A,
+                   const WInput &wi, const double *dA, double *pair_f,
+                   double *vir_atom);
 // F_a = sum_k (f_ak - f_rev(ak)) into forces (stride 3)
 void launch_gather(hipStream_t st, const NListDev &nl, const double *pair_f,
                    double *forces);
+// sums[0] = sum e_row, sums[1..9] = column sums of vir (n x 9); part holds
+// 128 x 10 partial sums; deterministic
+void launch_energy_virial_sums(hipStream_t st, int n, const double *e_row,
+                               const double *vir, double *part, double *sums);
 // column sums of an (n x k) row-major matrix into out[k], deterministic
 void launch_colsum(hipStream_t st, int n, int k, const double *x, double *out);
 // final assembly of energy, virial (LAMMPS order) and GP stress

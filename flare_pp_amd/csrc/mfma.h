@@ -7,13 +7,20 @@
 
 namespace flare {
 
-// Z (M x N, row-major FP32) = X (M x K, row-major FP32) * Bsym, where Bsym
-// (K x N, row-major FP32) is symmetric with K == N, on BF16 MFMA with FP32
-// accumulation: each operand is split into hi + lo BF16 parts and the
-// three products hi*hi + hi*lo + lo*hi are accumulated, which keeps ~16
-// significant bits per operand.
-void gemm_bf16x3(hipStream_t st, int M, int N, int K, const float *X,
-                 const float *Bsym, float *Z, void *workspace);
-size_t gemm_bf16x3_workspace(int M, int N, int K);
+// K padding of the BF16 operands (multiple of the GEMM's K step).
+int bf16_padded_k(int K);
+
+// Z (M x N, FP32, leading dimension ldz) = (Ah + Al) . (Bh + Bl)^T on BF16
+// MFMA with FP32 accumulation, three products hi*hi + hi*lo + lo*hi.
+// Ah/Al: M x Kp BF16 rows, Bh/Bl: N x Kp BF16 rows (row n of a symmetric
+// right operand is its column n); Kp a multiple of bf16_padded_k's step and
+// the padding zero.
+void gemm_bf16x3(hipStream_t st, int M, int N, int Kp, const void *Ah,
+                 const void *Al, const void *Bh, const void *Bl, float *Z,
+                 int ldz);
+
+// hi/lo BF16 split of a rows x cols FP64 matrix into rows of kp (zero padded)
+void split_bf16(hipStream_t st, int rows, int cols, int kp, const double *x,
+                void *hi, void *lo);
 
 }  // namespace flare

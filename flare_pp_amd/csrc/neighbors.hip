@@ -11,6 +11,8 @@
 // scan -> fill -> per-atom rank sort by key -> reverse-pair index.  One
 // wavefront per atom enumerates the candidates of the 27 (or, along thin
 // axes, all) neighboring bins cooperatively.
+#include <hipcub/hipcub.hpp>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -71,31 +73,6 @@ __global__ void bin_fill_kernel(int n, const int *__restrict__ bin_of,
   int b = bin_of[i];
   int slot = atomicAdd(&fillp[b], 1);
   bin_atoms[bin_start[b] + slot] = i;
-}
-
-// Exclusive scan of n ints into out[0..n] (out[n] = total), one workgroup.
-__global__ void __launch_bounds__(1024)
-scan_kernel(int n, const int *__restrict__ in, int *__restrict__ out) {
-  __shared__ int part[1024];
-  int tid = threadIdx.x;
-  int per = (n + blockDim.x - 1) / blockDim.x;
-  int lo = tid * per, hi = min(n, lo + per);
-  int s = 0;
-  for (int k = lo; k < hi; k++) s += in[k];
-  part[tid] = s;
-  __syncthreads();
-  for (int off = 1; off < (int)blockDim.x; off <<= 1) {
-    int v = tid >= off ? part[tid - off] : 0;
-    __syncthreads();
-    part[tid] += v;
-    __syncthreads();
-  }
-  int run = part[tid] - s;  // exclusive prefix of this thread's chunk
-  for (int k = lo; k < hi; k++) {
-    out[k] = run;
-    run += in[k];
-  }
-  if (tid == blockDim.x - 1) out[n] = part[tid];
 }
 
 namespace {
@@ -311,88 +288,123 @@ __global__ void nbr_reverse_kernel(int64_t P, int sweep,
 }
 
 // Species-grouped row map (b2.cpp:104-176 regroup: atoms of species 0 in
-// original order, then species 1, ...).  One workgroup; each thread owns a
-// contiguous chunk of atoms.
-__global__ void __launch_bounds__(1024)
-species_rows_kernel(int n, const int *__restrict__ species,
-                    int *__restrict__ row_of, int *__restrict__ atom_of_row,
-                    int *__restrict__ species_of_row, int *__restrict__ counts) {
-  __shared__ int part[kMaxSpecies][1025];
-  __shared__ int bad;
+// original order, then species 1, ...): per-block species counts, a scan
+// over blocks, and per-block ranks from wavefront ballots.
+constexpr int kRowBlock = 256;
+
+__global__ void __launch_bounds__(kRowBlock)
+species_count_kernel(int n, const int *__restrict__ species,
+                     int *__restrict__ bcount, int *__restrict__ bad) {
+  __shared__ int cnt[kMaxSpecies];
   const int t = threadIdx.x;
-  const int nt = blockDim.x;
-  if (t == 0) bad = 0;
-  const int per = (n + nt - 1) / nt;
-  const int lo = t * per, hi = min(n, lo + per);
-  int loc[kMaxSpecies];
-  for (int s = 0; s < kMaxSpecies; s++) loc[s] = 0;
+  if (t < kMaxSpecies) cnt[t] = 0;
   __syncthreads();
-  for (int k = lo; k < hi; k++) {
-    int s = species[k];
+  const int a = blockIdx.x * kRowBlock + t;
+  if (a < n) {
+    const int s = species[a];
     if (s < 0 || s >= kMaxSpecies)
-      bad = 1;
+      atomicOr(bad, 1);
     else
-      loc[s]++;
-  }
-  for (int s = 0; s < kMaxSpecies; s++) part[s][t + 1] = loc[s];
-  if (t == 0)
-    for (int s = 0; s < kMaxSpecies; s++) part[s][0] = 0;
-  __syncthreads();
-  if (t < kMaxSpecies) {  // serial exclusive scan per species (nt values)
-    int run = 0;
-    for (int q = 0; q <= nt; q++) {
-      int v = part[t][q];
-      part[t][q] = run;
-      run += v;
-    }
-    part[t][nt] = run;  // total of species t
+      atomicAdd(&cnt[s], 1);
   }
   __syncthreads();
-  int off[kMaxSpecies];
-  int acc = 0;
-  for (int s = 0; s < kMaxSpecies; s++) {
-    off[s] = acc;
-    acc += part[s][nt];
-  }
-  if (!bad) {
-    int run[kMaxSpecies];
-    for (int s = 0; s < kMaxSpecies; s++) run[s] = off[s] + part[s][t + 1];
-    for (int k = lo; k < hi; k++) {
-      int s = species[k];
-      int row = run[s]++;
-      row_of[k] = row;
-      atom_of_row[row] = k;
-      species_of_row[row] = s;
+  if (t < kMaxSpecies) bcount[blockIdx.x * kMaxSpecies + t] = cnt[t];
+}
+
+// one wavefront per species: exclusive scan of the block counts
+__global__ void __launch_bounds__(64 * kMaxSpecies)
+species_scan_kernel(int nblk, const int *__restrict__ bcount,
+                    int *__restrict__ boff, int *__restrict__ counts) {
+  const int lane = threadIdx.x & 63, s = threadIdx.x >> 6;
+  int run = 0;
+  for (int b0 = 0; b0 < nblk; b0 += 64) {
+    const int b = b0 + lane;
+    const int v = b < nblk ? bcount[b * kMaxSpecies + s] : 0;
+    int incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      int u = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += u;
     }
+    if (b < nblk) boff[b * kMaxSpecies + s] = run + incl - v;
+    run += __shfl(incl, 63, 64);
   }
-  if (t == 0) {
-    for (int s = 0; s < kMaxSpecies; s++) counts[s] = part[s][nt];
-    counts[kMaxSpecies] = bad;
+  if (lane == 0) counts[s] = run;
+}
+
+__global__ void __launch_bounds__(kRowBlock)
+species_rows_kernel(int n, const int *__restrict__ species,
+                    const int *__restrict__ boff, const int *__restrict__ counts,
+                    int *__restrict__ row_of, int *__restrict__ atom_of_row,
+                    int *__restrict__ species_of_row) {
+  __shared__ int wcnt[kRowBlock / 64][kMaxSpecies];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int a = blockIdx.x * kRowBlock + t;
+  const int s = a < n ? species[a] : -1;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  int my_rank = 0;
+  for (int q = 0; q < kMaxSpecies; q++) {
+    const uint64_t m = __ballot(s == q);
+    if (s == q) my_rank = __popcll(m & lt);
+    if (lane == 0) wcnt[wv][q] = __popcll(m);
+  }
+  __syncthreads();
+  if (s >= 0) {
+    int base = 0;
+    for (int q = 0; q < s; q++) base += counts[q];
+    for (int w = 0; w < wv; w++) my_rank += wcnt[w][s];
+    const int row = base + boff[blockIdx.x * kMaxSpecies + s] + my_rank;
+    row_of[a] = row;
+    atom_of_row[row] = a;
+    species_of_row[row] = s;
   }
 }
 
 int species_rows_device(hipStream_t st, NListBuffers &b, int n,
                         const int *d_species, int *host_counts) {
+  const int nblk = std::max(1, (n + kRowBlock - 1) / kRowBlock);
   b.row_of.ensure((size_t)std::max(n, 1) * sizeof(int));
   b.atom_of_row.ensure((size_t)std::max(n, 1) * sizeof(int));
   b.species_of_row.ensure((size_t)std::max(n, 1) * sizeof(int));
   b.species_count.ensure((kMaxSpecies + 1) * sizeof(int));
-  hipLaunchKernelGGL(species_rows_kernel, dim3(1), dim3(1024), 0, st, n,
-                     d_species, b.row_of.as<int>(), b.atom_of_row.as<int>(),
-                     b.species_of_row.as<int>(), b.species_count.as<int>());
+  b.species_blk.ensure((size_t)2 * nblk * kMaxSpecies * sizeof(int));
+  int *bcount = b.species_blk.as<int>(), *boff = bcount + nblk * kMaxSpecies;
+  int *counts = b.species_count.as<int>();
+  FLARE_HIP_CHECK(hipMemsetAsync(counts, 0, (kMaxSpecies + 1) * sizeof(int), st));
+  if (n > 0) {
+    hipLaunchKernelGGL(species_count_kernel, dim3(nblk), dim3(kRowBlock), 0, st,
+                       n, d_species, bcount, counts + kMaxSpecies);
+    hipLaunchKernelGGL(species_scan_kernel, dim3(1), dim3(64 * kMaxSpecies), 0,
+                       st, nblk, bcount, boff, counts);
+  }
   FLARE_HIP_CHECK(hipGetLastError());
   int h[kMaxSpecies + 1];
-  FLARE_HIP_CHECK(hipMemcpyAsync(h, b.species_count.ptr, sizeof(h),
-                                 hipMemcpyDeviceToHost, st));
+  FLARE_HIP_CHECK(hipMemcpyAsync(h, counts, sizeof(h), hipMemcpyDeviceToHost, st));
   FLARE_HIP_CHECK(hipStreamSynchronize(st));
   FLARE_REQUIRE(h[kMaxSpecies] == 0,
                 "species codes must lie in [0, " + std::to_string(kMaxSpecies) + ")");
+  if (n > 0) {
+    hipLaunchKernelGGL(species_rows_kernel, dim3(nblk), dim3(kRowBlock), 0, st, n,
+                       d_species, boff, counts, b.row_of.as<int>(),
+                       b.atom_of_row.as<int>(), b.species_of_row.as<int>());
+    FLARE_HIP_CHECK(hipGetLastError());
+  }
   int ns = 0;
   for (int s = 0; s < kMaxSpecies; s++) {
     host_counts[s] = h[s];
     if (h[s] > 0) ns = s + 1;
   }
   return ns;
+}
+
+// exclusive prefix sum of in[0..n) into out[0..n], out[n] = total; in[n]
+// must be readable and zero
+static void exclusive_scan(hipStream_t st, DevBuf &tmp, const int *in, int *out,
+                           int n) {
+  size_t bytes = 0;
+  FLARE_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n + 1, st));
+  tmp.ensure(bytes);
+  FLARE_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp.ptr, bytes, in, out, n + 1, st));
 }
 
 // ---------------------------------------------------------------------------
@@ -416,11 +428,11 @@ void nlist_build_device(hipStream_t st, NListBuffers &b, int n,
 
   b.wrapped.ensure((size_t)n * 3 * sizeof(double));
   b.bin_of.ensure((size_t)n * sizeof(int));
-  b.bin_count.ensure((size_t)(nbins + 1) * sizeof(int));
+  b.bin_count.ensure((size_t)(nbins + 2) * sizeof(int));
   b.bin_start.ensure((size_t)(nbins + 1) * sizeof(int));
   b.bin_fill.ensure((size_t)(nbins + 1) * sizeof(int));
   b.bin_atoms.ensure((size_t)std::max(n, 1) * sizeof(int));
-  b.count.ensure((size_t)std::max(n, 1) * sizeof(int));
+  b.count.ensure((size_t)(n + 1) * sizeof(int));
   b.cum.ensure((size_t)(n + 1) * sizeof(int));
 
   const int tb = 256;
@@ -429,15 +441,14 @@ void nlist_build_device(hipStream_t st, NListBuffers &b, int n,
     hipLaunchKernelGGL(wrap_bin_kernel, dim3(g1), dim3(tb), 0, st, n, d_pos, m,
                        b.wrapped.as<double>(), b.bin_of.as<int>());
   }
-  FLARE_HIP_CHECK(hipMemsetAsync(b.bin_count.ptr, 0, (nbins + 1) * sizeof(int), st));
+  FLARE_HIP_CHECK(hipMemsetAsync(b.bin_count.ptr, 0, (nbins + 2) * sizeof(int), st));
   FLARE_HIP_CHECK(hipMemsetAsync(b.bin_fill.ptr, 0, (nbins + 1) * sizeof(int), st));
   if (n > 0)
     hipLaunchKernelGGL(bin_count_kernel, dim3(g1), dim3(tb), 0, st, n,
                        b.bin_of.as<int>(), b.bin_count.as<int>());
-  // bin_count holds counts shifted by one; scan the first nbins entries of
-  // bin_count[1..] into bin_start[0..nbins]
-  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, nbins,
-                     b.bin_count.as<int>() + 1, b.bin_start.as<int>());
+  // bin_count[1..nbins] holds the counts; bin_count has one spare zero entry
+  exclusive_scan(st, b.scan_tmp, b.bin_count.as<int>() + 1, b.bin_start.as<int>(),
+                 nbins);
   if (n > 0)
     hipLaunchKernelGGL(bin_fill_kernel, dim3(g1), dim3(tb), 0, st, n,
                        b.bin_of.as<int>(), b.bin_start.as<int>(),
@@ -449,8 +460,8 @@ void nlist_build_device(hipStream_t st, NListBuffers &b, int n,
                        b.bin_start.as<int>(), b.bin_atoms.as<int>(),
                        b.count.as<int>(), (const int *)nullptr,
                        (int64_t *)nullptr, (double *)nullptr);
-  hipLaunchKernelGGL(scan_kernel, dim3(1), dim3(1024), 0, st, n,
-                     b.count.as<int>(), b.cum.as<int>());
+  FLARE_HIP_CHECK(hipMemsetAsync(b.count.as<int>() + n, 0, sizeof(int), st));
+  exclusive_scan(st, b.scan_tmp, b.count.as<int>(), b.cum.as<int>(), n);
   int P = 0;
   FLARE_HIP_CHECK(hipMemcpyAsync(&P, b.cum.as<int>() + n, sizeof(int),
                                  hipMemcpyDeviceToHost, st));
