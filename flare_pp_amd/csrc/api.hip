@@ -399,7 +399,8 @@ struct Forward {
 };
 
 // out_mode 0 writes B rows, 1 writes B/|B|; with bf16 true the latter is
-// also split into BF16 hi/lo rows (K padded) for the matrix-core GEMM.
+// also split into BF16 hi/lo rows (K padded) for the matrix-core GEMM;
+// out_mode 2 writes only that split.
 static Forward run_forward(flare_ctx *c, const flare::B2Dev &p,
                            const flare_nlist *nl, const int *d_species,
                            int prec, const std::string &tag,
@@ -409,7 +410,8 @@ static Forward run_forward(flare_ctx *c, const flare::B2Dev &p,
   const size_t ts = prec == flare::kPrecF64 ? sizeof(double) : sizeof(float);
   const int n = nl->n;
   f.A = c->buf(tag + ".A", std::max<size_t>(1, (size_t)n * p.W) * ts);
-  f.B = B_out ? B_out : c->buf(tag + ".B", std::max<size_t>(1, (size_t)n * p.F) * ts);
+  if (out_mode != 2)
+    f.B = B_out ? B_out : c->buf(tag + ".B", std::max<size_t>(1, (size_t)n * p.F) * ts);
   f.norm2 = c->buf_t<double>(tag + ".norm2", n);
   f.kp = flare::bf16_padded_k(p.F);
   if (bf16) {
@@ -447,10 +449,10 @@ static void run_forces(flare_ctx *c, const flare::B2Dev &p,
   c->mark_begin(mode == 0 ? "forces_plain" : "forces_adjoint");
   flare::launch_forces(c->stream, prec, mode, p, nl->dev(), d_species,
                        nl->b.row_of.as<int>(), c->ynorm_d.as<double>(),
-                       c->ynorm_f.as<float>(), fw.A, wi, dA, pf, vir);
+                       c->ynorm_f.as<float>(), fw.A, wi, dA, pf);
   c->mark_end();
   c->mark_begin("gather_reduce");
-  if (forces) flare::launch_gather(c->stream, nl->dev(), pf, forces);
+  flare::launch_gather(c->stream, nl->dev(), pf, forces, vir);
   double *sums = c->buf_t<double>(tag + ".sums", 10);
   if (n > 0) {
     double *part = c->buf_t<double>(tag + ".part", 128 * 10);
@@ -474,7 +476,7 @@ struct flare_pair {
   std::string radial = "chebyshev", cutoff_name = "quadratic";
   std::vector<double> cutoffs;
   DevBuf beta, beta_f32, beta_h, beta_l;  // FP64, FP32, BF16 hi/lo (K padded)
-  bool have_last = false;
+  bool have_last = false, last_mixed = false;
   int last_prec = 0, last_n = 0;
   int last_off[flare::kMaxSpecies + 1] = {0};
 };
@@ -1043,7 +1045,7 @@ int flare_pair_compute(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
     // GEMM rows: B itself in FP64 (as pair_flare), B/|B| in the FP32 forms
     const int normalized = prec == flare::kPrecF32 ? 1 : 0;
     Forward fw = run_forward(ctx, p, nl, d_species, prec, "pair", nullptr,
-                             normalized, mixed);
+                             mixed ? 2 : normalized, mixed);
     int off[flare::kMaxSpecies + 1];
     nl->row_offsets(p.S, off);
     // beta in the working precision
@@ -1097,14 +1099,20 @@ int flare_pair_compute(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
     }
     ctx->mark_end();
     double *E_row = ctx->buf_t<double>("pair.E", n);
-    flare::WInput wi{pr->power == 2 ? 1 : 2, normalized, X, Z,
+    flare::WInput wi{mixed ? 3 : pr->power == 2 ? 1 : 2, normalized, X, Z,
                      pr->power == 1 ? beta : nullptr, nullptr, fw.norm2, E_row};
+    if (mixed) {
+      wi.xh = fw.xh;
+      wi.xl = fw.xl;
+      wi.kp = kp;
+    }
     run_forces(ctx, p, nl, d_species, prec, mode, fw, wi, d_forces, d_energy,
                d_virial, nullptr, "pair");
     if (d_eatom)
       flare::launch_rows_to_atoms(ctx->stream, n, nl->b.atom_of_row.as<int>(),
                                   E_row, d_eatom);
     pr->have_last = true;
+    pr->last_mixed = mixed;
     pr->last_prec = prec;
     pr->last_n = n;
     for (int s = 0; s <= p.S; s++) pr->last_off[s] = off[s];
@@ -1124,6 +1132,10 @@ int flare_pair_beta_grad(flare_ctx *ctx, flare_pair *pr, double *d_grad) {
     if (pr->last_prec == flare::kPrecF64)
       flare::launch_normalize_rows_f64(ctx->stream, n, F,
                                        ctx->ws["pair.B"]->as<double>(), norm2, dhat);
+    else if (pr->last_mixed)  // B/|B| as the GEMM's BF16 hi + lo rows
+      flare::launch_bf16_pair_to_f64(ctx->stream, n, F, flare::bf16_padded_k(F),
+                                     ctx->ws["pair.xh"]->ptr, ctx->ws["pair.xl"]->ptr,
+                                     dhat);
     else  // the FP32 forms keep the normalised rows B/|B|
       flare::launch_to_f64(ctx->stream, (int64_t)n * F,
                            ctx->ws["pair.B"]->as<float>(), dhat);

@@ -23,7 +23,8 @@ namespace flare {
 // handed to the contraction with beta / the sparse set:
 //   out.mode 0: B itself;   1: d = B/|B| (zero when |B|^2 < 1e-8, the
 //   empty-environment threshold of pair_flare.cpp:135), plus optionally its
-//   BF16 hi + lo split into rows of out.kp (zero padded).
+//   BF16 hi + lo split into rows of out.kp (zero padded);  2: only that
+//   split (the mixed-precision form reads d nowhere else).
 struct FwdOut {
   int mode;
   int kp;
@@ -52,13 +53,16 @@ __device__ inline void b2_contract_store(const B2Dev &p, const T *Al, T *Bs,
   nrm = wave_sum(nrm);
   if (lane == 0) norm2_out[row] = nrm;
   __syncthreads();
-  T *Bi = B_out + (int64_t)row * F;
   if (o.mode == 0) {
+    T *Bi = B_out + (int64_t)row * F;
     for (int f = lane; f < F; f += kWave) Bi[f] = Bs[f];
     return;
   }
   const T sc = nrm < 1e-8 ? T(0) : (T)(1.0 / sqrt(nrm));
-  for (int f = lane; f < F; f += kWave) Bi[f] = Bs[f] * sc;
+  if (o.mode == 1) {
+    T *Bi = B_out + (int64_t)row * F;
+    for (int f = lane; f < F; f += kWave) Bi[f] = Bs[f] * sc;
+  }
   if (o.xh) {
     __bf16 *xh = o.xh + (int64_t)row * o.kp, *xl = o.xl + (int64_t)row * o.kp;
     for (int f = lane; f < o.kp; f += kWave) {
@@ -198,12 +202,100 @@ struct Mma<double> {
   static __device__ inline int row(int g, int j) { return g + 4 * j; }
 };
 
-template <typename T, int MT, int NT, bool KAHAN, int NMAX>
+// B2 on the matrix cores: for every l the block of features is the Gram
+// matrix of the single-bond rows, B_l[c1][c2] = sum_m A[c1, l^2+m] A[c2, l^2+m]
+// (M = N = channels, K = 2l+1), upper triangle c1 <= c2 scattered to
+// p(c1, c2) (L+1) + l in Bs.  Returns this lane's share of |B|^2.
+template <typename T, int MT, int LFIX>
+__device__ inline double b2_gram_mfma(const B2Dev &p, const T *Al, T *Bs) {
+  typedef typename Mma<T>::V V;
+  const int lane = threadIdx.x;
+  const int col = lane & 15, g = lane >> 4;
+  const int H = p.H, nr = p.nr;
+  const int L = LFIX >= 0 ? LFIX : p.L;
+  const int L1 = L + 1;
+  double nrm = 0;
+#pragma unroll
+  for (int t1 = 0; t1 < MT; t1++)
+#pragma unroll
+    for (int t2 = t1; t2 < MT; t2++) {
+      const int ca = t1 * 16 + col, cb = t2 * 16 + col;
+      for (int l = 0; l <= L; l++) {
+        const int K = 2 * l + 1, l2 = l * l;
+        V acc = V{0, 0, 0, 0};
+        for (int k0 = 0; k0 < K; k0 += 4) {
+          const int k = k0 + g;
+          T a = 0, b = 0;
+          if (k < K) {
+            if (ca < nr) a = Al[ca * H + l2 + k];
+            if (cb < nr) b = Al[cb * H + l2 + k];
+          }
+          acc = Mma<T>::run(a, b, acc);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int c1 = t1 * 16 + Mma<T>::row(g, q), c2 = cb;
+          if (c1 <= c2 && c2 < nr) {
+            const T v = acc[q];
+            Bs[b2_pair_index(c1, c2, nr) * L1 + l] = v;
+            nrm += (double)v * (double)v;
+          }
+        }
+      }
+    }
+  return nrm;
+}
+
+// Output rows from Bs (LDS) and |B|^2 (see b2_contract_store); the BF16
+// split is written four values (8 bytes) per lane store.
+template <typename T>
+__device__ inline void b2_store_rows(const B2Dev &p, const T *Bs, double nrm,
+                                     int row, const FwdOut &o,
+                                     T *__restrict__ B_out,
+                                     double *__restrict__ norm2_out) {
+  const int lane = threadIdx.x;
+  const int F = p.F;
+  if (lane == 0) norm2_out[row] = nrm;
+  if (o.mode == 0) {
+    T *Bi = B_out + (int64_t)row * F;
+    for (int f = lane; f < F; f += kWave) Bi[f] = Bs[f];
+    return;
+  }
+  const T sc = nrm < 1e-8 ? T(0) : (T)(1.0 / sqrt(nrm));
+  if (o.mode == 1) {
+    T *Bi = B_out + (int64_t)row * F;
+    for (int f = lane; f < F; f += kWave) Bi[f] = Bs[f] * sc;
+  }
+  if (o.xh) {
+    uint2 *xh = reinterpret_cast<uint2 *>(o.xh + (int64_t)row * o.kp);
+    uint2 *xl = reinterpret_cast<uint2 *>(o.xl + (int64_t)row * o.kp);
+    for (int f4 = lane; 4 * f4 < o.kp; f4 += kWave) {
+      union {
+        __bf16 h[4];
+        uint2 u;
+      } H, Lo;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int f = 4 * f4 + j;
+        float x = 0.0f;
+        if (f < F) x = (float)(Bs[f] * sc);
+        const __bf16 h = (__bf16)x;
+        H.h[j] = h;
+        Lo.h[j] = (__bf16)(x - (float)h);
+      }
+      xh[f4] = H.u;
+      xl[f4] = Lo.u;
+    }
+  }
+}
+
+// LFIX >= 0: l_max fixed at compile time; EXACT: n_max == NMAX.
+template <typename T, int MT, int NT, bool KAHAN, int NMAX, int LFIX, bool EXACT>
 __global__ void __launch_bounds__(64)
 b2_forward_mfma_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
                        const int *__restrict__ row_of,
-                       const T *__restrict__ ynorm, const int *__restrict__ b2_idx,
-                       FwdOut out, T *__restrict__ A_out, T *__restrict__ B_out,
+                       const T *__restrict__ ynorm, FwdOut out,
+                       T *__restrict__ A_out, T *__restrict__ B_out,
                        double *__restrict__ norm2_out) {
   typedef typename Mma<T>::V V;
   constexpr int KP = MT * 16 + 1;  // padded row of G
@@ -215,7 +307,9 @@ b2_forward_mfma_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
   T *Bs = Al + p.nr * p.H;  // F
   const int i = blockIdx.x;
   const int lane = threadIdx.x;
-  const int N = p.N, H = p.H, W = p.W, F = p.F, S = p.S, nr = p.nr;
+  const int N = EXACT ? NMAX : p.N;
+  const int H = LFIX >= 0 ? (LFIX + 1) * (LFIX + 1) : p.H;
+  const int W = p.W, S = p.S, nr = p.nr;
   V acc[MT][NT], cmp[MT][NT];
 #pragma unroll
   for (int a = 0; a < MT; a++)
@@ -233,25 +327,46 @@ b2_forward_mfma_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
     {
       T *gr = Gs + lane * KP;
       T *yr = Ys + lane * YP;
-      for (int c = 0; c < MT * 16; c++) gr[c] = 0;
-      for (int c = 0; c < NT * 16; c++) yr[c] = 0;
+      bool ok = false;
+      int sj = 0;
+      const double *rv = nullptr;
       if (lane < cnt) {
         const int q = c0 + lane;
-        const double *rv = nl.rel + 4 * (int64_t)q;
-        const int sj = nl.nspec[q];
-        if (pair_inside(p, si, sj, rv)) {
-          const T r = (T)rv[0];
-          const T rinv = T(1) / r;
-          T g[NMAX];
-          radial_basis<T, NMAX>(r, (T)p.cut[si * S + sj], (T)p.r1, N,
-                                p.cutoff_kind, g, (T *)nullptr);
+        rv = nl.rel + 4 * (int64_t)q;
+        sj = nl.nspec[q];
+        ok = pair_inside(p, si, sj, rv);
+      }
+      if (ok) {
+        const T r = (T)rv[0];
+        const T rinv = T(1) / r;
+        T g[NMAX];
+        radial_basis<T, NMAX, EXACT>(r, (T)p.cut[si * S + sj], (T)p.r1, N,
+                                     p.cutoff_kind, g, (T *)nullptr);
+        if (EXACT) {
+          // channel c = s NMAX + n holds g_n for the neighbor's species only
+#pragma unroll
+          for (int c = 0; c < MT * 16; c++)
+            gr[c] = (c / NMAX == sj) ? g[c % NMAX] : T(0);
+        } else {
+          for (int c = 0; c < MT * 16; c++) gr[c] = 0;
 #pragma unroll
           for (int n = 0; n < NMAX; n++)
             if (n < N) gr[sj * N + n] = g[n];
-          ylm_visit<T, false>((T)rv[1] * rinv, (T)rv[2] * rinv,
-                              (T)rv[3] * rinv, rinv, p.L, ynorm,
-                              [&](int idx, T y, T, T, T) { yr[idx] = y; });
         }
+        if (LFIX < 0)
+          for (int c = 0; c < NT * 16; c++) yr[c] = 0;
+        ylm_visit<T, false, LFIX>((T)rv[1] * rinv, (T)rv[2] * rinv,
+                                  (T)rv[3] * rinv, rinv, p.L, ynorm,
+                                  [&](int idx, T y, T, T, T) { yr[idx] = y; });
+        if (LFIX >= 0) {
+#pragma unroll
+          for (int c = (LFIX + 1) * (LFIX + 1); c < NT * 16; c++) yr[c] = 0;
+        }
+      } else {
+#pragma unroll
+        for (int c = 0; c < MT * 16; c++) gr[c] = 0;
+#pragma unroll
+        for (int c = 0; c < NT * 16; c++) yr[c] = 0;
       }
     }
     __syncthreads();
@@ -296,37 +411,60 @@ b2_forward_mfma_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
         }
       }
   __syncthreads();
-  b2_contract_store<T>(p, Al, Bs, b2_idx, row_of[i], out, B_out, norm2_out);
+  double nrm = b2_gram_mfma<T, MT, LFIX>(p, Al, Bs);
+  nrm = wave_sum(nrm);
+  __syncthreads();
+  b2_store_rows<T>(p, Bs, nrm, row_of[i], out, B_out, norm2_out);
+}
+
+template <typename T, bool KAHAN, int MT, int NT, int NMAX, int LFIX, bool EXACT>
+static void launch_mfma_k(hipStream_t st, const B2Dev &p, const NListDev &nl,
+                          const int *species, const int *row_of, const T *ynorm,
+                          const FwdOut &out, void *A_out, void *B_out,
+                          double *norm2_out, size_t lds) {
+  hipLaunchKernelGGL((b2_forward_mfma_kernel<T, MT, NT, KAHAN, NMAX, LFIX, EXACT>),
+                     dim3(nl.n), dim3(kWave), lds, st, p, nl, species, row_of,
+                     ynorm, out, (T *)A_out, (T *)B_out, norm2_out);
 }
 
 template <typename T, bool KAHAN, int MT, int NT>
 static void launch_mfma_t(hipStream_t st, const B2Dev &p, const NListDev &nl,
                           const int *species, const int *row_of, const T *ynorm,
-                          const int *b2_idx, const FwdOut &out, void *A_out,
-                          void *B_out, double *norm2_out) {
+                          const FwdOut &out, void *A_out, void *B_out,
+                          double *norm2_out) {
   size_t lds = sizeof(T) * ((size_t)kWave * (MT * 16 + 1 + NT * 16 + 1) +
                             (size_t)p.nr * p.H + (size_t)p.F);
   FLARE_REQUIRE(lds <= 64 * 1024, "descriptor settings too large");
-  if (p.N <= 8)
-    hipLaunchKernelGGL((b2_forward_mfma_kernel<T, MT, NT, KAHAN, 8>), dim3(nl.n),
-                       dim3(kWave), lds, st, p, nl, species, row_of, ynorm,
-                       b2_idx, out, (T *)A_out, (T *)B_out, norm2_out);
-  else
-    hipLaunchKernelGGL((b2_forward_mfma_kernel<T, MT, NT, KAHAN, kMaxN>),
-                       dim3(nl.n), dim3(kWave), lds, st, p, nl, species, row_of,
-                       ynorm, b2_idx, out, (T *)A_out, (T *)B_out, norm2_out);
+  // compile-time (n_max, l_max) variants of the common settings
+  if (p.N == 8 && p.L == 3 && NT == 1) {
+    launch_mfma_k<T, KAHAN, MT, NT, 8, 3, true>(st, p, nl, species, row_of, ynorm,
+                                                out, A_out, B_out, norm2_out, lds);
+  } else if (p.N == 8 && p.L == 4 && NT == 2) {
+    launch_mfma_k<T, KAHAN, MT, NT, 8, 4, true>(st, p, nl, species, row_of, ynorm,
+                                                out, A_out, B_out, norm2_out, lds);
+  } else if (p.N == 8 && p.L == 6 && NT == 4) {
+    launch_mfma_k<T, KAHAN, MT, NT, 8, 6, true>(st, p, nl, species, row_of, ynorm,
+                                                out, A_out, B_out, norm2_out, lds);
+  } else if (p.N <= 8) {
+    launch_mfma_k<T, KAHAN, MT, NT, 8, -1, false>(st, p, nl, species, row_of, ynorm,
+                                                  out, A_out, B_out, norm2_out, lds);
+  } else {
+    launch_mfma_k<T, KAHAN, MT, NT, kMaxN, -1, false>(st, p, nl, species, row_of,
+                                                      ynorm, out, A_out, B_out,
+                                                      norm2_out, lds);
+  }
 }
 
 template <typename T, bool KAHAN>
 static bool launch_mfma(hipStream_t st, const B2Dev &p, const NListDev &nl,
                         const int *species, const int *row_of, const T *ynorm,
-                        const int *b2_idx, const FwdOut &out, void *A_out,
-                        void *B_out, double *norm2_out) {
+                        const FwdOut &out, void *A_out, void *B_out,
+                        double *norm2_out) {
   const int MT = (p.nr + 15) / 16, NT = (p.H + 15) / 16;
 #define FLARE_MFMA_CASE(m, n)                                                 \
   if (MT == m && NT == n) {                                                   \
-    launch_mfma_t<T, KAHAN, m, n>(st, p, nl, species, row_of, ynorm, b2_idx,  \
-                                  out, A_out, B_out, norm2_out);              \
+    launch_mfma_t<T, KAHAN, m, n>(st, p, nl, species, row_of, ynorm, out,     \
+                                  A_out, B_out, norm2_out);                   \
     return true;                                                              \
   }
   FLARE_MFMA_CASE(1, 1)
@@ -392,21 +530,27 @@ pair_dA_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
     ins[t] = inside;
   }
   __syncthreads();
+  // each thread owns fixed (n, lm) columns and walks the block's pairs, so
+  // consecutive threads write consecutive addresses of every (pair, xyz) row
   const int per = 3 * NH;
-  const int64_t npair = min((int64_t)pb, nl.P - p0);
-  const int64_t total = npair * per;
+  const int npair = (int)min((int64_t)pb, nl.P - p0);
   double *o = out + p0 * per;
-  for (int64_t v = t; v < total; v += blockDim.x) {
-    const int k = (int)(v / per);
-    const int rem = (int)(v - (int64_t)k * per);
-    const int c = rem / NH;
-    const int nl_ = rem - c * NH;
-    const int n = nl_ / H, lm = nl_ - (nl_ / H) * H;
-    double val = 0;
-    if (ins[k])
-      val = dg[k * N + n] * uhat[3 * k + c] * Y[k * H + lm] +
-            g[k * N + n] * Yg[k * 3 * H + c * H + lm];
-    o[v] = val;
+  for (int col = t; col < NH; col += blockDim.x) {
+    const int n = col / H, lm = col - n * H;
+    for (int k = 0; k < npair; k++) {
+      double *ok = o + (int64_t)k * per + col;
+      if (ins[k]) {
+        const double gn = g[k * N + n], dgn = dg[k * N + n], y = Y[k * H + lm];
+        const double *yg = Yg + k * 3 * H + lm;
+        ok[0] = dgn * uhat[3 * k] * y + gn * yg[0];
+        ok[NH] = dgn * uhat[3 * k + 1] * y + gn * yg[H];
+        ok[2 * NH] = dgn * uhat[3 * k + 2] * y + gn * yg[2 * H];
+      } else {
+        ok[0] = 0;
+        ok[NH] = 0;
+        ok[2 * NH] = 0;
+      }
+    }
   }
 }
 
@@ -510,9 +654,9 @@ void launch_b2_forward(hipStream_t st, const B2Dev &p, const NListDev &nl,
   // matrix-core path when the channel and harmonic counts fit its tiles
   bool done = prec == kPrecF64
                   ? launch_mfma<double, false>(st, p, nl, species, row_of, ynorm,
-                                               b2_idx, out, A_out, B_out, norm2_out)
+                                               out, A_out, B_out, norm2_out)
                   : launch_mfma<float, true>(st, p, nl, species, row_of, ynorm_f,
-                                             b2_idx, out, A_out, B_out, norm2_out);
+                                             out, A_out, B_out, norm2_out);
   if (!done) {
     size_t tsz = prec == kPrecF64 ? sizeof(double) : sizeof(float);
     size_t lds = tsz * (2 * (size_t)p.W + kWave * (size_t)(p.N + p.H) + p.F) +

@@ -28,7 +28,8 @@ constexpr double kPi = 3.14159265358979323846;
 // cutoffs.cpp:44-70).  x = (r - r1) / (rcut - r1); T_0 = 1, T_1 = x,
 // T_n = 2 x T_{n-1} - T_{n-2}; g_n = T_n f_c, dg_n/dr = T_n' f_c + T_n f_c'.
 // The Cartesian gradient is dg_n/dr * r_vec / r.
-template <typename T, int NMAX = kMaxN>
+// EXACT: N == NMAX (no per-n guard).
+template <typename T, int NMAX = kMaxN, bool EXACT = false>
 FLARE_HD inline void radial_basis(T r, T rcut, T r1, int N, int cutoff_kind,
                                   T *g, T *dg) {
   T fc = 0, dfc = 0;
@@ -56,7 +57,7 @@ FLARE_HD inline void radial_basis(T r, T rcut, T r1, int N, int cutoff_kind,
 #pragma unroll
 #endif
   for (int n = 0; n < NMAX; n++) {
-    if (n < N) {
+    if (EXACT || n < N) {
       T tv, dv;
       if (n == 0) {
         tv = 1;
@@ -97,15 +98,24 @@ inline void ylm_norm_table(double *out) {
 // with Q_l^m = d^m P_l / dt^m.  Visits every (l, m) in order of m and then l
 // and calls fn(index, Y, dY/dx, dY/dy, dY/dz); rinv = 1/|r|, (ux, uy, uz)
 // the unit vector.  With GRAD false the gradient arguments are zero.
-template <typename T, bool GRAD, typename Fn>
-FLARE_HD inline void ylm_visit(T ux, T uy, T uz, T rinv, int L,
+// LFIX >= 0 fixes l_max at compile time (the loops unroll and the recurrence
+// divisions become multiplications by constants); LFIX < 0 reads Lrt.
+template <typename T, bool GRAD, int LFIX = -1, typename Fn>
+FLARE_HD inline void ylm_visit(T ux, T uy, T uz, T rinv, int Lrt,
                                const T *norm, Fn &&fn) {
+  const int L = LFIX >= 0 ? LFIX : Lrt;
   T Cm = 1, Sm = 0;    // (ux + i uy)^m
   T Cm1 = 0, Sm1 = 0;  // (ux + i uy)^(m-1)
   T dfact = 1;         // (2m-1)!!
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
   for (int m = 0; m <= L; m++) {
     T q1 = 0, q2 = 0;  // Q_{l-1}^m, Q_{l-2}^m
     T p1 = 0, p2 = 0;  // Q_{l-1}^{m+1}, Q_{l-2}^{m+1}
+#if defined(__HIP_DEVICE_COMPILE__)
+#pragma unroll
+#endif
     for (int l = m; l <= L; l++) {
       T q, p;
       if (l == m) {
@@ -115,11 +125,19 @@ FLARE_HD inline void ylm_visit(T ux, T uy, T uz, T rinv, int L,
         q = T(2 * m + 1) * uz * dfact;
         p = dfact * T(2 * m + 1);  // (2(m+1)-1)!!
       } else {
-        q = (T(2 * l - 1) * uz * q1 - T(l + m - 1) * q2) / T(l - m);
-        if (l == m + 2)
-          p = T(2 * m + 3) * uz * p1;
-        else
-          p = (T(2 * l - 1) * uz * p1 - T(l + m) * p2) / T(l - m - 1);
+        if (LFIX >= 0) {
+          q = (T(2 * l - 1) * uz * q1 - T(l + m - 1) * q2) * (T(1) / T(l - m));
+          if (l == m + 2)
+            p = T(2 * m + 3) * uz * p1;
+          else
+            p = (T(2 * l - 1) * uz * p1 - T(l + m) * p2) * (T(1) / T(l - m - 1));
+        } else {
+          q = (T(2 * l - 1) * uz * q1 - T(l + m - 1) * q2) / T(l - m);
+          if (l == m + 2)
+            p = T(2 * m + 3) * uz * p1;
+          else
+            p = (T(2 * l - 1) * uz * p1 - T(l + m) * p2) / T(l - m - 1);
+        }
       }
       q2 = q1;
       q1 = q;

@@ -25,33 +25,131 @@ template <typename T>
 struct WSource {
   int normalized;
   const T *X, *Z, *beta1, *w;
+  const __bf16 *xh, *xl;  // X as BF16 hi + lo rows of kp (mixed form)
+  int kp;
   const double *norm2;
   double *E_row;
 };
 
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+typedef __attribute__((ext_vector_type(4))) double f64x4;
+
+template <typename T>
+struct MmaF;
+template <>
+struct MmaF<float> {
+  typedef f32x4 V;
+  static __device__ inline V run(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int g, int j) { return 4 * g + j; }
+};
+template <>
+struct MmaF<double> {
+  typedef f64x4 V;
+  static __device__ inline V run(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // the FP64 instruction interleaves result rows across elements (gfx950)
+  static __device__ inline int row(int g, int j) { return g + 4 * j; }
+};
+
+// dst[o] = load(o) for o < n (dst in LDS); each lane issues KB loads
+// before storing any of them.
+template <int KB, typename T, typename Fn>
+__device__ inline void stage_lds(int n, Fn &&load, T *dst) {
+  const int lane = threadIdx.x;
+  for (int base = 0; base < n; base += KB * kWave) {
+    T t[KB];
+#pragma unroll
+    for (int j = 0; j < KB; j++) {
+      const int o = base + lane + j * kWave;
+      t[j] = 0;
+      if (o < n) t[j] = load(o);
+    }
+#pragma unroll
+    for (int j = 0; j < KB; j++) {
+      const int o = base + lane + j * kWave;
+      if (o < n) dst[o] = t[j];
+    }
+  }
+}
+
+// u[c, lm] = sum_c2 W_l[c][c2] A[c2, lm] with W_l[c][c2] =
+// w[p(c,c2) (L+1) + l] (1 + d_{c c2}) (lammps_descriptor.cpp:272-303), one
+// matrix-core product per l: M = channels (c), K = channels (c2), N = the
+// 2l+1 harmonics of that l.  Result stored as u2[(s H + lm) N + n] for
+// channel c = s N + n, so the per-pair loop reads the n of one lm together.
+// SCALE: each u[., lm] is multiplied by the harmonic's normalisation
+// nrm_lm[lm] = N_l|m|, the form adjoint_pair_fixed reads.
+template <typename T, int LFIX, bool SCALE>
+__device__ inline void u_mfma(const B2Dev &p, const T *A, const T *w,
+                              const T *nrm_lm, T *u2) {
+  typedef typename MmaF<T>::V V;
+  const int lane = threadIdx.x;
+  const int col = lane & 15, g = lane >> 4;
+  const int H = LFIX >= 0 ? (LFIX + 1) * (LFIX + 1) : p.H;
+  const int L1 = LFIX >= 0 ? LFIX + 1 : p.L + 1;
+  const int N = p.N, nr = p.nr;
+  const int MT = (nr + 15) >> 4, KS = (nr + 3) >> 2;
+#pragma unroll
+  for (int l = 0; l < L1; l++) {
+    const int nl = 2 * l + 1, l2 = l * l;
+    for (int nt = 0; nt * 16 < nl; nt++) {
+      const int j = nt * 16 + col;
+      for (int mt = 0; mt < MT; mt++) {
+        V acc = V{0, 0, 0, 0};
+        const int c = mt * 16 + col;
+        for (int ks = 0; ks < KS; ks++) {
+          const int c2 = 4 * ks + g;
+          T a = 0, b = 0;
+          if (c < nr && c2 < nr) {
+            T wv = w[b2_pair_index(c, c2, nr) * L1 + l];
+            a = (c == c2) ? 2 * wv : wv;
+          }
+          if (c2 < nr && j < nl) b = A[c2 * H + l2 + j];
+          acc = MmaF<T>::run(a, b, acc);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int cr = mt * 16 + MmaF<T>::row(g, q);
+          if (cr < nr && j < nl) {
+            const int s = cr / N, n = cr - (cr / N) * N;
+            T val = acc[q];
+            if (SCALE) val *= nrm_lm[l2 + j];
+            u2[(s * H + l2 + j) * N + n] = val;
+          }
+        }
+      }
+    }
+  }
+}
+
 namespace {
 
 // Adjoint contraction for one pair, everything recomputed in registers.
-template <typename T, int NMAX>
+// LFIX >= 0: l_max fixed at compile time; EXACT: n_max == NMAX.
+template <typename T, int NMAX, int LFIX, bool EXACT>
 __device__ inline void adjoint_pair(const B2Dev &p, T r, T x, T y, T z,
                                     T rcut, const T *__restrict__ ub,
                                     const T *__restrict__ ynorm, T &f0, T &f1,
                                     T &f2) {
-  const int N = p.N, H = p.H;
+  const int N = EXACT ? NMAX : p.N;
   const T rinv = T(1) / r;
   const T ux = x * rinv, uy = y * rinv, uz = z * rinv;
   T g[NMAX], dg[NMAX], v[NMAX];
-  radial_basis<T, NMAX>(r, rcut, (T)p.r1, N, p.cutoff_kind, g, dg);
+  radial_basis<T, NMAX, EXACT>(r, rcut, (T)p.r1, N, p.cutoff_kind, g, dg);
 #pragma unroll
   for (int n = 0; n < NMAX; n++) v[n] = 0;
   T a0 = 0, a1 = 0, a2 = 0;
-  ylm_visit<T, true>(ux, uy, uz, rinv, p.L, ynorm,
+  ylm_visit<T, true, LFIX>(ux, uy, uz, rinv, p.L, ynorm,
                      [&](int lm, T Y, T Yx, T Yy, T Yz) {
                        T q = 0;
+                       const T *ul = ub + lm * N;
 #pragma unroll
                        for (int n = 0; n < NMAX; n++) {
-                         if (n < N) {
-                           T un = ub[n * H + lm];
+                         if (EXACT || n < N) {
+                           T un = ul[n];
                            q += un * g[n];
                            v[n] += un * Y;
                          }
@@ -63,15 +161,126 @@ __device__ inline void adjoint_pair(const B2Dev &p, T r, T x, T y, T z,
   T fr = 0;
 #pragma unroll
   for (int n = 0; n < NMAX; n++)
-    if (n < N) fr += v[n] * dg[n];
+    if (EXACT || n < N) fr += v[n] * dg[n];
   f0 = a0 + fr * ux;
   f1 = a1 + fr * uy;
   f2 = a2 + fr * uz;
 }
 
+// The same contraction with l_max = L and n_max = NMAX fixed at compile time
+// and the harmonics left unnormalised: with u' = u N_l|m| (see u_mfma) and
+// Ytilde = Q_l^|m|(u_z) {Re,Im}(u_x + i u_y)^|m|,
+//   v_n = sum_lm u'_n,lm Ytilde_lm,   q'_lm = sum_n u'_n,lm g_n,
+// and the harmonic gradient sum_lm q_lm grad Y_lm = (G - u (u.G)) / r with
+// G = sum_lm q'_lm dYtilde/du: for m = 0 dYtilde/du = (0, 0, Q'),
+// cosine type (m Q C_{m-1}, -m Q S_{m-1}, Q' C_m), sine type
+// (m Q S_{m-1}, m Q C_{m-1}, Q' S_m), Q' = dQ/du_z.  The tangential
+// projection is applied once per pair instead of once per harmonic.
+template <typename T>
+__device__ inline void load_u_row8(const T *p, T *out);
+template <>
+__device__ inline void load_u_row8<float>(const float *p, float *out) {
+  const float4 a = reinterpret_cast<const float4 *>(p)[0];
+  const float4 b = reinterpret_cast<const float4 *>(p)[1];
+  out[0] = a.x; out[1] = a.y; out[2] = a.z; out[3] = a.w;
+  out[4] = b.x; out[5] = b.y; out[6] = b.z; out[7] = b.w;
+}
+template <>
+__device__ inline void load_u_row8<double>(const double *p, double *out) {
+#pragma unroll
+  for (int h = 0; h < 4; h++) {
+    const double2 a = reinterpret_cast<const double2 *>(p)[h];
+    out[2 * h] = a.x;
+    out[2 * h + 1] = a.y;
+  }
+}
+
+template <typename T, int L>
+__device__ inline void adjoint_pair_fixed(const B2Dev &p, T r, T x, T y, T z,
+                                          T rcut, const T *__restrict__ ub,
+                                          T &f0, T &f1, T &f2) {
+  constexpr int NM = 8;
+  const T rinv = T(1) / r;
+  const T ux = x * rinv, uy = y * rinv, uz = z * rinv;
+  T g[NM], dg[NM], v[NM];
+  radial_basis<T, NM, true>(r, rcut, (T)p.r1, NM, p.cutoff_kind, g, dg);
+#pragma unroll
+  for (int n = 0; n < NM; n++) v[n] = 0;
+  T G0 = 0, G1 = 0, G2 = 0;
+  T Cm = 1, Sm = 0, Cm1 = 0, Sm1 = 0, dfact = 1;
+  auto term = [&](int lm, T Yt, T &qo) {
+    T u[NM];
+    load_u_row8<T>(ub + lm * NM, u);
+    T q = u[0] * g[0];
+#pragma unroll
+    for (int n = 1; n < NM; n++) q += u[n] * g[n];
+#pragma unroll
+    for (int n = 0; n < NM; n++) v[n] += u[n] * Yt;
+    qo = q;
+  };
+#pragma unroll
+  for (int m = 0; m <= L; m++) {
+    T q1 = 0, q2 = 0, p1 = 0, p2 = 0;
+#pragma unroll
+    for (int l = m; l <= L; l++) {
+      T Q, P;
+      if (l == m) {
+        Q = dfact;
+        P = 0;
+      } else if (l == m + 1) {
+        Q = T(2 * m + 1) * uz * dfact;
+        P = dfact * T(2 * m + 1);
+      } else {
+        Q = (T(2 * l - 1) * uz * q1 - T(l + m - 1) * q2) * (T(1) / T(l - m));
+        if (l == m + 2)
+          P = T(2 * m + 3) * uz * p1;
+        else
+          P = (T(2 * l - 1) * uz * p1 - T(l + m) * p2) * (T(1) / T(l - m - 1));
+      }
+      q2 = q1;
+      q1 = Q;
+      p2 = p1;
+      p1 = P;
+      const int base = l * l + l;
+      T qc;
+      if (m == 0) {
+        term(base, Q, qc);
+        G2 += qc * P;
+      } else {
+        const T Qf = Q * T(m);
+        term(base + m, Q * Cm, qc);  // cosine type
+        T t = qc * Qf;
+        G0 += t * Cm1;
+        G1 -= t * Sm1;
+        G2 += qc * P * Cm;
+        term(base - m, Q * Sm, qc);  // sine type
+        t = qc * Qf;
+        G0 += t * Sm1;
+        G1 += t * Cm1;
+        G2 += qc * P * Sm;
+      }
+    }
+    const T cn = Cm * ux - Sm * uy;
+    const T sn = Sm * ux + Cm * uy;
+    Cm1 = Cm;
+    Sm1 = Sm;
+    Cm = cn;
+    Sm = sn;
+    dfact *= T(2 * m + 1);
+  }
+  T fr = 0;
+#pragma unroll
+  for (int n = 0; n < NM; n++) fr += v[n] * dg[n];
+  const T uG = ux * G0 + uy * G1 + uz * G2;
+  f0 = fr * ux + rinv * (G0 - ux * uG);
+  f1 = fr * uy + rinv * (G1 - uy * uG);
+  f2 = fr * uz + rinv * (G2 - uz * uG);
+}
+
 }  // namespace
 
-// One wavefront per atom.  Dynamic LDS (T): A[W], w[F], u[W], x[F].
+// One wavefront per atom.  Dynamic LDS (T): A[W], w[F], u[W], x[F], n[H].
+// WM 3 is WM 1 with X given as BF16 hi + lo rows (the GEMM's operand).
 // WM selects where w = dE_i/dB_i comes from:
 //   0  precomputed (sparse-GP path);
 //   1  mapped power 2 from the GEMM rows X and Z = beta X: E = X.Z/|X|^2,
@@ -79,41 +288,94 @@ __device__ inline void adjoint_pair(const B2Dev &p, T r, T x, T y, T z,
 //   2  mapped power 1 from X and beta_s: E = X.beta/|X|,
 //      w = (beta - E B/|B|) / |B|    (lammps_descriptor.cpp:262-270).
 // Environments with |B|^2 < 1e-8 carry no energy (pair_flare.cpp:135).
-template <typename T, int MODE, int NMAX, int WM>
+template <typename T, int MODE, int NMAX, int WM, int LFIX, bool EXACT>
 __global__ void __launch_bounds__(64)
 force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
              const int *__restrict__ row_of, const T *__restrict__ ynorm,
              const T *__restrict__ A_in, WSource<T> src,
-             const double *__restrict__ dA, double *__restrict__ pair_f,
-             double *__restrict__ vir_atom) {
+             const double *__restrict__ dA, double *__restrict__ pair_f) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int i = blockIdx.x;
   const int lane = threadIdx.x;
-  const int H = p.H, NH = p.NH, W = p.W, F = p.F, nr = p.nr, L1 = p.L + 1;
+  const int H = p.H, NH = p.NH, W = p.W, F = p.F;
   T *A = reinterpret_cast<T *>(smem);
   T *w = A + W;
   T *u = w + F;
+  T *xs = u + W;        // X row (rebuilt from hi + lo when BF16)
+  T *nrm_lm = xs + F;   // harmonic normalisation per lm
   const T *Ai = A_in + (int64_t)i * W;
   const int row = row_of[i];
-  for (int o = lane; o < W; o += kWave) A[o] = Ai[o];
+  constexpr bool kFixed = MODE == 1 && LFIX >= 0 && EXACT && NMAX == 8;
+  const int b = nl.cum[i], e = nl.cum[i + 1];
+  double4 rv_next = {0, 0, 0, 0};
+  int sj_next = 0;
+  if (MODE == 1 && b + lane < e) {
+    rv_next = reinterpret_cast<const double4 *>(nl.rel)[b + lane];
+    sj_next = nl.nspec[b + lane];
+  }
+  // Every global read of the prologue is issued in batches before its
+  // values are stored, so that a wavefront waits for memory a few times
+  // rather than once per element.
+  if (kFixed && lane < H) {
+    int l = 0;
+    while ((l + 1) * (l + 1) <= lane) l++;
+    const int m = lane - l * l - l;
+    nrm_lm[lane] = ynorm[l * (l + 1) / 2 + (m < 0 ? -m : m)];
+  }
+  stage_lds<8>(W, [&](int o) { return Ai[o]; }, A);
   if (WM == 0) {
     const T *wi = src.w + (int64_t)row * F;
-    for (int f = lane; f < F; f += kWave) w[f] = wi[f];
+    stage_lds<8>(F, [&](int f) { return wi[f]; }, w);
   } else {
     const double n2 = src.norm2[row];
-    const T *x = src.X + (int64_t)row * F;
+    if (WM == 3) {
+      // pairs of BF16 values: rows are padded to an even length (kp)
+      const uint32_t *xh = reinterpret_cast<const uint32_t *>(src.xh + (int64_t)row * src.kp);
+      const uint32_t *xl = reinterpret_cast<const uint32_t *>(src.xl + (int64_t)row * src.kp);
+      const int F2 = (F + 1) >> 1;
+      for (int base = 0; base < F2; base += 8 * kWave) {
+        uint32_t th[8], tl[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          const int q = base + lane + j * kWave;
+          th[j] = 0;
+          tl[j] = 0;
+          if (q < F2) {
+            th[j] = xh[q];
+            tl[j] = xl[q];
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          const int q = base + lane + j * kWave;
+          if (q < F2) {
+            const float v0 = __uint_as_float(th[j] << 16) + __uint_as_float(tl[j] << 16);
+            const float v1 = __uint_as_float(th[j] & 0xffff0000u) +
+                             __uint_as_float(tl[j] & 0xffff0000u);
+            xs[2 * q] = (T)v0;
+            if (2 * q + 1 < F) xs[2 * q + 1] = (T)v1;
+          }
+        }
+      }
+    } else {
+      const T *x = src.X + (int64_t)row * F;
+      stage_lds<8>(F, [&](int f) { return x[f]; }, xs);
+    }
+    // z (= beta X, or beta_s for power 1) staged into w, turned into w below
+    const T *z = WM != 2 ? src.Z + (int64_t)row * F
+                         : src.beta1 + (int64_t)species[i] * F;
+    stage_lds<8>(F, [&](int f) { return z[f]; }, w);
+    __syncthreads();
     if (n2 < 1e-8) {
       for (int f = lane; f < F; f += kWave) w[f] = 0;
       if (lane == 0) src.E_row[row] = 0;
     } else {
       const double bn = sqrt(n2);
-      const T *z = WM == 1 ? src.Z + (int64_t)row * F
-                           : src.beta1 + (int64_t)species[i] * F;
       double d = 0;
-      for (int f = lane; f < F; f += kWave) d += (double)x[f] * (double)z[f];
+      for (int f = lane; f < F; f += kWave) d += (double)xs[f] * (double)w[f];
       d = wave_sum(d);
       double E, sx, sz;
-      if (WM == 1) {
+      if (WM != 2) {
         E = src.normalized ? d : d / n2;
         sz = src.normalized ? 2.0 / bn : 2.0 / n2;
         sx = -E * sz;
@@ -123,63 +385,77 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
         sx = -E / bn * (src.normalized ? 1.0 : 1.0 / bn);
       }
       for (int f = lane; f < F; f += kWave)
-        w[f] = (T)(sz * (double)z[f] + sx * (double)x[f]);
+        w[f] = (T)(sz * (double)w[f] + sx * (double)xs[f]);
       if (lane == 0) src.E_row[row] = E;
     }
   }
   __syncthreads();
-  for (int o = lane; o < W; o += kWave) {
-    const int c = o / H, lm = o - (o / H) * H;
-    const int l = lm_to_l(lm);
-    T acc = 0;
-    for (int c2 = 0; c2 < nr; c2++) {
-      T t = w[b2_pair_index(c, c2, nr) * L1 + l] * A[c2 * H + lm];
-      acc += (c2 == c) ? 2 * t : t;
-    }
-    u[o] = acc;
-  }
+  u_mfma<T, LFIX, kFixed>(p, A, w, nrm_lm, u);
   __syncthreads();
   const int si = species[i];
-  const int b = nl.cum[i], e = nl.cum[i + 1];
-  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (MODE == 0) {
-    for (int k = b; k < e; k++) {
-      const double *rv = nl.rel + 4 * (int64_t)k;
-      const int sj = nl.nspec[k];
+    // 16 lanes per pair, four pairs at a time; partial sums over the
+    // pair's 3 x N x H derivative values, then a 16-lane reduction
+    const int sub = lane & 15, grp = lane >> 4;
+    for (int k0 = b; k0 < e; k0 += 4) {
+      const int k = k0 + grp;
       double f0 = 0, f1 = 0, f2 = 0;
-      if (pair_inside(p, si, sj, rv)) {
-        const double *d = dA + (int64_t)k * 3 * NH;
-        const T *ub = u + sj * NH;
-        for (int o = lane; o < NH; o += kWave) {
-          double uo = (double)ub[o];
-          f0 += d[o] * uo;
-          f1 += d[NH + o] * uo;
-          f2 += d[2 * NH + o] * uo;
+      bool inside = false;
+      if (k < e) {
+        const double *rv = nl.rel + 4 * (int64_t)k;
+        const int sj = nl.nspec[k];
+        inside = pair_inside(p, si, sj, rv);
+        if (inside) {
+          const double *d = dA + (int64_t)k * 3 * NH;
+          const T *ub = u + sj * NH;
+          int n = sub / H, lm = sub - n * H;  // (n, lm) of o, kept incrementally
+          for (int o = sub; o < NH; o += 16) {
+            const double uo = (double)ub[lm * p.N + n];
+            f0 += d[o] * uo;
+            f1 += d[NH + o] * uo;
+            f2 += d[2 * NH + o] * uo;
+            lm += 16;
+            while (lm >= H) {
+              lm -= H;
+              n++;
+            }
+          }
         }
-        f0 = wave_sum(f0);
-        f1 = wave_sum(f1);
-        f2 = wave_sum(f2);
       }
-      if (lane == 0) {
+#pragma unroll
+      for (int off = 8; off > 0; off >>= 1) {
+        f0 += __shfl_xor(f0, off, 16);
+        f1 += __shfl_xor(f1, off, 16);
+        f2 += __shfl_xor(f2, off, 16);
+      }
+      if (k < e && sub == 0) {
         pair_f[3 * (int64_t)k + 0] = f0;
         pair_f[3 * (int64_t)k + 1] = f1;
         pair_f[3 * (int64_t)k + 2] = f2;
-        const double r[3] = {rv[1], rv[2], rv[3]};
-        const double f[3] = {f0, f1, f2};
-        for (int a = 0; a < 3; a++)
-          for (int c = 0; c < 3; c++) v[3 * a + c] -= r[a] * f[c];
       }
     }
   } else {
+    // the next chunk's pair data is loaded one iteration ahead (the first
+    // chunk before the prologue)
     for (int k = b + lane; k < e; k += kWave) {
-      const double *rv = nl.rel + 4 * (int64_t)k;
-      const int sj = nl.nspec[k];
+      const double4 rv = rv_next;
+      const int sj = sj_next;
+      if (k + kWave < e) {
+        rv_next = reinterpret_cast<const double4 *>(nl.rel)[k + kWave];
+        sj_next = nl.nspec[k + kWave];
+      }
+      const double rva[4] = {rv.x, rv.y, rv.z, rv.w};
       double f[3] = {0, 0, 0};
-      if (pair_inside(p, si, sj, rv)) {
+      if (pair_inside(p, si, sj, rva)) {
         T f0, f1, f2;
-        adjoint_pair<T, NMAX>(p, (T)rv[0], (T)rv[1], (T)rv[2], (T)rv[3],
-                              (T)p.cut[si * p.S + sj], u + sj * NH, ynorm, f0,
-                              f1, f2);
+        if constexpr (kFixed)
+          adjoint_pair_fixed<T, (LFIX >= 0 ? LFIX : 0)>(
+              p, (T)rv.x, (T)rv.y, (T)rv.z, (T)rv.w,
+              (T)p.cut[si * p.S + sj], u + sj * NH, f0, f1, f2);
+        else
+          adjoint_pair<T, NMAX, LFIX, EXACT>(p, (T)rv.x, (T)rv.y, (T)rv.z,
+                                             (T)rv.w, (T)p.cut[si * p.S + sj],
+                                             u + sj * NH, ynorm, f0, f1, f2);
         f[0] = f0;
         f[1] = f1;
         f[2] = f2;
@@ -187,34 +463,58 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
       pair_f[3 * (int64_t)k + 0] = f[0];
       pair_f[3 * (int64_t)k + 1] = f[1];
       pair_f[3 * (int64_t)k + 2] = f[2];
-      const double r[3] = {rv[1], rv[2], rv[3]};
-      for (int a = 0; a < 3; a++)
-        for (int c = 0; c < 3; c++) v[3 * a + c] -= r[a] * f[c];
     }
-#pragma unroll
-    for (int q = 0; q < 9; q++) v[q] = wave_sum(v[q]);
   }
-  if (lane == 0)
-    for (int q = 0; q < 9; q++) vir_atom[9 * (int64_t)i + q] = v[q];
 }
 
-// 16 lanes per atom, four atoms per wavefront
+// 16 lanes per atom, four atoms per wavefront.  Also the per-atom virial
+// rows V_a[b][c] = -sum_k r_ak,b f_ak,c over the atom's own pairs (the
+// pair_flare tally of (x_i - x_j) f_ij, pair_flare.cpp:160-165).
 __global__ void __launch_bounds__(256)
 gather_kernel(int n, const int *__restrict__ cum, const int *__restrict__ rev,
-              const double *__restrict__ pf, double *__restrict__ forces) {
+              const double *__restrict__ rel, const double *__restrict__ pf,
+              double *__restrict__ forces, double *__restrict__ vir) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int a = t >> 4, sub = t & 15;
   double f0 = 0, f1 = 0, f2 = 0;
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (a < n) {
-    for (int k = cum[a] + sub; k < cum[a + 1]; k += 16) {
-      const int r = rev[k];
-      f0 += pf[3 * (int64_t)k];
-      f1 += pf[3 * (int64_t)k + 1];
-      f2 += pf[3 * (int64_t)k + 2];
-      if (r >= 0) {
-        f0 -= pf[3 * (int64_t)r];
-        f1 -= pf[3 * (int64_t)r + 1];
-        f2 -= pf[3 * (int64_t)r + 2];
+    const int e = cum[a + 1];
+    // up to four pairs per lane per batch, all loads issued before use
+    for (int k0 = cum[a] + sub; k0 < e; k0 += 64) {
+      int r[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int k = k0 + 16 * j;
+        r[j] = k < e ? rev[k] : -1;
+      }
+      double g[4][3], h[4][3];
+      double4 rv[4];
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int k = k0 + 16 * j;
+        for (int c = 0; c < 3; c++) {
+          g[j][c] = 0;
+          h[j][c] = 0;
+        }
+        rv[j] = double4{0, 0, 0, 0};
+        if (k < e) {
+          for (int c = 0; c < 3; c++) g[j][c] = pf[3 * (int64_t)k + c];
+          rv[j] = reinterpret_cast<const double4 *>(rel)[k];
+        }
+        if (r[j] >= 0)
+          for (int c = 0; c < 3; c++) h[j][c] = pf[3 * (int64_t)r[j] + c];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        f0 += g[j][0] - h[j][0];
+        f1 += g[j][1] - h[j][1];
+        f2 += g[j][2] - h[j][2];
+        const double x[3] = {rv[j].y, rv[j].z, rv[j].w};
+#pragma unroll
+        for (int b = 0; b < 3; b++)
+#pragma unroll
+          for (int c = 0; c < 3; c++) v[3 * b + c] -= x[b] * g[j][c];
       }
     }
   }
@@ -223,11 +523,17 @@ gather_kernel(int n, const int *__restrict__ cum, const int *__restrict__ rev,
     f0 += __shfl_xor(f0, off, 16);
     f1 += __shfl_xor(f1, off, 16);
     f2 += __shfl_xor(f2, off, 16);
+#pragma unroll
+    for (int q = 0; q < 9; q++) v[q] += __shfl_xor(v[q], off, 16);
   }
   if (a < n && sub == 0) {
-    forces[3 * (int64_t)a] = f0;
-    forces[3 * (int64_t)a + 1] = f1;
-    forces[3 * (int64_t)a + 2] = f2;
+    if (forces) {
+      forces[3 * (int64_t)a] = f0;
+      forces[3 * (int64_t)a + 1] = f1;
+      forces[3 * (int64_t)a + 2] = f2;
+    }
+#pragma unroll
+    for (int q = 0; q < 9; q++) vir[9 * (int64_t)a + q] = v[q];
   }
 }
 
@@ -257,14 +563,15 @@ partial_sums_kernel(int n, const double *__restrict__ e_row,
   }
 }
 
-__global__ void __launch_bounds__(64)
+// one wavefront per column, fixed lane partition
+__global__ void __launch_bounds__(640)
 partial_final_kernel(int nparts, const double *__restrict__ part,
                      double *__restrict__ sums) {
-  const int q = threadIdx.x;
-  if (q >= 10) return;
+  const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double s = 0;
-  for (int b = 0; b < nparts; b++) s += part[b * 10 + q];
-  sums[q] = s;
+  for (int b = lane; b < nparts; b += 64) s += part[b * 10 + q];
+  s = wave_sum(s);
+  if (lane == 0) sums[q] = s;
 }
 
 // one workgroup per column; fixed partition => deterministic
@@ -316,63 +623,76 @@ template <typename T, int MODE, int WM>
 static void launch_forces_t(hipStream_t st, const B2Dev &p, const NListDev &nl,
                             const int *species, const int *row_of,
                             const T *ynorm, const void *A, const WSource<T> &src,
-                            const double *dA, double *pair_f, double *vir_atom) {
-  size_t lds = sizeof(T) * (2 * (size_t)p.W + (size_t)p.F);
+                            const double *dA, double *pair_f) {
+  size_t lds = sizeof(T) * (2 * (size_t)p.W + 2 * (size_t)p.F + (size_t)p.H);
   FLARE_REQUIRE(lds <= 64 * 1024, "descriptor settings too large for the force kernel");
-  if (p.N <= 8)
-    hipLaunchKernelGGL((force_kernel<T, MODE, 8, WM>), dim3(nl.n), dim3(kWave),
-                       lds, st, p, nl, species, row_of, ynorm, (const T *)A, src,
-                       dA, pair_f, vir_atom);
+  FLARE_REQUIRE(p.nr <= 32, "at most 32 radial channels (n_species * n_max)");
+#define FLARE_FORCE_LAUNCH(NM, LF, EX)                                        \
+  hipLaunchKernelGGL((force_kernel<T, MODE, NM, WM, LF, EX>), dim3(nl.n),     \
+                     dim3(kWave), lds, st, p, nl, species, row_of, ynorm,     \
+                     (const T *)A, src, dA, pair_f)
+  // compile-time (n_max, l_max) variants of the common adjoint settings
+  if (MODE == 1 && p.N == 8 && p.L == 3)
+    FLARE_FORCE_LAUNCH(8, 3, true);
+  else if (MODE == 1 && p.N == 8 && p.L == 4)
+    FLARE_FORCE_LAUNCH(8, 4, true);
+  else if (MODE == 1 && p.N == 8 && p.L == 6)
+    FLARE_FORCE_LAUNCH(8, 6, true);
+  else if (p.N <= 8)
+    FLARE_FORCE_LAUNCH(8, -1, false);
   else
-    hipLaunchKernelGGL((force_kernel<T, MODE, kMaxN, WM>), dim3(nl.n),
-                       dim3(kWave), lds, st, p, nl, species, row_of, ynorm,
-                       (const T *)A, src, dA, pair_f, vir_atom);
+    FLARE_FORCE_LAUNCH(kMaxN, -1, false);
+#undef FLARE_FORCE_LAUNCH
 }
 
 template <typename T, int MODE>
 static void launch_forces_w(hipStream_t st, const B2Dev &p, const NListDev &nl,
                             const int *species, const int *row_of,
                             const T *ynorm, const void *A, const WInput &wi,
-                            const double *dA, double *pair_f, double *vir_atom) {
+                            const double *dA, double *pair_f) {
   WSource<T> src{wi.normalized, (const T *)wi.X, (const T *)wi.Z,
-                 (const T *)wi.beta1, (const T *)wi.w, wi.norm2, wi.E_row};
+                 (const T *)wi.beta1, (const T *)wi.w,
+                 (const __bf16 *)wi.xh, (const __bf16 *)wi.xl, wi.kp,
+                 wi.norm2, wi.E_row};
   if (wi.mode == 0)
     launch_forces_t<T, MODE, 0>(st, p, nl, species, row_of, ynorm, A, src, dA,
-                                pair_f, vir_atom);
+                                pair_f);
+  else if (wi.mode == 3)
+    launch_forces_t<T, MODE, 3>(st, p, nl, species, row_of, ynorm, A, src, dA,
+                                pair_f);
   else if (wi.mode == 1)
     launch_forces_t<T, MODE, 1>(st, p, nl, species, row_of, ynorm, A, src, dA,
-                                pair_f, vir_atom);
+                                pair_f);
   else
     launch_forces_t<T, MODE, 2>(st, p, nl, species, row_of, ynorm, A, src, dA,
-                                pair_f, vir_atom);
+                                pair_f);
 }
 
 void launch_forces(hipStream_t st, int prec, int mode, const B2Dev &p,
                    const NListDev &nl, const int *species, const int *row_of,
                    const double *ynorm, const float *ynorm_f, const void *A,
-                   const WInput &wi, const double *dA, double *pair_f,
-                   double *vir_atom) {
+                   const WInput &wi, const double *dA, double *pair_f) {
   if (nl.n == 0) return;
   if (prec == kPrecF64) {
     if (mode == 0)
       launch_forces_w<double, 0>(st, p, nl, species, row_of, ynorm, A, wi, dA,
-                                 pair_f, vir_atom);
+                                 pair_f);
     else
       launch_forces_w<double, 1>(st, p, nl, species, row_of, ynorm, A, wi, dA,
-                                 pair_f, vir_atom);
+                                 pair_f);
   } else {
     FLARE_REQUIRE(mode == 1, "the plain (materialised) form runs in FP64 only");
     launch_forces_w<float, 1>(st, p, nl, species, row_of, ynorm_f, A, wi, dA,
-                              pair_f, vir_atom);
+                              pair_f);
   }
   FLARE_HIP_CHECK(hipGetLastError());
 }
 
 void launch_gather(hipStream_t st, const NListDev &nl, const double *pair_f,
-                   double *forces) {
+                   double *forces, double *vir_atom) {
   if (nl.n == 0) return;
   hipLaunchKernelGGL(gather_kernel, dim3((nl.n * 16 + 255) / 256), dim3(256), 0,
-                     st, nl.n, nl.cum, nl.rev, pair_f, forces);
+                     st, nl.n, nl.cum, nl.rev, nl.rel, pair_f, forces, vir_atom);
   FLARE_HIP_CHECK(hipGetLastError());
 }
 
@@ -380,7 +700,7 @@ void launch_energy_virial_sums(hipStream_t st, int n, const double *e_row,
                                const double *vir, double *part, double *sums) {
   hipLaunchKernelGGL(partial_sums_kernel, dim3(kPartials), dim3(256), 0, st, n,
                      e_row, vir, part);
-  hipLaunchKernelGGL(partial_final_kernel, dim3(1), dim3(64), 0, st, kPartials,
+  hipLaunchKernelGGL(partial_final_kernel, dim3(1), dim3(640), 0, st, kPartials,
                      part, sums);
   FLARE_HIP_CHECK(hipGetLastError());
 }

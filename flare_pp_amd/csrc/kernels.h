@@ -91,25 +91,27 @@ void launch_b2_dervs(hipStream_t st, const B2Dev &p, const NListDev &nl,
 // forces.hip
 // Source of w = dE_i/dB_i for the force kernel.  mode 0: precomputed w
 // rows; 1: mapped power 2 from GEMM rows X and Z = beta X; 2: mapped power 1
-// from X and beta1 (one F-vector per species).  Modes 1 and 2 also write the
+// from X and beta1 (one F-vector per species); 3: as 1 with X rebuilt from
+// BF16 hi + lo rows (xh, xl, row stride kp).  Modes 1-3 also write the
 // per-row energies.  X is B, or B/|B| when `normalized`.
 struct WInput {
   int mode, normalized;
   const void *X, *Z, *beta1, *w;
   const double *norm2;
   double *E_row;
+  const void *xh = nullptr, *xl = nullptr;  // mode 3: X as BF16 hi + lo
+  int kp = 0;
 };
-// Per-atom u = dE/dA, pair forces f_ik and per-atom virial rows (9 values,
-// V[a][b] = -sum_k r_a f_b).  mode 0: plain (reads dA from `dA`),
-// mode 1: adjoint (recomputes the pair basis).
+// Per-atom u = dE/dA and pair forces f_ik.  mode 0: plain (reads dA from
+// `dA`), mode 1: adjoint (recomputes the pair basis).
 void launch_forces(hipStream_t st, int prec, int mode, const B2Dev &p,
                    const NListDev &nl, const int *species, const int *row_of,
                    const double *ynorm, const float *ynorm_f, const void *A,
-                   const WInput &wi, const double *dA, double *pair_f,
-                   double *vir_atom);
-// F_a = sum_k (f_ak - f_rev(ak)) into forces (stride 3)
+                   const WInput &wi, const double *dA, double *pair_f);
+// F_a = sum_k (f_ak - f_rev(ak)) into forces (stride 3; may be NULL) and
+// per-atom virial rows (9 values, V[a][b] = -sum_k r_a f_b)
 void launch_gather(hipStream_t st, const NListDev &nl, const double *pair_f,
-                   double *forces);
+                   double *forces, double *vir_atom);
 // sums[0] = sum e_row, sums[1..9] = column sums of vir (n x 9); part holds
 // 128 x 10 partial sums; deterministic
 void launch_energy_virial_sums(hipStream_t st, int n, const double *e_row,

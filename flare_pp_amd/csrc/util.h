@@ -18,6 +18,9 @@ void launch_normalize_rows_f32(hipStream_t st, int rows, int F, const float *X,
 void launch_normalize_rows_f64(hipStream_t st, int rows, int F, const double *X,
                                const double *norm2, double *out);
 void launch_to_f64(hipStream_t st, int64_t n, const float *x, double *y);
+// y (rows x F) = hi + lo of BF16 rows with stride kp
+void launch_bf16_pair_to_f64(hipStream_t st, int rows, int F, int kp,
+                             const void *hi, const void *lo, double *y);
 // zero rows whose |B|^2 is below thresh
 void launch_mask_rows(hipStream_t st, int rows, int F, const double *norm2,
                       double thresh, double *X);

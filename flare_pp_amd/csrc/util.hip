@@ -55,6 +55,16 @@ __global__ void to_f64_kernel(int64_t n, const float *__restrict__ x,
   if (q < n) y[q] = (double)x[q];
 }
 
+__global__ void bf16_pair_to_f64_kernel(int rows, int F, int kp,
+                                        const __bf16 *__restrict__ hi,
+                                        const __bf16 *__restrict__ lo,
+                                        double *__restrict__ y) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)rows * F) return;
+  int64_t r = q / F, c = q - r * F;
+  y[q] = (double)((float)hi[r * kp + c] + (float)lo[r * kp + c]);
+}
+
 __global__ void mask_rows_kernel(int rows, int F, const double *__restrict__ norm2,
                                  double thresh, double *__restrict__ X) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -176,6 +186,15 @@ void launch_normalize_rows_f64(hipStream_t st, int rows, int F, const double *X,
 void launch_to_f64(hipStream_t st, int64_t n, const float *x, double *y) {
   if (n == 0) return;
   hipLaunchKernelGGL(to_f64_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, n, x, y);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_bf16_pair_to_f64(hipStream_t st, int rows, int F, int kp,
+                             const void *hi, const void *lo, double *y) {
+  if (rows == 0) return;
+  hipLaunchKernelGGL(bf16_pair_to_f64_kernel, dim3(nblk((int64_t)rows * F, 256)),
+                     dim3(256), 0, st, rows, F, kp, (const __bf16 *)hi,
+                     (const __bf16 *)lo, y);
   FLARE_HIP_CHECK(hipGetLastError());
 }
 
