@@ -671,11 +671,10 @@ int flare_nlist_build(flare_ctx *ctx, flare_nlist *nl, int n_atoms,
     REQUIRE(nl->wh.volume > 0, "singular cell");
     nl->n = n_atoms;
     ctx->mark_begin("nlist");
-    flare::nlist_build_device(ctx->stream, nl->b, n_atoms, d_positions, d_species,
-                              nl->wh);
+    nl->n_species_present = flare::nlist_build_device(
+        ctx->stream, nl->b, n_atoms, d_positions, d_species, nl->wh,
+        nl->species_count);
     ctx->mark_end();
-    nl->n_species_present = flare::species_rows_device(
-        ctx->stream, nl->b, n_atoms, d_species, nl->species_count);
     nl->built = true;
   });
 }

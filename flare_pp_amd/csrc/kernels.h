@@ -53,22 +53,22 @@ struct WrapHost {
 };
 
 struct NListBuffers {
-  DevBuf wrapped, bin_of, bin_count, bin_start, bin_fill, bin_atoms, count,
-      cum, keys_tmp, rel_tmp, keys, rel, idx, nspec, center, rev;
+  DevBuf wrapped, bin_of, bin_count, bin_start, bin_fill, bin_atoms, bin_pos, count,
+      cum, keys_tmp, rel_tmp, keys_pad, rel_pad, keys, rel, idx, nspec, center,
+      rev;
   DevBuf row_of, atom_of_row, species_of_row, species_count, species_blk,
       scan_tmp;
   int P = 0;
 };
 
 // neighbors.hip
-void nlist_build_device(hipStream_t st, NListBuffers &b, int n,
-                        const double *d_pos, const int *d_species,
-                        const WrapHost &wh);
-// species-grouped row map: row_of[i], atom_of_row[row], species_of_row[row]
-// and per-species counts (kMaxSpecies ints, copied to host_counts).
-// Returns the number of species actually present (max species + 1).
-int species_rows_device(hipStream_t st, NListBuffers &b, int n,
-                        const int *d_species, int *host_counts);
+// Neighbor list (count, cum, idx, nspec, rel, center, rev, keys), wrapped
+// positions and the species-grouped row map row_of[i], atom_of_row[row],
+// species_of_row[row]; per-species counts (kMaxSpecies ints) are copied to
+// host_counts.  Returns the number of species present (max species + 1).
+int nlist_build_device(hipStream_t st, NListBuffers &b, int n,
+                       const double *d_pos, const int *d_species,
+                       const WrapHost &wh, int *host_counts);
 
 // descriptors.hip
 // A (atom order), GEMM rows (species-grouped: B when out_mode 0, B/|B| when

@@ -38,7 +38,8 @@ __device__ inline void rowmul3(const double *v, const double *b, double *o) {
 
 __global__ void wrap_bin_kernel(int n, const double *__restrict__ pos,
                                 WrapMats m, double *__restrict__ wrapped,
-                                int *__restrict__ bin_of) {
+                                int *__restrict__ bin_of,
+                                int *__restrict__ bin_count) {
 #pragma clang fp contract(off)
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -55,24 +56,25 @@ __global__ void wrap_bin_kernel(int n, const double *__restrict__ pos,
     int v = (int)floor(rw[c] * m.nb[c]);
     b[c] = v < 0 ? 0 : (v >= m.nb[c] ? m.nb[c] - 1 : v);
   }
-  bin_of[i] = (b[0] * m.nb[1] + b[1]) * m.nb[2] + b[2];
+  const int bin = (b[0] * m.nb[1] + b[1]) * m.nb[2] + b[2];
+  bin_of[i] = bin;
+  atomicAdd(&bin_count[bin + 1], 1);
 }
 
-__global__ void bin_count_kernel(int n, const int *__restrict__ bin_of,
-                                 int *__restrict__ bin_count) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) atomicAdd(&bin_count[bin_of[i] + 1], 1);
-}
-
+// bin_atoms / bin_pos: atoms and their wrapped positions grouped by bin
+// (order within a bin is arbitrary; the neighbor rows are sorted later)
 __global__ void bin_fill_kernel(int n, const int *__restrict__ bin_of,
                                 const int *__restrict__ bin_start,
+                                const double *__restrict__ wrapped,
                                 int *__restrict__ fillp,
-                                int *__restrict__ bin_atoms) {
+                                int *__restrict__ bin_atoms,
+                                double *__restrict__ bin_pos) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int b = bin_of[i];
-  int slot = atomicAdd(&fillp[b], 1);
-  bin_atoms[bin_start[b] + slot] = i;
+  int slot = bin_start[b] + atomicAdd(&fillp[b], 1);
+  bin_atoms[slot] = i;
+  for (int c = 0; c < 3; c++) bin_pos[3 * slot + c] = wrapped[3 * i + c];
 }
 
 namespace {
@@ -102,9 +104,19 @@ __device__ inline int axis_len(int nb, int sw) {
   return nb >= 3 ? 3 : nb * (2 * sw + 1);
 }
 
+// Sort key of a neighbor entry: j, then the image shifts, as bit fields
+// (8 bits per shift offset by the sweep, which is at most kMaxSweep), so
+// that integer order is the reference's (j, s1, s2, s3) order.
 __device__ inline int64_t pair_key(int j, int s1, int s2, int s3, int sw) {
-  int64_t u = 2 * sw + 1;
-  return (((int64_t)j * u + (s1 + sw)) * u + (s2 + sw)) * u + (s3 + sw);
+  return ((int64_t)j << 24) | ((int64_t)(s1 + sw) << 16) |
+         ((int64_t)(s2 + sw) << 8) | (int64_t)(s3 + sw);
+}
+__device__ inline void key_parts(int64_t k, int sw, int &j, int &s1, int &s2,
+                                 int &s3) {
+  j = (int)(k >> 24);
+  s1 = (int)((k >> 16) & 0xff) - sw;
+  s2 = (int)((k >> 8) & 0xff) - sw;
+  s3 = (int)(k & 0xff) - sw;
 }
 
 // Reference predicate and relative vector (structure.cpp:68-77)
@@ -230,6 +242,197 @@ nbr_enum_kernel(int n, WrapMats m, const double *__restrict__ wrapped,
   if (MODE == 0 && lane == 0) count[i] = found;
 }
 
+// Candidates per atom held in LDS by the fused enumerate + sort pass.
+constexpr int kNbrCap = 128;
+
+// Fused pass: enumerate one atom's neighbors (as nbr_enum_kernel), keep them
+// in LDS, rank-sort them by key and write them to the atom's padded slot
+// row (kNbrCap entries) of keys_pad / rel_pad.  count[i] is always written;
+// atoms with more than kNbrCap neighbors are left to the two-pass path
+// (stat[0] becomes the largest such count; it stays 0 otherwise).
+// THIN: some axis has fewer than three bins (general combination lists);
+// otherwise the 27 neighboring bins with fixed per-axis lists of three.
+template <bool THIN>
+__global__ void __launch_bounds__(256)
+nbr_enum_sort_kernel(int n, WrapMats m, const double *__restrict__ wrapped,
+                     const int *__restrict__ bin_of,
+                     const int *__restrict__ bin_start,
+                     const int *__restrict__ bin_atoms,
+                     const double *__restrict__ bin_pos, int *__restrict__ count,
+                     int64_t *__restrict__ keys_pad,
+                     double *__restrict__ rel_pad, int *__restrict__ stat) {
+  __shared__ int prefix_sh[4][kWave];
+  __shared__ int64_t skey_sh[4][kNbrCap];
+  __shared__ double srel_sh[4][kNbrCap][4];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wv = threadIdx.x / kWave;
+  const int i = blockIdx.x * 4 + wv;
+  if (i >= n) return;  // whole wave exits together
+  int *prefix = prefix_sh[wv];
+  int64_t *skey = skey_sh[wv];
+  double(*srel)[4] = srel_sh[wv];
+  const int sw = m.sweep;
+  const int hb = bin_of[i];
+  const int hx = hb / (m.nb[1] * m.nb[2]);
+  const int hy = (hb / m.nb[2]) % m.nb[1];
+  const int hz = hb % m.nb[2];
+  const int lx = THIN ? axis_len(m.nb[0], sw) : 3,
+            ly = THIN ? axis_len(m.nb[1], sw) : 3,
+            lz = THIN ? axis_len(m.nb[2], sw) : 3;
+  const int n_combo = lx * ly * lz;
+  const double wi[3] = {wrapped[3 * i], wrapped[3 * i + 1], wrapped[3 * i + 2]};
+  const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
+  int found = 0;
+
+  for (int c0 = 0; c0 < n_combo; c0 += kWave) {
+    int c = c0 + lane;
+    int csize = 0;
+    if (c < n_combo) {
+      int ex = c / (ly * lz), ey = (c / lz) % ly, ez = c % lz;
+      int bx, by, bz, s1, s2, s3;
+      axis_entry(m.nb[0], hx, sw, ex, bx, s1);
+      axis_entry(m.nb[1], hy, sw, ey, by, s2);
+      axis_entry(m.nb[2], hz, sw, ez, bz, s3);
+      const int cb = (bx * m.nb[1] + by) * m.nb[2] + bz;
+      csize = bin_start[cb + 1] - bin_start[cb];
+    }
+    int incl = csize;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      int v = __shfl_up(incl, off, kWave);
+      if (lane >= off) incl += v;
+    }
+    prefix[lane] = incl;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const int total = __shfl(incl, kWave - 1, kWave);
+    for (int t0 = 0; t0 < total; t0 += kWave) {
+      const int t = t0 + lane;
+      bool pass = false;
+      double o[4];
+      int64_t key = 0;
+      if (t < total) {
+        int lo = 0, hi = kWave - 1;
+        while (lo < hi) {
+          int mid = (lo + hi) >> 1;
+          if (prefix[mid] > t)
+            hi = mid;
+          else
+            lo = mid + 1;
+        }
+        const int cc = lo;
+        const int before = cc ? prefix[cc - 1] : 0;
+        const int cidx = c0 + cc;
+        int ex = cidx / (ly * lz), ey = (cidx / lz) % ly, ez = cidx % lz;
+        int bx, by, bz, t1, t2, t3;
+        axis_entry(m.nb[0], hx, sw, ex, bx, t1);
+        axis_entry(m.nb[1], hy, sw, ey, by, t2);
+        axis_entry(m.nb[2], hz, sw, ez, bz, t3);
+        const int b = (bx * m.nb[1] + by) * m.nb[2] + bz;
+        const int slot = bin_start[b] + (t - before);
+        const int j = bin_atoms[slot];
+        const double wj[3] = {bin_pos[3 * slot], bin_pos[3 * slot + 1],
+                              bin_pos[3 * slot + 2]};
+        pass = image_test(wi, wj, m.cell, t1, t2, t3, m.cutoff, o);
+        key = pair_key(j, t1, t2, t3, sw);
+      }
+      const uint64_t mask = __ballot(pass);
+      if (pass) {
+        const int slot = found + __popcll(mask & lt_mask);
+        if (slot < kNbrCap) {
+          skey[slot] = key;
+          srel[slot][0] = o[0];
+          srel[slot][1] = o[1];
+          srel[slot][2] = o[2];
+          srel[slot][3] = o[3];
+        }
+      }
+      found += __popcll(mask);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (lane == 0) {
+    count[i] = found;
+    if (found > kNbrCap) atomicMax(stat, found);  // rare: no contention
+  }
+  if (found > kNbrCap) return;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // rank = number of smaller keys (keys are distinct); every lane reads the
+  // same key per step, an LDS broadcast
+  int64_t k0 = lane < found ? skey[lane] : INT64_MAX;
+  int64_t k1 = lane + kWave < found ? skey[lane + kWave] : INT64_MAX;
+  int r0 = 0, r1 = 0;
+  for (int q = 0; q < found; q++) {
+    const int64_t kq = skey[q];
+    r0 += kq < k0;
+    r1 += kq < k1;
+  }
+  const int64_t row = (int64_t)i * kNbrCap;
+  if (lane < found) {
+    keys_pad[row + r0] = k0;
+    for (int c = 0; c < 4; c++) rel_pad[4 * (row + r0) + c] = srel[lane][c];
+  }
+  if (lane + kWave < found) {
+    keys_pad[row + r1] = k1;
+    for (int c = 0; c < 4; c++) rel_pad[4 * (row + r1) + c] = srel[lane + kWave][c];
+  }
+}
+
+// Padded sorted rows -> final CSR arrays, with the reverse-pair index found
+// by binary search in the neighbor's sorted row.  One wavefront per atom.
+__global__ void __launch_bounds__(256)
+nbr_compact_kernel(int n, int sweep, const int *__restrict__ cum,
+                   const int *__restrict__ count,
+                   const int64_t *__restrict__ keys_pad,
+                   const double *__restrict__ rel_pad,
+                   const int *__restrict__ species,
+                   int64_t *__restrict__ keys_out, double *__restrict__ rel_out,
+                   int *__restrict__ idx_out, int *__restrict__ nspec_out,
+                   int *__restrict__ center_out, int *__restrict__ rev_out) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int i = blockIdx.x * 4 + threadIdx.x / kWave;
+  if (i >= n) return;
+  const int b = cum[i], cnt = cum[i + 1] - b;
+  const int64_t row = (int64_t)i * kNbrCap;
+  for (int q = lane; q < cnt; q += kWave) {
+    const int64_t k = keys_pad[row + q];
+    const double4 rv = reinterpret_cast<const double4 *>(rel_pad)[row + q];
+    int j, s1, s2, s3;
+    key_parts(k, sweep, j, s1, s2, s3);
+    const int64_t want = pair_key(i, -s1, -s2, -s3, sweep);
+    const int dst = b + q;
+    // the entry of the two partners with the smaller key looks the other
+    // one up and writes both reverse indices (keys are never equal: i == j
+    // needs a non-zero shift)
+    if (k < want) {
+      const int64_t jrow = (int64_t)j * kNbrCap;
+      const int cj = count[j];
+      int lo = 0, hi = cj;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (keys_pad[jrow + mid] < want)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      if (lo < cj && keys_pad[jrow + lo] == want) {
+        rev_out[dst] = cum[j] + lo;
+        rev_out[cum[j] + lo] = dst;
+      } else {
+        rev_out[dst] = -1;
+      }
+    }
+    keys_out[dst] = k;
+    reinterpret_cast<double4 *>(rel_out)[dst] = rv;
+    idx_out[dst] = j;
+    nspec_out[dst] = species[j];
+    center_out[dst] = i;
+  }
+}
+
 // Rank sort of each atom's segment by key; writes the final CSR arrays.
 __global__ void __launch_bounds__(256)
 nbr_sort_kernel(int n, int sweep, const int *__restrict__ cum,
@@ -243,15 +446,13 @@ nbr_sort_kernel(int n, int sweep, const int *__restrict__ cum,
   const int i = blockIdx.x * 4 + threadIdx.x / kWave;
   if (i >= n) return;
   const int b = cum[i], e = cum[i + 1];
-  const int64_t u = 2 * sweep + 1;
-  const int64_t u3 = u * u * u;
   for (int q = b + lane; q < e; q += kWave) {
     int64_t k = keys_in[q];
     int rank = 0;
     for (int r = b; r < e; r++) rank += (keys_in[r] < k);
     int dst = b + rank;
     keys_out[dst] = k;
-    int j = (int)(k / u3);
+    int j = (int)(k >> 24);
     idx_out[dst] = j;
     nspec_out[dst] = species[j];
     center_out[dst] = i;
@@ -269,12 +470,10 @@ __global__ void nbr_reverse_kernel(int64_t P, int sweep,
                                    int *__restrict__ rev) {
   int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P) return;
-  const int64_t u = 2 * sweep + 1;
   int64_t k = keys[p];
-  int s3 = (int)(k % u) - sweep;
-  int s2 = (int)((k / u) % u) - sweep;
-  int s1 = (int)((k / (u * u)) % u) - sweep;
-  int j = idx[p], i = center[p];
+  int j, s1, s2, s3;
+  key_parts(k, sweep, j, s1, s2, s3);
+  int i = center[p];
   int64_t want = pair_key(i, -s1, -s2, -s3, sweep);
   int lo = cum[j], hi = cum[j + 1];
   while (lo < hi) {
@@ -360,43 +559,6 @@ species_rows_kernel(int n, const int *__restrict__ species,
   }
 }
 
-int species_rows_device(hipStream_t st, NListBuffers &b, int n,
-                        const int *d_species, int *host_counts) {
-  const int nblk = std::max(1, (n + kRowBlock - 1) / kRowBlock);
-  b.row_of.ensure((size_t)std::max(n, 1) * sizeof(int));
-  b.atom_of_row.ensure((size_t)std::max(n, 1) * sizeof(int));
-  b.species_of_row.ensure((size_t)std::max(n, 1) * sizeof(int));
-  b.species_count.ensure((kMaxSpecies + 1) * sizeof(int));
-  b.species_blk.ensure((size_t)2 * nblk * kMaxSpecies * sizeof(int));
-  int *bcount = b.species_blk.as<int>(), *boff = bcount + nblk * kMaxSpecies;
-  int *counts = b.species_count.as<int>();
-  FLARE_HIP_CHECK(hipMemsetAsync(counts, 0, (kMaxSpecies + 1) * sizeof(int), st));
-  if (n > 0) {
-    hipLaunchKernelGGL(species_count_kernel, dim3(nblk), dim3(kRowBlock), 0, st,
-                       n, d_species, bcount, counts + kMaxSpecies);
-    hipLaunchKernelGGL(species_scan_kernel, dim3(1), dim3(64 * kMaxSpecies), 0,
-                       st, nblk, bcount, boff, counts);
-  }
-  FLARE_HIP_CHECK(hipGetLastError());
-  int h[kMaxSpecies + 1];
-  FLARE_HIP_CHECK(hipMemcpyAsync(h, counts, sizeof(h), hipMemcpyDeviceToHost, st));
-  FLARE_HIP_CHECK(hipStreamSynchronize(st));
-  FLARE_REQUIRE(h[kMaxSpecies] == 0,
-                "species codes must lie in [0, " + std::to_string(kMaxSpecies) + ")");
-  if (n > 0) {
-    hipLaunchKernelGGL(species_rows_kernel, dim3(nblk), dim3(kRowBlock), 0, st, n,
-                       d_species, boff, counts, b.row_of.as<int>(),
-                       b.atom_of_row.as<int>(), b.species_of_row.as<int>());
-    FLARE_HIP_CHECK(hipGetLastError());
-  }
-  int ns = 0;
-  for (int s = 0; s < kMaxSpecies; s++) {
-    host_counts[s] = h[s];
-    if (h[s] > 0) ns = s + 1;
-  }
-  return ns;
-}
-
 // exclusive prefix sum of in[0..n) into out[0..n], out[n] = total; in[n]
 // must be readable and zero
 static void exclusive_scan(hipStream_t st, DevBuf &tmp, const int *in, int *out,
@@ -408,11 +570,12 @@ static void exclusive_scan(hipStream_t st, DevBuf &tmp, const int *in, int *out,
 }
 
 // ---------------------------------------------------------------------------
-// Host driver
+// Host driver.  One host synchronisation per build: the pair total, the
+// largest per-atom count and the species counts are read together.
 
-void nlist_build_device(hipStream_t st, NListBuffers &b, int n,
-                        const double *d_pos, const int *d_species,
-                        const WrapHost &wh) {
+int nlist_build_device(hipStream_t st, NListBuffers &b, int n,
+                       const double *d_pos, const int *d_species,
+                       const WrapHost &wh, int *host_counts) {
   WrapMats m;
   for (int q = 0; q < 9; q++) {
     m.ct[q] = wh.ct[q];
@@ -425,58 +588,100 @@ void nlist_build_device(hipStream_t st, NListBuffers &b, int n,
   m.sweep = wh.sweep;
   m.cutoff = wh.cutoff;
   const int nbins = wh.nb[0] * wh.nb[1] * wh.nb[2];
+  const int nblk = std::max(1, (n + kRowBlock - 1) / kRowBlock);
 
-  b.wrapped.ensure((size_t)n * 3 * sizeof(double));
-  b.bin_of.ensure((size_t)n * sizeof(int));
+  b.wrapped.ensure((size_t)std::max(n, 1) * 3 * sizeof(double));
+  b.bin_of.ensure((size_t)std::max(n, 1) * sizeof(int));
   b.bin_count.ensure((size_t)(nbins + 2) * sizeof(int));
   b.bin_start.ensure((size_t)(nbins + 1) * sizeof(int));
   b.bin_fill.ensure((size_t)(nbins + 1) * sizeof(int));
   b.bin_atoms.ensure((size_t)std::max(n, 1) * sizeof(int));
+  b.bin_pos.ensure((size_t)std::max(n, 1) * 3 * sizeof(double));
   b.count.ensure((size_t)(n + 1) * sizeof(int));
   b.cum.ensure((size_t)(n + 1) * sizeof(int));
+  b.keys_pad.ensure((size_t)std::max(n, 1) * kNbrCap * sizeof(int64_t));
+  b.rel_pad.ensure((size_t)std::max(n, 1) * kNbrCap * 4 * sizeof(double));
+  b.row_of.ensure((size_t)std::max(n, 1) * sizeof(int));
+  b.atom_of_row.ensure((size_t)std::max(n, 1) * sizeof(int));
+  b.species_of_row.ensure((size_t)std::max(n, 1) * sizeof(int));
+  // species_count: kMaxSpecies counts, bad-code flag, largest pair count
+  b.species_count.ensure((kMaxSpecies + 2) * sizeof(int));
+  b.species_blk.ensure((size_t)2 * nblk * kMaxSpecies * sizeof(int));
+  int *bcount = b.species_blk.as<int>(), *boff = bcount + nblk * kMaxSpecies;
+  int *counts = b.species_count.as<int>();
+  int *maxcount = counts + kMaxSpecies + 1;
 
-  const int tb = 256;
-  const int g1 = (n + tb - 1) / tb;
-  if (n > 0) {
-    hipLaunchKernelGGL(wrap_bin_kernel, dim3(g1), dim3(tb), 0, st, n, d_pos, m,
-                       b.wrapped.as<double>(), b.bin_of.as<int>());
-  }
+  FLARE_HIP_CHECK(hipMemsetAsync(counts, 0, (kMaxSpecies + 2) * sizeof(int), st));
   FLARE_HIP_CHECK(hipMemsetAsync(b.bin_count.ptr, 0, (nbins + 2) * sizeof(int), st));
   FLARE_HIP_CHECK(hipMemsetAsync(b.bin_fill.ptr, 0, (nbins + 1) * sizeof(int), st));
-  if (n > 0)
-    hipLaunchKernelGGL(bin_count_kernel, dim3(g1), dim3(tb), 0, st, n,
-                       b.bin_of.as<int>(), b.bin_count.as<int>());
+  const int tb = 256;
+  const int g1 = (n + tb - 1) / tb;
+  const int gw = (n + 3) / 4;
+  if (n > 0) {
+    hipLaunchKernelGGL(wrap_bin_kernel, dim3(g1), dim3(tb), 0, st, n, d_pos, m,
+                       b.wrapped.as<double>(), b.bin_of.as<int>(),
+                       b.bin_count.as<int>());
+    hipLaunchKernelGGL(species_count_kernel, dim3(nblk), dim3(kRowBlock), 0, st,
+                       n, d_species, bcount, counts + kMaxSpecies);
+    hipLaunchKernelGGL(species_scan_kernel, dim3(1), dim3(64 * kMaxSpecies), 0,
+                       st, nblk, bcount, boff, counts);
+  }
   // bin_count[1..nbins] holds the counts; bin_count has one spare zero entry
   exclusive_scan(st, b.scan_tmp, b.bin_count.as<int>() + 1, b.bin_start.as<int>(),
                  nbins);
-  if (n > 0)
+  if (n > 0) {
     hipLaunchKernelGGL(bin_fill_kernel, dim3(g1), dim3(tb), 0, st, n,
                        b.bin_of.as<int>(), b.bin_start.as<int>(),
-                       b.bin_fill.as<int>(), b.bin_atoms.as<int>());
-  const int gw = (n + 3) / 4;
-  if (n > 0)
-    hipLaunchKernelGGL(nbr_enum_kernel<0>, dim3(gw), dim3(256), 0, st, n, m,
-                       b.wrapped.as<double>(), b.bin_of.as<int>(),
-                       b.bin_start.as<int>(), b.bin_atoms.as<int>(),
-                       b.count.as<int>(), (const int *)nullptr,
-                       (int64_t *)nullptr, (double *)nullptr);
+                       b.wrapped.as<double>(), b.bin_fill.as<int>(),
+                       b.bin_atoms.as<int>(), b.bin_pos.as<double>());
+    const bool thin = wh.nb[0] < 3 || wh.nb[1] < 3 || wh.nb[2] < 3;
+    if (thin)
+      hipLaunchKernelGGL(nbr_enum_sort_kernel<true>, dim3(gw), dim3(256), 0, st,
+                         n, m, b.wrapped.as<double>(), b.bin_of.as<int>(),
+                         b.bin_start.as<int>(), b.bin_atoms.as<int>(),
+                         b.bin_pos.as<double>(), b.count.as<int>(),
+                         b.keys_pad.as<int64_t>(), b.rel_pad.as<double>(),
+                         maxcount);
+    else
+      hipLaunchKernelGGL(nbr_enum_sort_kernel<false>, dim3(gw), dim3(256), 0, st,
+                         n, m, b.wrapped.as<double>(), b.bin_of.as<int>(),
+                         b.bin_start.as<int>(), b.bin_atoms.as<int>(),
+                         b.bin_pos.as<double>(), b.count.as<int>(),
+                         b.keys_pad.as<int64_t>(), b.rel_pad.as<double>(),
+                         maxcount);
+  }
   FLARE_HIP_CHECK(hipMemsetAsync(b.count.as<int>() + n, 0, sizeof(int), st));
   exclusive_scan(st, b.scan_tmp, b.count.as<int>(), b.cum.as<int>(), n);
+  FLARE_HIP_CHECK(hipGetLastError());
+  int hc[kMaxSpecies + 2];
   int P = 0;
+  FLARE_HIP_CHECK(hipMemcpyAsync(hc, counts, sizeof(hc), hipMemcpyDeviceToHost, st));
   FLARE_HIP_CHECK(hipMemcpyAsync(&P, b.cum.as<int>() + n, sizeof(int),
                                  hipMemcpyDeviceToHost, st));
   FLARE_HIP_CHECK(hipStreamSynchronize(st));
+  FLARE_REQUIRE(hc[kMaxSpecies] == 0,
+                "species codes must lie in [0, " + std::to_string(kMaxSpecies) + ")");
   b.P = P;
   size_t Pa = std::max(P, 1);
-  b.keys_tmp.ensure(Pa * sizeof(int64_t));
-  b.rel_tmp.ensure(Pa * 4 * sizeof(double));
   b.keys.ensure(Pa * sizeof(int64_t));
   b.rel.ensure(Pa * 4 * sizeof(double));
   b.idx.ensure(Pa * sizeof(int));
   b.nspec.ensure(Pa * sizeof(int));
   b.center.ensure(Pa * sizeof(int));
   b.rev.ensure(Pa * sizeof(int));
-  if (n > 0) {
+  if (n > 0 && hc[kMaxSpecies + 1] <= kNbrCap) {
+    // entries without a partner keep -1 (the partners write each other)
+    FLARE_HIP_CHECK(hipMemsetAsync(b.rev.ptr, 0xff, Pa * sizeof(int), st));
+    hipLaunchKernelGGL(nbr_compact_kernel, dim3(gw), dim3(256), 0, st, n,
+                       wh.sweep, b.cum.as<int>(), b.count.as<int>(),
+                       b.keys_pad.as<int64_t>(), b.rel_pad.as<double>(),
+                       d_species, b.keys.as<int64_t>(), b.rel.as<double>(),
+                       b.idx.as<int>(), b.nspec.as<int>(), b.center.as<int>(),
+                       b.rev.as<int>());
+  } else if (n > 0) {
+    // two-pass path for atoms with more neighbors than the LDS rows hold
+    b.keys_tmp.ensure(Pa * sizeof(int64_t));
+    b.rel_tmp.ensure(Pa * 4 * sizeof(double));
     hipLaunchKernelGGL(nbr_enum_kernel<1>, dim3(gw), dim3(256), 0, st, n, m,
                        b.wrapped.as<double>(), b.bin_of.as<int>(),
                        b.bin_start.as<int>(), b.bin_atoms.as<int>(),
@@ -487,15 +692,25 @@ void nlist_build_device(hipStream_t st, NListBuffers &b, int n,
                        b.rel_tmp.as<double>(), d_species, b.keys.as<int64_t>(),
                        b.rel.as<double>(), b.idx.as<int>(), b.nspec.as<int>(),
                        b.center.as<int>());
+    if (P > 0) {
+      int gp = (P + tb - 1) / tb;
+      hipLaunchKernelGGL(nbr_reverse_kernel, dim3(gp), dim3(tb), 0, st,
+                         (int64_t)P, wh.sweep, b.cum.as<int>(),
+                         b.keys.as<int64_t>(), b.idx.as<int>(),
+                         b.center.as<int>(), b.rev.as<int>());
+    }
   }
-  if (P > 0) {
-    int gp = (P + tb - 1) / tb;
-    hipLaunchKernelGGL(nbr_reverse_kernel, dim3(gp), dim3(tb), 0, st,
-                       (int64_t)P, wh.sweep, b.cum.as<int>(),
-                       b.keys.as<int64_t>(), b.idx.as<int>(),
-                       b.center.as<int>(), b.rev.as<int>());
-  }
+  if (n > 0)
+    hipLaunchKernelGGL(species_rows_kernel, dim3(nblk), dim3(kRowBlock), 0, st, n,
+                       d_species, boff, counts, b.row_of.as<int>(),
+                       b.atom_of_row.as<int>(), b.species_of_row.as<int>());
   FLARE_HIP_CHECK(hipGetLastError());
+  int ns = 0;
+  for (int sp = 0; sp < kMaxSpecies; sp++) {
+    host_counts[sp] = hc[sp];
+    if (hc[sp] > 0) ns = sp + 1;
+  }
+  return ns;
 }
 
 }  // namespace flare

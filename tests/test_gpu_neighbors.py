@@ -65,3 +65,25 @@ def test_species_grouping(orc):
                            for k in range(3)])
     np.testing.assert_array_equal(rows, want)
     assert list(s.species_count[:3]) == [22118, 16589, 16589]
+
+
+def test_dense_two_pass_path(orc):
+    # about 190 neighbors per atom: more than the fused sort keeps in LDS
+    # (128), so the two-pass enumerate / rank-sort / reverse path runs
+    cell, species, pos = synth.fcc(4, 4.05, 0.05, [256], 5)
+    s, o = compare(orc, cell, species, pos, 9.0)
+    assert s.neighbor_count.max() > 128
+    from flare_pp_amd.structure import d2h
+    rev = d2h(s._view.d_reverse_pair, s.n_neighbors, np.int32)
+    np.testing.assert_array_equal(o.rel[rev, 1:], -o.rel[:, 1:])
+
+
+def test_fused_path_reverse_pairs(orc, cfg1):
+    c = cfg1
+    s, o = compare(orc, c["cell"], c["species"], c["positions"], 5.0)
+    assert s.neighbor_count.max() <= 128
+    from flare_pp_amd.structure import d2h
+    rev = d2h(s._view.d_reverse_pair, s.n_neighbors, np.int32)
+    assert rev.min() >= 0
+    np.testing.assert_array_equal(o.rel[rev, 1:], -o.rel[:, 1:])
+    np.testing.assert_array_equal(rev[rev], np.arange(len(rev)))
