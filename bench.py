@@ -34,6 +34,10 @@ def parse():
     ap.add_argument("--forms", default=",".join(FORMS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes")
+    ap.add_argument("--copy-calibration", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--traffic-child", default=None, help=argparse.SUPPRESS)
     return ap.parse_args()
 
 
@@ -53,37 +57,134 @@ def timed_steps(pair, ctx, cell, pos, spc, form, outs, steps, rebuild):
     return ev0.elapsed_time(ev1) / steps
 
 
-def roofline(form, stages, n_atoms, n_pairs, F, W):
-    """Dominant stage of the timed form and its algorithmic rate."""
-    per = {k: t / c for k, (t, c) in stages.items() if c}
+# single-kernel stages that can be the roofline entry, and the kernel each
+# one launches in a given form (name prefix in the rocprof kernel trace)
+KERNELS = {
+    ("forward", "adjoint_mixed"): "b2_forward_mfma_kernel<float",
+    ("forward", "adjoint_fp32"): "b2_forward_mfma_kernel<float",
+    ("forward", "adjoint_fp64"): "b2_forward_mfma_kernel<double",
+    ("forward", "plain"): "b2_forward_mfma_kernel<double",
+    ("gemm", "adjoint_mixed"): "gemm_bf16x3_kernel",
+    ("forces_adjoint", "adjoint_mixed"): "force_kernel<float, 1,",
+    ("forces_adjoint", "adjoint_fp32"): "force_kernel<float, 1,",
+    ("forces_adjoint", "adjoint_fp64"): "force_kernel<double, 1,",
+    ("forces_plain", "plain"): "force_kernel<double, 0,",
+    ("pair_dA", "plain"): "pair_dA_kernel",
+}
+
+
+def algorithmic(stage, form, n, P, S, F, W, kp):
+    """Minimum work of one launch of a stage's kernel: ("hbm", bytes) or
+    ("mfma", flops, precision)."""
+    fb = 8 if form in ("plain", "adjoint_fp64") else 4
+    if stage == "gemm":
+        prec = {"adjoint_mixed": "bf16", "adjoint_fp32": "fp32"}.get(form, "fp64")
+        return ("mfma", 2.0 * F * F * n, prec)
+    if stage == "forward":
+        rows = 2 * kp * 2 if form == "adjoint_mixed" else F * fb
+        return ("hbm", P * 36.0 + n * (16.0 + W * fb + rows + 8))
+    if stage == "forces_adjoint":
+        x = 2 * kp * 2 if form == "adjoint_mixed" else F * fb
+        return ("hbm", n * (W * fb + x + F * fb + 36.0) + P * 60.0)
+    if stage == "pair_dA":
+        return ("hbm", P * (3.0 * (W // S) * 8 + 40))
+    if stage == "forces_plain":
+        return ("hbm", P * (3.0 * (W // S) * 8 + 60) + n * (W + F) * 8.0)
+    return None
+
+
+def roofline(form, stages, n, P, S, F, W, kp):
+    """Dominant single-kernel stage of the form and its achieved rate."""
+    per = {k: t / c for k, (t, c) in stages.items() if c and (k, form) in KERNELS}
     name = max(per, key=per.get)
     sec = per[name] * 1e-3
-    fbytes = 8 if form in ("plain", "adjoint_fp64") else 4
-    if name == "gemm":
-        flops = 2.0 * F * F * n_atoms  # D . beta, 2F^2 per atom
-        prec = {"plain": "fp64", "adjoint_fp64": "fp64", "adjoint_fp32": "fp32",
-                "adjoint_mixed": "bf16"}[form]
-        ach = flops / sec / 1e12
-        peak = PEAKS_TFLOPS[prec]
-        return name, per[name], {"bound": "mfma", "achieved": ach, "peak": peak,
-                                 "unit": "TFLOP/s", "frac": ach / peak,
-                                 "traffic": None}
-    if name == "pair_dA":
-        algo = n_pairs * 3 * (W // 2) * 8.0  # dA/dr written once, FP64
-    elif name == "forces_plain":
-        algo = n_pairs * 3 * (W // 2) * 8.0 + n_atoms * (W + F) * 8.0 + n_pairs * 24
-    elif name == "forces_adjoint":
-        algo = n_atoms * (W + F) * fbytes + n_pairs * (32 + 4 + 24) + n_atoms * 72
-    elif name == "forward":
-        algo = n_pairs * (32 + 4) + n_atoms * (W + F) * fbytes
-    elif name == "nlist":
-        algo = n_pairs * (8 + 32 + 4 * 4) + n_atoms * 24 * 2
+    a = algorithmic(name, form, n, P, S, F, W, kp)
+    if a[0] == "mfma":
+        ach = a[1] / sec / 1e12
+        peak = PEAKS_TFLOPS[a[2]]
+        rl = {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+              "frac": ach / peak, "traffic": None}
     else:
-        algo = n_atoms * F * fbytes * 2
-    ach = algo / sec / 1e9
-    return name, per[name], {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
-                             "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
-                             "traffic": None}
+        ach = a[1] / sec / 1e9
+        rl = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS,
+              "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None}
+    return name, per[name], rl
+
+
+# ---------------------------------------------------------------------------
+# HBM traffic of one kernel launch from rocprofv3 counters (FETCH_SIZE and
+# WRITE_SIZE, KiB per dispatch, one counter per pass), each scaled by what
+# the same counter reads for a device copy of known size.
+
+COPY_BYTES = 256 * 1024 * 1024
+
+
+def pmc_pass(counter, child_args, match):
+    """Mean counter value (KiB) per dispatch of kernels whose name contains
+    `match`, from one rocprofv3 pass over a child run of this script."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    with tempfile.TemporaryDirectory() as d:
+        cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o",
+               "run", "--", sys.executable, os.path.abspath(__file__)] + child_args
+        subprocess.run(["timeout", "-s", "KILL", "90"] + cmd, check=True,
+                       stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
+        vals = []
+        for root, _, files in os.walk(d):
+            for f in files:
+                if f.endswith("counter_collection.csv"):
+                    for r in csv.DictReader(open(os.path.join(root, f))):
+                        if match in r["Kernel_Name"] and r["Counter_Name"] == counter:
+                            vals.append(float(r["Counter_Value"]))
+    if not vals:
+        raise RuntimeError(f"no {counter} samples for {match}")
+    return sum(vals) / len(vals)
+
+
+def measure_traffic(form, kernel):
+    scale = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        kib = pmc_pass(ctr, ["--copy-calibration"], "copyBuffer")
+        scale[ctr] = COPY_BYTES / kib  # bytes per counter unit
+    fetch = pmc_pass("FETCH_SIZE", ["--traffic-child", form], kernel)
+    write = pmc_pass("WRITE_SIZE", ["--traffic-child", form], kernel)
+    return {"bytes": fetch * scale["FETCH_SIZE"] + write * scale["WRITE_SIZE"],
+            "fetch_bytes": fetch * scale["FETCH_SIZE"],
+            "write_bytes": write * scale["WRITE_SIZE"],
+            "fetch_scale_bytes_per_unit": scale["FETCH_SIZE"],
+            "write_scale_bytes_per_unit": scale["WRITE_SIZE"]}
+
+
+def copy_calibration():
+    import torch
+    x = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda:0")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+
+
+def traffic_child(form):
+    import numpy as np
+    import torch
+    from flare_pp_amd import PairFLARE, pack_beta, synth
+    c = synth.config2()
+    n = len(c["species"])
+    packed = np.stack([pack_beta(b) for b in c["beta"]])
+    pair = PairFLARE.from_coefficients(2, 2, 8, 4, np.full((2, 2), 5.0), packed)
+    pos = torch.tensor(c["positions"], device="cuda:0")
+    spc = torch.tensor(c["species"], device="cuda:0", dtype=torch.int32)
+    e = torch.empty(1, dtype=torch.float64, device="cuda:0")
+    f = torch.empty((n, 3), dtype=torch.float64, device="cuda:0")
+    v = torch.empty(6, dtype=torch.float64, device="cuda:0")
+    for _ in range(3):
+        pair.build_neighbors(c["cell"], pos, spc)
+        pair.compute_device(spc, form=form, energy=e, forces=f, virial=v)
+    torch.cuda.synchronize()
 
 
 def cpu_baseline(c, packed, threads):
@@ -113,6 +214,10 @@ def cpu_baseline(c, packed, threads):
 
 def main():
     args = parse()
+    if args.copy_calibration:
+        return copy_calibration()
+    if args.traffic_child:
+        return traffic_child(args.traffic_child)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU "
                           "sharding is described in DESIGN.md, not run"}))
@@ -136,6 +241,7 @@ def main():
     v = torch.empty(6, dtype=torch.float64, device=dev)
     ctx = _capi.context(0)
     F, W = 680, 400
+    KP = (F + 31) // 32 * 32  # BF16 row length of the GEMM operands
 
     # reference result: plain FP64
     pair.build_neighbors(cell, pos, spc)
@@ -173,7 +279,7 @@ def main():
             "err_force_rms_meV_per_A": float(np.sqrt(np.mean(fd ** 2)) * 1e3),
             "err_force_max_meV_per_A": float(np.abs(fd).max() * 1e3),
         }
-        results[form]["roofline"] = roofline(form, stages, n, n_pairs, F, W)
+        results[form]["roofline"] = roofline(form, stages, n, n_pairs, 2, F, W, KP)
         print(f"# {form}: {ms:.3f} ms/step ({ms_reuse:.3f} reused nlist) "
               f"stages {results[form]['stage_ms']}", file=sys.stderr, flush=True)
     ctx.set_blocking(True)
@@ -181,7 +287,15 @@ def main():
     head = "adjoint_mixed" if "adjoint_mixed" in results else forms[-1]
     r = results[head]
     rl_name, rl_ms, rl = r["roofline"]
-    rl = dict(rl, kernel=rl_name, kernel_ms=rl_ms)
+    kname = KERNELS[(rl_name, head)]
+    rl = dict(rl, stage=rl_name, kernel=kname, kernel_ms=rl_ms)
+    if not args.no_traffic:
+        try:
+            t = measure_traffic(head, kname)
+            rl["traffic"] = t["bytes"]
+            rl["traffic_detail"] = t
+        except Exception as exc:  # profiler absent or refused: no figure
+            print(f"# traffic not measured: {exc}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(c, packed, args.cpu_threads)

@@ -1073,7 +1073,18 @@ int flare_pair_compute(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
     ctx->mark_begin("gemm");
     if (pr->power == 2) {
       Z = ctx->buf("pair.Z", std::max<size_t>(1, (size_t)n * F) * ts);
-      for (int s = 0; s < p.S; s++) {
+      if (mixed) {  // all species in one launch
+        int M[flare::kMaxSpecies];
+        int64_t a0[flare::kMaxSpecies], b0[flare::kMaxSpecies];
+        for (int s = 0; s < p.S; s++) {
+          M[s] = off[s + 1] - off[s];
+          a0[s] = off[s];
+          b0[s] = (int64_t)s * F;
+        }
+        flare::gemm_bf16x3_batched(ctx->stream, p.S, M, a0, b0, F, kp, fw.xh,
+                                   fw.xl, bh, bl, (float *)Z, F);
+      }
+      for (int s = 0; s < p.S && !mixed; s++) {
         int rows = off[s + 1] - off[s];
         if (!rows) continue;
         if (prec == flare::kPrecF64) {
@@ -1081,13 +1092,6 @@ int flare_pair_compute(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
                   (const double *)X + (size_t)off[s] * F, F,
                   (const double *)beta + s * per, F, 0.0,
                   (double *)Z + (size_t)off[s] * F, F);
-        } else if (mixed) {
-          flare::gemm_bf16x3(ctx->stream, rows, F, kp,
-                             (const __bf16 *)fw.xh + (size_t)off[s] * kp,
-                             (const __bf16 *)fw.xl + (size_t)off[s] * kp,
-                             (const __bf16 *)bh + (size_t)s * F * kp,
-                             (const __bf16 *)bl + (size_t)s * F * kp,
-                             (float *)Z + (size_t)off[s] * F, F);
         } else {
           sgemm_rm(ctx, false, false, rows, F, F, 1.0f,
                    (const float *)X + (size_t)off[s] * F, F,

@@ -220,6 +220,14 @@ __device__ inline double b2_gram_mfma(const B2Dev &p, const T *Al, T *Bs) {
 #pragma unroll
     for (int t2 = t1; t2 < MT; t2++) {
       const int ca = t1 * 16 + col, cb = t2 * 16 + col;
+      // feature offset p(c1, c2) (L+1) of this lane's four result entries
+      // (upper triangle only, -1 elsewhere), fixed over l
+      int fofs[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const int c1 = t1 * 16 + Mma<T>::row(g, q);
+        fofs[q] = (c1 <= cb && cb < nr) ? b2_pair_index(c1, cb, nr) * L1 : -1;
+      }
       for (int l = 0; l <= L; l++) {
         const int K = 2 * l + 1, l2 = l * l;
         V acc = V{0, 0, 0, 0};
@@ -234,10 +242,9 @@ __device__ inline double b2_gram_mfma(const B2Dev &p, const T *Al, T *Bs) {
         }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-          const int c1 = t1 * 16 + Mma<T>::row(g, q), c2 = cb;
-          if (c1 <= c2 && c2 < nr) {
+          if (fofs[q] >= 0) {
             const T v = acc[q];
-            Bs[b2_pair_index(c1, c2, nr) * L1 + l] = v;
+            Bs[fofs[q] + l] = v;
             nrm += (double)v * (double)v;
           }
         }

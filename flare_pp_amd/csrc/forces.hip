@@ -82,43 +82,48 @@ __device__ inline void stage_lds(int n, Fn &&load, T *dst) {
 // channel c = s N + n, so the per-pair loop reads the n of one lm together.
 // SCALE: each u[., lm] is multiplied by the harmonic's normalisation
 // nrm_lm[lm] = N_l|m|, the form adjoint_pair_fixed reads.
-template <typename T, int LFIX, bool SCALE>
+// ptab[c nr + c2] = p(c, c2) (L+1): the feature offset of a channel pair.
+template <typename T, int LFIX, bool SCALE, int NFIX = -1>
 __device__ inline void u_mfma(const B2Dev &p, const T *A, const T *w,
-                              const T *nrm_lm, T *u2) {
+                              const T *nrm_lm, const int *ptab, T *u2) {
   typedef typename MmaF<T>::V V;
   const int lane = threadIdx.x;
   const int col = lane & 15, g = lane >> 4;
   const int H = LFIX >= 0 ? (LFIX + 1) * (LFIX + 1) : p.H;
   const int L1 = LFIX >= 0 ? LFIX + 1 : p.L + 1;
-  const int N = p.N, nr = p.nr;
+  const int N = NFIX > 0 ? NFIX : p.N, nr = p.nr;
   const int MT = (nr + 15) >> 4, KS = (nr + 3) >> 2;
+  for (int mt = 0; mt < MT; mt++) {
+    const int c = mt * 16 + col;
+    // destination (s H) N + n of this lane's four result rows
+    int dst[4];
 #pragma unroll
-  for (int l = 0; l < L1; l++) {
-    const int nl = 2 * l + 1, l2 = l * l;
-    for (int nt = 0; nt * 16 < nl; nt++) {
-      const int j = nt * 16 + col;
-      for (int mt = 0; mt < MT; mt++) {
+    for (int q = 0; q < 4; q++) {
+      const int cr = mt * 16 + MmaF<T>::row(g, q);
+      const int s = cr / N, n = cr - s * N;
+      dst[q] = cr < nr ? s * H * N + n : -1;
+    }
+#pragma unroll 1
+    for (int l = 0; l < L1; l++) {
+      const int nl = 2 * l + 1, l2 = l * l;
+      for (int nt = 0; nt * 16 < nl; nt++) {
+        const int j = nt * 16 + col;
         V acc = V{0, 0, 0, 0};
-        const int c = mt * 16 + col;
         for (int ks = 0; ks < KS; ks++) {
           const int c2 = 4 * ks + g;
           T a = 0, b = 0;
           if (c < nr && c2 < nr) {
-            T wv = w[b2_pair_index(c, c2, nr) * L1 + l];
-            a = (c == c2) ? 2 * wv : wv;
+            const T wv = w[ptab[c * nr + c2] + l];
+            a = c == c2 ? 2 * wv : wv;
           }
           if (c2 < nr && j < nl) b = A[c2 * H + l2 + j];
           acc = MmaF<T>::run(a, b, acc);
         }
+        if (j < nl) {
+          const T sc = SCALE ? nrm_lm[l2 + j] : T(1);
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-          const int cr = mt * 16 + MmaF<T>::row(g, q);
-          if (cr < nr && j < nl) {
-            const int s = cr / N, n = cr - (cr / N) * N;
-            T val = acc[q];
-            if (SCALE) val *= nrm_lm[l2 + j];
-            u2[(s * H + l2 + j) * N + n] = val;
-          }
+          for (int q = 0; q < 4; q++)
+            if (dst[q] >= 0) u2[dst[q] + (l2 + j) * N] = acc[q] * sc;
         }
       }
     }
@@ -279,7 +284,8 @@ __device__ inline void adjoint_pair_fixed(const B2Dev &p, T r, T x, T y, T z,
 
 }  // namespace
 
-// One wavefront per atom.  Dynamic LDS (T): A[W], w[F], u[W], x[F], n[H].
+// One wavefront per atom.  Dynamic LDS (T): A[W], w[F], u[W], x[F], n[H],
+// then int ptab[nr * nr].
 // WM 3 is WM 1 with X given as BF16 hi + lo rows (the GEMM's operand).
 // WM selects where w = dE_i/dB_i comes from:
 //   0  precomputed (sparse-GP path);
@@ -303,6 +309,7 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
   T *u = w + F;
   T *xs = u + W;        // X row (rebuilt from hi + lo when BF16)
   T *nrm_lm = xs + F;   // harmonic normalisation per lm
+  int *ptab = reinterpret_cast<int *>(nrm_lm + H);  // nr x nr feature offsets
   const T *Ai = A_in + (int64_t)i * W;
   const int row = row_of[i];
   constexpr bool kFixed = MODE == 1 && LFIX >= 0 && EXACT && NMAX == 8;
@@ -321,6 +328,10 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
     while ((l + 1) * (l + 1) <= lane) l++;
     const int m = lane - l * l - l;
     nrm_lm[lane] = ynorm[l * (l + 1) / 2 + (m < 0 ? -m : m)];
+  }
+  for (int q = lane; q < p.nr * p.nr; q += kWave) {
+    const int c = q / p.nr, c2 = q - c * p.nr;
+    ptab[q] = b2_pair_index(c, c2, p.nr) * (p.L + 1);
   }
   stage_lds<8>(W, [&](int o) { return Ai[o]; }, A);
   if (WM == 0) {
@@ -390,7 +401,7 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
     }
   }
   __syncthreads();
-  u_mfma<T, LFIX, kFixed>(p, A, w, nrm_lm, u);
+  u_mfma<T, LFIX, kFixed, kFixed ? 8 : -1>(p, A, w, nrm_lm, ptab, u);
   __syncthreads();
   const int si = species[i];
   if (MODE == 0) {
@@ -435,14 +446,13 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
       }
     }
   } else {
-    // the next chunk's pair data is loaded one iteration ahead (the first
-    // chunk before the prologue)
+    // the first chunk's pair data was loaded before the prologue
     for (int k = b + lane; k < e; k += kWave) {
-      const double4 rv = rv_next;
-      const int sj = sj_next;
-      if (k + kWave < e) {
-        rv_next = reinterpret_cast<const double4 *>(nl.rel)[k + kWave];
-        sj_next = nl.nspec[k + kWave];
+      double4 rv = rv_next;
+      int sj = sj_next;
+      if (k != b + lane) {
+        rv = reinterpret_cast<const double4 *>(nl.rel)[k];
+        sj = nl.nspec[k];
       }
       const double rva[4] = {rv.x, rv.y, rv.z, rv.w};
       double f[3] = {0, 0, 0};
@@ -624,7 +634,8 @@ static void launch_forces_t(hipStream_t st, const B2Dev &p, const NListDev &nl,
                             const int *species, const int *row_of,
                             const T *ynorm, const void *A, const WSource<T> &src,
                             const double *dA, double *pair_f) {
-  size_t lds = sizeof(T) * (2 * (size_t)p.W + 2 * (size_t)p.F + (size_t)p.H);
+  size_t lds = sizeof(T) * (2 * (size_t)p.W + 2 * (size_t)p.F + (size_t)p.H) +
+               sizeof(int) * (size_t)p.nr * p.nr;
   FLARE_REQUIRE(lds <= 64 * 1024, "descriptor settings too large for the force kernel");
   FLARE_REQUIRE(p.nr <= 32, "at most 32 radial channels (n_species * n_max)");
 #define FLARE_FORCE_LAUNCH(NM, LF, EX)                                        \
