@@ -549,6 +549,14 @@ struct flare_sgp {
   std::vector<int> m_by_type, m_off;
   DevBuf P, Phat, pnorm, pnorm2, alpha, Kuu, Lcm, Linv_cm;
   bool have_alpha = false;
+  // low-precision copies of the normalised sparse rows, built on first use:
+  // FP32 rows; BF16 hi/lo rows of kp; per species the BF16 hi/lo transpose
+  // (F rows of mp[s], the species' count padded to 32) at tpoff[s]
+  bool have_lowp = false;
+  int kp = 0;
+  std::vector<int> mp;
+  std::vector<int64_t> tpoff;
+  DevBuf Phat_f, Phat_h, Phat_l, PhatT_h, PhatT_l;
 };
 
 // ===========================================================================
@@ -1074,15 +1082,13 @@ int flare_pair_compute(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
     if (pr->power == 2) {
       Z = ctx->buf("pair.Z", std::max<size_t>(1, (size_t)n * F) * ts);
       if (mixed) {  // all species in one launch
-        int M[flare::kMaxSpecies];
-        int64_t a0[flare::kMaxSpecies], b0[flare::kMaxSpecies];
-        for (int s = 0; s < p.S; s++) {
-          M[s] = off[s + 1] - off[s];
-          a0[s] = off[s];
-          b0[s] = (int64_t)s * F;
-        }
-        flare::gemm_bf16x3_batched(ctx->stream, p.S, M, a0, b0, F, kp, fw.xh,
-                                   fw.xl, bh, bl, (float *)Z, F);
+        flare::GemmProduct g[flare::kMaxSpecies];
+        for (int s = 0; s < p.S; s++)
+          g[s] = flare::GemmProduct{off[s + 1] - off[s], F, kp, F,
+                                    (int64_t)off[s] * kp, (int64_t)s * F * kp,
+                                    (int64_t)off[s] * F};
+        flare::gemm_bf16x3_batched(ctx->stream, p.S, g, fw.xh, fw.xl, bh, bl,
+                                   (float *)Z);
       }
       for (int s = 0; s < p.S && !mixed; s++) {
         int rows = off[s + 1] - off[s];
@@ -1364,17 +1370,162 @@ struct GPState {
   Forward fw;
   double *kval = nullptr;  // per species blocks rows_s x M_s
   std::vector<size_t> koff;
+  bool lowp = false;
 };
+
+static void sgp_build_lowp(flare_ctx *ctx, flare_sgp *gp) {
+  if (gp->have_lowp) return;
+  const int M = gp->M, F = gp->F, S = gp->S;
+  gp->kp = flare::bf16_padded_k(F);
+  gp->mp.assign(S, 0);
+  gp->tpoff.assign(S + 1, 0);
+  for (int s = 0; s < S; s++) {
+    gp->mp[s] = flare::bf16_padded_k(std::max(gp->m_by_type[s], 1));
+    gp->tpoff[s + 1] = gp->tpoff[s] + (int64_t)F * gp->mp[s];
+  }
+  gp->Phat_f.ensure((size_t)std::max(M, 1) * F * sizeof(float));
+  gp->Phat_h.ensure((size_t)std::max(M, 1) * gp->kp * 2);
+  gp->Phat_l.ensure((size_t)std::max(M, 1) * gp->kp * 2);
+  gp->PhatT_h.ensure((size_t)std::max<int64_t>(gp->tpoff[S], 1) * 2);
+  gp->PhatT_l.ensure((size_t)std::max<int64_t>(gp->tpoff[S], 1) * 2);
+  if (M) {
+    flare::launch_to_f32(ctx->stream, (int64_t)M * F, gp->Phat.as<double>(),
+                         gp->Phat_f.as<float>());
+    flare::split_bf16(ctx->stream, M, F, gp->kp, gp->Phat.as<double>(),
+                      gp->Phat_h.ptr, gp->Phat_l.ptr);
+  }
+  for (int s = 0; s < S; s++)
+    flare::launch_split_bf16_t(
+        ctx->stream, gp->m_by_type[s], F, gp->mp[s],
+        gp->Phat.as<double>() + (size_t)gp->m_off[s] * F,
+        (__bf16 *)gp->PhatT_h.ptr + gp->tpoff[s], (__bf16 *)gp->PhatT_l.ptr + gp->tpoff[s]);
+  gp->have_lowp = true;
+}
+
+// FP32 / mixed forms of the GP mean: normalised FP32 rows d, t = d . p_hat
+// in FP32 (sgemm) or on BF16 MFMA with hi+lo splitting, G = c P_hat the same
+// way, k, c, e, w in FP64 from t, forces from the FP32 adjoint kernel.
+static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p,
+                            const flare_nlist *nl, const int *d_species, int form,
+                            double *d_mean_efs) {
+  const bool mixed = form == FLARE_FORM_ADJOINT_MIXED;
+  const int n = nl->n, F = p.F;
+  sgp_build_lowp(ctx, gp);
+  GPState st;
+  st.lowp = true;
+  st.fw = run_forward(ctx, p, nl, d_species, flare::kPrecF32, "gpl", nullptr, 1,
+                      mixed);
+  const float *d = (const float *)st.fw.B;
+  int off[flare::kMaxSpecies + 1];
+  nl->row_offsets(p.S, off);
+  size_t ktot = 0, ctot = 0;
+  st.koff.assign(p.S + 1, 0);
+  std::vector<size_t> coff(p.S + 1, 0);
+  for (int s = 0; s < p.S; s++) {
+    st.koff[s] = ktot;
+    coff[s] = ctot;
+    ktot += (size_t)(off[s + 1] - off[s]) * gp->m_by_type[s];
+    ctot += (size_t)(off[s + 1] - off[s]) * gp->mp[s];
+  }
+  st.koff[p.S] = ktot;
+  coff[p.S] = ctot;
+  float *Kt = ctx->buf_t<float>("gpl.K", std::max<size_t>(ktot, 1));
+  st.kval = ctx->buf_t<double>("gpl.kval", std::max<size_t>(ktot, 1));
+  float *G = ctx->buf_t<float>("gpl.G", std::max<size_t>((size_t)n * F, 1));
+  double *e_row = ctx->buf_t<double>("gpl.e", std::max(n, 1));
+  float *cf = nullptr;
+  void *ch = nullptr, *cl = nullptr;
+  if (mixed) {
+    ch = ctx->buf("gpl.ch", std::max<size_t>(ctot, 1) * 2);
+    cl = ctx->buf("gpl.cl", std::max<size_t>(ctot, 1) * 2);
+  } else {
+    cf = ctx->buf_t<float>("gpl.c", std::max<size_t>(ktot, 1));
+  }
+  ctx->mark_begin("gemm");
+  if (mixed) {
+    flare::GemmProduct g[flare::kMaxSpecies];
+    for (int s = 0; s < p.S; s++)
+      g[s] = flare::GemmProduct{off[s + 1] - off[s], gp->m_by_type[s], gp->kp,
+                                gp->m_by_type[s], (int64_t)off[s] * gp->kp,
+                                (int64_t)gp->m_off[s] * gp->kp, (int64_t)st.koff[s]};
+    flare::gemm_bf16x3_batched(ctx->stream, p.S, g, st.fw.xh, st.fw.xl,
+                               gp->Phat_h.ptr, gp->Phat_l.ptr, Kt);
+  } else {
+    for (int s = 0; s < p.S; s++) {
+      const int rows = off[s + 1] - off[s], Ms = gp->m_by_type[s];
+      if (!rows || !Ms) continue;
+      sgemm_rm(ctx, false, true, rows, Ms, F, 1.0f, d + (size_t)off[s] * F, F,
+               gp->Phat_f.as<float>() + (size_t)gp->m_off[s] * F, F, 0.0f,
+               Kt + st.koff[s], Ms);
+    }
+  }
+  ctx->mark_end();
+  ctx->mark_begin("gp_epilogue");
+  for (int s = 0; s < p.S; s++) {
+    const int rows = off[s + 1] - off[s], Ms = gp->m_by_type[s], o = gp->m_off[s];
+    if (!rows) continue;
+    if (!Ms) {
+      FLARE_HIP_CHECK(hipMemsetAsync(e_row + off[s], 0, rows * sizeof(double), ctx->stream));
+      if (mixed) {
+        FLARE_HIP_CHECK(hipMemsetAsync((__bf16 *)ch + coff[s], 0,
+                                       (size_t)rows * gp->mp[s] * 2, ctx->stream));
+        FLARE_HIP_CHECK(hipMemsetAsync((__bf16 *)cl + coff[s], 0,
+                                       (size_t)rows * gp->mp[s] * 2, ctx->stream));
+      }
+      continue;
+    }
+    flare::launch_gp_epilogue_lp(
+        ctx->stream, rows, Ms, gp->mp[s], Kt + st.koff[s], st.fw.norm2 + off[s],
+        gp->pnorm.as<double>() + o, gp->alpha.as<double>() + o, gp->sig2,
+        gp->power, st.kval + st.koff[s], mixed ? nullptr : cf + st.koff[s],
+        mixed ? (void *)((__bf16 *)ch + coff[s]) : nullptr,
+        mixed ? (void *)((__bf16 *)cl + coff[s]) : nullptr, e_row + off[s]);
+  }
+  ctx->mark_end();
+  ctx->mark_begin("gemm");
+  if (mixed) {
+    flare::GemmProduct g[flare::kMaxSpecies];
+    for (int s = 0; s < p.S; s++)
+      g[s] = flare::GemmProduct{off[s + 1] - off[s], F, gp->mp[s], F,
+                                (int64_t)coff[s], gp->tpoff[s], (int64_t)off[s] * F};
+    flare::gemm_bf16x3_batched(ctx->stream, p.S, g, ch, cl, gp->PhatT_h.ptr,
+                               gp->PhatT_l.ptr, G);
+  } else {
+    for (int s = 0; s < p.S; s++) {
+      const int rows = off[s + 1] - off[s], Ms = gp->m_by_type[s];
+      if (!rows) continue;
+      if (!Ms) {
+        FLARE_HIP_CHECK(hipMemsetAsync(G + (size_t)off[s] * F, 0,
+                                       (size_t)rows * F * sizeof(float), ctx->stream));
+        continue;
+      }
+      sgemm_rm(ctx, false, false, rows, F, Ms, 1.0f, cf + st.koff[s], Ms,
+               gp->Phat_f.as<float>() + (size_t)gp->m_off[s] * F, F, 0.0f,
+               G + (size_t)off[s] * F, F);
+    }
+  }
+  ctx->mark_end();
+  float *w = ctx->buf_t<float>("gpl.w", std::max<size_t>((size_t)n * F, 1));
+  double *E_row = ctx->buf_t<double>("gpl.E", std::max(n, 1));
+  flare::launch_gp_w_lp(ctx->stream, n, F, G, e_row, d, st.fw.norm2, gp->sig2,
+                        gp->power, w, E_row);
+  flare::WInput wi{0, 1, nullptr, nullptr, nullptr, w, st.fw.norm2, E_row};
+  run_forces(ctx, p, nl, d_species, flare::kPrecF32, 1, st.fw, wi, d_mean_efs + 1,
+             d_mean_efs, nullptr, d_mean_efs + 1 + 3 * (size_t)n, "gpl");
+  return st;
+}
 
 static GPState gp_mean(flare_ctx *ctx, flare_sgp *gp, const flare_b2_params *params,
                        const flare_nlist *nl, const int *d_species, int form,
                        double *d_mean_efs, bool want_k) {
-  REQUIRE(form == FLARE_FORM_PLAIN_FP64 || form == FLARE_FORM_ADJOINT_FP64,
-          "the sparse-GP path runs the FP64 forms");
+  REQUIRE(form >= FLARE_FORM_PLAIN_FP64 && form <= FLARE_FORM_ADJOINT_MIXED,
+          "unknown form");
   flare::B2Dev p = make_b2(params, 0);
   check_species(nl, p.S);
   REQUIRE(p.S == gp->S && p.F == gp->F, "descriptor settings differ from the sparse set");
   REQUIRE(gp->have_alpha, "alpha not set");
+  if (form == FLARE_FORM_ADJOINT_FP32 || form == FLARE_FORM_ADJOINT_MIXED)
+    return gp_mean_lowp(ctx, gp, p, nl, d_species, form, d_mean_efs);
   const int n = nl->n, F = p.F;
   GPState st;
   st.fw = run_forward(ctx, p, nl, d_species, flare::kPrecF64, "gp");
@@ -1450,8 +1601,12 @@ int flare_sgp_predict_local_uncertainties(flare_ctx *ctx, flare_sgp *gp,
     int off[flare::kMaxSpecies + 1];
     nl->row_offsets(gp->S, off);
     double *kself = ctx->buf_t<double>("gp.kself", n);
-    flare::launch_kself(ctx->stream, n, F, (const double *)st.fw.B, st.fw.norm2,
-                        gp->sig2, gp->power, kself);
+    if (st.lowp)
+      flare::launch_kself_lp(ctx->stream, n, F, (const float *)st.fw.B, st.fw.norm2,
+                             gp->sig2, gp->power, kself);
+    else
+      flare::launch_kself(ctx->stream, n, F, (const double *)st.fw.B, st.fw.norm2,
+                          gp->sig2, gp->power, kself);
     double *Q = ctx->buf_t<double>("gp.Q", st.koff[gp->S]);
     for (int s = 0; s < gp->S; s++) {
       int rows = off[s + 1] - off[s], Ms = gp->m_by_type[s], o = gp->m_off[s];

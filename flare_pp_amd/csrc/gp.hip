@@ -200,6 +200,98 @@ __global__ void unpack_beta_kernel(int F, const double *__restrict__ packed,
   }
 }
 
+// Low-precision forms.  K holds t = d_i . p_hat_u directly (FP32, from the
+// FP32 or BF16x3 product of normalised rows); everything derived from it is
+// formed in FP64: k = sig2 t^xi (kept in FP64 for the variance), the
+// adjoint weights c = alpha t^(xi-1) (written FP32, or as BF16 hi + lo rows
+// of Mp for the matrix-core product) and e = sum alpha t^xi.
+__global__ void __launch_bounds__(64)
+gp_epilogue_lp_kernel(int rows, int M, int Mp, const float *__restrict__ K,
+                      const double *__restrict__ norm2_rows,
+                      const double *__restrict__ pnorm,
+                      const double *__restrict__ alpha, double sig2,
+                      double power, double *__restrict__ kval,
+                      float *__restrict__ cf, __bf16 *__restrict__ ch,
+                      __bf16 *__restrict__ cl, double *__restrict__ e_row) {
+  const int r = blockIdx.x;
+  const int lane = threadIdx.x;
+  const bool empty = sqrt(norm2_rows[r]) < 1e-8;
+  double e = 0;
+  const int Mw = ch ? Mp : M;
+  for (int u = lane; u < Mw; u += kWave) {
+    double kv = 0, cv = 0;
+    if (u < M && !empty && !(pnorm[u] < 1e-8)) {
+      const double t = (double)K[(int64_t)r * M + u];
+      const double tx = powi(t, power);
+      kv = sig2 * tx;
+      cv = alpha[u] * powi(t, power - 1.0);
+      e += alpha[u] * tx;
+    }
+    if (u < M && kval) kval[(int64_t)r * M + u] = kv;
+    if (cf && u < M) cf[(int64_t)r * M + u] = (float)cv;
+    if (ch) {
+      const float x = (float)cv;
+      const __bf16 h = (__bf16)x;
+      ch[(int64_t)r * Mp + u] = h;
+      cl[(int64_t)r * Mp + u] = (__bf16)(x - (float)h);
+    }
+  }
+  e = wave_sum(e);
+  if (lane == 0) e_row[r] = e;
+}
+
+// w = sig2 xi / |B| (G - e d) with d = B/|B| the FP32 row and G = c P_hat.
+__global__ void __launch_bounds__(64)
+gp_w_lp_kernel(int rows, int F, const float *__restrict__ G,
+               const double *__restrict__ e_row, const float *__restrict__ d,
+               const double *__restrict__ norm2, double sig2, double power,
+               float *__restrict__ w, double *__restrict__ E_row) {
+  const int r = blockIdx.x;
+  const int lane = threadIdx.x;
+  const double nb = sqrt(norm2[r]);
+  const bool empty = nb < 1e-8;
+  const double e = e_row[r];
+  const double s = empty ? 0.0 : sig2 * power / nb;
+  for (int f = lane; f < F; f += kWave) {
+    double v = 0;
+    if (!empty)
+      v = s * ((double)G[(int64_t)r * F + f] - e * (double)d[(int64_t)r * F + f]);
+    w[(int64_t)r * F + f] = (float)v;
+  }
+  if (lane == 0) E_row[r] = empty ? 0.0 : sig2 * e;
+}
+
+// k(i, i) = sig2 (d.d)^xi from the FP32 normalised row
+__global__ void __launch_bounds__(64)
+kself_lp_kernel(int rows, int F, const float *__restrict__ d,
+                const double *__restrict__ norm2, double sig2, double power,
+                double *__restrict__ kself) {
+  const int r = blockIdx.x;
+  const int lane = threadIdx.x;
+  double s = 0;
+  for (int f = lane; f < F; f += kWave) {
+    const double x = (double)d[(int64_t)r * F + f];
+    s += x * x;
+  }
+  s = wave_sum(s);
+  if (lane == 0) kself[r] = sqrt(norm2[r]) < 1e-8 ? 0.0 : sig2 * powi(s, power);
+}
+
+// BF16 hi/lo split of the transpose of a (rows x cols) FP64 block:
+// out[c * ldo + r] for c < cols, r < ldo (zero for r >= rows)
+__global__ void split_bf16_t_kernel(int rows, int cols, int ldo,
+                                    const double *__restrict__ x,
+                                    __bf16 *__restrict__ hi,
+                                    __bf16 *__restrict__ lo) {
+  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)cols * ldo) return;
+  const int c = (int)(q / ldo), r = (int)(q - (int64_t)c * ldo);
+  const float v = r < rows ? (float)x[(int64_t)r * cols + c] : 0.0f;
+  const __bf16 h = (__bf16)v;
+  hi[q] = h;
+  lo[q] = (__bf16)(v - (float)h);
+}
+
 __global__ void to_f32_kernel(int64_t n, const double *__restrict__ x,
                               float *__restrict__ y) {
   int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -298,6 +390,45 @@ void launch_unpack_beta(hipStream_t st, int F, int power, const double *packed,
   }
   hipLaunchKernelGGL(unpack_beta_kernel, dim3(nblk(F, 64)), dim3(64), 0, st, F,
                      packed, full);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_gp_epilogue_lp(hipStream_t st, int rows, int M, int Mp,
+                           const float *K, const double *norm2_rows,
+                           const double *pnorm, const double *alpha, double sig2,
+                           double power, double *kval, float *cf, void *ch,
+                           void *cl, double *e_row) {
+  if (rows == 0) return;
+  hipLaunchKernelGGL(gp_epilogue_lp_kernel, dim3(rows), dim3(kWave), 0, st, rows,
+                     M, Mp, K, norm2_rows, pnorm, alpha, sig2, power, kval, cf,
+                     (__bf16 *)ch, (__bf16 *)cl, e_row);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_gp_w_lp(hipStream_t st, int rows, int F, const float *G,
+                    const double *e_row, const float *d, const double *norm2,
+                    double sig2, double power, float *w, double *E_row) {
+  if (rows == 0) return;
+  hipLaunchKernelGGL(gp_w_lp_kernel, dim3(rows), dim3(kWave), 0, st, rows, F, G,
+                     e_row, d, norm2, sig2, power, w, E_row);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_kself_lp(hipStream_t st, int rows, int F, const float *d,
+                     const double *norm2, double sig2, double power,
+                     double *kself) {
+  if (rows == 0) return;
+  hipLaunchKernelGGL(kself_lp_kernel, dim3(rows), dim3(kWave), 0, st, rows, F, d,
+                     norm2, sig2, power, kself);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_split_bf16_t(hipStream_t st, int rows, int cols, int ldo,
+                         const double *x, void *hi, void *lo) {
+  if (cols == 0 || ldo == 0) return;
+  const int64_t n = (int64_t)cols * ldo;
+  hipLaunchKernelGGL(split_bf16_t_kernel, dim3(nblk(n, 256)), dim3(256), 0, st,
+                     rows, cols, ldo, x, (__bf16 *)hi, (__bf16 *)lo);
   FLARE_HIP_CHECK(hipGetLastError());
 }
 

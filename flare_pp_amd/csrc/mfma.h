@@ -13,16 +13,21 @@ int bf16_padded_k(int K);
 
 constexpr int kGemmMaxBatch = 8;
 
-// For q < nb independent products (one launch):
-//   Z[a_row0[q] + i][:] = (Ah + Al)[a_row0[q] + i] . (Bh + Bl)[b_row0[q] + :]^T
-// for i < M[q], N columns, FP32 result with leading dimension ldz, on BF16
-// MFMA with FP32 accumulation, three products hi*hi + hi*lo + lo*hi.
-// A and B rows have Kp BF16 values (a multiple of bf16_padded_k's step, the
-// padding zero); row n of a symmetric right operand is its column n.
-void gemm_bf16x3_batched(hipStream_t st, int nb, const int *M,
-                         const int64_t *a_row0, const int64_t *b_row0, int N,
-                         int Kp, const void *Ah, const void *Al, const void *Bh,
-                         const void *Bl, float *Z, int ldz);
+// One product Z = (Ah + Al) . (Bh + Bl)^T: A has M rows and B has N rows of
+// K BF16 values (K a multiple of bf16_padded_k's step, the padding zero),
+// starting at element offsets a_off / b_off; Z (FP32) starts at z_off with
+// leading dimension ldz.  Row n of a symmetric right operand is its
+// column n.
+struct GemmProduct {
+  int M, N, K, ldz;
+  int64_t a_off, b_off, z_off;
+};
+
+// Up to kGemmMaxBatch products in one launch on BF16 MFMA with FP32
+// accumulation, three products hi*hi + hi*lo + lo*hi each.
+void gemm_bf16x3_batched(hipStream_t st, int nb, const GemmProduct *prods,
+                         const void *Ah, const void *Al, const void *Bh,
+                         const void *Bl, float *Z);
 
 // hi/lo BF16 split of a rows x cols FP64 matrix into rows of kp (zero padded)
 void split_bf16(hipStream_t st, int rows, int cols, int kp, const double *x,

@@ -24,13 +24,11 @@ namespace flare {
 
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
 
-// Up to kGemmMaxBatch independent products Z_q = A_q B_q^T sharing N and K:
-// rows a_row0[q] .. of A and Z, rows b_row0[q] .. of B; tile0 is the
-// exclusive prefix of the tile counts.
+// Up to kGemmMaxBatch independent products (the GemmProduct list) and the
+// exclusive prefix of their tile counts.
 struct GemmBatch {
   int nb;
-  int M[kGemmMaxBatch];
-  int64_t a_row0[kGemmMaxBatch], b_row0[kGemmMaxBatch];
+  GemmProduct g[kGemmMaxBatch];
   int tile0[kGemmMaxBatch + 1];
 };
 typedef __attribute__((ext_vector_type(4))) float floatx4;
@@ -121,28 +119,28 @@ __device__ inline void tile_of_block(int n_row_tiles, int n_col_tiles, int &rt,
 }
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
-gemm_bf16x3_kernel(GemmBatch bt, int N, int Kp, const __bf16 *__restrict__ Ah0,
+gemm_bf16x3_kernel(GemmBatch bt, const __bf16 *__restrict__ Ah0,
                    const __bf16 *__restrict__ Al0, const __bf16 *__restrict__ Bh0,
-                   const __bf16 *__restrict__ Bl0, float *__restrict__ Z0,
-                   int ldz) {
+                   const __bf16 *__restrict__ Bl0, float *__restrict__ Z0) {
   __shared__ __align__(16) __bf16 sAh[kBM][kLdk], sAl[kBM][kLdk];
   __shared__ __align__(16) __bf16 sBh[kBN][kLdk], sBl[kBN][kLdk];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int wm = (wv >> 1) * 64, wn = (wv & 1) * 128;
-  const int nct = (N + kBN - 1) / kBN;
   // global tile id over all products (column tiles of a row block adjacent)
   int T, dummy;
   tile_of_block(bt.tile0[bt.nb], 1, T, dummy);
   if (T >= bt.tile0[bt.nb]) return;
   int q = 0;
   while (T >= bt.tile0[q + 1]) q++;
+  const GemmProduct &gp = bt.g[q];
+  const int M = gp.M, N = gp.N, Kp = gp.K, ldz = gp.ldz;
+  const int nct = (N + kBN - 1) / kBN;
   const int local = T - bt.tile0[q];
   const int rt = local / nct, ct = local - rt * nct;
-  const int M = bt.M[q];
-  const __bf16 *Ah = Ah0 + bt.a_row0[q] * Kp, *Al = Al0 + bt.a_row0[q] * Kp;
-  const __bf16 *Bh = Bh0 + bt.b_row0[q] * Kp, *Bl = Bl0 + bt.b_row0[q] * Kp;
-  float *Z = Z0 + bt.a_row0[q] * ldz;
+  const __bf16 *Ah = Ah0 + gp.a_off, *Al = Al0 + gp.a_off;
+  const __bf16 *Bh = Bh0 + gp.b_off, *Bl = Bl0 + gp.b_off;
+  float *Z = Z0 + gp.z_off;
   const int m0 = rt * kBM, n0 = ct * kBN;
   floatx4 acc[4][8];
 #pragma unroll
@@ -210,31 +208,29 @@ __global__ void split_bf16_kernel(int rows, int cols, int kp,
 
 int bf16_padded_k(int K) { return (K + kBK - 1) / kBK * kBK; }
 
-void gemm_bf16x3_batched(hipStream_t st, int nb, const int *M,
-                         const int64_t *a_row0, const int64_t *b_row0, int N,
-                         int Kp, const void *Ah, const void *Al, const void *Bh,
-                         const void *Bl, float *Z, int ldz) {
-  FLARE_REQUIRE(nb >= 1 && nb <= kGemmMaxBatch, "gemm_bf16x3: 1 to 8 products");
-  FLARE_REQUIRE(Kp % kBK == 0, "gemm_bf16x3: K must be padded to 32");
-  if (N == 0) return;
+void gemm_bf16x3_batched(hipStream_t st, int nb, const GemmProduct *prods,
+                         const void *Ah, const void *Al, const void *Bh,
+                         const void *Bl, float *Z) {
+  FLARE_REQUIRE(nb >= 0 && nb <= kGemmMaxBatch, "gemm_bf16x3: at most 8 products");
   GemmBatch bt;
-  bt.nb = nb;
-  const int nct = (N + kBN - 1) / kBN;
+  bt.nb = 0;
   int tiles = 0;
   for (int q = 0; q < nb; q++) {
-    bt.M[q] = M[q];
-    bt.a_row0[q] = a_row0[q];
-    bt.b_row0[q] = b_row0[q];
-    bt.tile0[q] = tiles;
-    tiles += (M[q] + kBM - 1) / kBM * nct;
+    const GemmProduct &g = prods[q];
+    FLARE_REQUIRE(g.K % kBK == 0, "gemm_bf16x3: K must be padded to 32");
+    if (g.M <= 0 || g.N <= 0) continue;
+    bt.g[bt.nb] = g;
+    bt.tile0[bt.nb] = tiles;
+    tiles += (g.M + kBM - 1) / kBM * ((g.N + kBN - 1) / kBN);
+    bt.nb++;
   }
-  bt.tile0[nb] = tiles;
+  bt.tile0[bt.nb] = tiles;
   if (tiles == 0) return;
   // round up to a multiple of 8 so every XCD gets whole slots
   const int grid = (tiles + 7) / 8 * 8;
-  hipLaunchKernelGGL(gemm_bf16x3_kernel, dim3(grid), dim3(256), 0, st, bt, N, Kp,
+  hipLaunchKernelGGL(gemm_bf16x3_kernel, dim3(grid), dim3(256), 0, st, bt,
                      (const __bf16 *)Ah, (const __bf16 *)Al, (const __bf16 *)Bh,
-                     (const __bf16 *)Bl, Z, ldz);
+                     (const __bf16 *)Bl, Z);
   FLARE_HIP_CHECK(hipGetLastError());
 }
 

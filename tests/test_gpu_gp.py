@@ -122,3 +122,79 @@ def test_config3_mean(orc):
     e_plain = gp.predict_mean(s, form="plain").copy()
     e_adj = gp.predict_mean(s, form="adjoint_fp64")
     assert nrel(e_adj, e_plain) < 1e-11
+
+
+# Low-precision forms.  FP32: t = d . p_hat in FP32; mixed: t and G = c P_hat
+# on BF16 MFMA with hi + lo splitting (FP32 accumulation).  k, c, e, w, the
+# Cholesky factor and the variance step stay FP64.  Tolerances below are the
+# stated error budget of each form; observed errors are printed.
+LOWP_TOL = {
+    # form: (energy rel, force rel to max |F|, variance abs / sigma^2).  The
+    # variance k_ii - |L^-1 k_i|^2 cancels: an error dt in t = d . p_hat
+    # reaches it multiplied by |L^-1|, so its budget is the loosest.
+    "adjoint_fp32": (1e-5, 1e-4, 5e-3),
+    "adjoint_mixed": (1e-4, 1e-3, 5e-2),
+}
+
+
+@pytest.mark.parametrize("form", ["adjoint_fp32", "adjoint_mixed"])
+def test_config1_lowp_forms(orc, cfg1, form):
+    c = cfg1
+    s, gp, b2 = build(c["cell"], c["species"], c["positions"], c["settings"], 5.0,
+                      c["sigma"], 2, c["sparse_atoms"], c["alpha"])
+    o, g, P, Pn, m = oracle_side(orc, c["cell"], c["species"], c["positions"],
+                                 c["settings"], 5.0, c["sparse_atoms"])
+    V = orc.volume(c["cell"])
+    want = orc.predict_mean(o, m, P, Pn, c["sigma"], 2, V, c["alpha"])
+    Li_o = orc.cholesky_inverse(orc.envs_envs(1, m, P, Pn, m, P, Pn, c["sigma"], 2),
+                                1e-8)
+    var_o = orc.cluster_uncertainties(o, m, P, Pn, c["sigma"], 2, Li_o)
+    n = s.noa
+    efs, (var,) = gp.predict_local_uncertainties(s, form=form)
+    e_err = abs(efs[0] - want[0]) / abs(want[0])
+    f_err = nrel(efs[1:1 + 3 * n], want[1:1 + 3 * n])
+    v_err = np.abs(var - var_o).max() / c["sigma"] ** 2
+    print(f"{form}: energy rel {e_err:.3e}, force rel {f_err:.3e}, "
+          f"variance {v_err:.3e} sigma^2", flush=True)
+    tol = LOWP_TOL[form]
+    assert e_err < tol[0] and f_err < tol[1] and v_err < tol[2]
+
+
+@pytest.mark.parametrize("form", ["adjoint_fp32", "adjoint_mixed"])
+def test_config3_lowp_mean(form):
+    # 32,000 atoms, 500 sparse: the low-precision forms against the FP64 GPU
+    # form (itself pinned to the oracle above); errors in meV/atom, meV/A
+    c = synth.config3()
+    s, gp, b2 = build(c["cell"], c["species"], c["positions"], c["settings"], 5.0,
+                      c["sigma"], c["power"], c["sparse_atoms"], c["alpha"])
+    n = s.noa
+    ref = gp.predict_mean(s, form="adjoint_fp64").copy()
+    efs = gp.predict_mean(s, form=form)
+    de = abs(efs[0] - ref[0]) / n * 1e3
+    df = (efs[1:1 + 3 * n] - ref[1:1 + 3 * n]) * 1e3
+    print(f"{form}: energy {de:.3e} meV/atom, force RMS {np.sqrt(np.mean(df**2)):.3e}"
+          f" max {np.abs(df).max():.3e} meV/A (|F| max "
+          f"{np.abs(ref[1:1 + 3 * n]).max() * 1e3:.3e})", flush=True)
+    assert np.sqrt(np.mean(df ** 2)) < 1.0  # meV/A
+    assert nrel(efs[1:1 + 3 * n], ref[1:1 + 3 * n]) < LOWP_TOL[form][1]
+
+
+def test_config4_variance_mixed():
+    # 55,296 atoms, 3 species, 2,000 sparse: BF16x3 kernel matrix against the
+    # FP64 path; relative variance error
+    c = synth.config4()
+    s, gp, b2 = build(c["cell"], c["species"], c["positions"], c["settings"],
+                      c["cutoff"], c["sigma"], c["power"], c["sparse_atoms"],
+                      c["alpha"])
+    efs64, (var64,) = gp.predict_local_uncertainties(s, form="adjoint_fp64")
+    efs64 = efs64.copy()
+    var64 = var64.copy()
+    for form in ("adjoint_fp32", "adjoint_mixed"):
+        efs, (var,) = gp.predict_local_uncertainties(s, form=form)
+        n = s.noa
+        dv = np.abs(var - var64)
+        rel = dv / np.maximum(np.abs(var64), 1e-12)
+        print(f"{form}: variance abs max {dv.max():.3e} (sigma^2 = "
+              f"{c['sigma'] ** 2}), rel median {np.median(rel):.3e}, "
+              f"energy {abs(efs[0] - efs64[0]) / n * 1e3:.3e} meV/atom", flush=True)
+        assert dv.max() < LOWP_TOL[form][2] * c["sigma"] ** 2
