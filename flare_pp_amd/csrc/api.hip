@@ -550,13 +550,14 @@ struct flare_sgp {
   DevBuf P, Phat, pnorm, pnorm2, alpha, Kuu, Lcm, Linv_cm;
   bool have_alpha = false;
   // low-precision copies of the normalised sparse rows, built on first use:
-  // FP32 rows; BF16 hi/lo rows of kp; per species the BF16 hi/lo transpose
-  // (F rows of mp[s], the species' count padded to 32) at tpoff[s]
+  // FP32 rows; FP16 hi/lo rows of kp with per-row inverse scales; per
+  // species the FP16 hi/lo transpose (F rows of mp[s], the species' count
+  // padded to 32, at tpoff[s]) with per-feature inverse scales (F per species)
   bool have_lowp = false;
   int kp = 0;
   std::vector<int> mp;
   std::vector<int64_t> tpoff;
-  DevBuf Phat_f, Phat_h, Phat_l, PhatT_h, PhatT_l;
+  DevBuf Phat_f, Phat_h, Phat_l, Phat_is, PhatT_h, PhatT_l, PhatT_is;
 };
 
 // ===========================================================================
@@ -1386,25 +1387,29 @@ static void sgp_build_lowp(flare_ctx *ctx, flare_sgp *gp) {
   gp->Phat_f.ensure((size_t)std::max(M, 1) * F * sizeof(float));
   gp->Phat_h.ensure((size_t)std::max(M, 1) * gp->kp * 2);
   gp->Phat_l.ensure((size_t)std::max(M, 1) * gp->kp * 2);
+  gp->Phat_is.ensure((size_t)std::max(M, 1) * sizeof(float));
   gp->PhatT_h.ensure((size_t)std::max<int64_t>(gp->tpoff[S], 1) * 2);
   gp->PhatT_l.ensure((size_t)std::max<int64_t>(gp->tpoff[S], 1) * 2);
+  gp->PhatT_is.ensure((size_t)S * F * sizeof(float));
   if (M) {
     flare::launch_to_f32(ctx->stream, (int64_t)M * F, gp->Phat.as<double>(),
                          gp->Phat_f.as<float>());
-    flare::split_bf16(ctx->stream, M, F, gp->kp, gp->Phat.as<double>(),
-                      gp->Phat_h.ptr, gp->Phat_l.ptr);
+    flare::split_f16_rows(ctx->stream, M, F, F, gp->kp, gp->Phat.as<double>(),
+                          gp->Phat_h.ptr, gp->Phat_l.ptr, gp->Phat_is.as<float>());
   }
   for (int s = 0; s < S; s++)
-    flare::launch_split_bf16_t(
-        ctx->stream, gp->m_by_type[s], F, gp->mp[s],
-        gp->Phat.as<double>() + (size_t)gp->m_off[s] * F,
-        (__bf16 *)gp->PhatT_h.ptr + gp->tpoff[s], (__bf16 *)gp->PhatT_l.ptr + gp->tpoff[s]);
+    flare::split_f16_cols(ctx->stream, gp->m_by_type[s], F, gp->mp[s],
+                          gp->Phat.as<double>() + (size_t)gp->m_off[s] * F,
+                          (_Float16 *)gp->PhatT_h.ptr + gp->tpoff[s],
+                          (_Float16 *)gp->PhatT_l.ptr + gp->tpoff[s],
+                          gp->PhatT_is.as<float>() + (size_t)s * F);
   gp->have_lowp = true;
 }
 
 // FP32 / mixed forms of the GP mean: normalised FP32 rows d, t = d . p_hat
-// in FP32 (sgemm) or on BF16 MFMA with hi+lo splitting, G = c P_hat the same
-// way, k, c, e, w in FP64 from t, forces from the FP32 adjoint kernel.
+// in FP32 (sgemm) or on FP16 MFMA with hi+lo splitting and power-of-two row
+// scaling, G = c P_hat the same way, k, c, e, w in FP64 from t, forces from
+// the FP32 adjoint kernel.
 static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p,
                             const flare_nlist *nl, const int *d_species, int form,
                             double *d_mean_efs) {
@@ -1413,8 +1418,7 @@ static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p
   sgp_build_lowp(ctx, gp);
   GPState st;
   st.lowp = true;
-  st.fw = run_forward(ctx, p, nl, d_species, flare::kPrecF32, "gpl", nullptr, 1,
-                      mixed);
+  st.fw = run_forward(ctx, p, nl, d_species, flare::kPrecF32, "gpl", nullptr, 1);
   const float *d = (const float *)st.fw.B;
   int off[flare::kMaxSpecies + 1];
   nl->row_offsets(p.S, off);
@@ -1433,23 +1437,32 @@ static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p
   st.kval = ctx->buf_t<double>("gpl.kval", std::max<size_t>(ktot, 1));
   float *G = ctx->buf_t<float>("gpl.G", std::max<size_t>((size_t)n * F, 1));
   double *e_row = ctx->buf_t<double>("gpl.e", std::max(n, 1));
-  float *cf = nullptr;
+  float *cf = nullptr, *c_is = nullptr;
   void *ch = nullptr, *cl = nullptr;
   if (mixed) {
     ch = ctx->buf("gpl.ch", std::max<size_t>(ctot, 1) * 2);
     cl = ctx->buf("gpl.cl", std::max<size_t>(ctot, 1) * 2);
+    c_is = ctx->buf_t<float>("gpl.c_is", std::max(n, 1));
   } else {
     cf = ctx->buf_t<float>("gpl.c", std::max<size_t>(ktot, 1));
   }
   ctx->mark_begin("gemm");
   if (mixed) {
+    const int kp = gp->kp;
+    void *dh = ctx->buf("gpl.dh", std::max<size_t>((size_t)n * kp, 1) * 2);
+    void *dl = ctx->buf("gpl.dl", std::max<size_t>((size_t)n * kp, 1) * 2);
+    float *d_is = ctx->buf_t<float>("gpl.d_is", std::max(n, 1));
+    flare::split_f16_rows_f32(ctx->stream, n, F, F, kp, d, dh, dl, d_is);
     flare::GemmProduct g[flare::kMaxSpecies];
-    for (int s = 0; s < p.S; s++)
-      g[s] = flare::GemmProduct{off[s + 1] - off[s], gp->m_by_type[s], gp->kp,
-                                gp->m_by_type[s], (int64_t)off[s] * gp->kp,
-                                (int64_t)gp->m_off[s] * gp->kp, (int64_t)st.koff[s]};
-    flare::gemm_bf16x3_batched(ctx->stream, p.S, g, st.fw.xh, st.fw.xl,
-                               gp->Phat_h.ptr, gp->Phat_l.ptr, Kt);
+    for (int s = 0; s < p.S; s++) {
+      g[s] = flare::GemmProduct{off[s + 1] - off[s], gp->m_by_type[s], kp,
+                                gp->m_by_type[s], (int64_t)off[s] * kp,
+                                (int64_t)gp->m_off[s] * kp, (int64_t)st.koff[s]};
+      g[s].rs = d_is + off[s];
+      g[s].cs = gp->Phat_is.as<float>() + gp->m_off[s];
+    }
+    flare::gemm_f16x3_batched(ctx->stream, p.S, g, dh, dl, gp->Phat_h.ptr,
+                              gp->Phat_l.ptr, Kt);
   } else {
     for (int s = 0; s < p.S; s++) {
       const int rows = off[s + 1] - off[s], Ms = gp->m_by_type[s];
@@ -1467,10 +1480,12 @@ static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p
     if (!Ms) {
       FLARE_HIP_CHECK(hipMemsetAsync(e_row + off[s], 0, rows * sizeof(double), ctx->stream));
       if (mixed) {
-        FLARE_HIP_CHECK(hipMemsetAsync((__bf16 *)ch + coff[s], 0,
+        FLARE_HIP_CHECK(hipMemsetAsync((_Float16 *)ch + coff[s], 0,
                                        (size_t)rows * gp->mp[s] * 2, ctx->stream));
-        FLARE_HIP_CHECK(hipMemsetAsync((__bf16 *)cl + coff[s], 0,
+        FLARE_HIP_CHECK(hipMemsetAsync((_Float16 *)cl + coff[s], 0,
                                        (size_t)rows * gp->mp[s] * 2, ctx->stream));
+        FLARE_HIP_CHECK(hipMemsetAsync(c_is + off[s], 0, rows * sizeof(float),
+                                       ctx->stream));
       }
       continue;
     }
@@ -1478,18 +1493,22 @@ static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p
         ctx->stream, rows, Ms, gp->mp[s], Kt + st.koff[s], st.fw.norm2 + off[s],
         gp->pnorm.as<double>() + o, gp->alpha.as<double>() + o, gp->sig2,
         gp->power, st.kval + st.koff[s], mixed ? nullptr : cf + st.koff[s],
-        mixed ? (void *)((__bf16 *)ch + coff[s]) : nullptr,
-        mixed ? (void *)((__bf16 *)cl + coff[s]) : nullptr, e_row + off[s]);
+        mixed ? (void *)((_Float16 *)ch + coff[s]) : nullptr,
+        mixed ? (void *)((_Float16 *)cl + coff[s]) : nullptr,
+        mixed ? c_is + off[s] : nullptr, e_row + off[s]);
   }
   ctx->mark_end();
   ctx->mark_begin("gemm");
   if (mixed) {
     flare::GemmProduct g[flare::kMaxSpecies];
-    for (int s = 0; s < p.S; s++)
+    for (int s = 0; s < p.S; s++) {
       g[s] = flare::GemmProduct{off[s + 1] - off[s], F, gp->mp[s], F,
                                 (int64_t)coff[s], gp->tpoff[s], (int64_t)off[s] * F};
-    flare::gemm_bf16x3_batched(ctx->stream, p.S, g, ch, cl, gp->PhatT_h.ptr,
-                               gp->PhatT_l.ptr, G);
+      g[s].rs = c_is + off[s];
+      g[s].cs = gp->PhatT_is.as<float>() + (size_t)s * F;
+    }
+    flare::gemm_f16x3_batched(ctx->stream, p.S, g, ch, cl, gp->PhatT_h.ptr,
+                              gp->PhatT_l.ptr, G);
   } else {
     for (int s = 0; s < p.S; s++) {
       const int rows = off[s + 1] - off[s], Ms = gp->m_by_type[s];

@@ -201,43 +201,62 @@ __global__ void unpack_beta_kernel(int F, const double *__restrict__ packed,
 }
 
 // Low-precision forms.  K holds t = d_i . p_hat_u directly (FP32, from the
-// FP32 or BF16x3 product of normalised rows); everything derived from it is
-// formed in FP64: k = sig2 t^xi (kept in FP64 for the variance), the
-// adjoint weights c = alpha t^(xi-1) (written FP32, or as BF16 hi + lo rows
-// of Mp for the matrix-core product) and e = sum alpha t^xi.
+// FP32 or split FP16 product of normalised rows); everything derived from it
+// is formed in FP64: k = sig2 t^xi (kept in FP64 for the variance), the
+// adjoint weights c = alpha t^(xi-1) (written FP32, or as FP16 hi + lo rows
+// of Mp scaled by a power of two that brings the row's largest |c| into
+// [0.5, 1), with the inverse scale in c_is) and e = sum alpha t^xi.
 __global__ void __launch_bounds__(64)
 gp_epilogue_lp_kernel(int rows, int M, int Mp, const float *__restrict__ K,
                       const double *__restrict__ norm2_rows,
                       const double *__restrict__ pnorm,
                       const double *__restrict__ alpha, double sig2,
                       double power, double *__restrict__ kval,
-                      float *__restrict__ cf, __bf16 *__restrict__ ch,
-                      __bf16 *__restrict__ cl, double *__restrict__ e_row) {
+                      float *__restrict__ cf, _Float16 *__restrict__ ch,
+                      _Float16 *__restrict__ cl, float *__restrict__ c_is,
+                      double *__restrict__ e_row) {
   const int r = blockIdx.x;
   const int lane = threadIdx.x;
   const bool empty = sqrt(norm2_rows[r]) < 1e-8;
-  double e = 0;
-  const int Mw = ch ? Mp : M;
-  for (int u = lane; u < Mw; u += kWave) {
-    double kv = 0, cv = 0;
+  auto cval = [&](int u, double &kv, double &cv, double &ev) {
+    kv = cv = ev = 0;
     if (u < M && !empty && !(pnorm[u] < 1e-8)) {
       const double t = (double)K[(int64_t)r * M + u];
       const double tx = powi(t, power);
       kv = sig2 * tx;
       cv = alpha[u] * powi(t, power - 1.0);
-      e += alpha[u] * tx;
+      ev = alpha[u] * tx;
     }
-    if (u < M && kval) kval[(int64_t)r * M + u] = kv;
-    if (cf && u < M) cf[(int64_t)r * M + u] = (float)cv;
-    if (ch) {
-      const float x = (float)cv;
-      const __bf16 h = (__bf16)x;
-      ch[(int64_t)r * Mp + u] = h;
-      cl[(int64_t)r * Mp + u] = (__bf16)(x - (float)h);
-    }
+  };
+  double e = 0;
+  float cmax = 0.0f;
+  for (int u = lane; u < M; u += kWave) {
+    double kv, cv, ev;
+    cval(u, kv, cv, ev);
+    e += ev;
+    if (kval) kval[(int64_t)r * M + u] = kv;
+    if (cf) cf[(int64_t)r * M + u] = (float)cv;
+    cmax = fmaxf(cmax, fabsf((float)cv));
   }
   e = wave_sum(e);
   if (lane == 0) e_row[r] = e;
+  if (!ch) return;
+  for (int off = 32; off > 0; off >>= 1) cmax = fmaxf(cmax, __shfl_xor(cmax, off));
+  int ex = 0;
+  if (cmax > 0.0f) {
+    frexpf(cmax, &ex);
+    ex = -ex;
+  }
+  const float sc = ldexpf(1.0f, ex);
+  for (int u = lane; u < Mp; u += kWave) {
+    double kv, cv, ev;
+    cval(u, kv, cv, ev);
+    const float x = (float)cv * sc;
+    const _Float16 h = (_Float16)x;
+    ch[(int64_t)r * Mp + u] = h;
+    cl[(int64_t)r * Mp + u] = (_Float16)(x - (float)h);
+  }
+  if (lane == 0) c_is[r] = ldexpf(1.0f, -ex);
 }
 
 // w = sig2 xi / |B| (G - e d) with d = B/|B| the FP32 row and G = c P_hat.
@@ -275,21 +294,6 @@ kself_lp_kernel(int rows, int F, const float *__restrict__ d,
   }
   s = wave_sum(s);
   if (lane == 0) kself[r] = sqrt(norm2[r]) < 1e-8 ? 0.0 : sig2 * powi(s, power);
-}
-
-// BF16 hi/lo split of the transpose of a (rows x cols) FP64 block:
-// out[c * ldo + r] for c < cols, r < ldo (zero for r >= rows)
-__global__ void split_bf16_t_kernel(int rows, int cols, int ldo,
-                                    const double *__restrict__ x,
-                                    __bf16 *__restrict__ hi,
-                                    __bf16 *__restrict__ lo) {
-  int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= (int64_t)cols * ldo) return;
-  const int c = (int)(q / ldo), r = (int)(q - (int64_t)c * ldo);
-  const float v = r < rows ? (float)x[(int64_t)r * cols + c] : 0.0f;
-  const __bf16 h = (__bf16)v;
-  hi[q] = h;
-  lo[q] = (__bf16)(v - (float)h);
 }
 
 __global__ void to_f32_kernel(int64_t n, const double *__restrict__ x,
@@ -397,11 +401,11 @@ void launch_gp_epilogue_lp(hipStream_t st, int rows, int M, int Mp,
                            const float *K, const double *norm2_rows,
                            const double *pnorm, const double *alpha, double sig2,
                            double power, double *kval, float *cf, void *ch,
-                           void *cl, double *e_row) {
+                           void *cl, float *c_is, double *e_row) {
   if (rows == 0) return;
   hipLaunchKernelGGL(gp_epilogue_lp_kernel, dim3(rows), dim3(kWave), 0, st, rows,
                      M, Mp, K, norm2_rows, pnorm, alpha, sig2, power, kval, cf,
-                     (__bf16 *)ch, (__bf16 *)cl, e_row);
+                     (_Float16 *)ch, (_Float16 *)cl, c_is, e_row);
   FLARE_HIP_CHECK(hipGetLastError());
 }
 
@@ -420,15 +424,6 @@ void launch_kself_lp(hipStream_t st, int rows, int F, const float *d,
   if (rows == 0) return;
   hipLaunchKernelGGL(kself_lp_kernel, dim3(rows), dim3(kWave), 0, st, rows, F, d,
                      norm2, sig2, power, kself);
-  FLARE_HIP_CHECK(hipGetLastError());
-}
-
-void launch_split_bf16_t(hipStream_t st, int rows, int cols, int ldo,
-                         const double *x, void *hi, void *lo) {
-  if (cols == 0 || ldo == 0) return;
-  const int64_t n = (int64_t)cols * ldo;
-  hipLaunchKernelGGL(split_bf16_t_kernel, dim3(nblk(n, 256)), dim3(256), 0, st,
-                     rows, cols, ldo, x, (__bf16 *)hi, (__bf16 *)lo);
   FLARE_HIP_CHECK(hipGetLastError());
 }
 

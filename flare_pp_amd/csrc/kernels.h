@@ -154,12 +154,13 @@ void launch_unpack_beta(hipStream_t st, int F, int power, const double *packed,
                         double *full);
 void launch_to_f32(hipStream_t st, int64_t n, const double *x, float *y);
 // Low-precision GP forms: epilogue from t (FP32, rows x M) into kval (FP64),
-// c as FP32 rows (cf) or BF16 hi/lo rows of Mp (ch, cl), e (FP64)
+// c as FP32 rows (cf) or as FP16 hi/lo rows of Mp scaled per row (ch, cl,
+// inverse scales c_is), e (FP64)
 void launch_gp_epilogue_lp(hipStream_t st, int rows, int M, int Mp,
                            const float *K, const double *norm2_rows,
                            const double *pnorm, const double *alpha, double sig2,
                            double power, double *kval, float *cf, void *ch,
-                           void *cl, double *e_row);
+                           void *cl, float *c_is, double *e_row);
 // w (FP32) = sig2 xi / |B| (G - e d), d the FP32 normalised rows
 void launch_gp_w_lp(hipStream_t st, int rows, int F, const float *G,
                     const double *e_row, const float *d, const double *norm2,
@@ -167,8 +168,6 @@ void launch_gp_w_lp(hipStream_t st, int rows, int F, const float *G,
 void launch_kself_lp(hipStream_t st, int rows, int F, const float *d,
                      const double *norm2, double sig2, double power,
                      double *kself);
-// BF16 hi/lo of the transpose of rows x cols FP64 (cols rows of ldo, padded)
-void launch_split_bf16_t(hipStream_t st, int rows, int cols, int ldo,
-                         const double *x, void *hi, void *lo);
+
 
 }  // namespace flare

@@ -18,9 +18,12 @@ constexpr int kGemmMaxBatch = 8;
 // starting at element offsets a_off / b_off; Z (FP32) starts at z_off with
 // leading dimension ldz.  Row n of a symmetric right operand is its
 // column n.
+// rs / cs (optional, FP32, M and N values) multiply row i / column j of the
+// result: the inverse scales of pre-scaled operand rows.
 struct GemmProduct {
   int M, N, K, ldz;
   int64_t a_off, b_off, z_off;
+  const float *rs = nullptr, *cs = nullptr;
 };
 
 // Up to kGemmMaxBatch products in one launch on BF16 MFMA with FP32
@@ -28,6 +31,20 @@ struct GemmProduct {
 void gemm_bf16x3_batched(hipStream_t st, int nb, const GemmProduct *prods,
                          const void *Ah, const void *Al, const void *Bh,
                          const void *Bl, float *Z);
+// The same on FP16 MFMA (operands from split_f16_*).
+void gemm_f16x3_batched(hipStream_t st, int nb, const GemmProduct *prods,
+                        const void *Ah, const void *Al, const void *Bh,
+                        const void *Bl, float *Z);
+// FP16 hi/lo split with one power-of-two scale per row (largest element in
+// [0.5, 1)); inv_scale[r] = 1 / scale.  x rows of stride ldx, out rows of kp.
+void split_f16_rows(hipStream_t st, int rows, int cols, int ldx, int kp,
+                    const double *x, void *hi, void *lo, float *inv_scale);
+void split_f16_rows_f32(hipStream_t st, int rows, int cols, int ldx, int kp,
+                        const float *x, void *hi, void *lo, float *inv_scale);
+// the same for the transpose of a rows x cols FP64 block (cols output rows
+// of ldo values, one scale per output row)
+void split_f16_cols(hipStream_t st, int rows, int cols, int ldo, const double *x,
+                    void *hi, void *lo, float *inv_scale);
 
 // hi/lo BF16 split of a rows x cols FP64 matrix into rows of kp (zero padded)
 void split_bf16(hipStream_t st, int rows, int cols, int kp, const double *x,

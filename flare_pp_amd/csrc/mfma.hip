@@ -1,11 +1,15 @@
-// mfma.hip -- BF16x3 matrix-core GEMM for the mapped contraction Z = D beta.
+// mfma.hip -- split-precision matrix-core GEMM: the mapped contraction
+// Z = D beta and the sparse-GP products D P_hat^T and C P_hat.
 //
-// Operands arrive pre-split into BF16 hi + lo parts (D by the forward
-// kernel, beta once per potential), K-padded with zeros to a multiple of 32:
+// Operands arrive pre-split into 16-bit hi + lo parts, K-padded with zeros
+// to a multiple of 32:
 //   Z (M x N, FP32) = (Ah + Al)(M x Kp) . (Bh + Bl)(N x Kp)^T,
-// accumulating hi*hi + hi*lo + lo*hi (lo*lo, ~2^-16 relative, is dropped).
-// beta is symmetric, so its row n is column n and both operands are read
-// with K contiguous.
+// accumulating hi*hi + hi*lo + lo*hi (lo*lo is dropped).  BF16 parts keep
+// ~16 bits (2^-16 relative per product); FP16 parts keep ~22 bits, and rows
+// are pre-scaled by powers of two (exact) so that their largest element
+// sits in [0.5, 1) away from the FP16 subnormal range; the store multiplies
+// by the inverse row and column scales.  beta is symmetric, so its row n is
+// column n and both operands are read with K contiguous.
 //
 // Tile 128 x 256 x 32 per workgroup of 256 threads: four wavefronts in
 // 2 x 2, each 64 x 128 = 4 x 8 tiles of v_mfma_f32_16x16x32_bf16.  Global
@@ -13,7 +17,7 @@
 // the MFMAs of the current step; workgroup ids are remapped so that the
 // column tiles of a row block share one XCD's L2.
 //
-// Operand layouts of v_mfma_f32_16x16x32_bf16 (64 lanes):
+// Operand layouts of v_mfma_f32_16x16x32_{bf16,f16} (64 lanes):
 //   A[i][k]: lane = i + 16 (k / 8), element k % 8
 //   B[k][j]: lane = j + 16 (k / 8), element k % 8
 //   D[i][j]: lane = j + 16 (i / 4), element i % 4
@@ -23,6 +27,26 @@
 namespace flare {
 
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(8))) _Float16 f16x8;
+
+template <typename E>
+struct Frag;
+template <>
+struct Frag<__bf16> {
+  typedef bf16x8 V;
+};
+template <>
+struct Frag<_Float16> {
+  typedef f16x8 V;
+};
+__device__ inline __attribute__((ext_vector_type(4))) float mma32(bf16x8 a, bf16x8 b,
+                                                                   __attribute__((ext_vector_type(4))) float c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ inline __attribute__((ext_vector_type(4))) float mma32(f16x8 a, f16x8 b,
+                                                                   __attribute__((ext_vector_type(4))) float c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
 
 // Up to kGemmMaxBatch independent products (the GemmProduct list) and the
 // exclusive prefix of their tile counts.
@@ -42,10 +66,11 @@ struct Tile {
   uint4 b[2 * kChunksB];  // Bh, Bl
 };
 
-__device__ inline void load_tile(Tile &t, const __bf16 *__restrict__ Ah,
-                                 const __bf16 *__restrict__ Al,
-                                 const __bf16 *__restrict__ Bh,
-                                 const __bf16 *__restrict__ Bl, int M, int N,
+template <typename E>
+__device__ inline void load_tile(Tile &t, const E *__restrict__ Ah,
+                                 const E *__restrict__ Al,
+                                 const E *__restrict__ Bh,
+                                 const E *__restrict__ Bl, int M, int N,
                                  int Kp, int m0, int n0, int k0) {
   const int tid = threadIdx.x;
   const uint4 z = {0, 0, 0, 0};
@@ -77,9 +102,10 @@ __device__ inline void load_tile(Tile &t, const __bf16 *__restrict__ Ah,
   }
 }
 
-__device__ inline void store_tile(const Tile &t, __bf16 (*sAh)[kLdk],
-                                  __bf16 (*sAl)[kLdk], __bf16 (*sBh)[kLdk],
-                                  __bf16 (*sBl)[kLdk]) {
+template <typename E>
+__device__ inline void store_tile(const Tile &t, E (*sAh)[kLdk],
+                                  E (*sAl)[kLdk], E (*sBh)[kLdk],
+                                  E (*sBl)[kLdk]) {
   const int tid = threadIdx.x;
 #pragma unroll
   for (int h = 0; h < kChunksA; h++) {
@@ -118,12 +144,14 @@ __device__ inline void tile_of_block(int n_row_tiles, int n_col_tiles, int &rt,
   ct = T - rt * n_col_tiles;
 }
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
-gemm_bf16x3_kernel(GemmBatch bt, const __bf16 *__restrict__ Ah0,
-                   const __bf16 *__restrict__ Al0, const __bf16 *__restrict__ Bh0,
-                   const __bf16 *__restrict__ Bl0, float *__restrict__ Z0) {
-  __shared__ __align__(16) __bf16 sAh[kBM][kLdk], sAl[kBM][kLdk];
-  __shared__ __align__(16) __bf16 sBh[kBN][kLdk], sBl[kBN][kLdk];
+template <typename E>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+gemm_x3_kernel(GemmBatch bt, const E *__restrict__ Ah0,
+               const E *__restrict__ Al0, const E *__restrict__ Bh0,
+               const E *__restrict__ Bl0, float *__restrict__ Z0) {
+  typedef typename Frag<E>::V V8;
+  __shared__ __align__(16) E sAh[kBM][kLdk], sAl[kBM][kLdk];
+  __shared__ __align__(16) E sBh[kBN][kLdk], sBl[kBN][kLdk];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int wm = (wv >> 1) * 64, wn = (wv & 1) * 128;
@@ -138,8 +166,8 @@ gemm_bf16x3_kernel(GemmBatch bt, const __bf16 *__restrict__ Ah0,
   const int nct = (N + kBN - 1) / kBN;
   const int local = T - bt.tile0[q];
   const int rt = local / nct, ct = local - rt * nct;
-  const __bf16 *Ah = Ah0 + gp.a_off, *Al = Al0 + gp.a_off;
-  const __bf16 *Bh = Bh0 + gp.b_off, *Bl = Bl0 + gp.b_off;
+  const E *Ah = Ah0 + gp.a_off, *Al = Al0 + gp.a_off;
+  const E *Bh = Bh0 + gp.b_off, *Bl = Bl0 + gp.b_off;
   float *Z = Z0 + gp.z_off;
   const int m0 = rt * kBM, n0 = ct * kBN;
   floatx4 acc[4][8];
@@ -157,38 +185,55 @@ gemm_bf16x3_kernel(GemmBatch bt, const __bf16 *__restrict__ Ah0,
     store_tile(t, sAh, sAl, sBh, sBl);
     __syncthreads();
     if (s + 1 < steps) load_tile(t, Ah, Al, Bh, Bl, M, N, Kp, m0, n0, (s + 1) * kBK);
-    bf16x8 ah[4], al[4];
+    V8 ah[4], al[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      ah[i] = *reinterpret_cast<const bf16x8 *>(&sAh[wm + 16 * i + fr][fk]);
-      al[i] = *reinterpret_cast<const bf16x8 *>(&sAl[wm + 16 * i + fr][fk]);
+      ah[i] = *reinterpret_cast<const V8 *>(&sAh[wm + 16 * i + fr][fk]);
+      al[i] = *reinterpret_cast<const V8 *>(&sAl[wm + 16 * i + fr][fk]);
     }
 #pragma unroll
     for (int j = 0; j < 8; j++) {
-      const bf16x8 bh = *reinterpret_cast<const bf16x8 *>(&sBh[wn + 16 * j + fr][fk]);
-      const bf16x8 bl = *reinterpret_cast<const bf16x8 *>(&sBl[wn + 16 * j + fr][fk]);
+      const V8 bh = *reinterpret_cast<const V8 *>(&sBh[wn + 16 * j + fr][fk]);
+      const V8 bl = *reinterpret_cast<const V8 *>(&sBl[wn + 16 * j + fr][fk]);
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh, acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl, acc[i][j], 0, 0, 0);
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh, acc[i][j], 0, 0, 0);
+        acc[i][j] = mma32(al[i], bh, acc[i][j]);
+        acc[i][j] = mma32(ah[i], bl, acc[i][j]);
+        acc[i][j] = mma32(ah[i], bh, acc[i][j]);
       }
     }
   }
   const int dcol = lane & 15;
   const int drow = (lane >> 4) * 4;
+  if (!gp.rs && !gp.cs) {
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+    for (int i = 0; i < 4; i++)
 #pragma unroll
-    for (int j = 0; j < 8; j++) {
-      const int col = n0 + wn + 16 * j + dcol;
-      if (col >= N) continue;
+      for (int j = 0; j < 8; j++) {
+        const int col = n0 + wn + 16 * j + dcol;
+        if (col >= N) continue;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int row = m0 + wm + 16 * i + drow + q;
+          if (row < M) Z[(int64_t)row * ldz + col] = acc[i][j][q];
+        }
+      }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
 #pragma unroll
       for (int q = 0; q < 4; q++) {
         const int row = m0 + wm + 16 * i + drow + q;
-        if (row < M) Z[(int64_t)row * ldz + col] = acc[i][j][q];
+        if (row >= M) continue;
+        const float rsc = gp.rs ? gp.rs[row] : 1.0f;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          const int col = n0 + wn + 16 * j + dcol;
+          if (col < N)
+            Z[(int64_t)row * ldz + col] = acc[i][j][q] * rsc * (gp.cs ? gp.cs[col] : 1.0f);
+        }
       }
-    }
+  }
 }
 
 // hi/lo split of a (rows x cols) FP64 matrix into rows of kp BF16 (zero
@@ -208,16 +253,17 @@ __global__ void split_bf16_kernel(int rows, int cols, int kp,
 
 int bf16_padded_k(int K) { return (K + kBK - 1) / kBK * kBK; }
 
-void gemm_bf16x3_batched(hipStream_t st, int nb, const GemmProduct *prods,
-                         const void *Ah, const void *Al, const void *Bh,
-                         const void *Bl, float *Z) {
-  FLARE_REQUIRE(nb >= 0 && nb <= kGemmMaxBatch, "gemm_bf16x3: at most 8 products");
+template <typename E>
+static void gemm_x3_batched(hipStream_t st, int nb, const GemmProduct *prods,
+                            const void *Ah, const void *Al, const void *Bh,
+                            const void *Bl, float *Z) {
+  FLARE_REQUIRE(nb >= 0 && nb <= kGemmMaxBatch, "gemm_x3: at most 8 products");
   GemmBatch bt;
   bt.nb = 0;
   int tiles = 0;
   for (int q = 0; q < nb; q++) {
     const GemmProduct &g = prods[q];
-    FLARE_REQUIRE(g.K % kBK == 0, "gemm_bf16x3: K must be padded to 32");
+    FLARE_REQUIRE(g.K % kBK == 0, "gemm_x3: K must be padded to 32");
     if (g.M <= 0 || g.N <= 0) continue;
     bt.g[bt.nb] = g;
     bt.tile0[bt.nb] = tiles;
@@ -228,9 +274,100 @@ void gemm_bf16x3_batched(hipStream_t st, int nb, const GemmProduct *prods,
   if (tiles == 0) return;
   // round up to a multiple of 8 so every XCD gets whole slots
   const int grid = (tiles + 7) / 8 * 8;
-  hipLaunchKernelGGL(gemm_bf16x3_kernel, dim3(grid), dim3(256), 0, st, bt,
-                     (const __bf16 *)Ah, (const __bf16 *)Al, (const __bf16 *)Bh,
-                     (const __bf16 *)Bl, Z);
+  hipLaunchKernelGGL(gemm_x3_kernel<E>, dim3(grid), dim3(256), 0, st, bt,
+                     (const E *)Ah, (const E *)Al, (const E *)Bh, (const E *)Bl, Z);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void gemm_bf16x3_batched(hipStream_t st, int nb, const GemmProduct *prods,
+                         const void *Ah, const void *Al, const void *Bh,
+                         const void *Bl, float *Z) {
+  gemm_x3_batched<__bf16>(st, nb, prods, Ah, Al, Bh, Bl, Z);
+}
+
+void gemm_f16x3_batched(hipStream_t st, int nb, const GemmProduct *prods,
+                        const void *Ah, const void *Al, const void *Bh,
+                        const void *Bl, float *Z) {
+  gemm_x3_batched<_Float16>(st, nb, prods, Ah, Al, Bh, Bl, Z);
+}
+
+// FP16 hi/lo split of rows with a power-of-two scale per row: the row's
+// largest |x| is brought into [0.5, 1); inv_scale[r] = 2^-e.  One wavefront
+// per row; X is FP64 or FP32 (row stride ldx), out rows of kp (zero padded).
+template <typename T>
+__global__ void __launch_bounds__(64)
+split_f16_rows_kernel(int rows, int cols, int ldx, int kp, const T *__restrict__ x,
+                      _Float16 *__restrict__ hi, _Float16 *__restrict__ lo,
+                      float *__restrict__ inv_scale) {
+  const int r = blockIdx.x, lane = threadIdx.x;
+  const T *xr = x + (int64_t)r * ldx;
+  float m = 0.0f;
+  for (int c = lane; c < cols; c += 64) m = fmaxf(m, fabsf((float)xr[c]));
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  int e = 0;
+  if (m > 0.0f) {
+    frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
+    e = -e;
+  }
+  const float sc = ldexpf(1.0f, e);
+  for (int c = lane; c < kp; c += 64) {
+    const float v = c < cols ? (float)xr[c] * sc : 0.0f;
+    const _Float16 h = (_Float16)v;
+    hi[(int64_t)r * kp + c] = h;
+    lo[(int64_t)r * kp + c] = (_Float16)(v - (float)h);
+  }
+  if (lane == 0) inv_scale[r] = ldexpf(1.0f, -e);
+}
+
+// The same for the transpose of a (rows x cols) FP64 block: output row c
+// (c < cols) holds column c, ldo values (zero for r >= rows), scaled by
+// its own largest element.  One wavefront per output row.
+__global__ void __launch_bounds__(64)
+split_f16_cols_kernel(int rows, int cols, int ldo, const double *__restrict__ x,
+                      _Float16 *__restrict__ hi, _Float16 *__restrict__ lo,
+                      float *__restrict__ inv_scale) {
+  const int c = blockIdx.x, lane = threadIdx.x;
+  float m = 0.0f;
+  for (int r = lane; r < rows; r += 64) m = fmaxf(m, fabsf((float)x[(int64_t)r * cols + c]));
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  int e = 0;
+  if (m > 0.0f) {
+    frexpf(m, &e);
+    e = -e;
+  }
+  const float sc = ldexpf(1.0f, e);
+  for (int r = lane; r < ldo; r += 64) {
+    const float v = r < rows ? (float)x[(int64_t)r * cols + c] * sc : 0.0f;
+    const _Float16 h = (_Float16)v;
+    hi[(int64_t)c * ldo + r] = h;
+    lo[(int64_t)c * ldo + r] = (_Float16)(v - (float)h);
+  }
+  if (lane == 0) inv_scale[c] = ldexpf(1.0f, -e);
+}
+
+void split_f16_rows(hipStream_t st, int rows, int cols, int ldx, int kp,
+                    const double *x, void *hi, void *lo, float *inv_scale) {
+  if (rows == 0) return;
+  hipLaunchKernelGGL(split_f16_rows_kernel<double>, dim3(rows), dim3(64), 0, st,
+                     rows, cols, ldx, kp, x, (_Float16 *)hi, (_Float16 *)lo,
+                     inv_scale);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void split_f16_rows_f32(hipStream_t st, int rows, int cols, int ldx, int kp,
+                        const float *x, void *hi, void *lo, float *inv_scale) {
+  if (rows == 0) return;
+  hipLaunchKernelGGL(split_f16_rows_kernel<float>, dim3(rows), dim3(64), 0, st,
+                     rows, cols, ldx, kp, x, (_Float16 *)hi, (_Float16 *)lo,
+                     inv_scale);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void split_f16_cols(hipStream_t st, int rows, int cols, int ldo, const double *x,
+                    void *hi, void *lo, float *inv_scale) {
+  if (cols == 0) return;
+  hipLaunchKernelGGL(split_f16_cols_kernel, dim3(cols), dim3(64), 0, st, rows,
+                     cols, ldo, x, (_Float16 *)hi, (_Float16 *)lo, inv_scale);
   FLARE_HIP_CHECK(hipGetLastError());
 }
 
