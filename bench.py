@@ -1,12 +1,15 @@
 #!/usr/bin/env python
 """Force-evaluation benchmark of the mapped flare potential on one MI355X.
 
-Workload: 32,000-atom 2-species fcc (20^3 conventional cells, a = 4.05 A,
-seeded Gaussian displacements sigma = 0.08 A, 50/50 species by seeded
-shuffle), B2 {2 species, n_max 8, l_max 4, rc 5.0 A} -> 680 features, per
-species symmetric beta 680 x 680 with A ~ N(0, 1e-3).  One step = one force
-evaluation (energy, forces, virial) from positions resident in HBM,
-including the neighbor-list build.
+Workload (default, --workload 32k): 32,000-atom 2-species fcc (20^3
+conventional cells, a = 4.05 A, seeded Gaussian displacements sigma = 0.08 A,
+50/50 species by seeded shuffle), B2 {2 species, n_max 8, l_max 4, rc 5.0 A}
+-> 680 features, per species symmetric beta 680 x 680 with A ~ N(0, 1e-3).
+--workload 262k: 262,144-atom 3-species disordered box (random sequential
+addition at 0.06 atoms/A^3, d_min 2.2 A), B2 {3, 8, 6, rc 5.5 A} -> 2,100
+features, beta 2,100 x 2,100 per species, entries of std 1e-4.  One step =
+one force evaluation (energy, forces, virial) from positions resident in
+HBM, including the neighbor-list build.
 
 Forms timed: plain FP64 (drop-in, materialised dA/dr), adjoint FP64, adjoint
 FP32, adjoint + mixed precision (the headline `value`).  Prints one JSON line.
@@ -22,6 +25,36 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FORMS = ["plain", "adjoint_fp64", "adjoint_fp32", "adjoint_mixed"]
+
+# name -> (generator in flare_pp_amd.synth, S, N, L, rc, description)
+WORKLOADS = {
+    "32k": ("config2", 2, 8, 4, 5.0,
+            "mapped pair_flare force eval, 32,000-atom 2-species fcc, B2 {2,8,4} "
+            "rc 5.0 (F=680), beta 680^2/species"),
+    "262k": ("config5", 3, 8, 6, 5.5,
+             "mapped pair_flare force eval, 262,144-atom 3-species RSA liquid "
+             "(0.06/A^3, d_min 2.2 A), B2 {3,8,6} rc 5.5 (F=2100), beta "
+             "2100^2/species"),
+}
+
+
+def load_workload(name, path=None):
+    """Structure and packed beta of a workload (from `path` when given, as
+    written by save_workload)."""
+    import numpy as np
+    from flare_pp_amd import pack_beta, synth
+    if path:
+        z = np.load(path)
+        return {k: z[k] for k in z.files}
+    c = getattr(synth, WORKLOADS[name][0])()
+    c["packed"] = np.stack([pack_beta(b) for b in c["beta"]])
+    return c
+
+
+def save_workload(c, path):
+    import numpy as np
+    np.savez(path, cell=c["cell"], species=c["species"], positions=c["positions"],
+             beta=c["beta"], packed=c["packed"])
 HBM_PEAK_GBS = 8000.0
 PEAKS_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}
 
@@ -32,6 +65,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--forms", default=",".join(FORMS))
+    ap.add_argument("--workload", default="32k", choices=sorted(WORKLOADS),
+                    help="32k: the headline configuration; 262k: the scale-up box")
+    ap.add_argument("--workload-file", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-traffic", action="store_true",
@@ -144,13 +180,14 @@ def pmc_pass(counter, child_args, match):
     return sum(vals) / len(vals)
 
 
-def measure_traffic(form, kernel):
+def measure_traffic(form, kernel, workload, wfile):
     scale = {}
     for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
         kib = pmc_pass(ctr, ["--copy-calibration"], "copyBuffer")
         scale[ctr] = COPY_BYTES / kib  # bytes per counter unit
-    fetch = pmc_pass("FETCH_SIZE", ["--traffic-child", form], kernel)
-    write = pmc_pass("WRITE_SIZE", ["--traffic-child", form], kernel)
+    child = ["--traffic-child", form, "--workload", workload, "--workload-file", wfile]
+    fetch = pmc_pass("FETCH_SIZE", child, kernel)
+    write = pmc_pass("WRITE_SIZE", child, kernel)
     return {"bytes": fetch * scale["FETCH_SIZE"] + write * scale["WRITE_SIZE"],
             "fetch_bytes": fetch * scale["FETCH_SIZE"],
             "write_bytes": write * scale["WRITE_SIZE"],
@@ -168,14 +205,18 @@ def copy_calibration():
     torch.cuda.synchronize()
 
 
-def traffic_child(form):
+def make_pair(name, c):
     import numpy as np
+    from flare_pp_amd import PairFLARE
+    _, S, N, L, rc, _ = WORKLOADS[name]
+    return PairFLARE.from_coefficients(2, S, N, L, np.full((S, S), rc), c["packed"])
+
+
+def traffic_child(form, workload, wfile):
     import torch
-    from flare_pp_amd import PairFLARE, pack_beta, synth
-    c = synth.config2()
+    c = load_workload(workload, wfile)
     n = len(c["species"])
-    packed = np.stack([pack_beta(b) for b in c["beta"]])
-    pair = PairFLARE.from_coefficients(2, 2, 8, 4, np.full((2, 2), 5.0), packed)
+    pair = make_pair(workload, c)
     pos = torch.tensor(c["positions"], device="cuda:0")
     spc = torch.tensor(c["species"], device="cuda:0", dtype=torch.int32)
     e = torch.empty(1, dtype=torch.float64, device="cuda:0")
@@ -187,15 +228,15 @@ def traffic_child(form):
     torch.cuda.synchronize()
 
 
-def cpu_baseline(c, packed, threads):
+def cpu_baseline(name, c, threads):
     """The oracle restatement of PairFLARE::compute on the host cores."""
     from oracle import pyoracle
     import numpy as np
+    _, S, N, L, rc, _ = WORKLOADS[name]
     nthreads = threads or min(16, os.cpu_count() or 1)
-    nl = pyoracle.NeighborList(c["cell"], c["species"], c["positions"], 5.0,
+    nl = pyoracle.NeighborList(c["cell"], c["species"], c["positions"], rc,
                                nthreads=nthreads)
-    beta_full = c["beta"]
-    args = (nl, c["species"], 2, 8, 4, 2, np.full((2, 2), 5.0), beta_full)
+    args = (nl, c["species"], S, N, L, 2, np.full((S, S), rc), c["beta"])
     pyoracle.pair_compute(*args, packed=False, nthreads=nthreads)  # warm-up
     times = []
     t_end = time.time() + 20.0
@@ -207,7 +248,7 @@ def cpu_baseline(c, packed, threads):
     n = len(c["species"])
     return {"value": n / med, "unit": "atoms/s", "cores": nthreads,
             "kind": "port",
-            "sample": f"full 32,000-atom force evaluation (E, F, virial), "
+            "sample": f"full {n:,}-atom force evaluation (E, F, virial), "
                       f"median of {len(times)} after 1 warm-up, neighbor list "
                       f"prebuilt, FP64, OpenMP over atoms"}
 
@@ -217,7 +258,7 @@ def main():
     if args.copy_calibration:
         return copy_calibration()
     if args.traffic_child:
-        return traffic_child(args.traffic_child)
+        return traffic_child(args.traffic_child, args.workload, args.workload_file)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU "
                           "sharding is described in DESIGN.md, not run"}))
@@ -225,13 +266,14 @@ def main():
     import numpy as np
     import torch
 
-    from flare_pp_amd import PairFLARE, _capi, pack_beta, synth
+    from flare_pp_amd import _capi
 
     forms = [f for f in args.forms.split(",") if f]
-    c = synth.config2()
+    wl = args.workload
+    _, S, N, L, rc, wdesc = WORKLOADS[wl]
+    c = load_workload(wl)
     n = len(c["species"])
-    packed = np.stack([pack_beta(b) for b in c["beta"]])
-    pair = PairFLARE.from_coefficients(2, 2, 8, 4, np.full((2, 2), 5.0), packed)
+    pair = make_pair(wl, c)
     dev = "cuda:0"
     pos = torch.tensor(c["positions"], device=dev)
     spc = torch.tensor(c["species"], device=dev, dtype=torch.int32)
@@ -240,7 +282,8 @@ def main():
     f = torch.empty((n, 3), dtype=torch.float64, device=dev)
     v = torch.empty(6, dtype=torch.float64, device=dev)
     ctx = _capi.context(0)
-    F, W = 680, 400
+    nr = S * N
+    F, W = nr * (nr + 1) // 2 * (L + 1), nr * (L + 1) ** 2
     KP = (F + 31) // 32 * 32  # BF16 row length of the GEMM operands
 
     # reference result: plain FP64
@@ -279,7 +322,7 @@ def main():
             "err_force_rms_meV_per_A": float(np.sqrt(np.mean(fd ** 2)) * 1e3),
             "err_force_max_meV_per_A": float(np.abs(fd).max() * 1e3),
         }
-        results[form]["roofline"] = roofline(form, stages, n, n_pairs, 2, F, W, KP)
+        results[form]["roofline"] = roofline(form, stages, n, n_pairs, S, F, W, KP)
         print(f"# {form}: {ms:.3f} ms/step ({ms_reuse:.3f} reused nlist) "
               f"stages {results[form]['stage_ms']}", file=sys.stderr, flush=True)
     ctx.set_blocking(True)
@@ -290,18 +333,22 @@ def main():
     kname = KERNELS[(rl_name, head)]
     rl = dict(rl, stage=rl_name, kernel=kname, kernel_ms=rl_ms)
     if not args.no_traffic:
+        import tempfile
         try:
-            t = measure_traffic(head, kname)
+            with tempfile.TemporaryDirectory() as d:
+                wfile = os.path.join(d, "workload.npz")
+                save_workload(c, wfile)
+                t = measure_traffic(head, kname, wl, wfile)
             rl["traffic"] = t["bytes"]
             rl["traffic_detail"] = t
         except Exception as exc:  # profiler absent or refused: no figure
             print(f"# traffic not measured: {exc}", file=sys.stderr, flush=True)
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(c, packed, args.cpu_threads)
+        cpu = cpu_baseline(wl, c, args.cpu_threads)
     plain = results.get("plain")
     line = {
-        "metric": "force-eval atoms/s at 32k atoms: adjoint+mixed-prec vs plain "
+        "metric": f"force-eval atoms/s at {wl} atoms: adjoint+mixed-prec vs plain "
                   "FP64 on 1 GPU",
         "value": r["atoms_per_s"],
         "unit": "atoms/s",
@@ -313,11 +360,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "bf16x3+fp32" if head == "adjoint_mixed" else head,
-        "data": "synthetic (seeded fcc structure and random symmetric beta)",
-        "config": {"workload": "mapped pair_flare force eval, 32,000-atom 2-species "
-                               "fcc, B2 {2,8,4} rc 5.0 (F=680), beta 680^2/species",
-                   "n_atoms": n, "n_pairs": n_pairs, "form": head,
-                   "includes_neighbor_build": True},
+        "data": "synthetic (seeded structure and random symmetric beta)",
+        "config": {"workload": wdesc, "n_atoms": n, "n_pairs": n_pairs,
+                   "form": head, "includes_neighbor_build": True},
         "speedup_vs_plain_fp64": (r["atoms_per_s"] / plain["atoms_per_s"]
                                   if plain else None),
         "forms": {k: {kk: vv for kk, vv in x.items() if kk != "roofline"}

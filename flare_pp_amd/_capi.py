@@ -146,6 +146,8 @@ SIGNATURES = {
                                                    _vp, _i, _vp, _vp]),
     "flare_envs_envs": (_i, [_vp, _i, _i, _pi, _vp, _pi, _vp, _d, _d, _vp]),
     "flare_sgp_mapping_coefficients": (_i, [_vp, _vp, _vp]),
+    "flare_synth_rsa": (_i, [_i, _d, _d, ctypes.c_uint64, ctypes.c_int64, _vp]),
+    "flare_ctx_set_plain_chunk": (_i, [_vp, _i64]),
 }
 
 _lib = None
@@ -209,6 +211,10 @@ class Context:
 
     def set_blocking(self, blocking):
         check(lib().flare_ctx_set_blocking(self.handle, int(bool(blocking))))
+
+    def set_plain_chunk(self, nbytes):
+        """Budget of the plain form's per-pair derivative tensor (bytes)."""
+        check(lib().flare_ctx_set_plain_chunk(self.handle, int(nbytes)))
 
     def synchronize(self):
         check(lib().flare_ctx_synchronize(self.handle))

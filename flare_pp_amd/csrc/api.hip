@@ -64,6 +64,7 @@ struct flare_ctx {
   hipStream_t stream = nullptr;
   rocblas_handle blas = nullptr;
   int blocking = 1;
+  int64_t plain_chunk_bytes = int64_t(8) << 30;  // plain-form dA/dr budget
   // stage timing with HIP events on the context stream (when enabled)
   int profiling = 0;
   struct Stage {
@@ -435,22 +436,55 @@ static void run_forces(flare_ctx *c, const flare::B2Dev &p,
                        double *stress6, const std::string &tag) {
   const int n = nl->n;
   const int64_t P = nl->b.P;
-  const double *dA = nullptr;
-  if (mode == 0) {
-    double *d = c->buf_t<double>(tag + ".dA", (size_t)std::max<int64_t>(P, 1) * 3 * p.NH);
-    c->mark_begin("pair_dA");
-    flare::launch_pair_dA(c->stream, p, nl->dev(), d_species,
-                          c->ynorm_d.as<double>(), d);
-    c->mark_end();
-    dA = d;
-  }
   double *pf = c->buf_t<double>(tag + ".pf", (size_t)std::max<int64_t>(P, 1) * 3);
-  double *vir = c->buf_t<double>(tag + ".vir", (size_t)n * 9);
-  c->mark_begin(mode == 0 ? "forces_plain" : "forces_adjoint");
-  flare::launch_forces(c->stream, prec, mode, p, nl->dev(), d_species,
-                       nl->b.row_of.as<int>(), c->ynorm_d.as<double>(),
-                       c->ynorm_f.as<float>(), fw.A, wi, dA, pf);
-  c->mark_end();
+  double *vir = c->buf_t<double>(tag + ".vir", (size_t)std::max(n, 1) * 9);
+  if (mode == 0) {
+    // plain form: dA/dr materialised for a range of atoms at a time, at most
+    // c->plain_chunk_bytes of derivatives (one range when everything fits;
+    // an atom with more pairs than the budget still gets a range of its own)
+    const int64_t per_pair = 3 * (int64_t)p.NH * (int64_t)sizeof(double);
+    const int64_t max_pairs = std::max<int64_t>(1, c->plain_chunk_bytes / per_pair);
+    std::vector<int> bounds{0, n};
+    std::vector<int> cum_host;
+    int64_t chunk_pairs = P;
+    if (P > max_pairs) {
+      cum_host.resize(n + 1);
+      FLARE_HIP_CHECK(hipMemcpyAsync(cum_host.data(), nl->b.cum.ptr,
+                                     (n + 1) * sizeof(int), hipMemcpyDeviceToHost,
+                                     c->stream));
+      FLARE_HIP_CHECK(hipStreamSynchronize(c->stream));
+      bounds.assign(1, 0);
+      chunk_pairs = 0;
+      for (int a0 = 0; a0 < n;) {
+        int a1 = a0 + 1;
+        while (a1 < n && (int64_t)cum_host[a1 + 1] - cum_host[a0] <= max_pairs) a1++;
+        chunk_pairs = std::max<int64_t>(chunk_pairs, cum_host[a1] - cum_host[a0]);
+        bounds.push_back(a1);
+        a0 = a1;
+      }
+    }
+    double *d = c->buf_t<double>(tag + ".dA", (size_t)std::max<int64_t>(chunk_pairs, 1) * 3 * p.NH);
+    for (size_t q = 0; q + 1 < bounds.size(); q++) {
+      const int a0 = bounds[q], a1 = bounds[q + 1];
+      const int64_t p0 = cum_host.empty() ? 0 : cum_host[a0];
+      const int64_t p1 = cum_host.empty() ? P : cum_host[a1];
+      c->mark_begin("pair_dA");
+      flare::launch_pair_dA(c->stream, p, nl->dev(), d_species,
+                            c->ynorm_d.as<double>(), p0, p1, d);
+      c->mark_end();
+      c->mark_begin("forces_plain");
+      flare::launch_forces(c->stream, prec, mode, p, nl->dev(), d_species,
+                           nl->b.row_of.as<int>(), c->ynorm_d.as<double>(),
+                           c->ynorm_f.as<float>(), fw.A, wi, d, p0, a0, a1, pf);
+      c->mark_end();
+    }
+  } else {
+    c->mark_begin("forces_adjoint");
+    flare::launch_forces(c->stream, prec, mode, p, nl->dev(), d_species,
+                         nl->b.row_of.as<int>(), c->ynorm_d.as<double>(),
+                         c->ynorm_f.as<float>(), fw.A, wi, nullptr, 0, 0, n, pf);
+    c->mark_end();
+  }
   c->mark_begin("gather_reduce");
   flare::launch_gather(c->stream, nl->dev(), pf, forces, vir);
   double *sums = c->buf_t<double>(tag + ".sums", 10);
@@ -610,6 +644,14 @@ int flare_ctx_set_blocking(flare_ctx *ctx, int blocking) {
   return guarded([&] {
     REQUIRE(ctx, "null context");
     ctx->blocking = blocking ? 1 : 0;
+  });
+}
+
+int flare_ctx_set_plain_chunk(flare_ctx *ctx, int64_t bytes) {
+  return guarded([&] {
+    REQUIRE(ctx, "null context");
+    REQUIRE(bytes > 0, "chunk size must be positive");
+    ctx->plain_chunk_bytes = bytes;
   });
 }
 

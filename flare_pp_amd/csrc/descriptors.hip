@@ -308,10 +308,11 @@ b2_forward_mfma_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
   constexpr int KP = MT * 16 + 1;  // padded row of G
   constexpr int YP = NT * 16 + 1;  // padded row of Y
   extern __shared__ __align__(16) unsigned char smem[];
+  // B (F values) reuses the pair staging rows, dead once A is formed
   T *Gs = reinterpret_cast<T *>(smem);
   T *Ys = Gs + kWave * KP;
-  T *Al = Ys + kWave * YP;  // nr x H
-  T *Bs = Al + p.nr * p.H;  // F
+  T *Bs = Gs;
+  T *Al = Gs + max(kWave * (KP + YP), p.F);  // nr x H
   const int i = blockIdx.x;
   const int lane = threadIdx.x;
   const int N = EXACT ? NMAX : p.N;
@@ -439,8 +440,9 @@ static void launch_mfma_t(hipStream_t st, const B2Dev &p, const NListDev &nl,
                           const int *species, const int *row_of, const T *ynorm,
                           const FwdOut &out, void *A_out, void *B_out,
                           double *norm2_out) {
-  size_t lds = sizeof(T) * ((size_t)kWave * (MT * 16 + 1 + NT * 16 + 1) +
-                            (size_t)p.nr * p.H + (size_t)p.F);
+  size_t lds = sizeof(T) * (std::max<size_t>((size_t)kWave * (MT * 16 + 1 + NT * 16 + 1),
+                                             (size_t)p.F) +
+                            (size_t)p.nr * p.H);
   FLARE_REQUIRE(lds <= 64 * 1024, "descriptor settings too large");
   // compile-time (n_max, l_max) variants of the common settings
   if (p.N == 8 && p.L == 3 && NT == 1) {
@@ -486,12 +488,13 @@ static bool launch_mfma(hipStream_t st, const B2Dev &p, const NListDev &nl,
   return false;
 }
 
-// Plain form: dA/dr for `pb` (<= 64) pairs per workgroup of 256 threads.
-// out[(p*3 + c)*NH + n*H + lm]; zero for pairs outside the species cutoff.
+// Plain form: dA/dr for `pb` (<= 64) pairs per workgroup of 256 threads,
+// pairs p_begin <= p < p_end: out[((p - p_begin)*3 + c)*NH + n*H + lm];
+// zero for pairs outside the species cutoff.
 __global__ void __launch_bounds__(256)
 pair_dA_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
-               const double *__restrict__ ynorm, int pb,
-               double *__restrict__ out) {
+               const double *__restrict__ ynorm, int pb, int64_t p_begin,
+               int64_t p_end, double *__restrict__ out) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = p.N, H = p.H, NH = p.NH;
   double *g = reinterpret_cast<double *>(smem);  // pb*N
@@ -500,12 +503,12 @@ pair_dA_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
   double *Yg = Y + pb * H;                       // pb*3*H
   double *uhat = Yg + pb * 3 * H;                // pb*3
   int *ins = reinterpret_cast<int *>(uhat + pb * 3);
-  const int64_t p0 = (int64_t)blockIdx.x * pb;
+  const int64_t p0 = p_begin + (int64_t)blockIdx.x * pb;
   const int t = threadIdx.x;
   if (t < pb) {
     const int64_t q = p0 + t;
     int inside = 0;
-    if (q < nl.P) {
+    if (q < p_end) {
       const double *rv = nl.rel + 4 * q;
       const int si = species[nl.center[q]], sj = nl.nspec[q];
       if (pair_inside(p, si, sj, rv)) {
@@ -540,8 +543,8 @@ pair_dA_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
   // each thread owns fixed (n, lm) columns and walks the block's pairs, so
   // consecutive threads write consecutive addresses of every (pair, xyz) row
   const int per = 3 * NH;
-  const int npair = (int)min((int64_t)pb, nl.P - p0);
-  double *o = out + p0 * per;
+  const int npair = (int)min((int64_t)pb, p_end - p0);
+  double *o = out + (p0 - p_begin) * per;
   for (int col = t; col < NH; col += blockDim.x) {
     const int n = col / H, lm = col - n * H;
     for (int k = 0; k < npair; k++) {
@@ -683,17 +686,18 @@ void launch_b2_forward(hipStream_t st, const B2Dev &p, const NListDev &nl,
 }
 
 void launch_pair_dA(hipStream_t st, const B2Dev &p, const NListDev &nl,
-                    const int *species, const double *ynorm, double *out) {
-  if (nl.P == 0) return;
+                    const int *species, const double *ynorm, int64_t p_begin,
+                    int64_t p_end, double *out) {
+  if (p_end <= p_begin) return;
   // pairs per workgroup: as many as fit 64 KB of LDS, at most 64
   const size_t per_pair =
       sizeof(double) * (2 * (size_t)p.N + 4 * (size_t)p.H + 3) + sizeof(int);
   int pb = (int)std::min<size_t>(kWave, (64 * 1024) / per_pair);
   FLARE_REQUIRE(pb >= 1, "descriptor settings too large for the plain form");
   size_t lds = per_pair * pb;
-  int64_t blocks = (nl.P + pb - 1) / pb;
+  int64_t blocks = (p_end - p_begin + pb - 1) / pb;
   hipLaunchKernelGGL(pair_dA_kernel, dim3((unsigned)blocks), dim3(256), lds,
-                     st, p, nl, species, ynorm, pb, out);
+                     st, p, nl, species, ynorm, pb, p_begin, p_end, out);
   FLARE_HIP_CHECK(hipGetLastError());
 }
 

@@ -299,9 +299,10 @@ __global__ void __launch_bounds__(64)
 force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
              const int *__restrict__ row_of, const T *__restrict__ ynorm,
              const T *__restrict__ A_in, WSource<T> src,
-             const double *__restrict__ dA, double *__restrict__ pair_f) {
+             const double *__restrict__ dA, int64_t dA_p0, int a0,
+             double *__restrict__ pair_f) {
   extern __shared__ __align__(16) unsigned char smem[];
-  const int i = blockIdx.x;
+  const int i = a0 + blockIdx.x;
   const int lane = threadIdx.x;
   const int H = p.H, NH = p.NH, W = p.W, F = p.F;
   T *A = reinterpret_cast<T *>(smem);
@@ -417,7 +418,7 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
         const int sj = nl.nspec[k];
         inside = pair_inside(p, si, sj, rv);
         if (inside) {
-          const double *d = dA + (int64_t)k * 3 * NH;
+          const double *d = dA + (k - dA_p0) * 3 * NH;
           const T *ub = u + sj * NH;
           int n = sub / H, lm = sub - n * H;  // (n, lm) of o, kept incrementally
           for (int o = sub; o < NH; o += 16) {
@@ -633,15 +634,16 @@ template <typename T, int MODE, int WM>
 static void launch_forces_t(hipStream_t st, const B2Dev &p, const NListDev &nl,
                             const int *species, const int *row_of,
                             const T *ynorm, const void *A, const WSource<T> &src,
-                            const double *dA, double *pair_f) {
+                            const double *dA, int64_t dA_p0, int a0, int a1,
+                            double *pair_f) {
   size_t lds = sizeof(T) * (2 * (size_t)p.W + 2 * (size_t)p.F + (size_t)p.H) +
                sizeof(int) * (size_t)p.nr * p.nr;
   FLARE_REQUIRE(lds <= 64 * 1024, "descriptor settings too large for the force kernel");
   FLARE_REQUIRE(p.nr <= 32, "at most 32 radial channels (n_species * n_max)");
 #define FLARE_FORCE_LAUNCH(NM, LF, EX)                                        \
-  hipLaunchKernelGGL((force_kernel<T, MODE, NM, WM, LF, EX>), dim3(nl.n),     \
+  hipLaunchKernelGGL((force_kernel<T, MODE, NM, WM, LF, EX>), dim3(a1 - a0),   \
                      dim3(kWave), lds, st, p, nl, species, row_of, ynorm,     \
-                     (const T *)A, src, dA, pair_f)
+                     (const T *)A, src, dA, dA_p0, a0, pair_f)
   // compile-time (n_max, l_max) variants of the common adjoint settings
   if (MODE == 1 && p.N == 8 && p.L == 3)
     FLARE_FORCE_LAUNCH(8, 3, true);
@@ -660,41 +662,43 @@ template <typename T, int MODE>
 static void launch_forces_w(hipStream_t st, const B2Dev &p, const NListDev &nl,
                             const int *species, const int *row_of,
                             const T *ynorm, const void *A, const WInput &wi,
-                            const double *dA, double *pair_f) {
+                            const double *dA, int64_t dA_p0, int a0, int a1,
+                            double *pair_f) {
   WSource<T> src{wi.normalized, (const T *)wi.X, (const T *)wi.Z,
                  (const T *)wi.beta1, (const T *)wi.w,
                  (const __bf16 *)wi.xh, (const __bf16 *)wi.xl, wi.kp,
                  wi.norm2, wi.E_row};
   if (wi.mode == 0)
     launch_forces_t<T, MODE, 0>(st, p, nl, species, row_of, ynorm, A, src, dA,
-                                pair_f);
+                                dA_p0, a0, a1, pair_f);
   else if (wi.mode == 3)
     launch_forces_t<T, MODE, 3>(st, p, nl, species, row_of, ynorm, A, src, dA,
-                                pair_f);
+                                dA_p0, a0, a1, pair_f);
   else if (wi.mode == 1)
     launch_forces_t<T, MODE, 1>(st, p, nl, species, row_of, ynorm, A, src, dA,
-                                pair_f);
+                                dA_p0, a0, a1, pair_f);
   else
     launch_forces_t<T, MODE, 2>(st, p, nl, species, row_of, ynorm, A, src, dA,
-                                pair_f);
+                                dA_p0, a0, a1, pair_f);
 }
 
 void launch_forces(hipStream_t st, int prec, int mode, const B2Dev &p,
                    const NListDev &nl, const int *species, const int *row_of,
                    const double *ynorm, const float *ynorm_f, const void *A,
-                   const WInput &wi, const double *dA, double *pair_f) {
-  if (nl.n == 0) return;
+                   const WInput &wi, const double *dA, int64_t dA_p0,
+                   int a0, int a1, double *pair_f) {
+  if (a1 <= a0) return;
   if (prec == kPrecF64) {
     if (mode == 0)
       launch_forces_w<double, 0>(st, p, nl, species, row_of, ynorm, A, wi, dA,
-                                 pair_f);
+                                 dA_p0, a0, a1, pair_f);
     else
       launch_forces_w<double, 1>(st, p, nl, species, row_of, ynorm, A, wi, dA,
-                                 pair_f);
+                                 dA_p0, a0, a1, pair_f);
   } else {
     FLARE_REQUIRE(mode == 1, "the plain (materialised) form runs in FP64 only");
     launch_forces_w<float, 1>(st, p, nl, species, row_of, ynorm_f, A, wi, dA,
-                              pair_f);
+                              dA_p0, a0, a1, pair_f);
   }
   FLARE_HIP_CHECK(hipGetLastError());
 }

@@ -80,8 +80,10 @@ void launch_b2_forward(hipStream_t st, const B2Dev &p, const NListDev &nl,
                        const int *b2_idx, int prec, int out_mode, int kp,
                        void *xh, void *xl, void *A_out, void *B_out,
                        double *norm2_out);
+// dA/dr of pairs [p_begin, p_end) (plain form)
 void launch_pair_dA(hipStream_t st, const B2Dev &p, const NListDev &nl,
-                    const int *species, const double *ynorm, double *out);
+                    const int *species, const double *ynorm, int64_t p_begin,
+                    int64_t p_end, double *out);
 void launch_b2_dervs(hipStream_t st, const B2Dev &p, const NListDev &nl,
                      const int *species, const int *row_of,
                      const double *ynorm, const int *b2_idx, int64_t nq,
@@ -102,12 +104,14 @@ struct WInput {
   const void *xh = nullptr, *xl = nullptr;  // mode 3: X as BF16 hi + lo
   int kp = 0;
 };
-// Per-atom u = dE/dA and pair forces f_ik.  mode 0: plain (reads dA from
-// `dA`), mode 1: adjoint (recomputes the pair basis).
+// Per-atom u = dE/dA and pair forces f_ik for atoms [a_begin, a_end).
+// mode 0: plain (reads dA of pair k at dA[(k - dA_first_pair) * 3 NH]),
+// mode 1: adjoint (recomputes the pair basis).
 void launch_forces(hipStream_t st, int prec, int mode, const B2Dev &p,
                    const NListDev &nl, const int *species, const int *row_of,
                    const double *ynorm, const float *ynorm_f, const void *A,
-                   const WInput &wi, const double *dA, double *pair_f);
+                   const WInput &wi, const double *dA, int64_t dA_first_pair,
+                   int a_begin, int a_end, double *pair_f);
 // F_a = sum_k (f_ak - f_rev(ak)) into forces (stride 3; may be NULL) and
 // per-atom virial rows (9 values, V[a][b] = -sum_k r_a f_b)
 void launch_gather(hipStream_t st, const NListDev &nl, const double *pair_f,

@@ -97,3 +97,30 @@ def config4(seed=4, n_sparse=2000):
     return dict(cell=cell, species=species, positions=pos, cutoff=4.5,
                 settings=[3, 8, 3], sparse_atoms=sparse_atoms,
                 alpha=rng.normal(0.0, 1.0, n_sparse), sigma=2.0, power=2)
+
+
+def rsa_positions(n, box, d_min, seed):
+    """Random sequential addition of n points in a periodic cube (edge box)
+    with minimum distance d_min, from the library's deterministic host
+    generator (flare_synth_rsa)."""
+    import ctypes
+    from . import _capi
+    pos = np.empty((n, 3), dtype=np.float64)
+    _capi.check(_capi.lib().flare_synth_rsa(int(n), float(box), float(d_min),
+                                            int(seed), 0,
+                                            pos.ctypes.data_as(ctypes.c_void_p)))
+    return pos
+
+
+def config5(seed=5, n=262144, density=0.06, d_min=2.2):
+    """Disordered 3-species box: 262,144 atoms at 0.06 atoms/A^3 (cube of
+    (n / density)^(1/3) = 163.5 A), random sequential addition with a 2.2 A
+    minimum distance, species drawn uniformly; B2 {3, 8, 6}, rc 5.5
+    (F = 2,100); per-species symmetric beta with entries of std 1e-4."""
+    box = (n / density) ** (1.0 / 3.0)
+    pos = rsa_positions(n, box, d_min, seed)
+    rng = np.random.default_rng(seed + 1000)
+    species = rng.integers(0, 3, n).astype(np.int32)
+    beta = symmetric_beta(3, 2100, 1e-4, seed + 2000)
+    return dict(cell=np.eye(3) * box, species=species, positions=pos, cutoff=5.5,
+                settings=[3, 8, 6], beta=beta)

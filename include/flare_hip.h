@@ -106,6 +106,11 @@ int flare_ctx_synchronize(flare_ctx *ctx);
  * (flare_ctx_synchronize waits); 1 (default): they return with results
  * ready.  No counterpart in the reference (its calls are synchronous). */
 int flare_ctx_set_blocking(flare_ctx *ctx, int blocking);
+/* Memory budget (bytes, default 8 GiB) of the per-pair derivative tensor
+ * that the plain form materialises: larger lists are processed in ranges of
+ * atoms.  Plays the role of pair_flare_kokkos' batch size
+ * (pair_flare_kokkos.cpp:117-124); results do not depend on it. */
+int flare_ctx_set_plain_chunk(flare_ctx *ctx, int64_t bytes);
 /* Stage timing: with profiling enabled every stage (nlist, forward, gemm,
  * energy_w, pair_dA, forces_*, gather_reduce) is bracketed by HIP events on
  * the context stream; enabling or disabling resets the totals.  Stage
@@ -235,6 +240,15 @@ int flare_envs_envs(flare_ctx *ctx, int n_species, int n_descriptors,
  * order, S x beta_size */
 int flare_sgp_mapping_coefficients(flare_ctx *ctx, const flare_sgp *gp,
                                    double *d_out);
+
+/* ---- synthetic structures ------------------------------------------------
+ * No counterpart in the reference: generator of the disordered benchmark
+ * box.  Random sequential addition of n points in a periodic cube of edge
+ * box with minimum distance d_min (host only, deterministic for a seed);
+ * positions: host n x 3.  max_attempts <= 0: unlimited.  FLARE_EINVAL when
+ * the box is thinner than three grid cells or the attempts run out. */
+int flare_synth_rsa(int n, double box, double d_min, uint64_t seed,
+                    int64_t max_attempts, double *positions);
 
 #ifdef __cplusplus
 }

@@ -114,3 +114,47 @@ def test_config2_full(orc):
               f"force RMS {rms * 1e3:.3e} meV/A max {np.abs(f - f0).max() * 1e3:.3e}",
               flush=True)
         assert rms < 1e-3  # 1 meV/A
+
+
+def test_plain_form_in_atom_ranges(orc):
+    # a derivative budget of ~40 pairs forces the plain form through many
+    # atom ranges (and single-atom ranges); results must not change
+    from flare_pp_amd import _capi
+    cell, species, pos = small_two_species(seed=9)
+    pair = make_pair(2, 4, 3, seed=4)
+    e0, f0, v0 = pair.compute(cell, species, pos, form="plain")[:3]
+    ctx = _capi.context(0)
+    per_pair = 3 * (4 * 16) * 8
+    try:
+        for budget in (40 * per_pair, 1):
+            ctx.set_plain_chunk(budget)
+            e, f, v = pair.compute(cell, species, pos, form="plain")[:3]
+            assert e == e0 and np.array_equal(f, f0) and np.array_equal(v, v0)
+    finally:
+        ctx.set_plain_chunk(8 << 30)
+    e1, f1, v1, _ = oracle_pair(orc, pair, cell, species, pos)
+    assert nrel(f0, f1) < 1e-10
+
+
+def test_config5_scaleup_forms():
+    # 262,144-atom 3-species RSA liquid, l_max 6 (F = 2,100): the plain form
+    # runs through atom ranges (its dA/dr is ~100 GB); the FP64 adjoint agrees
+    # with it norm-wise to 1e-10 and the mixed form stays within 1 meV/A RMS.
+    # The plain form is pinned to the oracle on the smaller cases above.
+    c = synth.config5()
+    packed = np.stack([pack_beta(b) for b in c["beta"]])
+    pair = PairFLARE.from_coefficients(2, 3, 8, 6, np.full((3, 3), 5.5), packed)
+    cell, species, pos = c["cell"], c["species"], c["positions"]
+    n = len(species)
+    print(f"config5 n={n}", flush=True)
+    e0, f0, v0 = pair.compute(cell, species, pos, form="plain")
+    assert np.abs(f0.sum(axis=0)).max() < 1e-9 * np.abs(f0).sum()
+    e1, f1, v1 = pair.compute(cell, species, pos, form="adjoint_fp64")
+    assert abs(e1 - e0) < 1e-10 * abs(e0) and nrel(f1, f0) < 1e-10
+    assert nrel(v1, v0) < 1e-10
+    e2, f2, v2 = pair.compute(cell, species, pos, form="adjoint_mixed")
+    d = (f2 - f0) * 1e3
+    rms = float(np.sqrt(np.mean(d ** 2)))
+    print(f"mixed: energy {abs(e2 - e0) / n * 1e3:.3e} meV/atom, force RMS "
+          f"{rms:.3e} max {np.abs(d).max():.3e} meV/A", flush=True)
+    assert rms < 1.0
