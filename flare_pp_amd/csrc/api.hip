@@ -330,6 +330,27 @@ struct flare_nlist {
   int species_count[flare::kMaxSpecies] = {0};
   int n_species_present = 0;
   bool built = false;
+  // a build is enqueued without waiting; its host-side summary (pinned
+  // stats_host, ready at stats_ev) is read on the list's next use
+  bool pending = false;
+  int *stats_host = nullptr;
+  hipEvent_t stats_ev = nullptr;
+  const int *d_species = nullptr;
+
+  ~flare_nlist() {
+    if (stats_host) (void)hipHostFree(stats_host);
+    if (stats_ev) (void)hipEventDestroy(stats_ev);
+  }
+  void resolve() {
+    if (!pending) return;
+    pending = false;
+    built = false;
+    ctx->use();
+    FLARE_HIP_CHECK(hipEventSynchronize(stats_ev));
+    n_species_present = flare::nlist_resolve(ctx->stream, b, n, d_species, wh,
+                                             stats_host, species_count);
+    built = true;
+  }
 
   flare::NListDev dev() const {
     flare::NListDev d;
@@ -384,8 +405,14 @@ static flare::B2Dev make_b2(const flare_b2_params *p, int pair_pred) {
   return d;
 }
 
-static void check_species(const flare_nlist *nl, int S) {
+// Completes a pending build (host-side summary) before the list is used.
+static void use_nlist(const flare_nlist *nl) {
   REQUIRE(nl && nl->built, "neighbor list not built");
+  const_cast<flare_nlist *>(nl)->resolve();
+}
+
+static void check_species(const flare_nlist *nl, int S) {
+  use_nlist(nl);
   for (int s = S; s < flare::kMaxSpecies; s++)
     REQUIRE(nl->species_count[s] == 0,
             "structure holds species codes beyond n_species");
@@ -717,23 +744,35 @@ int flare_nlist_build(flare_ctx *ctx, flare_nlist *nl, int n_atoms,
     REQUIRE(cutoff > 0 && std::isfinite(cutoff), "cutoff must be positive");
     for (int q = 0; q < 9; q++) REQUIRE(std::isfinite(cell[q]), "non-finite cell");
     ctx->use();
+    nl->resolve();  // an earlier build's fallback must not run on new data
     nl->built = false;
     nl->wh = make_wrap(cell, cutoff);
     REQUIRE(nl->wh.volume > 0, "singular cell");
     nl->n = n_atoms;
+    nl->ctx = ctx;
+    nl->d_species = d_species;
+    if (!nl->stats_host) {
+      FLARE_HIP_CHECK(hipHostMalloc((void **)&nl->stats_host,
+                                    flare::kNlistStats * sizeof(int), hipHostMallocDefault));
+      FLARE_HIP_CHECK(hipEventCreateWithFlags(&nl->stats_ev, hipEventDisableTiming));
+    }
     ctx->mark_begin("nlist");
-    nl->n_species_present = flare::nlist_build_device(
-        ctx->stream, nl->b, n_atoms, d_positions, d_species, nl->wh,
-        nl->species_count);
+    flare::nlist_build_async(ctx->stream, nl->b, n_atoms, d_positions, d_species,
+                             nl->wh, nl->stats_host, nl->stats_ev);
     ctx->mark_end();
+    nl->pending = true;
     nl->built = true;
+    if (ctx->blocking) {
+      nl->resolve();
+      ctx->sync();
+    }
   });
 }
 
 int flare_nlist_get_view(const flare_nlist *nl, flare_nlist_view *v) {
   return guarded([&] {
     REQUIRE(nl && v, "null argument");
-    REQUIRE(nl->built, "neighbor list not built");
+    use_nlist(nl);
     v->n_atoms = nl->n;
     v->n_pairs = nl->b.P;
     v->sweep = nl->wh.sweep;

@@ -176,8 +176,9 @@ b2_forward_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
 // result D[i][j] sits in lane j + 16 (i / 4), element i % 4 for the FP32
 // instruction and in lane j + 16 (i % 4), element i / 4 for the FP64 one
 // (both measured on gfx950).
-// FP32: each K step is accumulated from zero and Kahan-added into the
-// running (sum, compensation) pair; FP64 accumulates in the matrix core.
+// FP32: every two K steps (8 pairs) are accumulated from zero and
+// Kahan-added into the running (sum, compensation) pair; FP64 accumulates
+// in the matrix core.
 
 typedef __attribute__((ext_vector_type(4))) float f32x4;
 typedef __attribute__((ext_vector_type(4))) double f64x4;
@@ -378,26 +379,37 @@ b2_forward_mfma_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
       }
     }
     __syncthreads();
+    // K steps of 4 pairs; FP32 sums two steps (8 pairs) in the matrix core
+    // and Kahan-adds them into the running (sum, compensation) pair
     const int steps = (cnt + 3) >> 2;
-    for (int st = 0; st < steps; st++) {
+    for (int st = 0; st < steps; st += 2) {
       const int kk = 4 * st + krow;
-      T a[MT], b[NT];
+      const bool two = st + 1 < steps;  // uniform
+      T a[MT], b[NT], a2[MT], b2[NT];
 #pragma unroll
-      for (int m = 0; m < MT; m++) a[m] = Gs[kk * KP + m * 16 + col];
+      for (int m = 0; m < MT; m++) {
+        a[m] = Gs[kk * KP + m * 16 + col];
+        a2[m] = two ? Gs[(kk + 4) * KP + m * 16 + col] : T(0);
+      }
 #pragma unroll
-      for (int n = 0; n < NT; n++) b[n] = Ys[kk * YP + n * 16 + col];
+      for (int n = 0; n < NT; n++) {
+        b[n] = Ys[kk * YP + n * 16 + col];
+        b2[n] = two ? Ys[(kk + 4) * YP + n * 16 + col] : T(0);
+      }
 #pragma unroll
       for (int m = 0; m < MT; m++)
 #pragma unroll
         for (int n = 0; n < NT; n++) {
           if (KAHAN) {
             V part = Mma<T>::run(a[m], b[n], V{0, 0, 0, 0});
+            part = Mma<T>::run(a2[m], b2[n], part);
             V y = part - cmp[m][n];
             V t = acc[m][n] + y;
             cmp[m][n] = (t - acc[m][n]) - y;
             acc[m][n] = t;
           } else {
             acc[m][n] = Mma<T>::run(a[m], b[n], acc[m][n]);
+            acc[m][n] = Mma<T>::run(a2[m], b2[n], acc[m][n]);
           }
         }
     }

@@ -75,6 +75,37 @@ __device__ inline void stage_lds(int n, Fn &&load, T *dst) {
   }
 }
 
+// dst[0..n) = src[0..n) (dst in LDS) in 16-byte vectors, KB per lane per
+// batch, all loads of a batch issued before its stores; scalar when src or
+// dst is not 16-byte aligned or n * sizeof(T) is not a multiple of 16.
+template <int KB, typename T>
+__device__ inline void stage_lds_vec(int n, const T *__restrict__ src, T *dst) {
+  const bool vec = ((n * (int)sizeof(T)) & 15) == 0 &&
+                   ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15) == 0;
+  if (!vec) {
+    stage_lds<KB>(n, [&](int o) { return src[o]; }, dst);
+    return;
+  }
+  const int lane = threadIdx.x;
+  const int nv = n * (int)sizeof(T) / 16;
+  const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+  uint4 *d4 = reinterpret_cast<uint4 *>(dst);
+  for (int base = 0; base < nv; base += KB * kWave) {
+    uint4 t[KB];
+#pragma unroll
+    for (int j = 0; j < KB; j++) {
+      const int o = base + lane + j * kWave;
+      t[j] = uint4{0, 0, 0, 0};
+      if (o < nv) t[j] = s4[o];
+    }
+#pragma unroll
+    for (int j = 0; j < KB; j++) {
+      const int o = base + lane + j * kWave;
+      if (o < nv) d4[o] = t[j];
+    }
+  }
+}
+
 // u[c, lm] = sum_c2 W_l[c][c2] A[c2, lm] with W_l[c][c2] =
 // w[p(c,c2) (L+1) + l] (1 + d_{c c2}) (lammps_descriptor.cpp:272-303), one
 // matrix-core product per l: M = channels (c), K = channels (c2), N = the
@@ -334,10 +365,66 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
     const int c = q / p.nr, c2 = q - c * p.nr;
     ptab[q] = b2_pair_index(c, c2, p.nr) * (p.L + 1);
   }
-  stage_lds<8>(W, [&](int o) { return Ai[o]; }, A);
+  stage_lds_vec<4>(W, Ai, A);
   if (WM == 0) {
     const T *wi = src.w + (int64_t)row * F;
-    stage_lds<8>(F, [&](int f) { return wi[f]; }, w);
+    stage_lds_vec<4>(F, wi, w);
+  } else if (WM == 3 && (F & 3) == 0 && F <= 16 * kWave) {
+    // mixed form, rows of up to 1,024 features: each lane keeps its 4 x 4
+    // features of x (= hi + lo) and z in registers through E and w
+    const double n2 = src.norm2[row];
+    const uint2 *xh = reinterpret_cast<const uint2 *>(src.xh + (int64_t)row * src.kp);
+    const uint2 *xl = reinterpret_cast<const uint2 *>(src.xl + (int64_t)row * src.kp);
+    const float4 *z4 = reinterpret_cast<const float4 *>(src.Z + (int64_t)row * F);
+    const int F4 = F >> 2;
+    uint2 th[4], tl[4];
+    float4 zv[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int q = lane + j * kWave;
+      th[j] = uint2{0, 0};
+      tl[j] = uint2{0, 0};
+      zv[j] = float4{0, 0, 0, 0};
+      if (q < F4) {
+        th[j] = xh[q];
+        tl[j] = xl[q];
+        zv[j] = z4[q];
+      }
+    }
+    float xv[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      xv[j][0] = __uint_as_float(th[j].x << 16) + __uint_as_float(tl[j].x << 16);
+      xv[j][1] = __uint_as_float(th[j].x & 0xffff0000u) + __uint_as_float(tl[j].x & 0xffff0000u);
+      xv[j][2] = __uint_as_float(th[j].y << 16) + __uint_as_float(tl[j].y << 16);
+      xv[j][3] = __uint_as_float(th[j].y & 0xffff0000u) + __uint_as_float(tl[j].y & 0xffff0000u);
+    }
+    // four-term FP32 partial sums, accumulated in FP64
+    double d = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+      d += (double)(xv[j][0] * zv[j].x + xv[j][1] * zv[j].y + xv[j][2] * zv[j].z +
+                    xv[j][3] * zv[j].w);
+    d = wave_sum(d);
+    double E = 0, sx = 0, sz = 0;
+    if (!(n2 < 1e-8)) {
+      const double bn = sqrt(n2);
+      E = src.normalized ? d : d / n2;
+      sz = src.normalized ? 2.0 / bn : 2.0 / n2;
+      sx = -E * sz;
+    }
+    const float szf = (float)sz, sxf = (float)sx;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int q = lane + j * kWave;
+      if (q < F4) {
+        w[4 * q + 0] = szf * zv[j].x + sxf * xv[j][0];
+        w[4 * q + 1] = szf * zv[j].y + sxf * xv[j][1];
+        w[4 * q + 2] = szf * zv[j].z + sxf * xv[j][2];
+        w[4 * q + 3] = szf * zv[j].w + sxf * xv[j][3];
+      }
+    }
+    if (lane == 0) src.E_row[row] = E;
   } else {
     const double n2 = src.norm2[row];
     if (WM == 3) {
@@ -371,12 +458,12 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
       }
     } else {
       const T *x = src.X + (int64_t)row * F;
-      stage_lds<8>(F, [&](int f) { return x[f]; }, xs);
+      stage_lds_vec<4>(F, x, xs);
     }
     // z (= beta X, or beta_s for power 1) staged into w, turned into w below
     const T *z = WM != 2 ? src.Z + (int64_t)row * F
                          : src.beta1 + (int64_t)species[i] * F;
-    stage_lds<8>(F, [&](int f) { return z[f]; }, w);
+    stage_lds_vec<4>(F, z, w);
     __syncthreads();
     if (n2 < 1e-8) {
       for (int f = lane; f < F; f += kWave) w[f] = 0;

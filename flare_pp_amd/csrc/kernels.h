@@ -62,13 +62,22 @@ struct NListBuffers {
 };
 
 // neighbors.hip
-// Neighbor list (count, cum, idx, nspec, rel, center, rev, keys), wrapped
-// positions and the species-grouped row map row_of[i], atom_of_row[row],
-// species_of_row[row]; per-species counts (kMaxSpecies ints) are copied to
-// host_counts.  Returns the number of species present (max species + 1).
-int nlist_build_device(hipStream_t st, NListBuffers &b, int n,
-                       const double *d_pos, const int *d_species,
-                       const WrapHost &wh, int *host_counts);
+// Ints of the host-side build summary: species counts (kMaxSpecies), a
+// bad-species flag, the largest over-capacity neighbor count, the pair total.
+constexpr int kNlistStats = kMaxSpecies + 3;
+// Enqueues the neighbor list (count, cum, idx, nspec, rel, center, rev,
+// keys), wrapped positions and the species-grouped row map row_of[i],
+// atom_of_row[row], species_of_row[row] without waiting; the summary is
+// copied to stats_host (pinned, kNlistStats ints) before stats_ev.
+void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_pos,
+                       const int *d_species, const WrapHost &wh, int *stats_host,
+                       hipEvent_t stats_ev);
+// After stats_ev: validates the species codes, sets b.P, completes the list
+// with the two-pass path when an atom had more neighbors than the fused
+// path holds, copies the species counts to host_counts and returns the
+// number of species present (max species + 1).
+int nlist_resolve(hipStream_t st, NListBuffers &b, int n, const int *d_species,
+                  const WrapHost &wh, const int *stats, int *host_counts);
 
 // descriptors.hip
 // A (atom order), GEMM rows (species-grouped: B when out_mode 0, B/|B| when

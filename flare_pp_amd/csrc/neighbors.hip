@@ -36,29 +36,116 @@ __device__ inline void rowmul3(const double *v, const double *b, double *o) {
   }
 }
 
-__global__ void wrap_bin_kernel(int n, const double *__restrict__ pos,
-                                WrapMats m, double *__restrict__ wrapped,
-                                int *__restrict__ bin_of,
-                                int *__restrict__ bin_count) {
+// Per atom: wrapped position and bin, counted into bin_count[bin + 1]; per
+// block of kRowBlock atoms the species counts (bcount, bad code -> *bad).
+constexpr int kRowBlock = 256;
+
+__global__ void __launch_bounds__(kRowBlock)
+wrap_bin_kernel(int n, const double *__restrict__ pos, WrapMats m,
+                const int *__restrict__ species, double *__restrict__ wrapped,
+                int *__restrict__ bin_of, int *__restrict__ bin_count,
+                int *__restrict__ bcount, int *__restrict__ bad) {
 #pragma clang fp contract(off)
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
-  double t[3], rel[3], rw[3], t2[3], w[3];
-  rowmul3(p, m.ct, t);
-  rowmul3(t, m.cdi, rel);
-  for (int c = 0; c < 3; c++) rw[c] = rel[c] - floor(rel[c]);
-  rowmul3(rw, m.cd, t2);
-  rowmul3(t2, m.cti, w);
-  for (int c = 0; c < 3; c++) wrapped[3 * i + c] = w[c];
-  int b[3];
-  for (int c = 0; c < 3; c++) {
-    int v = (int)floor(rw[c] * m.nb[c]);
-    b[c] = v < 0 ? 0 : (v >= m.nb[c] ? m.nb[c] - 1 : v);
+  __shared__ int cnt[kMaxSpecies];
+  const int t = threadIdx.x;
+  if (t < kMaxSpecies) cnt[t] = 0;
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + t;
+  if (i < n) {
+    double p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    double tt[3], rel[3], rw[3], t2[3], w[3];
+    rowmul3(p, m.ct, tt);
+    rowmul3(tt, m.cdi, rel);
+    for (int c = 0; c < 3; c++) rw[c] = rel[c] - floor(rel[c]);
+    rowmul3(rw, m.cd, t2);
+    rowmul3(t2, m.cti, w);
+    for (int c = 0; c < 3; c++) wrapped[3 * i + c] = w[c];
+    int b[3];
+    for (int c = 0; c < 3; c++) {
+      int v = (int)floor(rw[c] * m.nb[c]);
+      b[c] = v < 0 ? 0 : (v >= m.nb[c] ? m.nb[c] - 1 : v);
+    }
+    const int bin = (b[0] * m.nb[1] + b[1]) * m.nb[2] + b[2];
+    bin_of[i] = bin;
+    atomicAdd(&bin_count[bin + 1], 1);
+    const int sp = species[i];
+    if (sp < 0 || sp >= kMaxSpecies)
+      atomicOr(bad, 1);
+    else
+      atomicAdd(&cnt[sp], 1);
   }
-  const int bin = (b[0] * m.nb[1] + b[1]) * m.nb[2] + b[2];
-  bin_of[i] = bin;
-  atomicAdd(&bin_count[bin + 1], 1);
+  __syncthreads();
+  if (t < kMaxSpecies) bcount[blockIdx.x * kMaxSpecies + t] = cnt[t];
+}
+
+// Zero the build's counters in one launch: a[0..na), b[0..nb), c[0..nc).
+__global__ void clear3_kernel(int *__restrict__ a, int na, int *__restrict__ b,
+                              int nb, int *__restrict__ c, int nc) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < na) a[q] = 0;
+  if (q < nb) b[q] = 0;
+  if (q < nc) c[q] = 0;
+}
+
+// Single-workgroup exclusive scan: out[0..n] with out[n] = total, each of
+// the 1024 threads summing a contiguous segment.  Used for small arrays
+// (bins, per-atom counts) where one workgroup beats a device-wide scan.
+constexpr int kScanThreads = 1024;
+
+__device__ inline void block_exclusive_scan(int n, const int *__restrict__ in,
+                                            int *__restrict__ out, int *sh) {
+  const int t = threadIdx.x;
+  const int per = (n + kScanThreads - 1) / kScanThreads;
+  const int lo = min(n, t * per), hi = min(n, lo + per);
+  int s = 0;
+  for (int q = lo; q < hi; q++) s += in[q];
+  sh[t] = s;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {  // Hillis-Steele
+    int v = t >= off ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += v;
+    __syncthreads();
+  }
+  int run = sh[t] - s;  // exclusive prefix of this segment
+  for (int q = lo; q < hi; q++) {
+    const int v = in[q];
+    out[q] = run;
+    run += v;
+  }
+  if (t == kScanThreads - 1) out[n] = sh[t];
+  __syncthreads();
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+scan_small_kernel(int n, const int *__restrict__ in, int *__restrict__ out) {
+  __shared__ int sh[kScanThreads];
+  block_exclusive_scan(n, in, out, sh);
+}
+
+// After the count scan: the species scan of the per-block counts (boff,
+// counts[s]) and the pair total cum[n] into stat_P, so that every number
+// the host needs sits in one small buffer.
+__global__ void __launch_bounds__(64 * kMaxSpecies)
+nbr_stats_kernel(int n, const int *__restrict__ cum, int nblk,
+                 const int *__restrict__ bcount, int *__restrict__ boff,
+                 int *__restrict__ counts, int *__restrict__ stat_P) {
+  const int lane = threadIdx.x & 63, sp = threadIdx.x >> 6;
+  if (threadIdx.x == 0) *stat_P = cum[n];
+  int run = 0;
+  for (int b0 = 0; b0 < nblk; b0 += 64) {
+    const int b = b0 + lane;
+    const int v = b < nblk ? bcount[b * kMaxSpecies + sp] : 0;
+    int incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      int u = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += u;
+    }
+    if (b < nblk) boff[b * kMaxSpecies + sp] = run + incl - v;
+    run += __shfl(incl, 63, 64);
+  }
+  if (lane == 0) counts[sp] = run;
 }
 
 // bin_atoms / bin_pos: atoms and their wrapped positions grouped by bin
@@ -382,9 +469,11 @@ nbr_enum_sort_kernel(int n, WrapMats m, const double *__restrict__ wrapped,
 }
 
 // Padded sorted rows -> final CSR arrays, with the reverse-pair index found
-// by binary search in the neighbor's sorted row.  One wavefront per atom.
+// by binary search in the neighbor's sorted row.  One wavefront per atom;
+// does nothing when *overflow is set.
 __global__ void __launch_bounds__(256)
-nbr_compact_kernel(int n, int sweep, const int *__restrict__ cum,
+nbr_compact_kernel(int n, int sweep, const int *__restrict__ overflow,
+                   const int *__restrict__ cum,
                    const int *__restrict__ count,
                    const int64_t *__restrict__ keys_pad,
                    const double *__restrict__ rel_pad,
@@ -394,7 +483,9 @@ nbr_compact_kernel(int n, int sweep, const int *__restrict__ cum,
                    int *__restrict__ center_out, int *__restrict__ rev_out) {
   const int lane = threadIdx.x & (kWave - 1);
   const int i = blockIdx.x * 4 + threadIdx.x / kWave;
-  if (i >= n) return;
+  // some atom has more than kNbrCap neighbors: the two-pass path rebuilds
+  // the arrays (the output capacity n * kNbrCap may then be too small)
+  if (i >= n || *overflow) return;
   const int b = cum[i], cnt = cum[i + 1] - b;
   const int64_t row = (int64_t)i * kNbrCap;
   for (int q = lane; q < cnt; q += kWave) {
@@ -487,50 +578,9 @@ __global__ void nbr_reverse_kernel(int64_t P, int sweep,
 }
 
 // Species-grouped row map (b2.cpp:104-176 regroup: atoms of species 0 in
-// original order, then species 1, ...): per-block species counts, a scan
-// over blocks, and per-block ranks from wavefront ballots.
-constexpr int kRowBlock = 256;
-
-__global__ void __launch_bounds__(kRowBlock)
-species_count_kernel(int n, const int *__restrict__ species,
-                     int *__restrict__ bcount, int *__restrict__ bad) {
-  __shared__ int cnt[kMaxSpecies];
-  const int t = threadIdx.x;
-  if (t < kMaxSpecies) cnt[t] = 0;
-  __syncthreads();
-  const int a = blockIdx.x * kRowBlock + t;
-  if (a < n) {
-    const int s = species[a];
-    if (s < 0 || s >= kMaxSpecies)
-      atomicOr(bad, 1);
-    else
-      atomicAdd(&cnt[s], 1);
-  }
-  __syncthreads();
-  if (t < kMaxSpecies) bcount[blockIdx.x * kMaxSpecies + t] = cnt[t];
-}
-
-// one wavefront per species: exclusive scan of the block counts
-__global__ void __launch_bounds__(64 * kMaxSpecies)
-species_scan_kernel(int nblk, const int *__restrict__ bcount,
-                    int *__restrict__ boff, int *__restrict__ counts) {
-  const int lane = threadIdx.x & 63, s = threadIdx.x >> 6;
-  int run = 0;
-  for (int b0 = 0; b0 < nblk; b0 += 64) {
-    const int b = b0 + lane;
-    const int v = b < nblk ? bcount[b * kMaxSpecies + s] : 0;
-    int incl = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      int u = __shfl_up(incl, off, 64);
-      if (lane >= off) incl += u;
-    }
-    if (b < nblk) boff[b * kMaxSpecies + s] = run + incl - v;
-    run += __shfl(incl, 63, 64);
-  }
-  if (lane == 0) counts[s] = run;
-}
-
+// original order, then species 1, ...): per-block species counts (in
+// wrap_bin_kernel), their scan (nbr_stats_kernel), and per-block ranks from
+// wavefront ballots.
 __global__ void __launch_bounds__(kRowBlock)
 species_rows_kernel(int n, const int *__restrict__ species,
                     const int *__restrict__ boff, const int *__restrict__ counts,
@@ -570,12 +620,14 @@ static void exclusive_scan(hipStream_t st, DevBuf &tmp, const int *in, int *out,
 }
 
 // ---------------------------------------------------------------------------
-// Host driver.  One host synchronisation per build: the pair total, the
-// largest per-atom count and the species counts are read together.
+// Host driver.  The build is enqueued without waiting: the fused path
+// writes the final arrays into capacity n * kNbrCap, and the numbers the
+// host needs (species counts, bad-code flag, largest over-capacity count,
+// pair total) are copied to stats_host behind an event.  nlist_resolve
+// reads them and, in the rare case of an over-capacity atom, rebuilds the
+// arrays with the two-pass path.
 
-int nlist_build_device(hipStream_t st, NListBuffers &b, int n,
-                       const double *d_pos, const int *d_species,
-                       const WrapHost &wh, int *host_counts) {
+static WrapMats wrap_mats(const WrapHost &wh) {
   WrapMats m;
   for (int q = 0; q < 9; q++) {
     m.ct[q] = wh.ct[q];
@@ -587,8 +639,16 @@ int nlist_build_device(hipStream_t st, NListBuffers &b, int n,
   for (int k = 0; k < 3; k++) m.nb[k] = wh.nb[k];
   m.sweep = wh.sweep;
   m.cutoff = wh.cutoff;
+  return m;
+}
+
+void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_pos,
+                       const int *d_species, const WrapHost &wh, int *stats_host,
+                       hipEvent_t stats_ev) {
+  const WrapMats m = wrap_mats(wh);
   const int nbins = wh.nb[0] * wh.nb[1] * wh.nb[2];
   const int nblk = std::max(1, (n + kRowBlock - 1) / kRowBlock);
+  const size_t cap = (size_t)std::max(n, 1) * kNbrCap;
 
   b.wrapped.ensure((size_t)std::max(n, 1) * 3 * sizeof(double));
   b.bin_of.ensure((size_t)std::max(n, 1) * sizeof(int));
@@ -599,36 +659,43 @@ int nlist_build_device(hipStream_t st, NListBuffers &b, int n,
   b.bin_pos.ensure((size_t)std::max(n, 1) * 3 * sizeof(double));
   b.count.ensure((size_t)(n + 1) * sizeof(int));
   b.cum.ensure((size_t)(n + 1) * sizeof(int));
-  b.keys_pad.ensure((size_t)std::max(n, 1) * kNbrCap * sizeof(int64_t));
-  b.rel_pad.ensure((size_t)std::max(n, 1) * kNbrCap * 4 * sizeof(double));
+  b.keys_pad.ensure(cap * sizeof(int64_t));
+  b.rel_pad.ensure(cap * 4 * sizeof(double));
+  b.keys.ensure(cap * sizeof(int64_t));
+  b.rel.ensure(cap * 4 * sizeof(double));
+  b.idx.ensure(cap * sizeof(int));
+  b.nspec.ensure(cap * sizeof(int));
+  b.center.ensure(cap * sizeof(int));
+  b.rev.ensure(cap * sizeof(int));
   b.row_of.ensure((size_t)std::max(n, 1) * sizeof(int));
   b.atom_of_row.ensure((size_t)std::max(n, 1) * sizeof(int));
   b.species_of_row.ensure((size_t)std::max(n, 1) * sizeof(int));
-  // species_count: kMaxSpecies counts, bad-code flag, largest pair count
-  b.species_count.ensure((kMaxSpecies + 2) * sizeof(int));
+  // counts[0..8): species counts, [8]: bad species code, [9]: largest
+  // over-capacity neighbor count, [10]: pair total
+  b.species_count.ensure(kNlistStats * sizeof(int));
   b.species_blk.ensure((size_t)2 * nblk * kMaxSpecies * sizeof(int));
   int *bcount = b.species_blk.as<int>(), *boff = bcount + nblk * kMaxSpecies;
   int *counts = b.species_count.as<int>();
-  int *maxcount = counts + kMaxSpecies + 1;
+  int *overflow = counts + kMaxSpecies + 1;
 
-  FLARE_HIP_CHECK(hipMemsetAsync(counts, 0, (kMaxSpecies + 2) * sizeof(int), st));
-  FLARE_HIP_CHECK(hipMemsetAsync(b.bin_count.ptr, 0, (nbins + 2) * sizeof(int), st));
-  FLARE_HIP_CHECK(hipMemsetAsync(b.bin_fill.ptr, 0, (nbins + 1) * sizeof(int), st));
+  const int nclear = std::max(nbins + 2, kNlistStats);
+  hipLaunchKernelGGL(clear3_kernel, dim3((nclear + 255) / 256), dim3(256), 0, st,
+                     counts, kNlistStats, b.bin_count.as<int>(), nbins + 2,
+                     b.bin_fill.as<int>(), nbins + 1);
   const int tb = 256;
   const int g1 = (n + tb - 1) / tb;
   const int gw = (n + 3) / 4;
-  if (n > 0) {
-    hipLaunchKernelGGL(wrap_bin_kernel, dim3(g1), dim3(tb), 0, st, n, d_pos, m,
-                       b.wrapped.as<double>(), b.bin_of.as<int>(),
-                       b.bin_count.as<int>());
-    hipLaunchKernelGGL(species_count_kernel, dim3(nblk), dim3(kRowBlock), 0, st,
-                       n, d_species, bcount, counts + kMaxSpecies);
-    hipLaunchKernelGGL(species_scan_kernel, dim3(1), dim3(64 * kMaxSpecies), 0,
-                       st, nblk, bcount, boff, counts);
-  }
+  if (n > 0)
+    hipLaunchKernelGGL(wrap_bin_kernel, dim3(g1), dim3(kRowBlock), 0, st, n, d_pos,
+                       m, d_species, b.wrapped.as<double>(), b.bin_of.as<int>(),
+                       b.bin_count.as<int>(), bcount, counts + kMaxSpecies);
   // bin_count[1..nbins] holds the counts; bin_count has one spare zero entry
-  exclusive_scan(st, b.scan_tmp, b.bin_count.as<int>() + 1, b.bin_start.as<int>(),
-                 nbins);
+  if (nbins <= 16 * kScanThreads)
+    hipLaunchKernelGGL(scan_small_kernel, dim3(1), dim3(kScanThreads), 0, st, nbins,
+                       b.bin_count.as<int>() + 1, b.bin_start.as<int>());
+  else
+    exclusive_scan(st, b.scan_tmp, b.bin_count.as<int>() + 1, b.bin_start.as<int>(),
+                   nbins);
   if (n > 0) {
     hipLaunchKernelGGL(bin_fill_kernel, dim3(g1), dim3(tb), 0, st, n,
                        b.bin_of.as<int>(), b.bin_start.as<int>(),
@@ -641,47 +708,58 @@ int nlist_build_device(hipStream_t st, NListBuffers &b, int n,
                          b.bin_start.as<int>(), b.bin_atoms.as<int>(),
                          b.bin_pos.as<double>(), b.count.as<int>(),
                          b.keys_pad.as<int64_t>(), b.rel_pad.as<double>(),
-                         maxcount);
+                         overflow);
     else
       hipLaunchKernelGGL(nbr_enum_sort_kernel<false>, dim3(gw), dim3(256), 0, st,
                          n, m, b.wrapped.as<double>(), b.bin_of.as<int>(),
                          b.bin_start.as<int>(), b.bin_atoms.as<int>(),
                          b.bin_pos.as<double>(), b.count.as<int>(),
                          b.keys_pad.as<int64_t>(), b.rel_pad.as<double>(),
-                         maxcount);
+                         overflow);
   }
   FLARE_HIP_CHECK(hipMemsetAsync(b.count.as<int>() + n, 0, sizeof(int), st));
   exclusive_scan(st, b.scan_tmp, b.count.as<int>(), b.cum.as<int>(), n);
-  FLARE_HIP_CHECK(hipGetLastError());
-  int hc[kMaxSpecies + 2];
-  int P = 0;
-  FLARE_HIP_CHECK(hipMemcpyAsync(hc, counts, sizeof(hc), hipMemcpyDeviceToHost, st));
-  FLARE_HIP_CHECK(hipMemcpyAsync(&P, b.cum.as<int>() + n, sizeof(int),
+  hipLaunchKernelGGL(nbr_stats_kernel, dim3(1), dim3(64 * kMaxSpecies), 0, st, n,
+                     b.cum.as<int>(), n > 0 ? nblk : 0, bcount, boff, counts,
+                     counts + kMaxSpecies + 2);
+  FLARE_HIP_CHECK(hipMemcpyAsync(stats_host, counts, kNlistStats * sizeof(int),
                                  hipMemcpyDeviceToHost, st));
-  FLARE_HIP_CHECK(hipStreamSynchronize(st));
-  FLARE_REQUIRE(hc[kMaxSpecies] == 0,
-                "species codes must lie in [0, " + std::to_string(kMaxSpecies) + ")");
-  b.P = P;
-  size_t Pa = std::max(P, 1);
-  b.keys.ensure(Pa * sizeof(int64_t));
-  b.rel.ensure(Pa * 4 * sizeof(double));
-  b.idx.ensure(Pa * sizeof(int));
-  b.nspec.ensure(Pa * sizeof(int));
-  b.center.ensure(Pa * sizeof(int));
-  b.rev.ensure(Pa * sizeof(int));
-  if (n > 0 && hc[kMaxSpecies + 1] <= kNbrCap) {
+  FLARE_HIP_CHECK(hipEventRecord(stats_ev, st));
+  if (n > 0) {
     // entries without a partner keep -1 (the partners write each other)
-    FLARE_HIP_CHECK(hipMemsetAsync(b.rev.ptr, 0xff, Pa * sizeof(int), st));
+    FLARE_HIP_CHECK(hipMemsetAsync(b.rev.ptr, 0xff, cap * sizeof(int), st));
     hipLaunchKernelGGL(nbr_compact_kernel, dim3(gw), dim3(256), 0, st, n,
-                       wh.sweep, b.cum.as<int>(), b.count.as<int>(),
+                       wh.sweep, overflow, b.cum.as<int>(), b.count.as<int>(),
                        b.keys_pad.as<int64_t>(), b.rel_pad.as<double>(),
                        d_species, b.keys.as<int64_t>(), b.rel.as<double>(),
                        b.idx.as<int>(), b.nspec.as<int>(), b.center.as<int>(),
                        b.rev.as<int>());
-  } else if (n > 0) {
+    hipLaunchKernelGGL(species_rows_kernel, dim3(nblk), dim3(kRowBlock), 0, st, n,
+                       d_species, boff, counts, b.row_of.as<int>(),
+                       b.atom_of_row.as<int>(), b.species_of_row.as<int>());
+  }
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+int nlist_resolve(hipStream_t st, NListBuffers &b, int n, const int *d_species,
+                  const WrapHost &wh, const int *stats, int *host_counts) {
+  FLARE_REQUIRE(stats[kMaxSpecies] == 0,
+                "species codes must lie in [0, " + std::to_string(kMaxSpecies) + ")");
+  const int P = stats[kMaxSpecies + 2];
+  b.P = P;
+  if (n > 0 && stats[kMaxSpecies + 1] > kNbrCap) {
     // two-pass path for atoms with more neighbors than the LDS rows hold
+    const WrapMats m = wrap_mats(wh);
+    const size_t Pa = std::max(P, 1);
+    b.keys.ensure(Pa * sizeof(int64_t));
+    b.rel.ensure(Pa * 4 * sizeof(double));
+    b.idx.ensure(Pa * sizeof(int));
+    b.nspec.ensure(Pa * sizeof(int));
+    b.center.ensure(Pa * sizeof(int));
+    b.rev.ensure(Pa * sizeof(int));
     b.keys_tmp.ensure(Pa * sizeof(int64_t));
     b.rel_tmp.ensure(Pa * 4 * sizeof(double));
+    const int gw = (n + 3) / 4;
     hipLaunchKernelGGL(nbr_enum_kernel<1>, dim3(gw), dim3(256), 0, st, n, m,
                        b.wrapped.as<double>(), b.bin_of.as<int>(),
                        b.bin_start.as<int>(), b.bin_atoms.as<int>(),
@@ -693,22 +771,18 @@ int nlist_build_device(hipStream_t st, NListBuffers &b, int n,
                        b.rel.as<double>(), b.idx.as<int>(), b.nspec.as<int>(),
                        b.center.as<int>());
     if (P > 0) {
-      int gp = (P + tb - 1) / tb;
-      hipLaunchKernelGGL(nbr_reverse_kernel, dim3(gp), dim3(tb), 0, st,
+      const int gp = (P + 255) / 256;
+      hipLaunchKernelGGL(nbr_reverse_kernel, dim3(gp), dim3(256), 0, st,
                          (int64_t)P, wh.sweep, b.cum.as<int>(),
                          b.keys.as<int64_t>(), b.idx.as<int>(),
                          b.center.as<int>(), b.rev.as<int>());
     }
+    FLARE_HIP_CHECK(hipGetLastError());
   }
-  if (n > 0)
-    hipLaunchKernelGGL(species_rows_kernel, dim3(nblk), dim3(kRowBlock), 0, st, n,
-                       d_species, boff, counts, b.row_of.as<int>(),
-                       b.atom_of_row.as<int>(), b.species_of_row.as<int>());
-  FLARE_HIP_CHECK(hipGetLastError());
   int ns = 0;
   for (int sp = 0; sp < kMaxSpecies; sp++) {
-    host_counts[sp] = hc[sp];
-    if (hc[sp] > 0) ns = sp + 1;
+    host_counts[sp] = stats[sp];
+    if (stats[sp] > 0) ns = sp + 1;
   }
   return ns;
 }
