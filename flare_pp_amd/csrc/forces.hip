@@ -114,7 +114,9 @@ __device__ inline void stage_lds_vec(int n, const T *__restrict__ src, T *dst) {
 // SCALE: each u[., lm] is multiplied by the harmonic's normalisation
 // nrm_lm[lm] = N_l|m|, the form adjoint_pair_fixed reads.
 // ptab[c nr + c2] = p(c, c2) (L+1): the feature offset of a channel pair.
-template <typename T, int LFIX, bool SCALE, int NFIX = -1>
+// CHANNEL_MAJOR: store u[c H + lm] (the dA/dr layout of the plain form)
+// instead of u2[(s H + lm) N + n].
+template <typename T, int LFIX, bool SCALE, int NFIX = -1, bool CHANNEL_MAJOR = false>
 __device__ inline void u_mfma(const B2Dev &p, const T *A, const T *w,
                               const T *nrm_lm, const int *ptab, T *u2) {
   typedef typename MmaF<T>::V V;
@@ -132,7 +134,7 @@ __device__ inline void u_mfma(const B2Dev &p, const T *A, const T *w,
     for (int q = 0; q < 4; q++) {
       const int cr = mt * 16 + MmaF<T>::row(g, q);
       const int s = cr / N, n = cr - s * N;
-      dst[q] = cr < nr ? s * H * N + n : -1;
+      dst[q] = cr < nr ? (CHANNEL_MAJOR ? cr * H : s * H * N + n) : -1;
     }
 #pragma unroll 1
     for (int l = 0; l < L1; l++) {
@@ -154,7 +156,7 @@ __device__ inline void u_mfma(const B2Dev &p, const T *A, const T *w,
           const T sc = SCALE ? nrm_lm[l2 + j] : T(1);
 #pragma unroll
           for (int q = 0; q < 4; q++)
-            if (dst[q] >= 0) u2[dst[q] + (l2 + j) * N] = acc[q] * sc;
+            if (dst[q] >= 0) u2[dst[q] + (l2 + j) * (CHANNEL_MAJOR ? 1 : N)] = acc[q] * sc;
         }
       }
     }
@@ -489,7 +491,7 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
     }
   }
   __syncthreads();
-  u_mfma<T, LFIX, kFixed, kFixed ? 8 : -1>(p, A, w, nrm_lm, ptab, u);
+  u_mfma<T, LFIX, kFixed, kFixed ? 8 : -1, MODE == 0>(p, A, w, nrm_lm, ptab, u);
   __syncthreads();
   const int si = species[i];
   if (MODE == 0) {
@@ -505,18 +507,29 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
         const int sj = nl.nspec[k];
         inside = pair_inside(p, si, sj, rv);
         if (inside) {
+          // u is stored channel-major here: u[sj] rows match dA/dr's (n, lm)
           const double *d = dA + (k - dA_p0) * 3 * NH;
           const T *ub = u + sj * NH;
-          int n = sub / H, lm = sub - n * H;  // (n, lm) of o, kept incrementally
-          for (int o = sub; o < NH; o += 16) {
-            const double uo = (double)ub[lm * p.N + n];
-            f0 += d[o] * uo;
-            f1 += d[NH + o] * uo;
-            f2 += d[2 * NH + o] * uo;
-            lm += 16;
-            while (lm >= H) {
-              lm -= H;
-              n++;
+          constexpr int kPer = 16;  // values per lane held in flight
+          for (int o0 = sub; o0 < NH; o0 += 16 * kPer) {
+            double d0[kPer], d1[kPer], d2[kPer];
+#pragma unroll
+            for (int t = 0; t < kPer; t++) {
+              const int o = o0 + 16 * t;
+              d0[t] = d1[t] = d2[t] = 0;
+              if (o < NH) {
+                d0[t] = d[o];
+                d1[t] = d[NH + o];
+                d2[t] = d[2 * NH + o];
+              }
+            }
+#pragma unroll
+            for (int t = 0; t < kPer; t++) {
+              const int o = o0 + 16 * t;
+              const double uo = o < NH ? (double)ub[o] : 0.0;
+              f0 += d0[t] * uo;
+              f1 += d1[t] * uo;
+              f2 += d2[t] * uo;
             }
           }
         }
