@@ -91,6 +91,27 @@ class PairInfo(ctypes.Structure):
     ]
 
 
+class StdInfo(ctypes.Structure):
+    _fields_ = [
+        ("use_map", ctypes.c_int),
+        ("power", ctypes.c_int),
+        ("n_hyps", ctypes.c_int),
+        ("hyps", ctypes.c_double * 4),
+        ("n_species", ctypes.c_int),
+        ("n_max", ctypes.c_int),
+        ("l_max", ctypes.c_int),
+        ("n_descriptors", ctypes.c_int),
+        ("beta_size", ctypes.c_int64),
+        ("n_kernels", ctypes.c_int),
+        ("n_clusters", ctypes.c_int),
+        ("n_clusters_by_type", ctypes.c_int * MAX_SPECIES),
+        ("cutoff_function", ctypes.c_int),
+        ("cutoff", ctypes.c_double),
+        ("radial_basis", ctypes.c_char * 32),
+        ("cutoff_name", ctypes.c_char * 32),
+    ]
+
+
 _vp = ctypes.c_void_p
 _i = ctypes.c_int
 _d = ctypes.c_double
@@ -148,6 +169,18 @@ SIGNATURES = {
     "flare_sgp_mapping_coefficients": (_i, [_vp, _vp, _vp]),
     "flare_synth_rsa": (_i, [_i, _d, _d, ctypes.c_uint64, ctypes.c_int64, _vp]),
     "flare_ctx_set_plain_chunk": (_i, [_vp, _i64]),
+    "flare_std_read": (_i, [_cp, _cp, ctypes.POINTER(StdInfo), _pd, _pd, _pd]),
+    "flare_std_load": (_i, [_vp, _cp, _cp, _pvp]),
+    "flare_std_create": (_i, [_vp, ctypes.POINTER(StdInfo), _pd, _pd, _pd, _pvp]),
+    "flare_std_destroy": (_i, [_vp]),
+    "flare_std_get_info": (_i, [_vp, ctypes.POINTER(StdInfo)]),
+    "flare_std_compute": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "flare_sgp_varmap_coefficients": (_i, [_vp, _vp, _vp]),
+    "flare_write_varmap_coefficients": (_i, [_cp, _cp, _i, _pd, _i, _i, _i, _cp,
+                                             _cp, _pd, _i64, _pd]),
+    "flare_write_L_inverse": (_i, [_cp, _cp, _d, _i, _pd, _i, _i, _i, _cp, _cp,
+                                   _pd, _i, _pd]),
+    "flare_write_sparse_descriptors": (_i, [_cp, _cp, _i, _i, _pi, _pd, _pd]),
 }
 
 _lib = None

@@ -40,3 +40,15 @@ def local_stds(atom_indices, variances, sigma):
     full = np.zeros((len(sv), 3))
     full[:, 0] = stds / abs(sigma)
     return full
+
+
+def build_map(sparse_gp, filename="lmp.flare", contributor="user"):
+    """SGP_Calculator.build_map (sparse_gp_calculator.py:159-163): the
+    pair_style coefficient file plus the L^-1 and sparse-descriptor files of
+    compute flare/std/atom (L_inv_<filename>, sparse_desc_<filename>)."""
+    import os
+    d, b = os.path.split(str(filename))
+    sparse_gp.write_mapping_coefficients(filename, contributor, 0)
+    sparse_gp.write_L_inverse(os.path.join(d, f"L_inv_{b}"), contributor)
+    sparse_gp.write_sparse_descriptors(os.path.join(d, f"sparse_desc_{b}"),
+                                       contributor)

@@ -621,6 +621,307 @@ struct flare_sgp {
   DevBuf Phat_f, Phat_h, Phat_l, Phat_is, PhatT_h, PhatT_l, PhatT_is;
 };
 
+// ---------------------------------------------------------------------------
+// compute flare/std/atom model (compute_flare_std_atom.cpp:297-650)
+
+struct flare_std {
+  flare_ctx *ctx = nullptr;
+  int use_map = 1, power = 2, S = 1, N = 1, L = 0, F = 0, cutoff_kind = 0;
+  int n_hyps = 0, n_kernels = 1, M = 0;
+  double hyps[4] = {0, 0, 0, 0};
+  double cutoff = 0;
+  std::string radial = "chebyshev", cutoff_name = "quadratic";
+  std::vector<double> cutoffs;
+  std::vector<int> m_by_type, m_off;
+  DevBuf beta;  // variance map: S x F x F, row-major as stored
+  DevBuf Phat;  // L^-1 form: normalised sparse rows, M x F
+  DevBuf Linv;  // L^-1 form: M x M row-major, lower
+};
+
+namespace {
+
+// Line-oriented reader with the grab semantics of the reference readers
+// (compute_flare_std_atom.cpp:652-664): header lines one at a time, value
+// blocks line by line until the expected count is reached.
+struct LineFile {
+  FILE *f = nullptr;
+  std::string path;
+  explicit LineFile(const char *p) : path(p ? p : "") {
+    f = p ? std::fopen(p, "r") : nullptr;
+    if (!f) throw Error{FLARE_EIO, "Cannot open variance file " + path};
+  }
+  ~LineFile() {
+    if (f) std::fclose(f);
+  }
+  LineFile(const LineFile &) = delete;
+  LineFile &operator=(const LineFile &) = delete;
+  bool next(std::string &line) {
+    line.clear();
+    char buf[4096];
+    bool any = false;
+    while (std::fgets(buf, sizeof buf, f)) {
+      any = true;
+      line += buf;
+      if (line.back() == '\n') break;
+    }
+    return any;
+  }
+  std::string need(const char *what) {
+    std::string l;
+    if (!next(l)) throw Error{FLARE_EFORMAT, path + ": file ends before " + what};
+    return l;
+  }
+  // a count line; blank lines before it are skipped
+  std::string need_nonblank(const char *what) {
+    for (;;) {
+      std::string l = need(what);
+      if (l.find_first_not_of(" \t\r\n\f") != std::string::npos) return l;
+    }
+  }
+  void grab(int64_t n, double *out, const char *what) {
+    int64_t i = 0;
+    std::string line;
+    while (i < n) {
+      if (!next(line)) throw Error{FLARE_EFORMAT, path + ": file ends inside " + what};
+      const char *q = line.c_str();
+      for (;;) {
+        while (*q == ' ' || *q == '\t' || *q == '\n' || *q == '\r' || *q == '\f') q++;
+        if (!*q) break;
+        char *end = nullptr;
+        double v = std::strtod(q, &end);
+        if (end == q) throw Error{FLARE_EFORMAT, path + ": bad number in " + what};
+        if (i >= n)
+          throw Error{FLARE_EFORMAT, path + ": more values than expected in " + what};
+        if (out) out[i] = v;
+        i++;
+        q = end;
+      }
+    }
+  }
+};
+
+std::vector<long long> ints_of(const std::string &line, size_t want, const char *what) {
+  std::istringstream is(line);
+  std::vector<long long> v;
+  long long x;
+  while (v.size() < want && (is >> x)) v.push_back(x);
+  if (v.size() < want) throw Error{FLARE_EFORMAT, std::string("malformed ") + what + " line"};
+  return v;
+}
+
+// hyperparameter line: sscanf "%lg %lg %lg %lg" (compute_flare_std_atom.cpp:
+// 367-374): sigma first, up to four values
+void parse_hyps(const std::string &line, flare_std &sd) {
+  std::istringstream is(line);
+  double x;
+  int k = 0;
+  while (k < 4 && (is >> x)) sd.hyps[k++] = x;
+  if (k == 0) throw Error{FLARE_EFORMAT, "malformed hyperparameter line"};
+}
+
+void set_basis(flare_std &sd) {
+  if (sd.radial != "chebyshev")
+    throw Error{FLARE_EINVAL, "unsupported radial basis '" + sd.radial + "'"};
+  if (sd.cutoff_name == "quadratic")
+    sd.cutoff_kind = FLARE_CUTOFF_QUADRATIC;
+  else if (sd.cutoff_name == "cosine")
+    sd.cutoff_kind = FLARE_CUTOFF_COSINE;
+  else
+    throw Error{FLARE_EINVAL, "unsupported cutoff function '" + sd.cutoff_name + "'"};
+  REQUIRE(sd.S >= 1 && sd.S <= flare::kMaxSpecies, "n_species must be in [1, 8]");
+  REQUIRE(sd.N >= 1 && sd.N <= flare::kMaxN, "n_max must be in [1, 16]");
+  REQUIRE(sd.L >= 0 && sd.L <= flare::kMaxL, "l_max must be in [0, 10]");
+  sd.F = flare::b2_n_descriptors(sd.S, sd.N, sd.L);
+}
+
+void read_cutoffs(LineFile &lf, flare_std &sd) {
+  sd.cutoffs.assign((size_t)sd.S * sd.S, 0.0);
+  lf.grab((int64_t)sd.S * sd.S, sd.cutoffs.data(), "cutoffs");
+  sd.cutoff = -1;
+  for (double v : sd.cutoffs) sd.cutoff = std::max(sd.cutoff, v);
+}
+
+// ComputeFlareStdAtom::read_file (compute_flare_std_atom.cpp:349-451)
+void read_varmap(const char *path, flare_std &sd, std::vector<double> *beta) {
+  LineFile lf(path);
+  lf.need("header");
+  sd.use_map = 1;
+  sd.power = 2;
+  sd.n_hyps = (int)ints_of(lf.need("hyperparameter count"), 1, "hyperparameter count")[0];
+  parse_hyps(lf.need("hyperparameters"), sd);
+  sd.radial = first_word(lf.need("radial basis"));
+  auto v = ints_of(lf.need("descriptor settings"), 4, "'n_species n_max l_max beta_size'");
+  sd.S = (int)v[0];
+  sd.N = (int)v[1];
+  sd.L = (int)v[2];
+  int64_t beta_size = v[3];
+  sd.cutoff_name = first_word(lf.need("cutoff function"));
+  set_basis(sd);
+  read_cutoffs(lf, sd);
+  if ((int64_t)sd.F * sd.F != beta_size)
+    throw Error{FLARE_EFORMAT, "Beta size doesn't match the number of descriptors."};
+  if (beta) {
+    beta->assign((size_t)sd.S * beta_size, 0.0);
+    lf.grab(sd.S * beta_size, beta->data(), "variance map");
+  }
+}
+
+// ComputeFlareStdAtom::read_L_inverse (compute_flare_std_atom.cpp:453-563)
+void read_L_inverse(const char *path, flare_std &sd, std::vector<double> *Linv) {
+  LineFile lf(path);
+  lf.need("header");
+  sd.use_map = 0;
+  sd.power = (int)ints_of(lf.need("power"), 1, "power")[0];
+  sd.n_hyps = (int)ints_of(lf.need("hyperparameter count"), 1, "hyperparameter count")[0];
+  parse_hyps(lf.need("hyperparameters"), sd);
+  sd.radial = first_word(lf.need("radial basis"));
+  auto v = ints_of(lf.need("descriptor settings"), 4, "'n_species n_max l_max n_kernels'");
+  sd.S = (int)v[0];
+  sd.N = (int)v[1];
+  sd.L = (int)v[2];
+  sd.n_kernels = (int)v[3];
+  sd.cutoff_name = first_word(lf.need("cutoff function"));
+  set_basis(sd);
+  if (sd.n_kernels != 1) throw Error{FLARE_EINVAL, "one kernel is supported"};
+  read_cutoffs(lf, sd);
+  sd.M = (int)ints_of(lf.need_nonblank("sparse count"), 1, "sparse count")[0];
+  REQUIRE(sd.M >= 0, "negative sparse count");
+  const int64_t M = sd.M;
+  std::vector<double> packed((size_t)(M * (M + 1) / 2));
+  lf.grab(M * (M + 1) / 2, packed.data(), "L inverse");
+  if (Linv) {
+    Linv->assign((size_t)(M * M), 0.0);
+    int64_t c = 0;
+    for (int64_t j = 0; j < M; j++)
+      for (int64_t k = 0; k <= j; k++) (*Linv)[j * M + k] = packed[c++];
+  }
+}
+
+// ComputeFlareStdAtom::read_sparse_descriptors (compute_flare_std_atom.cpp:
+// 565-650)
+void read_sparse(const char *path, flare_std &sd, std::vector<double> *rows) {
+  LineFile lf(path);
+  lf.need("header");
+  int n_kern = (int)ints_of(lf.need("kernel count"), 1, "kernel count")[0];
+  if (n_kern != sd.n_kernels)
+    throw Error{FLARE_EFORMAT, "n_kernals in sparse_descriptors and L_inv not match"};
+  auto v = ints_of(lf.need_nonblank("kernel line"), 3, "'kernel n_clusters n_types'");
+  if ((int)v[1] != sd.M)
+    throw Error{FLARE_EFORMAT, "n_clusters in sparse_descriptors and L_inv not match"};
+  const int n_types = (int)v[2];
+  if (n_types != sd.S)
+    throw Error{FLARE_EFORMAT, "n_types in sparse_descriptors differs from n_species"};
+  sd.m_by_type.assign(sd.S, 0);
+  sd.m_off.assign(sd.S + 1, 0);
+  if (rows) rows->assign((size_t)sd.M * sd.F, 0.0);
+  for (int s = 0; s < n_types; s++) {
+    int c = (int)ints_of(lf.need_nonblank("cluster count"), 1, "cluster count")[0];
+    REQUIRE(c >= 0 && sd.m_off[s] + c <= sd.M, "cluster counts exceed n_clusters");
+    sd.m_by_type[s] = c;
+    sd.m_off[s + 1] = sd.m_off[s] + c;
+    lf.grab((int64_t)c * sd.F, rows ? rows->data() + (size_t)sd.m_off[s] * sd.F : nullptr,
+            "sparse descriptors");
+  }
+  if (sd.m_off[sd.S] != sd.M)
+    throw Error{FLARE_EFORMAT, "cluster counts do not add up to n_clusters"};
+}
+
+void read_std_files(const char *path1, const char *path2, flare_std &sd,
+                    std::vector<double> *v1, std::vector<double> *v2) {
+  if (!path2) {
+    read_varmap(path1, sd, v1);
+  } else {
+    read_L_inverse(path1, sd, v1);
+    read_sparse(path2, sd, v2);
+  }
+}
+
+void fill_std_info(const flare_std &sd, flare_std_info *o) {
+  std::memset(o, 0, sizeof(*o));
+  o->use_map = sd.use_map;
+  o->power = sd.power;
+  o->n_hyps = sd.n_hyps;
+  for (int k = 0; k < 4; k++) o->hyps[k] = sd.hyps[k];
+  o->n_species = sd.S;
+  o->n_max = sd.N;
+  o->l_max = sd.L;
+  o->n_descriptors = sd.F;
+  o->beta_size = sd.use_map ? (int64_t)sd.F * sd.F : 0;
+  o->n_kernels = sd.n_kernels;
+  o->n_clusters = sd.use_map ? 0 : sd.M;
+  for (int s = 0; s < sd.S && !sd.use_map; s++) o->n_clusters_by_type[s] = sd.m_by_type[s];
+  o->cutoff_function = sd.cutoff_kind;
+  o->cutoff = sd.cutoff;
+  std::snprintf(o->radial_basis, sizeof(o->radial_basis), "%s", sd.radial.c_str());
+  std::snprintf(o->cutoff_name, sizeof(o->cutoff_name), "%s", sd.cutoff_name.c_str());
+}
+
+void upload_std(flare_std &sd, const double *v1, const double *v2) {
+  flare_ctx *c = sd.ctx;
+  if (sd.use_map) {
+    const size_t nb = (size_t)sd.S * sd.F * sd.F * sizeof(double);
+    sd.beta.ensure(nb);
+    FLARE_HIP_CHECK(hipMemcpyAsync(sd.beta.ptr, v1, nb, hipMemcpyHostToDevice, c->stream));
+  } else {
+    const size_t M = sd.M;
+    sd.Phat.ensure(std::max<size_t>(M, 1) * sd.F * sizeof(double));
+    sd.Linv.ensure(std::max<size_t>(M * M, 1) * sizeof(double));
+    if (M) {
+      FLARE_HIP_CHECK(hipMemcpyAsync(sd.Phat.ptr, v2, M * sd.F * sizeof(double),
+                                     hipMemcpyHostToDevice, c->stream));
+      FLARE_HIP_CHECK(hipMemcpyAsync(sd.Linv.ptr, v1, M * M * sizeof(double),
+                                     hipMemcpyHostToDevice, c->stream));
+    }
+  }
+  c->sync();
+}
+
+// one line of values: "%.16e " each, five to a line with the counter c
+// running on; the layout of write_L_inverse / write_sparse_descriptors
+// (sparse_gp.cpp:1329-1336, 1378-1388)
+void put_value(FILE *f, double v, int &count) {
+  std::fprintf(f, "%.16e ", v);
+  if (count % 5 == 0) std::fprintf(f, "\n");
+  count++;
+}
+
+FILE *open_out(const char *path) {
+  FILE *f = std::fopen(path, "w");
+  if (!f) throw Error{FLARE_EIO, std::string("cannot write ") + path};
+  return f;
+}
+
+void close_out(FILE *f, const char *path) {
+  if (std::fclose(f) != 0) throw Error{FLARE_EIO, std::string("cannot write ") + path};
+}
+
+std::string date_now() {
+  time_t now = std::time(nullptr);
+  std::string t(std::ctime(&now));
+  if (!t.empty() && t.back() == '\n') t.pop_back();
+  return t;
+}
+
+// hyperparameter count and values (sparse_gp.cpp:1230-1236, 1306-1312)
+void put_hyps(FILE *f, int n_hyps, const double *hyps) {
+  std::fprintf(f, "%d\n", n_hyps);
+  for (int i = 0; i < n_hyps; i++) std::fprintf(f, "%.16e ", hyps[i]);
+  std::fprintf(f, "\n");
+}
+
+// B2::write_to_file (b2.cpp:52-74)
+void put_b2_settings(FILE *f, const char *radial, int S, int N, int L,
+                     long long coeff_size, const char *cutoff_fn,
+                     const double *cutoffs) {
+  std::fprintf(f, "%s\n", radial ? radial : "chebyshev");
+  std::fprintf(f, "%d %d %d %lld\n", S, N, L, coeff_size);
+  std::fprintf(f, "%s\n", cutoff_fn ? cutoff_fn : "quadratic");
+  for (int q = 0; q < S * S; q++) std::fprintf(f, "%.2f ", cutoffs[q]);
+  std::fprintf(f, "\n");
+}
+
+}  // namespace
+
 // ===========================================================================
 extern "C" {
 
@@ -1792,6 +2093,302 @@ int flare_sgp_mapping_coefficients(flare_ctx *ctx, const flare_sgp *gp,
       }
     }
     ctx->sync();
+  });
+}
+
+
+// ---- compute flare/std/atom ------------------------------------------------
+
+int flare_std_read(const char *path1, const char *path2, flare_std_info *info,
+                   double *cutoffs, double *values1, double *values2) {
+  return guarded([&] {
+    REQUIRE(path1 && info, "null argument");
+    flare_std sd;
+    std::vector<double> v1, v2;
+    read_std_files(path1, path2, sd, values1 ? &v1 : nullptr,
+                   values2 && path2 ? &v2 : nullptr);
+    fill_std_info(sd, info);
+    if (cutoffs) std::copy(sd.cutoffs.begin(), sd.cutoffs.end(), cutoffs);
+    if (values1) std::copy(v1.begin(), v1.end(), values1);
+    if (values2 && path2) std::copy(v2.begin(), v2.end(), values2);
+  });
+}
+
+int flare_std_load(flare_ctx *ctx, const char *path1, const char *path2,
+                   flare_std **out) {
+  return guarded([&] {
+    REQUIRE(ctx && path1 && out, "null argument");
+    *out = nullptr;
+    ctx->use();
+    std::unique_ptr<flare_std> sd(new flare_std);
+    sd->ctx = ctx;
+    std::vector<double> v1, v2;
+    read_std_files(path1, path2, *sd, &v1, &v2);
+    upload_std(*sd, v1.data(), v2.data());
+    *out = sd.release();
+  });
+}
+
+int flare_std_create(flare_ctx *ctx, const flare_std_info *info,
+                     const double *cutoffs, const double *values1,
+                     const double *values2, flare_std **out) {
+  return guarded([&] {
+    REQUIRE(ctx && info && cutoffs && out, "null argument");
+    *out = nullptr;
+    ctx->use();
+    std::unique_ptr<flare_std> sd(new flare_std);
+    sd->ctx = ctx;
+    sd->use_map = info->use_map ? 1 : 0;
+    sd->power = sd->use_map ? 2 : info->power;
+    sd->n_hyps = info->n_hyps;
+    REQUIRE(info->n_hyps >= 1, "at least one hyperparameter (sigma) is needed");
+    for (int k = 0; k < 4; k++) sd->hyps[k] = k < info->n_hyps ? info->hyps[k] : 0.0;
+    sd->S = info->n_species;
+    sd->N = info->n_max;
+    sd->L = info->l_max;
+    sd->cutoff_name = info->cutoff_function == FLARE_CUTOFF_COSINE ? "cosine" : "quadratic";
+    REQUIRE(info->cutoff_function == FLARE_CUTOFF_QUADRATIC ||
+                info->cutoff_function == FLARE_CUTOFF_COSINE,
+            "unknown cutoff function");
+    set_basis(*sd);
+    sd->cutoffs.assign(cutoffs, cutoffs + sd->S * sd->S);
+    sd->cutoff = -1;
+    for (double v : sd->cutoffs) sd->cutoff = std::max(sd->cutoff, v);
+    if (!sd->use_map) {
+      sd->M = info->n_clusters;
+      REQUIRE(sd->M >= 0, "negative sparse count");
+      sd->m_by_type.assign(sd->S, 0);
+      sd->m_off.assign(sd->S + 1, 0);
+      for (int s = 0; s < sd->S; s++) {
+        REQUIRE(info->n_clusters_by_type[s] >= 0, "negative cluster count");
+        sd->m_by_type[s] = info->n_clusters_by_type[s];
+        sd->m_off[s + 1] = sd->m_off[s] + sd->m_by_type[s];
+      }
+      REQUIRE(sd->m_off[sd->S] == sd->M, "cluster counts do not add up to n_clusters");
+      REQUIRE(sd->M == 0 || (values1 && values2), "null L^-1 or sparse rows");
+    } else {
+      REQUIRE(values1, "null variance map");
+    }
+    upload_std(*sd, values1, values2);
+    *out = sd.release();
+  });
+}
+
+int flare_std_destroy(flare_std *sd) {
+  return guarded([&] {
+    if (!sd) return;
+    sd->ctx->use();
+    (void)hipStreamSynchronize(sd->ctx->stream);
+    delete sd;
+  });
+}
+
+int flare_std_get_info(const flare_std *sd, flare_std_info *out) {
+  return guarded([&] {
+    REQUIRE(sd && out, "null argument");
+    fill_std_info(*sd, out);
+  });
+}
+
+int flare_std_compute(flare_ctx *ctx, flare_std *sd, const flare_nlist *nl,
+                      const int *d_species, double *d_stds) {
+  return guarded([&] {
+    REQUIRE(ctx && sd && nl && d_stds, "null argument");
+    flare_b2_params bp;
+    bp.n_species = sd->S;
+    bp.n_max = sd->N;
+    bp.l_max = sd->L;
+    bp.radial_basis = FLARE_RADIAL_CHEBYSHEV;
+    bp.cutoff_function = sd->cutoff_kind;
+    bp.radial_r1 = 0.0;  // radial_hyps = {0, cutoff} (compute_flare_std_atom.cpp:414-417)
+    bp.cutoffs = sd->cutoffs.data();
+    flare::B2Dev p = make_b2(&bp, 1);  // rsq < rc^2, as the pair style
+    check_species(nl, p.S);
+    ctx->use();
+    const int n = nl->n, F = p.F;
+    Forward fw = run_forward(ctx, p, nl, d_species, flare::kPrecF64, "std");
+    const double *B = (const double *)fw.B;
+    int off[flare::kMaxSpecies + 1];
+    nl->row_offsets(p.S, off);
+    const int *aor = nl->b.atom_of_row.as<int>();
+    const double sig2 = sd->hyps[0] * sd->hyps[0];
+    if (sd->use_map) {
+      double *Z = ctx->buf_t<double>("std.Z", (size_t)n * F);
+      ctx->mark_begin("gemm");
+      for (int s = 0; s < p.S; s++) {
+        int rows = off[s + 1] - off[s];
+        gemm_rm(ctx, false, true, rows, F, F, 1.0, B + (size_t)off[s] * F, F,
+                sd->beta.as<double>() + (size_t)s * F * F, F, 0.0,
+                Z + (size_t)off[s] * F, F);
+      }
+      ctx->mark_end();
+      ctx->mark_begin("std");
+      flare::launch_std_map(ctx->stream, n, F, B, Z, fw.norm2, sig2, aor, d_stds);
+      ctx->mark_end();
+    } else {
+      const int M = sd->M;
+      size_t tmax = 1, qmax = 1;
+      for (int s = 0; s < p.S; s++) {
+        size_t rows = off[s + 1] - off[s];
+        tmax = std::max(tmax, rows * sd->m_by_type[s]);
+        qmax = std::max(qmax, rows * (size_t)(M - sd->m_off[s]));
+      }
+      double *T = ctx->buf_t<double>("std.T", tmax);
+      double *Q = ctx->buf_t<double>("std.Q", qmax);
+      for (int s = 0; s < p.S; s++) {
+        const int rows = off[s + 1] - off[s], Ms = sd->m_by_type[s], o = sd->m_off[s];
+        if (!rows) continue;
+        const double *Bs = B + (size_t)off[s] * F;
+        if (!Ms) {  // no sparse environments of this species: k = 0
+          flare::launch_std_linv(ctx->stream, rows, 0, Q, fw.norm2 + off[s], sig2,
+                                 aor + off[s], d_stds);
+          continue;
+        }
+        ctx->mark_begin("gemm");
+        // t = P_hat_s B (then / |B|, ^power), q = L^-1[:, species block] k
+        gemm_rm(ctx, false, true, rows, Ms, F, 1.0, Bs, F,
+                sd->Phat.as<double>() + (size_t)o * F, F, 0.0, T, Ms);
+        flare::launch_std_kvec(ctx->stream, rows, Ms, T, fw.norm2 + off[s],
+                               (double)sd->power);
+        gemm_rm(ctx, false, true, rows, M - o, Ms, 1.0, T, Ms,
+                sd->Linv.as<double>() + (size_t)o * M + o, M, 0.0, Q, M - o);
+        ctx->mark_end();
+        ctx->mark_begin("std");
+        flare::launch_std_linv(ctx->stream, rows, M - o, Q, fw.norm2 + off[s], sig2,
+                               aor + off[s], d_stds);
+        ctx->mark_end();
+      }
+    }
+    ctx->finish();
+  });
+}
+
+int flare_sgp_varmap_coefficients(flare_ctx *ctx, const flare_sgp *gp,
+                                  double *d_out) {
+  return guarded([&] {
+    REQUIRE(ctx && gp && d_out, "null argument");
+    if (gp->power != 1.0)
+      throw Error{FLARE_EINVAL,
+                  "Mapping coefficients of the normalized dot product kernel are "
+                  "implemented for power 1 only."};
+    ctx->use();
+    const int F = gp->F, M = gp->M;
+    const size_t FF = (size_t)F * F;
+    // first sparse norm of each species decides the sigma^2 I term
+    std::vector<double> pn(std::max(M, 1));
+    if (M)
+      FLARE_HIP_CHECK(hipMemcpyAsync(pn.data(), gp->pnorm.ptr, M * sizeof(double),
+                                     hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    for (int s = 0; s < gp->S; s++) {
+      const int Ms = gp->m_by_type[s], o = gp->m_off[s];
+      double *dst = d_out + (size_t)s * FF;
+      FLARE_HIP_CHECK(hipMemsetAsync(dst, 0, FF * sizeof(double), ctx->stream));
+      if (!Ms) continue;
+      // Y = L^-1_s P_hat_s, beta = -sigma^4 Y^T Y (K_uu^-1 = L^-T L^-1,
+      // sparse_gp.cpp:588; empty rows of P_hat are zero)
+      double *Y = ctx->buf_t<double>("varmap.Y", (size_t)Ms * F);
+      gemm_rm(ctx, true, false, Ms, F, Ms, 1.0,
+              gp->Linv_cm.as<double>() + (size_t)o * M + o, M,
+              gp->Phat.as<double>() + (size_t)o * F, F, 0.0, Y, F);
+      gemm_rm(ctx, true, false, F, F, Ms, -gp->sig2 * gp->sig2, Y, F, Y, F, 0.0,
+              dst, F);
+      if (!(pn[o] < 1e-8)) flare::launch_add_diag_rm(ctx->stream, F, dst, gp->sig2);
+    }
+    ctx->sync();
+  });
+}
+
+int flare_write_varmap_coefficients(const char *path, const char *contributor,
+                                    int n_hyps, const double *hyps,
+                                    int n_species, int n_max, int l_max,
+                                    const char *radial_basis,
+                                    const char *cutoff_function,
+                                    const double *cutoffs, int64_t beta_size,
+                                    const double *beta) {
+  return guarded([&] {
+    REQUIRE(path && cutoffs && beta && (hyps || n_hyps == 0), "null argument");
+    FILE *f = open_out(path);
+    // sparse_gp.cpp:1208-1281
+    std::fprintf(f, "DATE: %s CONTRIBUTOR: %s\n", date_now().c_str(),
+                 contributor ? contributor : "");
+    put_hyps(f, n_hyps, hyps);
+    put_b2_settings(f, radial_basis, n_species, n_max, l_max, (long long)beta_size,
+                    cutoff_function, cutoffs);
+    int count = 0;
+    for (int s = 0; s < n_species; s++) {
+      if (count != 0) {
+        std::fprintf(f, "\n");
+        count = 0;
+      }
+      for (int64_t j = 0; j < beta_size; j++) {
+        double v = beta[(int64_t)s * beta_size + j];
+        std::fprintf(f, v > 0 ? "  %.16e" : " %.16e", v);
+        if (++count == 5) {
+          count = 0;
+          std::fprintf(f, "\n");
+        }
+      }
+    }
+    close_out(f, path);
+  });
+}
+
+int flare_write_L_inverse(const char *path, const char *contributor,
+                          double power, int n_hyps, const double *hyps,
+                          int n_species, int n_max, int l_max,
+                          const char *radial_basis,
+                          const char *cutoff_function, const double *cutoffs,
+                          int n_clusters, const double *L_inv) {
+  return guarded([&] {
+    REQUIRE(path && cutoffs && (L_inv || n_clusters == 0) && (hyps || n_hyps == 0),
+            "null argument");
+    FILE *f = open_out(path);
+    // sparse_gp.cpp:1283-1339
+    std::fprintf(f, "L_inverse_block file DATE: %s CONTRIBUTOR: %s\n",
+                 date_now().c_str(), contributor ? contributor : "");
+    std::fprintf(f, "%.0f\n", power);  // NormalizedDotProduct::write_info
+    put_hyps(f, n_hyps, hyps);
+    put_b2_settings(f, radial_basis, n_species, n_max, l_max, 1, cutoff_function,
+                    cutoffs);
+    std::fprintf(f, "%d\n", n_clusters);
+    int count = 1;
+    for (int64_t j = 0; j < n_clusters; j++)
+      for (int64_t k = 0; k <= j; k++) put_value(f, L_inv[j * n_clusters + k], count);
+    if (count % 5 != 0) std::fprintf(f, "\n");
+    close_out(f, path);
+  });
+}
+
+int flare_write_sparse_descriptors(const char *path, const char *contributor,
+                                   int n_species, int n_descriptors,
+                                   const int *n_clusters_by_type,
+                                   const double *descriptors,
+                                   const double *norms) {
+  return guarded([&] {
+    REQUIRE(path && n_clusters_by_type, "null argument");
+    int M = 0;
+    for (int s = 0; s < n_species; s++) M += n_clusters_by_type[s];
+    REQUIRE(M == 0 || (descriptors && norms), "null argument");
+    FILE *f = open_out(path);
+    // sparse_gp.cpp:1341-1396, one kernel
+    std::fprintf(f, "sparse_descriptors file DATE: %s CONTRIBUTOR: %s\n",
+                 date_now().c_str(), contributor ? contributor : "");
+    std::fprintf(f, "%d\n", 1);
+    std::fprintf(f, "%d %d %d\n", 0, M, n_species);
+    int count = 1, row = 0;
+    for (int s = 0; s < n_species; s++) {
+      std::fprintf(f, "%d\n", n_clusters_by_type[s]);
+      for (int j = 0; j < n_clusters_by_type[s]; j++, row++) {
+        const double nrm = norms[row];
+        for (int k = 0; k < n_descriptors; k++) {
+          double v = nrm < 1e-8 ? 0.0 : descriptors[(size_t)row * n_descriptors + k] / nrm;
+          put_value(f, v, count);
+        }
+      }
+      if (count % 5 != 1) std::fprintf(f, "\n");
+    }
+    close_out(f, path);
   });
 }
 

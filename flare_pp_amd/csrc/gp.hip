@@ -326,6 +326,106 @@ void launch_gp_w(hipStream_t st, int rows, int F, const double *G,
   FLARE_HIP_CHECK(hipGetLastError());
 }
 
+// ---- compute flare/std/atom (compute_flare_std_atom.cpp:172-212) ---------
+
+__device__ inline double signed_sqrt(double v) {
+  return v >= 0.0 ? sqrt(v) : -sqrt(fabs(v));
+}
+
+__global__ void __launch_bounds__(64)
+std_map_kernel(int rows, int F, const double *__restrict__ B,
+               const double *__restrict__ Z, const double *__restrict__ norm2,
+               double sig2, const int *__restrict__ atom_of_row,
+               double *__restrict__ stds) {
+  const int r = blockIdx.x;
+  const int lane = threadIdx.x;
+  double d = 0;
+  for (int f = lane; f < F; f += kWave)
+    d += B[(int64_t)r * F + f] * Z[(int64_t)r * F + f];
+  d = wave_sum(d);
+  if (lane == 0) {
+    const double n2 = norm2[r];
+    double v = 0;
+    if (!(n2 < 1e-8)) {
+      v = d / n2;  // compute_energy_and_u, power 2 (lammps_descriptor.cpp:267)
+      v /= sig2;
+      v = signed_sqrt(v);
+    }
+    stds[atom_of_row[r]] = v;
+  }
+}
+
+__global__ void __launch_bounds__(64)
+std_kvec_kernel(int rows, int M, double *__restrict__ T,
+                const double *__restrict__ norm2, double power) {
+  const int r = blockIdx.x;
+  const int lane = threadIdx.x;
+  const double n2 = norm2[r];
+  const bool empty = n2 < 1e-8;
+  const double nb = sqrt(n2);
+  for (int u = lane; u < M; u += kWave) {
+    const double t = T[(int64_t)r * M + u];
+    T[(int64_t)r * M + u] = empty ? 0.0 : powi(t / nb, power);
+  }
+}
+
+__global__ void __launch_bounds__(64)
+std_linv_kernel(int rows, int M, const double *__restrict__ Q,
+                const double *__restrict__ norm2, double sig2,
+                const int *__restrict__ atom_of_row, double *__restrict__ stds) {
+  const int r = blockIdx.x;
+  const int lane = threadIdx.x;
+  double s = 0;
+  for (int u = lane; u < M; u += kWave) {
+    const double q = Q[(int64_t)r * M + u];
+    s += q * q;
+  }
+  s = wave_sum(s);
+  if (lane == 0) {
+    double v = 0;
+    if (!(norm2[r] < 1e-8)) v = signed_sqrt(1.0 - sig2 * s);
+    stds[atom_of_row[r]] = v;
+  }
+}
+
+__global__ void add_diag_rm_kernel(int F, double *__restrict__ out, double v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < F) out[(int64_t)i * F + i] += v;
+}
+
+void launch_std_map(hipStream_t st, int rows, int F, const double *B,
+                    const double *Z, const double *norm2, double sig2,
+                    const int *atom_of_row, double *stds) {
+  if (rows == 0) return;
+  hipLaunchKernelGGL(std_map_kernel, dim3(rows), dim3(kWave), 0, st, rows, F, B,
+                     Z, norm2, sig2, atom_of_row, stds);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_std_kvec(hipStream_t st, int rows, int M, double *T,
+                     const double *norm2, double power) {
+  if (rows == 0 || M == 0) return;
+  hipLaunchKernelGGL(std_kvec_kernel, dim3(rows), dim3(kWave), 0, st, rows, M, T,
+                     norm2, power);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_std_linv(hipStream_t st, int rows, int M, const double *Q,
+                     const double *norm2, double sig2, const int *atom_of_row,
+                     double *stds) {
+  if (rows == 0) return;
+  hipLaunchKernelGGL(std_linv_kernel, dim3(rows), dim3(kWave), 0, st, rows, M, Q,
+                     norm2, sig2, atom_of_row, stds);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_add_diag_rm(hipStream_t st, int F, double *out, double v) {
+  if (F == 0) return;
+  hipLaunchKernelGGL(add_diag_rm_kernel, dim3((F + 255) / 256), dim3(256), 0, st,
+                     F, out, v);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
 void launch_row_sqsum_sub(hipStream_t st, int rows, int M, const double *Q,
                           const double *kself, double *var) {
   if (rows == 0) return;

@@ -166,6 +166,22 @@ void launch_pack_beta(hipStream_t st, int F, int power, const double *full,
 void launch_unpack_beta(hipStream_t st, int F, int power, const double *packed,
                         double *full);
 void launch_to_f32(hipStream_t st, int64_t n, const double *x, float *y);
+// compute flare/std/atom tails (compute_flare_std_atom.cpp:172-212), one row
+// per species-grouped descriptor row, written to stds[atom_of_row[r]]:
+//  map:  v = (B.Z / |B|^2) / sig2, Z = beta B
+//  kvec: T <- (T / |B|)^power in place (rows x M)
+//  linv: v = 1 - sig2 sum_u Q[r][u]^2
+// std = sqrt(v), -sqrt(|v|) when v < 0, 0 when |B|^2 < 1e-8.
+void launch_std_map(hipStream_t st, int rows, int F, const double *B,
+                    const double *Z, const double *norm2, double sig2,
+                    const int *atom_of_row, double *stds);
+void launch_std_kvec(hipStream_t st, int rows, int M, double *T,
+                     const double *norm2, double power);
+void launch_std_linv(hipStream_t st, int rows, int M, const double *Q,
+                     const double *norm2, double sig2, const int *atom_of_row,
+                     double *stds);
+// out[i][i] += v for i < F (row-major F x F)
+void launch_add_diag_rm(hipStream_t st, int F, double *out, double v);
 // Low-precision GP forms: epilogue from t (FP32, rows x M) into kval (FP64),
 // c as FP32 rows (cf) or as FP16 hi/lo rows of Mp scaled per row (ch, cl,
 // inverse scales c_is), e (FP64)

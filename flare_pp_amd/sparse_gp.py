@@ -232,6 +232,77 @@ class SparseGP:
             raise RuntimeError("alpha is not set")
         self._build()
 
+    # -- variance path (normalized_dot_product.cpp:761-852;
+    #    sparse_gp.cpp:1208-1396) ---------------------------------------------
+    @property
+    def hyperparameters(self):
+        """[sigma, energy noise, force noise, stress noise]
+        (sparse_gp.cpp:36-47)."""
+        return np.array([self.kernels[0].sigma, self.energy_noise,
+                         self.force_noise, self.stress_noise])
+
+    @property
+    def Kuu_inverse(self):
+        """L^-T L^-1 (sparse_gp.cpp:588)."""
+        Li = self.L_inv
+        return Li.T @ Li
+
+    def varmap_coefficients(self):
+        """Variance map S x F x F of this power-1 model
+        (normalized_dot_product.cpp:761-852), computed on the GPU."""
+        self._build()
+        sp = self.sparse_descriptors[0]
+        F, S = sp.n_descriptors, sp.n_types
+        ctx = _capi.context(self.device)
+        out = torch.empty((S, F, F), dtype=torch.float64, device=f"cuda:{self.device}")
+        check(lib().flare_sgp_varmap_coefficients(ctx.handle, self._h, dptr(out)))
+        return out.cpu().numpy()
+
+    def write_varmap_coefficients(self, file_name, contributor, kernel_index=0):
+        """SparseGP::write_varmap_coefficients (sparse_gp.cpp:1208-1281);
+        sets ``varmap_coeffs`` (S x F*F).  The kernel power must be 1."""
+        if kernel_index != 0:
+            raise ValueError("one kernel is supported")
+        beta = self.varmap_coefficients()
+        S = beta.shape[0]
+        self.varmap_coeffs = beta.reshape(S, -1)
+        calc = self.descriptor_calculators[0]
+        hyps = np.ascontiguousarray(self.hyperparameters)
+        cut = np.ascontiguousarray(calc.cutoffs.ravel())
+        vals = np.ascontiguousarray(self.varmap_coeffs)
+        pd = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        check(lib().flare_write_varmap_coefficients(
+            str(file_name).encode(), str(contributor).encode(), len(hyps), pd(hyps),
+            calc.n_species, calc.n_max, calc.l_max, calc.radial_basis.encode(),
+            calc.cutoff_function.encode(), pd(cut), vals.shape[1], pd(vals)))
+
+    def write_L_inverse(self, file_name, contributor):
+        """SparseGP::write_L_inverse (sparse_gp.cpp:1283-1339)."""
+        self._build()
+        Li = np.ascontiguousarray(self.L_inv)
+        calc = self.descriptor_calculators[0]
+        hyps = np.ascontiguousarray(self.hyperparameters)
+        cut = np.ascontiguousarray(calc.cutoffs.ravel())
+        pd = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        check(lib().flare_write_L_inverse(
+            str(file_name).encode(), str(contributor).encode(),
+            float(self.kernels[0].power), len(hyps), pd(hyps), calc.n_species,
+            calc.n_max, calc.l_max, calc.radial_basis.encode(),
+            calc.cutoff_function.encode(), pd(cut), Li.shape[0], pd(Li)))
+
+    def write_sparse_descriptors(self, file_name, contributor):
+        """SparseGP::write_sparse_descriptors (sparse_gp.cpp:1341-1396)."""
+        sp = self.sparse_descriptors[0]
+        P, m_by_type = sp.stacked()
+        P = np.ascontiguousarray(P, dtype=np.float64)
+        norms = np.ascontiguousarray(np.concatenate(
+            [np.asarray(n, dtype=np.float64) for n in sp.descriptor_norms]))
+        arr = (ctypes.c_int * sp.n_types)(*m_by_type)
+        pd = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        check(lib().flare_write_sparse_descriptors(
+            str(file_name).encode(), str(contributor).encode(), sp.n_types,
+            sp.n_descriptors, arr, pd(P), pd(norms)))
+
     def write_mapping_coefficients(self, file_name, contributor, kernel_index=0):
         from .pair import write_coefficients
         if kernel_index != 0:

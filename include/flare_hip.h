@@ -241,6 +241,89 @@ int flare_envs_envs(flare_ctx *ctx, int n_species, int n_descriptors,
 int flare_sgp_mapping_coefficients(flare_ctx *ctx, const flare_sgp *gp,
                                    double *d_out);
 
+/* ---- variance path: compute flare/std/atom ---------------------------------
+ * ComputeFlareStdAtom (compute_flare_std_atom.cpp:31-664) on the GPU.  Two
+ * model sources, as in ComputeFlareStdAtom::coeff (:297-314):
+ *  - a variance-map file (write_varmap_coefficients, sparse_gp.cpp:
+ *    1208-1281; reader :349-451): variance = B^T beta_s B / |B|^2 / sigma^2;
+ *  - an L^-1 file plus a sparse-descriptor file (write_L_inverse /
+ *    write_sparse_descriptors, sparse_gp.cpp:1283-1396; readers :453-650):
+ *    variance = 1 - sigma^2 |L^-1 k|^2, k = (P_hat_s B/|B|)^power on the
+ *    atom's species, zero elsewhere.
+ * Per-atom result: sqrt(variance), or -sqrt(|variance|) when negative; 0 for
+ * empty environments (|B|^2 < 1e-8).  FP64 throughout: both forms subtract
+ * nearly equal terms. */
+typedef struct flare_std flare_std;
+typedef struct flare_std_info {
+  int use_map;  /* 1: variance map; 0: L^-1 + sparse descriptors */
+  int power;    /* kernel power (L^-1 file); 2 for the variance map */
+  int n_hyps;
+  double hyps[4]; /* sigma, energy noise, force noise, stress noise */
+  int n_species, n_max, l_max, n_descriptors;
+  int64_t beta_size; /* variance map: F*F */
+  int n_kernels;     /* L^-1 and sparse-descriptor files (1 supported) */
+  int n_clusters;
+  int n_clusters_by_type[FLARE_MAX_SPECIES];
+  int cutoff_function;
+  double cutoff; /* largest pair cutoff */
+  char radial_basis[32];
+  char cutoff_name[32];
+} flare_std_info;
+
+/* Host-only parser of the files above.  path2 == NULL reads path1 as a
+ * variance map, otherwise path1 is the L^-1 file and path2 the sparse
+ * descriptors.  Fills info and, where non-NULL, cutoffs (S x S), values1
+ * (variance map: S x F x F row-major as stored; L^-1: n_clusters x
+ * n_clusters row-major, upper triangle zero) and values2 (sparse rows
+ * n_clusters x F, species-grouped).  Call with NULL arrays to get sizes. */
+int flare_std_read(const char *path1, const char *path2, flare_std_info *info,
+                   double *cutoffs, double *values1, double *values2);
+/* ComputeFlareStdAtom::coeff: load the model onto the device. */
+int flare_std_load(flare_ctx *ctx, const char *path1, const char *path2,
+                   flare_std **out);
+/* The same from memory: info fields use_map, power, n_hyps, hyps,
+ * n_species, n_max, l_max, cutoff_function, n_clusters and
+ * n_clusters_by_type are read; arrays as in flare_std_read (host). */
+int flare_std_create(flare_ctx *ctx, const flare_std_info *info,
+                     const double *cutoffs, const double *values1,
+                     const double *values2, flare_std **out);
+int flare_std_destroy(flare_std *sd);
+int flare_std_get_info(const flare_std *sd, flare_std_info *out);
+/* ComputeFlareStdAtom::compute_peratom (:99-213): d_stds n (atom order). */
+int flare_std_compute(flare_ctx *ctx, flare_std *sd, const flare_nlist *nl,
+                      const int *d_species, double *d_stds);
+
+/* Variance map of a power-1 sparse GP (normalized_dot_product.cpp:761-852):
+ * d_out S x F x F, beta_s = sigma^2 I - sigma^4 P_hat_s^T K_uu^-1 P_hat_s
+ * over non-empty sparse rows; the sigma^2 I term only when the species'
+ * first sparse environment is non-empty (:829-842).  FLARE_EINVAL for other
+ * powers (the reference prints a message and returns an empty matrix). */
+int flare_sgp_varmap_coefficients(flare_ctx *ctx, const flare_sgp *gp,
+                                  double *d_out);
+/* Writers, host arrays (sparse_gp.cpp:1208-1396 layouts; the DATE line uses
+ * the local time).  hyps: n_hyps values (sigma first). */
+int flare_write_varmap_coefficients(const char *path, const char *contributor,
+                                    int n_hyps, const double *hyps,
+                                    int n_species, int n_max, int l_max,
+                                    const char *radial_basis,
+                                    const char *cutoff_function,
+                                    const double *cutoffs, int64_t beta_size,
+                                    const double *beta);
+/* L_inv: n_clusters x n_clusters row-major; its lower triangle is written */
+int flare_write_L_inverse(const char *path, const char *contributor,
+                          double power, int n_hyps, const double *hyps,
+                          int n_species, int n_max, int l_max,
+                          const char *radial_basis,
+                          const char *cutoff_function, const double *cutoffs,
+                          int n_clusters, const double *L_inv);
+/* descriptors n_clusters x F (species-grouped, raw), norms n_clusters;
+ * rows are written divided by their norm, zero when the norm < 1e-8 */
+int flare_write_sparse_descriptors(const char *path, const char *contributor,
+                                   int n_species, int n_descriptors,
+                                   const int *n_clusters_by_type,
+                                   const double *descriptors,
+                                   const double *norms);
+
 /* ---- synthetic structures ------------------------------------------------
  * No counterpart in the reference: generator of the disordered benchmark
  * box.  Random sequential addition of n points in a periodic cube of edge

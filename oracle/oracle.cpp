@@ -27,6 +27,10 @@
 //   mapping / pair_flare       mapped E/F/S == GP mean for power 1 and 2
 //                              (tests/test_lammps.py:97-102), FD forces,
 //                              virial == -V * GP stress
+//   varmap / std/atom          std/atom from the L^-1 + sparse-descriptor
+//                              files and from the variance map equal the
+//                              signed square roots of the GP local variances
+//                              over sigma^2 (tests/test_lammps.py:207-237)
 // Bit-level Eigen reduction orders in GEMV/GEMM/LLT are not restated
 // (summations run in index order); those agree to rounding only.
 
@@ -1262,6 +1266,164 @@ extern "C" void orc_pair_compute_full(const orc_nlist *nl, const int *species,
     double v = 0;
     for (int t = 0; t < nt; t++) v += vbuf[(size_t)t * 6 + c];
     virial[c] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Variance map (normalized_dot_product.cpp:761-852): power-1 kernels only.
+// beta_s[k][l] = sum over non-empty i, j of species s of
+//   sig^4 p_ik p_jl (-Kinv_ij / |p_i| / |p_j|), plus sig^2 on the diagonal,
+// added inside the (i = 0, j = 0) iteration, i.e. only when the species'
+// first sparse environment is non-empty.
+
+extern "C" int orc_varmap_coefficients(int S, int F, const int *m_by_type,
+                                       const double *sparse,
+                                       const double *sparse_norms,
+                                       double sigma, const double *Kinv,
+                                       double *out) {
+  const double sig2 = sigma * sigma;
+  const double empty_thresh = 1e-8;
+  int M = 0;
+  for (int s = 0; s < S; s++) M += m_by_type[s];
+  const size_t FF = (size_t)F * F;
+  std::fill(out, out + FF * S, 0.0);
+  int K_ind = 0;
+  for (int s = 0; s < S; s++) {
+    double *row = out + FF * s;
+    const int nt = m_by_type[s];
+    for (int i = 0; i < nt; i++) {
+      const double *pi = sparse + (size_t)(K_ind + i) * F;
+      double pi_norm = sparse_norms[K_ind + i];
+      if (pi_norm < empty_thresh) continue;
+      for (int j = 0; j < nt; j++) {
+        const double *pj = sparse + (size_t)(K_ind + j) * F;
+        double pj_norm = sparse_norms[K_ind + j];
+        if (pj_norm < empty_thresh) continue;
+        double kij = Kinv[(size_t)(K_ind + i) * M + (K_ind + j)];
+        double kn = kij / pi_norm / pj_norm;
+        size_t bc = 0;
+        for (int k = 0; k < F; k++) {
+          double p_ik = pi[k];
+          for (int l = 0; l < F; l++) {
+            double p_jl = pj[l];
+            row[bc] += sig2 * sig2 * p_ik * p_jl * (-kn);
+            if (k == l && i == 0 && j == 0) row[bc] += sig2;
+            bc++;
+          }
+        }
+      }
+    }
+    K_ind += nt;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// compute flare/std/atom (compute_flare_std_atom.cpp:99-213).  Per atom: the
+// pair-style descriptor (rsq < rc(si,sj)^2, lammps_descriptor.cpp:7-107,
+// 209-248); empty environments (|B|^2 < 1e-8) keep std 0.
+//   use_map: variance = B^T beta B / |B|^2 / sig^2 with beta * B taken as
+//            Eigen's matrix-vector product of the file's row-major matrix
+//            (compute_flare_std_atom.cpp:186-191, 427-450;
+//            lammps_descriptor.cpp:265-268).
+//   else:    k_s = (P_s d)^power over the species' normalised sparse rows,
+//            zero elsewhere; variance = 1 - sig^2 |L^-1 k|^2 with the full
+//            lower-triangular L^-1 (compute_flare_std_atom.cpp:192-206).
+// std = sqrt(variance), negative square root of |variance| when negative.
+
+extern "C" void orc_std_atom(const orc_nlist *nl, const int *species, int S,
+                             int N, int L, const double *cutoffs,
+                             int cutoff_kind, double sigma, int use_map,
+                             const double *beta_var, int power,
+                             int n_clusters, const int *n_by_type,
+                             const double *sparse_normed, const double *Linv,
+                             int nthreads, double *stds) {
+  const int n = nl->n;
+  const int H = (L + 1) * (L + 1);
+  const int nr = S * N;
+  const int W = nr * H;
+  const int F = nr * (nr + 1) / 2 * (L + 1);
+  const double empty_thresh = 1e-8;
+  const double sig2 = sigma * sigma;
+#ifdef _OPENMP
+  int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
+#else
+  int nt = 1;
+  (void)nthreads;
+#endif
+  std::fill(stds, stds + n, 0.0);
+  std::vector<int> cum_types(S + 1, 0);
+  if (!use_map)
+    for (int s = 0; s < S; s++) cum_types[s + 1] = cum_types[s] + n_by_type[s];
+
+#pragma omp parallel num_threads(nt)
+  {
+    std::vector<double> g(N), gx(N), gy(N), gz(N), Y(H), Yx(H), Yy(H), Yz(H);
+    std::vector<double> A(W), B(F), Z(F), dn(F), kv, q;
+#pragma omp for schedule(static)
+    for (int i = 0; i < n; i++) {
+      int si = species[i];
+      std::fill(A.begin(), A.end(), 0.0);
+      for (int k = nl->cum[i]; k < nl->cum[i + 1]; k++) {
+        const double *rv = &nl->rel[(size_t)k * 4];
+        double x = rv[1], y = rv[2], z = rv[3];
+        double rsq = x * x + y * y + z * z;
+        int s = nl->nspec[k];
+        double rc = cutoffs[si * S + s];
+        if (!(rsq < rc * rc)) continue;
+        double r = std::sqrt(rsq);
+        orc_radial(x, y, z, r, rc, N, 0.0, cutoff_kind, g.data(), gx.data(),
+                   gy.data(), gz.data());
+        orc_ylm(x, y, z, L, Y.data(), Yx.data(), Yy.data(), Yz.data());
+        int dc = s * N * H;
+        for (int rn = 0; rn < N; rn++)
+          for (int a = 0; a < H; a++) A[dc++] += g[rn] * Y[a];
+      }
+      std::fill(B.begin(), B.end(), 0.0);
+      for (int n1 = nr - 1; n1 >= 0; n1--) {
+        int n1c = (n1 * (2 * nr - n1 + 1)) / 2;
+        for (int n2 = n1; n2 < nr; n2++) {
+          int n2c = n2 - n1;
+          for (int l = 0; l < L + 1; l++) {
+            int counter = l + (n1c + n2c) * (L + 1);
+            for (int m = 0; m < 2 * l + 1; m++)
+              B[counter] += A[n1 * H + l * l + m] * A[n2 * H + l * l + m];
+          }
+        }
+      }
+      double norm2 = 0;
+      for (int c = 0; c < F; c++) norm2 += B[c] * B[c];
+      if (norm2 < empty_thresh) continue;
+      double variance;
+      if (use_map) {
+        const double *beta = beta_var + (size_t)si * F * F;
+        std::fill(Z.begin(), Z.end(), 0.0);
+        for (int j = 0; j < F; j++) {
+          double bj = B[j];
+          for (int c = 0; c < F; c++) Z[c] += beta[(size_t)c * F + j] * bj;
+        }
+        variance = dotn(B.data(), Z.data(), F) / norm2;
+        variance /= sig2;
+      } else {
+        double bn = std::pow(norm2, 0.5);
+        for (int c = 0; c < F; c++) dn[c] = B[c] / bn;
+        kv.assign(n_clusters, 0.0);
+        for (int u = 0; u < n_by_type[si]; u++) {
+          int row = cum_types[si] + u;
+          double t = dotn(sparse_normed + (size_t)row * F, dn.data(), F);
+          kv[row] = std::pow(t, (double)power);
+        }
+        double qs = 0;
+        for (int u = 0; u < n_clusters; u++) {
+          double v = 0;
+          for (int w = 0; w <= u; w++) v += Linv[(size_t)u * n_clusters + w] * kv[w];
+          qs += v * v;
+        }
+        variance = 1.0 - sig2 * qs;
+      }
+      stds[i] = variance >= 0.0 ? std::pow(variance, 0.5)
+                                : -std::pow(std::fabs(variance), 0.5);
+    }
   }
 }
 

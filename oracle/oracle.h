@@ -154,6 +154,24 @@ void orc_pair_compute_full(const orc_nlist *nl, const int *species, int S,
                            int cutoff_kind, const double *beta_full,
                            int nthreads, double *energy, double *forces,
                            double *virial, double *eatom);
+/* ---- variance path: normalized_dot_product.cpp:761-852;
+ *      compute_flare_std_atom.cpp:99-213 ----------------------------------- */
+/* variance map of a power-1 kernel: out S x F x F (row-major).  Kinv is
+ * K_uu^-1 = L^-T L^-1 (M x M, sparse_gp.cpp:588). */
+int orc_varmap_coefficients(int S, int F, const int *m_by_type,
+                            const double *sparse, const double *sparse_norms,
+                            double sigma, const double *Kinv, double *out);
+/* per-atom signed stds (atom order).  use_map != 0: beta_var S x F x F as
+ * read from the variance-map file.  Otherwise: kernel power, n_clusters,
+ * n_by_type[S], normalised sparse rows (n_clusters x F, species-grouped) and
+ * L^-1 (n_clusters x n_clusters row-major, lower). */
+void orc_std_atom(const orc_nlist *nl, const int *species, int S, int N,
+                  int L, const double *cutoffs, int cutoff_kind, double sigma,
+                  int use_map, const double *beta_var, int power,
+                  int n_clusters, const int *n_by_type,
+                  const double *sparse_normed, const double *Linv,
+                  int nthreads, double *stds);
+
 /* unpack file beta (pair_flare.cpp:363-401): packed -> S x F x F */
 void orc_unpack_beta(int S, int F, int power, const double *packed,
                      double *full);

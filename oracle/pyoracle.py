@@ -101,6 +101,14 @@ def lib():
                                              _d, _d, _d]),
             "orc_unpack_beta": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        _d, _d]),
+            "orc_varmap_coefficients": (ctypes.c_int, [ctypes.c_int, ctypes.c_int,
+                                                       _i, _d, _d,
+                                                       ctypes.c_double, _d, _d]),
+            "orc_std_atom": (None, [_vp, _i, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_int, _d, ctypes.c_int,
+                                    ctypes.c_double, ctypes.c_int, _d,
+                                    ctypes.c_int, ctypes.c_int, _i, _d, _d,
+                                    ctypes.c_int, _d]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -366,3 +374,42 @@ def pair_compute(nlist, species, S, N, L, power, cutoffs, beta, cutoff="quadrati
        _pd(_f64(cutoffs).ravel()), cutoff_code(cutoff), _pd(_f64(beta)),
        nthreads, _pd(e), _pd(f), _pd(v), _pd(ea))
     return (e[0], f, v, ea) if want_eatom else (e[0], f, v)
+
+
+def varmap_coefficients(S, m_by_type, sparse, sparse_norms, sigma, Kinv):
+    """Variance map S x F x F of a power-1 kernel
+    (normalized_dot_product.cpp:761-852)."""
+    sparse = _f64(sparse)
+    F = sparse.shape[1]
+    out = np.zeros((S, F, F))
+    lib().orc_varmap_coefficients(S, F, _pi(_i32(m_by_type)), _pd(sparse),
+                                  _pd(_f64(sparse_norms)), sigma,
+                                  _pd(_f64(Kinv)), _pd(out))
+    return out
+
+
+def std_atom_map(nlist, species, S, N, L, cutoffs, sigma, beta_var,
+                 cutoff="quadratic", nthreads=0):
+    """compute flare/std/atom with a variance-map file
+    (compute_flare_std_atom.cpp:186-191)."""
+    n = len(species)
+    out = np.zeros(n)
+    lib().orc_std_atom(nlist.handle, _pi(_i32(species)), S, N, L,
+                       _pd(_f64(cutoffs).ravel()), cutoff_code(cutoff), sigma, 1,
+                       _pd(_f64(beta_var)), 0, 0, None, None, None, nthreads,
+                       _pd(out))
+    return out
+
+
+def std_atom_linv(nlist, species, S, N, L, cutoffs, sigma, power, n_by_type,
+                  sparse_normed, Linv, cutoff="quadratic", nthreads=0):
+    """compute flare/std/atom with L^-1 and sparse-descriptor files
+    (compute_flare_std_atom.cpp:192-206)."""
+    n = len(species)
+    out = np.zeros(n)
+    sp = _f64(sparse_normed)
+    lib().orc_std_atom(nlist.handle, _pi(_i32(species)), S, N, L,
+                       _pd(_f64(cutoffs).ravel()), cutoff_code(cutoff), sigma, 0,
+                       None, int(power), sp.shape[0], _pi(_i32(n_by_type)),
+                       _pd(sp), _pd(_f64(Linv)), nthreads, _pd(out))
+    return out
