@@ -455,16 +455,15 @@ static Forward run_forward(flare_ctx *c, const flare::B2Dev &p,
   return f;
 }
 
-// Forces / virial assembly shared by the mapped and GP paths.
-static void run_forces(flare_ctx *c, const flare::B2Dev &p,
-                       const flare_nlist *nl, const int *d_species, int prec,
-                       int mode, const Forward &fw, const flare::WInput &wi,
-                       double *forces, double *energy, double *virial6,
-                       double *stress6, const std::string &tag) {
+// Per-pair forces f_ik (P x 3) of every centre of nl, shared by the mapped
+// and GP paths.
+static double *run_pair_forces(flare_ctx *c, const flare::B2Dev &p,
+                               const flare_nlist *nl, const int *d_species,
+                               int prec, int mode, const Forward &fw,
+                               const flare::WInput &wi, const std::string &tag) {
   const int n = nl->n;
   const int64_t P = nl->b.P;
   double *pf = c->buf_t<double>(tag + ".pf", (size_t)std::max<int64_t>(P, 1) * 3);
-  double *vir = c->buf_t<double>(tag + ".vir", (size_t)std::max(n, 1) * 9);
   if (mode == 0) {
     // plain form: dA/dr materialised for a range of atoms at a time, at most
     // c->plain_chunk_bytes of derivatives (one range when everything fits;
@@ -512,6 +511,18 @@ static void run_forces(flare_ctx *c, const flare::B2Dev &p,
                          c->ynorm_f.as<float>(), fw.A, wi, nullptr, 0, 0, n, pf);
     c->mark_end();
   }
+  return pf;
+}
+
+// Forces (gather over own and reverse pairs), per-centre virial rows, the
+// energy and virial sums.  On a caller's list (no reverse pairs) `forces`
+// receives each centre's own-pair sums.
+static void run_gather_reduce(flare_ctx *c, const flare_nlist *nl, const double *pf,
+                              const flare::WInput &wi, double *forces, double *energy,
+                              double *virial6, double *stress6,
+                              const std::string &tag) {
+  const int n = nl->n;
+  double *vir = c->buf_t<double>(tag + ".vir", (size_t)std::max(n, 1) * 9);
   c->mark_begin("gather_reduce");
   flare::launch_gather(c->stream, nl->dev(), pf, forces, vir);
   double *sums = c->buf_t<double>(tag + ".sums", 10);
@@ -524,6 +535,16 @@ static void run_forces(flare_ctx *c, const flare::B2Dev &p,
   flare::launch_finish(c->stream, sums, 1.0 / nl->wh.volume, energy, virial6,
                        stress6);
   c->mark_end();
+}
+
+// Forces / virial assembly shared by the mapped and GP paths.
+static void run_forces(flare_ctx *c, const flare::B2Dev &p,
+                       const flare_nlist *nl, const int *d_species, int prec,
+                       int mode, const Forward &fw, const flare::WInput &wi,
+                       double *forces, double *energy, double *virial6,
+                       double *stress6, const std::string &tag) {
+  const double *pf = run_pair_forces(c, p, nl, d_species, prec, mode, fw, wi, tag);
+  run_gather_reduce(c, nl, pf, wi, forces, energy, virial6, stress6, tag);
 }
 
 // ---------------------------------------------------------------------------
@@ -540,6 +561,7 @@ struct flare_pair {
   bool have_last = false, last_mixed = false;
   int last_prec = 0, last_n = 0;
   int last_off[flare::kMaxSpecies + 1] = {0};
+  std::unique_ptr<flare_nlist> ext;  // list built from a caller's neighbor list
 };
 
 namespace {
@@ -1404,110 +1426,192 @@ int flare_pair_get_beta(flare_ctx *ctx, const flare_pair *pr, double *beta) {
   });
 }
 
+}  // extern "C"
+
+// Forward, GEMM and w of a mapped force evaluation over every centre of nl
+// (pair_flare.cpp:95-137; lammps_descriptor.cpp:209-304).
+struct PairEval {
+  flare::B2Dev p;
+  Forward fw;
+  flare::WInput wi;
+  int prec = 0, mode = 0;
+  bool mixed = false;
+  int off[flare::kMaxSpecies + 1];
+};
+
+static PairEval pair_eval(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
+                          const int *d_species, int form, const double *d_beta,
+                          int pair_pred) {
+  REQUIRE(form >= FLARE_FORM_PLAIN_FP64 && form <= FLARE_FORM_ADJOINT_MIXED,
+          "unknown form");
+  PairEval ev;
+  flare_b2_params bp;
+  bp.n_species = pr->S;
+  bp.n_max = pr->N;
+  bp.l_max = pr->L;
+  bp.radial_basis = FLARE_RADIAL_CHEBYSHEV;
+  bp.cutoff_function = pr->cutoff_kind;
+  bp.radial_r1 = 0.0;  // radial_hyps = {0, cutoff} (pair_flare.cpp:346-349)
+  bp.cutoffs = pr->cutoffs.data();
+  flare::B2Dev p = make_b2(&bp, pair_pred);
+  ev.p = p;
+  check_species(nl, p.S);
+  const int n = nl->n;
+  const int F = p.F;
+  const int prec = (form == FLARE_FORM_PLAIN_FP64 || form == FLARE_FORM_ADJOINT_FP64)
+                       ? flare::kPrecF64
+                       : flare::kPrecF32;
+  ev.prec = prec;
+  ev.mode = form == FLARE_FORM_PLAIN_FP64 ? 0 : 1;
+  const size_t per = pr->power == 1 ? (size_t)F : (size_t)F * F;
+  const size_t ts = prec == flare::kPrecF64 ? sizeof(double) : sizeof(float);
+  const bool mixed = form == FLARE_FORM_ADJOINT_MIXED && pr->power == 2;
+  ev.mixed = mixed;
+  // GEMM rows: B itself in FP64 (as pair_flare), B/|B| in the FP32 forms
+  const int normalized = prec == flare::kPrecF32 ? 1 : 0;
+  Forward fw = run_forward(ctx, p, nl, d_species, prec, "pair", nullptr,
+                           mixed ? 2 : normalized, mixed);
+  int *off = ev.off;
+  nl->row_offsets(p.S, off);
+  // beta in the working precision
+  const void *beta = pr->beta.ptr;
+  const void *bh = pr->beta_h.ptr, *bl = pr->beta_l.ptr;
+  const int kp = fw.kp;
+  if (prec == flare::kPrecF64) {
+    if (d_beta) beta = d_beta;
+  } else {
+    beta = pr->beta_f32.ptr;
+    if (d_beta) {
+      float *bf = ctx->buf_t<float>("pair.beta_override", p.S * per);
+      flare::launch_to_f32(ctx->stream, (int64_t)(p.S * per), d_beta, bf);
+      beta = bf;
+      if (mixed) {
+        void *h = ctx->buf("pair.beta_oh", (size_t)p.S * F * kp * 2);
+        void *l = ctx->buf("pair.beta_ol", (size_t)p.S * F * kp * 2);
+        flare::split_bf16(ctx->stream, p.S * F, F, kp, d_beta, h, l);
+        bh = h;
+        bl = l;
+      }
+    }
+  }
+  const void *X = fw.B;
+  void *Z = nullptr;
+  ctx->mark_begin("gemm");
+  if (pr->power == 2) {
+    Z = ctx->buf("pair.Z", std::max<size_t>(1, (size_t)n * F) * ts);
+    if (mixed) {  // all species in one launch
+      flare::GemmProduct g[flare::kMaxSpecies];
+      for (int s = 0; s < p.S; s++)
+        g[s] = flare::GemmProduct{off[s + 1] - off[s], F, kp, F,
+                                  (int64_t)off[s] * kp, (int64_t)s * F * kp,
+                                  (int64_t)off[s] * F};
+      flare::gemm_bf16x3_batched(ctx->stream, p.S, g, fw.xh, fw.xl, bh, bl,
+                                 (float *)Z);
+    }
+    for (int s = 0; s < p.S && !mixed; s++) {
+      int rows = off[s + 1] - off[s];
+      if (!rows) continue;
+      if (prec == flare::kPrecF64) {
+        gemm_rm(ctx, false, false, rows, F, F, 1.0,
+                (const double *)X + (size_t)off[s] * F, F,
+                (const double *)beta + s * per, F, 0.0,
+                (double *)Z + (size_t)off[s] * F, F);
+      } else {
+        sgemm_rm(ctx, false, false, rows, F, F, 1.0f,
+                 (const float *)X + (size_t)off[s] * F, F,
+                 (const float *)beta + s * per, F, 0.0f,
+                 (float *)Z + (size_t)off[s] * F, F);
+      }
+    }
+  }
+  ctx->mark_end();
+  double *E_row = ctx->buf_t<double>("pair.E", n);
+  flare::WInput wi{mixed ? 3 : pr->power == 2 ? 1 : 2, normalized, X, Z,
+                   pr->power == 1 ? beta : nullptr, nullptr, fw.norm2, E_row};
+  if (mixed) {
+    wi.xh = fw.xh;
+    wi.xl = fw.xl;
+    wi.kp = kp;
+  }
+  ev.fw = fw;
+  ev.wi = wi;
+  return ev;
+}
+
+static void pair_remember(flare_pair *pr, const PairEval &ev, int n) {
+  pr->have_last = true;
+  pr->last_mixed = ev.mixed;
+  pr->last_prec = ev.prec;
+  pr->last_n = n;
+  for (int s = 0; s <= ev.p.S; s++) pr->last_off[s] = ev.off[s];
+}
+
+extern "C" {
+
 int flare_pair_compute(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
                        const int *d_species, int form, const double *d_beta,
                        double *d_energy, double *d_forces, double *d_virial,
                        double *d_eatom) {
   return guarded([&] {
     REQUIRE(ctx && pr, "null argument");
-    REQUIRE(form >= FLARE_FORM_PLAIN_FP64 && form <= FLARE_FORM_ADJOINT_MIXED,
-            "unknown form");
-    flare_b2_params bp;
-    bp.n_species = pr->S;
-    bp.n_max = pr->N;
-    bp.l_max = pr->L;
-    bp.radial_basis = FLARE_RADIAL_CHEBYSHEV;
-    bp.cutoff_function = pr->cutoff_kind;
-    bp.radial_r1 = 0.0;  // radial_hyps = {0, cutoff} (pair_flare.cpp:346-349)
-    bp.cutoffs = pr->cutoffs.data();
-    flare::B2Dev p = make_b2(&bp, 1);
-    check_species(nl, p.S);
     ctx->use();
+    PairEval ev = pair_eval(ctx, pr, nl, d_species, form, d_beta, 1);
     const int n = nl->n;
-    const int F = p.F;
-    const int prec = (form == FLARE_FORM_PLAIN_FP64 || form == FLARE_FORM_ADJOINT_FP64)
-                         ? flare::kPrecF64
-                         : flare::kPrecF32;
-    const int mode = form == FLARE_FORM_PLAIN_FP64 ? 0 : 1;
-    const size_t per = pr->power == 1 ? (size_t)F : (size_t)F * F;
-    const size_t ts = prec == flare::kPrecF64 ? sizeof(double) : sizeof(float);
-    const bool mixed = form == FLARE_FORM_ADJOINT_MIXED && pr->power == 2;
-    // GEMM rows: B itself in FP64 (as pair_flare), B/|B| in the FP32 forms
-    const int normalized = prec == flare::kPrecF32 ? 1 : 0;
-    Forward fw = run_forward(ctx, p, nl, d_species, prec, "pair", nullptr,
-                             mixed ? 2 : normalized, mixed);
-    int off[flare::kMaxSpecies + 1];
-    nl->row_offsets(p.S, off);
-    // beta in the working precision
-    const void *beta = pr->beta.ptr;
-    const void *bh = pr->beta_h.ptr, *bl = pr->beta_l.ptr;
-    const int kp = fw.kp;
-    if (prec == flare::kPrecF64) {
-      if (d_beta) beta = d_beta;
-    } else {
-      beta = pr->beta_f32.ptr;
-      if (d_beta) {
-        float *bf = ctx->buf_t<float>("pair.beta_override", p.S * per);
-        flare::launch_to_f32(ctx->stream, (int64_t)(p.S * per), d_beta, bf);
-        beta = bf;
-        if (mixed) {
-          void *h = ctx->buf("pair.beta_oh", (size_t)p.S * F * kp * 2);
-          void *l = ctx->buf("pair.beta_ol", (size_t)p.S * F * kp * 2);
-          flare::split_bf16(ctx->stream, p.S * F, F, kp, d_beta, h, l);
-          bh = h;
-          bl = l;
-        }
-      }
-    }
-    const void *X = fw.B;
-    void *Z = nullptr;
-    ctx->mark_begin("gemm");
-    if (pr->power == 2) {
-      Z = ctx->buf("pair.Z", std::max<size_t>(1, (size_t)n * F) * ts);
-      if (mixed) {  // all species in one launch
-        flare::GemmProduct g[flare::kMaxSpecies];
-        for (int s = 0; s < p.S; s++)
-          g[s] = flare::GemmProduct{off[s + 1] - off[s], F, kp, F,
-                                    (int64_t)off[s] * kp, (int64_t)s * F * kp,
-                                    (int64_t)off[s] * F};
-        flare::gemm_bf16x3_batched(ctx->stream, p.S, g, fw.xh, fw.xl, bh, bl,
-                                   (float *)Z);
-      }
-      for (int s = 0; s < p.S && !mixed; s++) {
-        int rows = off[s + 1] - off[s];
-        if (!rows) continue;
-        if (prec == flare::kPrecF64) {
-          gemm_rm(ctx, false, false, rows, F, F, 1.0,
-                  (const double *)X + (size_t)off[s] * F, F,
-                  (const double *)beta + s * per, F, 0.0,
-                  (double *)Z + (size_t)off[s] * F, F);
-        } else {
-          sgemm_rm(ctx, false, false, rows, F, F, 1.0f,
-                   (const float *)X + (size_t)off[s] * F, F,
-                   (const float *)beta + s * per, F, 0.0f,
-                   (float *)Z + (size_t)off[s] * F, F);
-        }
-      }
-    }
-    ctx->mark_end();
-    double *E_row = ctx->buf_t<double>("pair.E", n);
-    flare::WInput wi{mixed ? 3 : pr->power == 2 ? 1 : 2, normalized, X, Z,
-                     pr->power == 1 ? beta : nullptr, nullptr, fw.norm2, E_row};
-    if (mixed) {
-      wi.xh = fw.xh;
-      wi.xl = fw.xl;
-      wi.kp = kp;
-    }
-    run_forces(ctx, p, nl, d_species, prec, mode, fw, wi, d_forces, d_energy,
-               d_virial, nullptr, "pair");
+    run_forces(ctx, ev.p, nl, d_species, ev.prec, ev.mode, ev.fw, ev.wi, d_forces,
+               d_energy, d_virial, nullptr, "pair");
     if (d_eatom)
       flare::launch_rows_to_atoms(ctx->stream, n, nl->b.atom_of_row.as<int>(),
-                                  E_row, d_eatom);
-    pr->have_last = true;
-    pr->last_mixed = mixed;
-    pr->last_prec = prec;
-    pr->last_n = n;
-    for (int s = 0; s <= p.S; s++) pr->last_off[s] = off[s];
+                                  ev.wi.E_row, d_eatom);
+    pair_remember(pr, ev, n);
+    ctx->finish();
+  });
+}
+
+int flare_pair_compute_neigh(flare_ctx *ctx, flare_pair *pr, int n_all, int inum,
+                             const int *d_ilist, const int *d_numneigh,
+                             const int *d_neighbors, int64_t stride_i,
+                             int64_t stride_j, const double *d_x,
+                             const int *d_type, int form, double *d_energy,
+                             double *d_forces, double *d_virial, double *d_eatom) {
+  return guarded([&] {
+    REQUIRE(ctx && pr, "null argument");
+    REQUIRE(n_all >= 0 && inum >= 0 && inum <= n_all, "need 0 <= inum <= n_all");
+    REQUIRE(inum == 0 || (d_ilist && d_numneigh && d_neighbors && d_x && d_type),
+            "null neighbor list or atom arrays");
+    REQUIRE(d_energy && d_forces && d_virial, "null output");
+    ctx->use();
+    // the caller's list as a CSR list over the inum centres
+    if (!pr->ext) {
+      pr->ext.reset(new flare_nlist);
+      pr->ext->ctx = ctx;
+    }
+    flare_nlist *nl = pr->ext.get();
+    flare::ExtList e{n_all, inum, d_ilist, d_numneigh, d_neighbors, stride_i,
+                     stride_j, d_x, d_type};
+    int *cspec = ctx->buf_t<int>("ext.cspec", inum);
+    ctx->mark_begin("nlist");
+    flare::nlist_from_external(ctx->stream, nl->b, e, pr->cutoffs.data(), pr->S, cspec,
+                               nl->species_count);
+    ctx->mark_end();
+    nl->n = inum;
+    nl->wh.volume = 1.0;  // no stress on this path
+    nl->pending = false;
+    nl->built = true;
+    PairEval ev = pair_eval(ctx, pr, nl, cspec, form, nullptr, 2);
+    const double *pf = run_pair_forces(ctx, ev.p, nl, cspec, ev.prec, ev.mode, ev.fw,
+                                       ev.wi, "pair");
+    double *own = ctx->buf_t<double>("ext.own", (size_t)inum * 3);
+    run_gather_reduce(ctx, nl, pf, ev.wi, own, d_energy, d_virial, nullptr, "pair");
+    // F_j -= f_ij over the pairs whose neighbor is j (ghosts included), then
+    // F_i += own sums; LAMMPS sums ghost forces back (newton on)
+    ctx->mark_begin("assemble");
+    int *in_cum = ctx->buf_t<int>("ext.in_cum", (size_t)n_all + 1);
+    int *in_pairs = ctx->buf_t<int>("ext.in_pairs", (size_t)std::max<int64_t>(nl->b.P, 1));
+    flare::ext_incoming(ctx->stream, nl->b, n_all, nl->b.P, in_cum, in_pairs);
+    flare::ext_assemble(ctx->stream, n_all, inum, d_ilist, in_cum, in_pairs, pf, own,
+                        nl->b.row_of.as<int>(), ev.wi.E_row, d_forces, d_eatom);
+    ctx->mark_end();
+    pair_remember(pr, ev, inum);
     ctx->finish();
   });
 }

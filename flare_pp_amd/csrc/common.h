@@ -47,7 +47,8 @@ __device__ inline T wave_sum(T v) {
 struct B2Dev {
   int S, N, L, H, NH, W, F, nr;
   int cutoff_kind;
-  int pair_pred;  // 0: r <= rc(si,sj) (b2.cpp:277); 1: r^2 < rc^2
+  int pair_pred;  // 0: r <= rc(si,sj) (b2.cpp:277); 1: r^2 < rc^2;
+                  // 2: every listed pair (a caller's list, filtered on entry)
                   // (pair_flare.cpp:115)
   double r1;
   double cut[kMaxSpecies * kMaxSpecies];
@@ -68,6 +69,8 @@ struct NListDev {
 
 __device__ inline bool pair_inside(const B2Dev &p, int si, int sj,
                                    const double *rv) {
+#pragma clang fp contract(off)
+  if (p.pair_pred == 2) return true;
   double rc = p.cut[si * p.S + sj];
   if (p.pair_pred == 0) return rv[0] <= rc;
   double rsq = rv[1] * rv[1] + rv[2] * rv[2] + rv[3] * rv[3];

@@ -79,6 +79,38 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
 int nlist_resolve(hipStream_t st, NListBuffers &b, int n, const int *d_species,
                   const WrapHost &wh, const int *stats, int *host_counts);
 
+// A caller's neighbor list over local + ghost atoms (LAMMPS: x, type,
+// ilist, numneigh, firstneigh as a 2-D array; pair_flare.cpp:80-118,
+// pair_flare_kokkos.cpp:633-662).  Neighbor jj of atom i is
+// neighbors[i * stride_i + jj * stride_j] with LAMMPS' special-bond bits
+// masked off; types are LAMMPS' 1-based codes.
+struct ExtList {
+  int n_all, inum;
+  const int *ilist, *numneigh, *neighbors;
+  int64_t stride_i, stride_j;
+  const double *x;
+  const int *type;
+};
+// Builds the CSR list of the inum centres (pairs with r^2 < rc(si,sj)^2 in
+// list order, relative vectors x_j - x_i, neighbor index j in [0, n_all)),
+// center = centre index, rev = -1, the species-grouped row map of the
+// centres and their 0-based species (cspec, inum).  Waits for the pair
+// count; host_counts receives the centres' species counts.  Returns P.
+int64_t nlist_from_external(hipStream_t st, NListBuffers &b, const ExtList &e,
+                            const double *cut_host, int S, int *cspec,
+                            int *host_counts);
+// Target-sorted view of the pairs for the force assembly: in_cum (n_all+1)
+// and the pair indices sorted by neighbor (stable, ascending pair index).
+void ext_incoming(hipStream_t st, NListBuffers &b, int n_all, int64_t P,
+                  int *in_cum, int *in_pairs);
+// forces[a] -= sum of pf over pairs whose neighbor is a (a < n_all), then
+// forces[ilist[c]] += own[c] (c < inum); eatom[ilist[c]] += E_row[row_of[c]]
+// when eatom is non-null.
+void ext_assemble(hipStream_t st, int n_all, int inum, const int *ilist,
+                  const int *in_cum, const int *in_pairs, const double *pf,
+                  const double *own, const int *row_of, const double *E_row,
+                  double *forces, double *eatom);
+
 // descriptors.hip
 // A (atom order), GEMM rows (species-grouped: B when out_mode 0, B/|B| when
 // 1, plus the BF16 hi/lo split of the latter into rows of kp when xh/xl are

@@ -787,4 +787,274 @@ int nlist_resolve(hipStream_t st, NListBuffers &b, int n, const int *d_species,
   return ns;
 }
 
+
+// ---------------------------------------------------------------------------
+// Caller-provided neighbor lists.  LAMMPS hands pair styles a full list over
+// local and ghost atoms; pair_flare keeps the neighbors with
+// r^2 < rc(si,sj)^2 in list order (pair_flare.cpp:95-118,
+// lammps_descriptor.cpp:51-66; the Kokkos short list of
+// pair_flare_kokkos.cpp:633-662).  One wavefront per centre; ballots keep
+// the list order when the kept pairs are compacted.
+
+constexpr int kNeighMask = 0x1FFFFFFF;  // LAMMPS NEIGHMASK
+
+struct ExtCut {
+  int S;
+  double rc2[kMaxSpecies * kMaxSpecies];
+};
+
+// neighbor jj of centre atom i: index, type and the pair test
+__device__ inline bool ext_pair(const ExtList &e, const ExtCut &c, int i, int ti,
+                                int jj, int &j, int &tj, double *o, int *bad) {
+#pragma clang fp contract(off)
+  j = e.neighbors[(int64_t)i * e.stride_i + (int64_t)jj * e.stride_j] & kNeighMask;
+  if (j < 0 || j >= e.n_all) {
+    atomicOr(bad, 2);
+    return false;
+  }
+  tj = e.type[j] - 1;
+  if (tj < 0 || tj >= c.S) {
+    atomicOr(bad, 1);
+    return false;
+  }
+  const double dx = e.x[3 * (int64_t)j] - e.x[3 * (int64_t)i];
+  const double dy = e.x[3 * (int64_t)j + 1] - e.x[3 * (int64_t)i + 1];
+  const double dz = e.x[3 * (int64_t)j + 2] - e.x[3 * (int64_t)i + 2];
+  const double rsq = dx * dx + dy * dy + dz * dz;
+  o[0] = sqrt(rsq);
+  o[1] = dx;
+  o[2] = dy;
+  o[3] = dz;
+  return rsq < c.rc2[ti * c.S + tj];
+}
+
+__global__ void __launch_bounds__(256)
+ext_count_kernel(ExtList e, ExtCut c, int *__restrict__ cnt,
+                 int *__restrict__ cspec, int *__restrict__ bad) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int ii = blockIdx.x * 4 + threadIdx.x / kWave;
+  if (ii >= e.inum) return;
+  const int i = e.ilist[ii];
+  int ti = -1;
+  if (i >= 0 && i < e.n_all) ti = e.type[i] - 1;
+  if (ti < 0 || ti >= c.S) {
+    if (lane == 0) {
+      atomicOr(bad, (i >= 0 && i < e.n_all) ? 1 : 2);
+      cnt[ii] = 0;
+      cspec[ii] = 0;
+    }
+    return;
+  }
+  const int jnum = e.numneigh[i];
+  int total = 0;
+  for (int jj0 = 0; jj0 < jnum; jj0 += kWave) {
+    const int jj = jj0 + lane;
+    bool in = false;
+    if (jj < jnum) {
+      int j, tj;
+      double o[4];
+      in = ext_pair(e, c, i, ti, jj, j, tj, o, bad);
+    }
+    total += __popcll(__ballot(in));
+  }
+  if (lane == 0) {
+    cnt[ii] = total;
+    cspec[ii] = ti;
+  }
+}
+
+__global__ void __launch_bounds__(256)
+ext_fill_kernel(ExtList e, ExtCut c, const int *__restrict__ cum,
+                const int *__restrict__ cspec, int *__restrict__ idx,
+                int *__restrict__ nspec, double *__restrict__ rel,
+                int *__restrict__ center, int *__restrict__ bad) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int ii = blockIdx.x * 4 + threadIdx.x / kWave;
+  if (ii >= e.inum) return;
+  const int i = e.ilist[ii];
+  if (i < 0 || i >= e.n_all) return;
+  const int ti = cspec[ii];
+  const int jnum = e.numneigh[i];
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  int pos = cum[ii];
+  for (int jj0 = 0; jj0 < jnum; jj0 += kWave) {
+    const int jj = jj0 + lane;
+    bool in = false;
+    int j = 0, tj = 0;
+    double o[4];
+    if (jj < jnum) in = ext_pair(e, c, i, ti, jj, j, tj, o, bad);
+    const uint64_t m = __ballot(in);
+    if (in) {
+      const int k = pos + __popcll(m & lt);
+      idx[k] = j;
+      nspec[k] = tj;
+      reinterpret_cast<double4 *>(rel)[k] = double4{o[0], o[1], o[2], o[3]};
+      center[k] = ii;
+    }
+    pos += __popcll(m);
+  }
+}
+
+// per-block species counts of the centres (the input of nbr_stats_kernel /
+// species_rows_kernel)
+__global__ void __launch_bounds__(kRowBlock)
+ext_species_kernel(int inum, const int *__restrict__ cspec, int *__restrict__ bcount) {
+  __shared__ int cnt[kMaxSpecies];
+  const int t = threadIdx.x;
+  if (t < kMaxSpecies) cnt[t] = 0;
+  __syncthreads();
+  const int ii = blockIdx.x * kRowBlock + t;
+  if (ii < inum) atomicAdd(&cnt[cspec[ii]], 1);
+  __syncthreads();
+  if (t < kMaxSpecies) bcount[blockIdx.x * kMaxSpecies + t] = cnt[t];
+}
+
+int64_t nlist_from_external(hipStream_t st, NListBuffers &b, const ExtList &e,
+                            const double *cut_host, int S, int *cspec,
+                            int *host_counts) {
+  FLARE_REQUIRE(S >= 1 && S <= kMaxSpecies, "n_species must be in [1, 8]");
+  ExtCut c;
+  c.S = S;
+  for (int q = 0; q < kMaxSpecies * kMaxSpecies; q++) c.rc2[q] = 0;
+  for (int q = 0; q < S * S; q++) c.rc2[q] = cut_host[q] * cut_host[q];
+  const int n = e.inum;
+  const int nblk = std::max(1, (n + kRowBlock - 1) / kRowBlock);
+  b.count.ensure((size_t)(n + 1) * sizeof(int));
+  b.cum.ensure((size_t)(n + 1) * sizeof(int));
+  b.row_of.ensure((size_t)std::max(n, 1) * sizeof(int));
+  b.atom_of_row.ensure((size_t)std::max(n, 1) * sizeof(int));
+  b.species_of_row.ensure((size_t)std::max(n, 1) * sizeof(int));
+  b.species_count.ensure(kNlistStats * sizeof(int));
+  b.species_blk.ensure((size_t)2 * nblk * kMaxSpecies * sizeof(int));
+  int *bcount = b.species_blk.as<int>(), *boff = bcount + nblk * kMaxSpecies;
+  int *counts = b.species_count.as<int>();
+  int *bad = counts + kMaxSpecies;
+  hipLaunchKernelGGL(clear3_kernel, dim3(1), dim3(256), 0, st, counts, kNlistStats,
+                     b.count.as<int>() + n, 1, (int *)nullptr, 0);
+  const int gw = (n + 3) / 4;
+  if (n > 0) {
+    hipLaunchKernelGGL(ext_count_kernel, dim3(gw), dim3(256), 0, st, e, c,
+                       b.count.as<int>(), cspec, bad);
+    hipLaunchKernelGGL(ext_species_kernel, dim3(nblk), dim3(kRowBlock), 0, st, n,
+                       cspec, bcount);
+  }
+  exclusive_scan(st, b.scan_tmp, b.count.as<int>(), b.cum.as<int>(), n);
+  hipLaunchKernelGGL(nbr_stats_kernel, dim3(1), dim3(64 * kMaxSpecies), 0, st, n,
+                     b.cum.as<int>(), n > 0 ? nblk : 0, bcount, boff, counts,
+                     counts + kMaxSpecies + 2);
+  int stats[kNlistStats];
+  FLARE_HIP_CHECK(hipMemcpyAsync(stats, counts, kNlistStats * sizeof(int),
+                                 hipMemcpyDeviceToHost, st));
+  FLARE_HIP_CHECK(hipStreamSynchronize(st));
+  FLARE_REQUIRE(!(stats[kMaxSpecies] & 1), "atom types must lie in [1, n_species]");
+  FLARE_REQUIRE(!(stats[kMaxSpecies] & 2), "neighbor or centre index outside [0, n_all)");
+  const int64_t P = stats[kMaxSpecies + 2];
+  b.P = (int)P;
+  const size_t Pa = (size_t)std::max<int64_t>(P, 1);
+  b.idx.ensure(Pa * sizeof(int));
+  b.nspec.ensure(Pa * sizeof(int));
+  b.center.ensure(Pa * sizeof(int));
+  b.rev.ensure(Pa * sizeof(int));
+  b.rel.ensure(Pa * 4 * sizeof(double));
+  if (P > 0)
+    FLARE_HIP_CHECK(hipMemsetAsync(b.rev.ptr, 0xff, P * sizeof(int), st));
+  if (n > 0) {
+    hipLaunchKernelGGL(ext_fill_kernel, dim3(gw), dim3(256), 0, st, e, c,
+                       b.cum.as<int>(), cspec, b.idx.as<int>(), b.nspec.as<int>(),
+                       b.rel.as<double>(), b.center.as<int>(), bad);
+    hipLaunchKernelGGL(species_rows_kernel, dim3(nblk), dim3(kRowBlock), 0, st, n,
+                       cspec, boff, counts, b.row_of.as<int>(),
+                       b.atom_of_row.as<int>(), b.species_of_row.as<int>());
+  }
+  FLARE_HIP_CHECK(hipGetLastError());
+  for (int sp = 0; sp < kMaxSpecies; sp++) host_counts[sp] = stats[sp];
+  return P;
+}
+
+__global__ void ext_hist_kernel(int64_t P, const int *__restrict__ idx,
+                                int *__restrict__ cnt) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < P) atomicAdd(&cnt[idx[k]], 1);
+}
+
+__global__ void iota_kernel(int64_t P, int *__restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < P) out[k] = (int)k;
+}
+
+void ext_incoming(hipStream_t st, NListBuffers &b, int n_all, int64_t P,
+                  int *in_cum, int *in_pairs) {
+  // counts (n_all + 1, last zero) -> in_cum; stable radix sort of the pair
+  // indices by neighbor index
+  b.bin_count.ensure((size_t)(n_all + 1) * sizeof(int));
+  int *cnt = b.bin_count.as<int>();
+  FLARE_HIP_CHECK(hipMemsetAsync(cnt, 0, (size_t)(n_all + 1) * sizeof(int), st));
+  const size_t Pa = (size_t)std::max<int64_t>(P, 1);
+  b.keys_tmp.ensure(Pa * sizeof(int64_t));  // iota values, then sorted keys
+  int *vals = b.keys_tmp.as<int>();
+  int *keys_sorted = vals + Pa;
+  if (P > 0) {
+    const int g = (int)((P + 255) / 256);
+    hipLaunchKernelGGL(ext_hist_kernel, dim3(g), dim3(256), 0, st, P,
+                       b.idx.as<int>(), cnt);
+    hipLaunchKernelGGL(iota_kernel, dim3(g), dim3(256), 0, st, P, vals);
+    int bits = 1;
+    while (bits < 31 && (1 << bits) < n_all) bits++;
+    size_t bytes = 0;
+    FLARE_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(
+        nullptr, bytes, b.idx.as<int>(), keys_sorted, vals, in_pairs, (int)P, 0, bits,
+        st));
+    b.scan_tmp.ensure(bytes);
+    FLARE_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(
+        b.scan_tmp.ptr, bytes, b.idx.as<int>(), keys_sorted, vals, in_pairs, (int)P,
+        0, bits, st));
+  }
+  exclusive_scan(st, b.scan_tmp, cnt, in_cum, n_all);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+__global__ void __launch_bounds__(256)
+ext_incoming_kernel(int n_all, const int *__restrict__ in_cum,
+                    const int *__restrict__ in_pairs, const double *__restrict__ pf,
+                    double *__restrict__ forces) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n_all) return;
+  double f0 = 0, f1 = 0, f2 = 0;
+  for (int q = in_cum[a]; q < in_cum[a + 1]; q++) {
+    const int64_t k = in_pairs[q];
+    f0 += pf[3 * k];
+    f1 += pf[3 * k + 1];
+    f2 += pf[3 * k + 2];
+  }
+  forces[3 * (int64_t)a] -= f0;
+  forces[3 * (int64_t)a + 1] -= f1;
+  forces[3 * (int64_t)a + 2] -= f2;
+}
+
+__global__ void __launch_bounds__(256)
+ext_own_kernel(int inum, const int *__restrict__ ilist, const double *__restrict__ own,
+               const int *__restrict__ row_of, const double *__restrict__ E_row,
+               double *__restrict__ forces, double *__restrict__ eatom) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= inum) return;
+  const int64_t a = ilist[c];
+  forces[3 * a] += own[3 * (int64_t)c];
+  forces[3 * a + 1] += own[3 * (int64_t)c + 1];
+  forces[3 * a + 2] += own[3 * (int64_t)c + 2];
+  if (eatom) eatom[a] += E_row[row_of[c]];
+}
+
+void ext_assemble(hipStream_t st, int n_all, int inum, const int *ilist,
+                  const int *in_cum, const int *in_pairs, const double *pf,
+                  const double *own, const int *row_of, const double *E_row,
+                  double *forces, double *eatom) {
+  if (n_all > 0)
+    hipLaunchKernelGGL(ext_incoming_kernel, dim3((n_all + 255) / 256), dim3(256), 0,
+                       st, n_all, in_cum, in_pairs, pf, forces);
+  if (inum > 0)
+    hipLaunchKernelGGL(ext_own_kernel, dim3((inum + 255) / 256), dim3(256), 0, st,
+                       inum, ilist, own, row_of, E_row, forces, eatom);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
 }  // namespace flare

@@ -126,6 +126,19 @@ class PairFLARE:
                                        _form(form), dptr(d_beta), dptr(energy),
                                        dptr(forces), dptr(virial), dptr(eatom)))
 
+    def compute_neigh_device(self, n_all, inum, d_ilist, d_numneigh, d_neighbors,
+                             stride_i, stride_j, d_x, d_type, form="plain",
+                             energy=None, forces=None, virial=None, eatom=None):
+        """PairFLARE::compute on a caller's full neighbor list with ghost atoms
+        (LAMMPS layout; forces and eatom accumulate, energy and virial are
+        written)."""
+        ctx = _capi.context(self.device)
+        check(lib().flare_pair_compute_neigh(
+            ctx.handle, self._h, int(n_all), int(inum), dptr(d_ilist),
+            dptr(d_numneigh), dptr(d_neighbors), int(stride_i), int(stride_j),
+            dptr(d_x), dptr(d_type), _form(form), dptr(energy), dptr(forces),
+            dptr(virial), dptr(eatom)))
+
     def beta_grad_device(self, out):
         ctx = _capi.context(self.device)
         check(lib().flare_pair_beta_grad(ctx.handle, self._h, dptr(out)))

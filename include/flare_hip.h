@@ -187,6 +187,30 @@ int flare_pair_compute(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
                        const int *d_species, int form, const double *d_beta,
                        double *d_energy, double *d_forces, double *d_virial,
                        double *d_eatom);
+/* PairFLARE::compute on the caller's own full neighbor list over local and
+ * ghost atoms (pair_flare.cpp:68-184; the Kokkos contract of
+ * pair_flare_kokkos.cpp:83-124, 633-662): what a LAMMPS pair_style
+ * flare/hip passes every step.
+ *   d_x       n_all x 3 positions, local atoms first, then ghosts (atom->x)
+ *   d_type    n_all LAMMPS types, 1-based (atom->type)
+ *   d_ilist   inum centre atoms (list->ilist)
+ *   d_numneigh, d_neighbors: neighbor jj of atom i is
+ *             d_neighbors[i * stride_i + jj * stride_j] for jj <
+ *             d_numneigh[i] (LAMMPS' special-bond bits are masked off), i.e.
+ *             Kokkos' d_neighbors(i, jj) in either layout; a CPU adaptor
+ *             packs firstneigh into such an array.
+ * Pairs with r^2 < rc(ti,tj)^2 are used in list order.  Outputs: d_forces
+ * (n_all x 3) and d_eatom (n_all, may be NULL) are ACCUMULATED (+=) like
+ * LAMMPS' f and eatom, ghost rows included (newton on: LAMMPS' reverse
+ * communication sums them onto their owners); d_energy (1) and d_virial (6:
+ * xx yy zz xy xz yz) are written with this call's totals. */
+int flare_pair_compute_neigh(flare_ctx *ctx, flare_pair *pr, int n_all,
+                             int inum, const int *d_ilist,
+                             const int *d_numneigh, const int *d_neighbors,
+                             int64_t stride_i, int64_t stride_j,
+                             const double *d_x, const int *d_type, int form,
+                             double *d_energy, double *d_forces,
+                             double *d_virial, double *d_eatom);
 /* dE/dbeta of the last flare_pair_compute: sum over atoms of species s of
  * d_i d_i^T, d = B/|B| (power 2, S x F x F) or d_i (power 1, S x F). */
 int flare_pair_beta_grad(flare_ctx *ctx, flare_pair *pr, double *d_grad);

@@ -1427,6 +1427,159 @@ extern "C" void orc_std_atom(const orc_nlist *nl, const int *species, int S,
   }
 }
 
+// ---------------------------------------------------------------------------
+// PairFLARE::compute on a LAMMPS full neighbor list with ghost atoms
+// (pair_flare.cpp:68-184; lammps_descriptor.cpp:7-107, 209-304), serial:
+// forces on all n_all atoms accumulate as f[i] += f_ij, f[j] -= f_ij in list
+// order; virial by ev_tally_xyz with del = x_i - x_j; energy by
+// ev_tally_full(i, 2 evdwl) (eatom[i] += evdwl).  firstneigh is given as CSR
+// (offsets[i] .. offsets[i] + numneigh[i]) indexed by atom i.
+
+extern "C" void orc_pair_compute_lmp(int n_all, const double *x,
+                                     const int *type, int inum,
+                                     const int *ilist, const int *numneigh,
+                                     const int *offsets, const int *neigh,
+                                     int S, int N, int L, int power,
+                                     const double *cutoffs, int cutoff_kind,
+                                     const double *beta_full, double *energy,
+                                     double *forces, double *virial,
+                                     double *eatom) {
+  const int H = (L + 1) * (L + 1);
+  const int nr = S * N;
+  const int W = nr * H;
+  const int F = nr * (nr + 1) / 2 * (L + 1);
+  const double empty_thresh = 1e-8;
+  std::vector<double> g(N), gx(N), gy(N), gz(N), Y(H), Yx(H), Yy(H), Yz(H);
+  std::vector<double> A(W), B(F), Z, w(F), u(W), dA;
+  double e = 0;
+  for (int c = 0; c < 6; c++) virial[c] = 0;
+  for (int ii = 0; ii < inum; ii++) {
+    const int i = ilist[ii];
+    const int itype = type[i];
+    const int jnum = numneigh[i];
+    const int *jlist = neigh + offsets[i];
+    const double xtmp = x[3 * i], ytmp = x[3 * i + 1], ztmp = x[3 * i + 2];
+    const int si = itype - 1;
+    int n_inner = 0;
+    for (int jj = 0; jj < jnum; jj++) {
+      int j = jlist[jj];
+      double cv = cutoffs[si * S + type[j] - 1];
+      double dx = x[3 * j] - xtmp, dy = x[3 * j + 1] - ytmp, dz = x[3 * j + 2] - ztmp;
+      double rsq = dx * dx + dy * dy + dz * dz;
+      if (rsq < cv * cv) n_inner++;
+    }
+    std::fill(A.begin(), A.end(), 0.0);
+    dA.assign((size_t)n_inner * 3 * W, 0.0);
+    int q = 0;
+    for (int jj = 0; jj < jnum; jj++) {
+      int j = jlist[jj];
+      double dx = x[3 * j] - xtmp, dy = x[3 * j + 1] - ytmp, dz = x[3 * j + 2] - ztmp;
+      double rsq = dx * dx + dy * dy + dz * dz;
+      double r = std::sqrt(rsq);
+      int s = type[j] - 1;
+      double rc = cutoffs[si * S + s];
+      double cutforcesq = rc * rc;
+      if (!(rsq < cutforcesq)) continue;
+      orc_radial(dx, dy, dz, r, rc, N, 0.0, cutoff_kind, g.data(), gx.data(),
+                 gy.data(), gz.data());
+      orc_ylm(dx, dy, dz, L, Y.data(), Yx.data(), Yy.data(), Yz.data());
+      int dc = s * N * H;
+      for (int rn = 0; rn < N; rn++)
+        for (int a = 0; a < H; a++) {
+          A[dc] += g[rn] * Y[a];
+          dA[(size_t)(q * 3 + 0) * W + dc] += gx[rn] * Y[a] + g[rn] * Yx[a];
+          dA[(size_t)(q * 3 + 1) * W + dc] += gy[rn] * Y[a] + g[rn] * Yy[a];
+          dA[(size_t)(q * 3 + 2) * W + dc] += gz[rn] * Y[a] + g[rn] * Yz[a];
+          dc++;
+        }
+      q++;
+    }
+    std::fill(B.begin(), B.end(), 0.0);
+    for (int n1 = nr - 1; n1 >= 0; n1--) {
+      int n1c = (n1 * (2 * nr - n1 + 1)) / 2;
+      for (int n2 = n1; n2 < nr; n2++) {
+        int n2c = n2 - n1;
+        for (int l = 0; l < L + 1; l++) {
+          int counter = l + (n1c + n2c) * (L + 1);
+          for (int m = 0; m < 2 * l + 1; m++)
+            B[counter] += A[n1 * H + l * l + m] * A[n2 * H + l * l + m];
+        }
+      }
+    }
+    double norm2 = 0;
+    for (int c = 0; c < F; c++) norm2 += B[c] * B[c];
+    if (norm2 < empty_thresh) continue;
+    const double *beta = beta_full + (size_t)si * (power == 1 ? F : (size_t)F * F);
+    double evdwl;
+    if (power == 1) {
+      double bn = std::pow(norm2, 0.5);
+      evdwl = dotn(B.data(), beta, F) / bn;
+      for (int c = 0; c < F; c++) w[c] = beta[c] / bn - evdwl * B[c] / norm2;
+    } else {
+      Z.assign(F, 0.0);
+      for (int j = 0; j < F; j++) {
+        const double *col = beta + (size_t)j * F;
+        double bj = B[j];
+        for (int c = 0; c < F; c++) Z[c] += col[c] * bj;
+      }
+      evdwl = dotn(B.data(), Z.data(), F) / norm2;
+      for (int c = 0; c < F; c++) w[c] = 2 * (Z[c] - evdwl * B[c]) / norm2;
+    }
+    std::fill(u.begin(), u.end(), 0.0);
+    for (int n1 = nr - 1; n1 >= 0; n1--)
+      for (int n2 = 0; n2 < nr; n2++) {
+        int n1c, n2c;
+        double factor;
+        if (n1 == n2) {
+          n1c = (n1 * (2 * nr - n1 + 1)) / 2;
+          n2c = n2 - n1;
+          factor = 1.0;
+        } else if (n1 < n2) {
+          n1c = (n1 * (2 * nr - n1 + 1)) / 2;
+          n2c = n2 - n1;
+          factor = 0.5;
+        } else {
+          n1c = (n2 * (2 * nr - n2 + 1)) / 2;
+          n2c = n1 - n2;
+          factor = 0.5;
+        }
+        for (int l = 0; l < L + 1; l++) {
+          int counter = l + (n1c + n2c) * (L + 1);
+          for (int m = 0; m < 2 * l + 1; m++)
+            u[n1 * H + l * l + m] += w[counter] * A[n2 * H + l * l + m] * factor;
+        }
+      }
+    for (int c = 0; c < W; c++) u[c] *= 2;
+    q = 0;
+    for (int jj = 0; jj < jnum; jj++) {
+      int j = jlist[jj];
+      double rc = cutoffs[si * S + type[j] - 1];
+      double delx = xtmp - x[3 * j], dely = ytmp - x[3 * j + 1], delz = ztmp - x[3 * j + 2];
+      double rsq = delx * delx + dely * dely + delz * delz;
+      if (!(rsq < rc * rc)) continue;
+      double fx = dotn(&dA[(size_t)(q * 3 + 0) * W], u.data(), W);
+      double fy = dotn(&dA[(size_t)(q * 3 + 1) * W], u.data(), W);
+      double fz = dotn(&dA[(size_t)(q * 3 + 2) * W], u.data(), W);
+      forces[3 * i] += fx;
+      forces[3 * i + 1] += fy;
+      forces[3 * i + 2] += fz;
+      forces[3 * j] -= fx;
+      forces[3 * j + 1] -= fy;
+      forces[3 * j + 2] -= fz;
+      virial[0] += delx * fx;
+      virial[1] += dely * fy;
+      virial[2] += delz * fz;
+      virial[3] += delx * fy;
+      virial[4] += delx * fz;
+      virial[5] += dely * fz;
+      q++;
+    }
+    e += evdwl;  // ev_tally_full(i, 2 evdwl): eng_vdwl += 0.5 * 2 evdwl
+    if (eatom) eatom[i] += evdwl;
+  }
+  *energy = e;
+}
+
 extern "C" void orc_pair_compute(const orc_nlist *nl, const int *species,
                                  int S, int N, int L, int power,
                                  const double *cutoffs, int cutoff_kind,

@@ -172,6 +172,18 @@ void orc_std_atom(const orc_nlist *nl, const int *species, int S, int N,
                   const double *sparse_normed, const double *Linv,
                   int nthreads, double *stds);
 
+/* PairFLARE::compute on a LAMMPS full list with ghosts (serial): x n_all x 3,
+ * LAMMPS types (1-based), centres ilist[inum], neighbors of atom i at
+ * neigh[offsets[i] .. offsets[i] + numneigh[i]).  forces (n_all x 3) and
+ * eatom (n_all, may be NULL) accumulate; energy and virial are written. */
+void orc_pair_compute_lmp(int n_all, const double *x, const int *type,
+                          int inum, const int *ilist, const int *numneigh,
+                          const int *offsets, const int *neigh, int S, int N,
+                          int L, int power, const double *cutoffs,
+                          int cutoff_kind, const double *beta_full,
+                          double *energy, double *forces, double *virial,
+                          double *eatom);
+
 /* unpack file beta (pair_flare.cpp:363-401): packed -> S x F x F */
 void orc_unpack_beta(int S, int F, int power, const double *packed,
                      double *full);

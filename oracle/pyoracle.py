@@ -101,6 +101,10 @@ def lib():
                                              _d, _d, _d]),
             "orc_unpack_beta": (None, [ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                        _d, _d]),
+            "orc_pair_compute_lmp": (None, [ctypes.c_int, _d, _i, ctypes.c_int, _i,
+                                            _i, _i, _i, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int, _d,
+                                            ctypes.c_int, _d, _d, _d, _d, _d]),
             "orc_varmap_coefficients": (ctypes.c_int, [ctypes.c_int, ctypes.c_int,
                                                        _i, _d, _d,
                                                        ctypes.c_double, _d, _d]),
@@ -413,3 +417,22 @@ def std_atom_linv(nlist, species, S, N, L, cutoffs, sigma, power, n_by_type,
                        None, int(power), sp.shape[0], _pi(_i32(n_by_type)),
                        _pd(sp), _pd(_f64(Linv)), nthreads, _pd(out))
     return out
+
+
+def pair_compute_lmp(x, type_, ilist, numneigh, offsets, neigh, S, N, L, power,
+                     cutoffs, beta_full, cutoff="quadratic"):
+    """PairFLARE::compute on a LAMMPS-style list with ghosts: energy, forces
+    (n_all x 3, ghost rows included), virial (xx yy zz xy xz yz), eatom."""
+    x = _f64(x)
+    n_all = len(x)
+    e = np.zeros(1)
+    f = np.zeros((n_all, 3))
+    v = np.zeros(6)
+    ea = np.zeros(n_all)
+    lib().orc_pair_compute_lmp(n_all, _pd(x), _pi(_i32(type_)), len(ilist),
+                               _pi(_i32(ilist)), _pi(_i32(numneigh)),
+                               _pi(_i32(offsets)), _pi(_i32(neigh)), S, N, L,
+                               power, _pd(_f64(cutoffs).ravel()),
+                               cutoff_code(cutoff), _pd(_f64(beta_full)), _pd(e),
+                               _pd(f), _pd(v), _pd(ea))
+    return e[0], f, v, ea

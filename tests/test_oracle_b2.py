@@ -241,3 +241,38 @@ def test_pair_empty_environment(orc):
                                      beta, packed=False, want_eatom=True)
     assert ea[0] == 0 and np.all(F[0] == 0)
     assert np.isfinite(E) and ea[1] != 0
+
+
+@pytest.mark.parametrize("power", [1, 2])
+def test_pair_on_lammps_ghost_list_equals_periodic(orc, power):
+    # PairFLARE::compute on a full list with ghost atoms, ghost forces summed
+    # onto their owners (LAMMPS' reverse communication, newton on), equals the
+    # periodic evaluation on the Structure list
+    from helpers import lammps_ghost_list
+    cell, species, pos = small_two_species(seed=6)
+    S, N, L = 2, 4, 3
+    cut = np.array([[5.0, 4.6], [4.6, 4.2]])
+    nr = S * N
+    F = nr * (nr + 1) // 2 * (L + 1)
+    rng = np.random.default_rng(2)
+    if power == 2:
+        a = rng.normal(0, 1e-3, (S, F, F))
+        beta = (a + a.transpose(0, 2, 1)) / 2
+    else:
+        beta = rng.normal(0, 1e-2, (S, F))
+    g = lammps_ghost_list(cell, species, pos, 6.0, 6.0, seed=3)
+    e, f_all, vir, ea = orc.pair_compute_lmp(g["x"], g["type"], g["ilist"],
+                                             g["numneigh"], g["offsets"], g["neigh"],
+                                             S, N, L, power, cut, beta)
+    n = len(species)
+    f = np.zeros((n, 3))
+    np.add.at(f, g["owner"], f_all)
+    nl = orc.NeighborList(cell, species, pos, cut.max())
+    E0, F0, V0, ea0 = orc.pair_compute(nl, species, S, N, L, power, cut, beta,
+                                       packed=False, want_eatom=True)
+    scale = np.abs(F0).max()
+    assert abs(e - E0) < 1e-12 * max(1.0, abs(E0))
+    np.testing.assert_allclose(f, F0, atol=1e-11 * scale)
+    np.testing.assert_allclose(vir, V0, atol=1e-11 * np.abs(V0).max())
+    np.testing.assert_allclose(ea[:n], ea0, atol=1e-12 * np.abs(ea0).max())
+    assert np.all(ea[n:] == 0)
