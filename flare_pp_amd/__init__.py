@@ -15,5 +15,6 @@ from ._capi import FlareError, FORMS  # noqa: F401
 from .structure import B2, ClusterDescriptor, DescriptorValues, Structure  # noqa: F401
 from .sparse_gp import NormalizedDotProduct, SparseGP  # noqa: F401
 from .pair import PairFLARE, pack_beta, write_coefficients  # noqa: F401
-from .calculator import build_map, efs_to_results, local_stds, sort_variances  # noqa: F401
+from .calculator import (build_map, efs_stds, efs_to_results, local_stds,  # noqa: F401
+                         sort_variances)
 from .std_atom import ComputeFlareStdAtom, read_std_files  # noqa: F401

@@ -178,6 +178,14 @@ SIGNATURES = {
     "flare_std_get_info": (_i, [_vp, ctypes.POINTER(StdInfo)]),
     "flare_std_compute": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "flare_sgp_varmap_coefficients": (_i, [_vp, _vp, _vp]),
+    "flare_sgp_envs_struc": (_i, [_vp, _vp, ctypes.POINTER(B2Params), _vp, _vp, _vp]),
+    "flare_sgp_self_kernel_struc": (_i, [_vp, _vp, ctypes.POINTER(B2Params), _vp, _vp,
+                                         _vp]),
+    "flare_sgp_set_R_inv": (_i, [_vp, _vp, _vp]),
+    "flare_sgp_predict_SOR": (_i, [_vp, _vp, ctypes.POINTER(B2Params), _vp, _vp, _vp,
+                                   _vp]),
+    "flare_sgp_predict_DTC": (_i, [_vp, _vp, ctypes.POINTER(B2Params), _vp, _vp, _vp,
+                                   _vp]),
     "flare_write_varmap_coefficients": (_i, [_cp, _cp, _i, _pd, _i, _i, _i, _cp,
                                              _cp, _pd, _i64, _pd]),
     "flare_write_L_inverse": (_i, [_cp, _cp, _d, _i, _pd, _i, _i, _i, _cp, _cp,

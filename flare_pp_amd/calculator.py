@@ -42,6 +42,13 @@ def local_stds(atom_indices, variances, sigma):
     return full
 
 
+def efs_stds(variance_efs):
+    """Per-force-component stds of the SOR / DTC variances (n x 3): signed
+    square roots of variance_efs[1:-6] (sparse_gp_calculator.py:81-94)."""
+    v = np.asarray(variance_efs)[1:-6]
+    return np.where(v > 0, np.sqrt(np.abs(v)), -np.sqrt(np.abs(v))).reshape(-1, 3)
+
+
 def build_map(sparse_gp, filename="lmp.flare", contributor="user"):
     """SGP_Calculator.build_map (sparse_gp_calculator.py:159-163): the
     pair_style coefficient file plus the L^-1 and sparse-descriptor files of

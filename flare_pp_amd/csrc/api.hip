@@ -114,6 +114,11 @@ struct flare_ctx {
     p->ensure(bytes);
     return p->ptr;
   }
+  DevBuf *ws_buf(const std::string &key) {
+    auto &p = ws[key];
+    if (!p) p.reset(new DevBuf);
+    return p.get();
+  }
   template <typename T>
   T *buf_t(const std::string &key, size_t count) {
     return reinterpret_cast<T *>(buf(key, std::max<size_t>(count, 1) * sizeof(T)));
@@ -632,6 +637,8 @@ struct flare_sgp {
   std::vector<int> m_by_type, m_off;
   DevBuf P, Phat, pnorm, pnorm2, alpha, Kuu, Lcm, Linv_cm;
   bool have_alpha = false;
+  DevBuf R_inv;  // M x M row-major, from training (sparse_gp.cpp:604-607)
+  bool have_R_inv = false;
   // low-precision copies of the normalised sparse rows, built on first use:
   // FP32 rows; FP16 hi/lo rows of kp with per-row inverse scales; per
   // species the FP16 hi/lo transpose (F rows of mp[s], the species' count
@@ -2493,6 +2500,316 @@ int flare_write_sparse_descriptors(const char *path, const char *contributor,
       if (count % 5 != 1) std::fprintf(f, "\n");
     }
     close_out(f, path);
+  });
+}
+
+
+}  // extern "C"
+
+// ---- explicit kernel matrices: SOR / DTC ------------------------------------
+
+// DescriptorValues on the device (b2.cpp:76-179 layout, all species' blocks
+// concatenated): rows, per-row pair ranges and the pair tensors.
+struct DVDev {
+  Forward fw;
+  int64_t Q = 0;
+  std::vector<int64_t> q0h;  // per row
+  std::vector<int> cnth;     // per row
+  std::vector<int> atom_of_row;
+  int off[flare::kMaxSpecies + 1];
+  int64_t qoff[flare::kMaxSpecies + 1];
+  int64_t *q0 = nullptr;
+  int *cnt = nullptr, *ni = nullptr, *pair_rowg = nullptr;
+  double *dB = nullptr, *fd = nullptr, *co = nullptr;
+};
+
+static DVDev dv_build(flare_ctx *ctx, const flare::B2Dev &p, const flare_nlist *nl,
+                      const int *d_species, const std::string &tag) {
+  DVDev dv;
+  const int n = nl->n;
+  dv.fw = run_forward(ctx, p, nl, d_species, flare::kPrecF64, tag);
+  B2Pairs r = b2_pairs(ctx, p, nl, d_species);
+  nl->row_offsets(p.S, dv.off);
+  dv.Q = r.total;
+  dv.q0h = r.q0;
+  dv.atom_of_row = r.atom_of_row;
+  dv.cnth.resize(n);
+  for (int row = 0; row < n; row++) dv.cnth[row] = r.count[r.atom_of_row[row]];
+  // species pair offsets from the row ranges
+  {
+    int64_t q = 0;
+    for (int s = 0; s < p.S; s++) {
+      dv.qoff[s] = q;
+      for (int row = dv.off[s]; row < dv.off[s + 1]; row++) q += dv.cnth[row];
+    }
+    dv.qoff[p.S] = q;
+  }
+  const size_t Qa = (size_t)std::max<int64_t>(dv.Q, 1);
+  dv.q0 = ctx->buf_t<int64_t>(tag + ".q0", n);
+  dv.cnt = ctx->buf_t<int>(tag + ".cnt", n);
+  if (n) {
+    FLARE_HIP_CHECK(hipMemcpyAsync(dv.q0, dv.q0h.data(), n * sizeof(int64_t),
+                                   hipMemcpyHostToDevice, ctx->stream));
+    FLARE_HIP_CHECK(hipMemcpyAsync(dv.cnt, dv.cnth.data(), n * sizeof(int),
+                                   hipMemcpyHostToDevice, ctx->stream));
+  }
+  std::vector<int> prow(Qa, 0);
+  for (int row = 0; row < n; row++)
+    for (int k = 0; k < dv.cnth[row]; k++) prow[dv.q0h[row] + k] = row;
+  dv.pair_rowg = ctx->buf_t<int>(tag + ".prow", Qa);
+  FLARE_HIP_CHECK(hipMemcpyAsync(dv.pair_rowg, prow.data(), Qa * sizeof(int),
+                                 hipMemcpyHostToDevice, ctx->stream));
+  dv.dB = ctx->buf_t<double>(tag + ".dB", Qa * 3 * p.F);
+  dv.fd = ctx->buf_t<double>(tag + ".fd", Qa * 3);
+  dv.co = ctx->buf_t<double>(tag + ".co", Qa * 3);
+  dv.ni = ctx->buf_t<int>(tag + ".ni", Qa);
+  if (dv.Q > 0) {
+    int64_t *fpair = ctx->buf_t<int64_t>(tag + ".fpair", dv.Q);
+    flare::launch_b2_fill_fpair(ctx->stream, p, nl->dev(), d_species,
+                                nl->b.row_of.as<int>(), dv.q0, fpair);
+    flare::launch_b2_dervs(ctx->stream, p, nl->dev(), d_species, nl->b.row_of.as<int>(),
+                           ctx->ynorm_d.as<double>(), ctx->b2_index(p.S, p.N, p.L),
+                           dv.Q, fpair, (const double *)dv.fw.A,
+                           (const double *)dv.fw.B, dv.dB, dv.fd, dv.co, dv.ni);
+  }
+  // keep the host vectors alive until the uploads are done
+  ctx->sync();
+  return dv;
+}
+
+// envs_struc of the sparse set against the structure: d_K M x (1+3n+6)
+// (normalized_dot_product.cpp:115-210).
+static void sgp_envs_struc(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p,
+                           const flare_nlist *nl, const DVDev &dv, double *d_K) {
+  const int n = nl->n, F = p.F, M = gp->M;
+  const int64_t n_out = 1 + 3 * (int64_t)n + 6;
+  FLARE_HIP_CHECK(hipMemsetAsync(d_K, 0, std::max<int64_t>(M * n_out, 1) * sizeof(double),
+                                 ctx->stream));
+  const double *B = (const double *)dv.fw.B;
+  const double vol_inv = 1.0 / nl->wh.volume;
+  std::vector<int> own0(std::max(n, 1)), ownc(std::max(n, 1));
+  for (int s = 0; s < p.S; s++) {
+    const int Ms = gp->m_by_type[s], o = gp->m_off[s];
+    const int ns = dv.off[s + 1] - dv.off[s];
+    const int64_t Qs = dv.qoff[s + 1] - dv.qoff[s];
+    if (!Ms || !ns) continue;
+    const double *Ps = gp->P.as<double>() + (size_t)o * F;
+    double *DOT = ctx->buf_t<double>("es.dot", (size_t)Ms * ns);
+    gemm_rm(ctx, false, true, Ms, ns, F, 1.0, Ps, F, B + (size_t)dv.off[s] * F, F, 0.0,
+            DOT, ns);
+    double *FD = ctx->buf_t<double>("es.fd", (size_t)Ms * 3 * std::max<int64_t>(Qs, 1));
+    if (Qs > 0)
+      gemm_rm(ctx, false, true, Ms, (int)(3 * Qs), F, 1.0, Ps, F,
+              dv.dB + (size_t)dv.qoff[s] * 3 * F, F, 0.0, FD, (int)(3 * Qs));
+    // pair -> row within the species; own pair ranges per atom
+    std::vector<int> prow(std::max<int64_t>(Qs, 1));
+    std::fill(ownc.begin(), ownc.end(), 0);
+    std::fill(own0.begin(), own0.end(), 0);
+    for (int row = dv.off[s]; row < dv.off[s + 1]; row++) {
+      const int64_t b = dv.q0h[row] - dv.qoff[s];
+      for (int k = 0; k < dv.cnth[row]; k++) prow[b + k] = row - dv.off[s];
+      own0[dv.atom_of_row[row]] = (int)b;
+      ownc[dv.atom_of_row[row]] = dv.cnth[row];
+    }
+    int *d_prow = ctx->buf_t<int>("es.prow", prow.size());
+    int *d_own0 = ctx->buf_t<int>("es.own0", own0.size());
+    int *d_ownc = ctx->buf_t<int>("es.ownc", ownc.size());
+    FLARE_HIP_CHECK(hipMemcpyAsync(d_prow, prow.data(), prow.size() * sizeof(int),
+                                   hipMemcpyHostToDevice, ctx->stream));
+    FLARE_HIP_CHECK(hipMemcpyAsync(d_own0, own0.data(), own0.size() * sizeof(int),
+                                   hipMemcpyHostToDevice, ctx->stream));
+    FLARE_HIP_CHECK(hipMemcpyAsync(d_ownc, ownc.data(), ownc.size() * sizeof(int),
+                                   hipMemcpyHostToDevice, ctx->stream));
+    double *Ks = d_K + (size_t)o * n_out;
+    flare::launch_envs_struc_epilogue(ctx->stream, Ms, ns, Qs, DOT, FD,
+                                      gp->pnorm.as<double>() + o, dv.fw.norm2 + dv.off[s],
+                                      dv.fd + 3 * dv.qoff[s], dv.co + 3 * dv.qoff[s],
+                                      d_prow, gp->sig2, gp->power, vol_inv, Ks, n_out, n);
+    int *in_cum = ctx->buf_t<int>("es.in_cum", (size_t)n + 1);
+    int *in_pairs = ctx->buf_t<int>("es.in_pairs", std::max<int64_t>(Qs, 1));
+    flare::index_by_target(ctx->stream, *ctx->ws_buf("es.w1"), *ctx->ws_buf("es.w2"),
+                           *ctx->ws_buf("es.w3"), dv.ni + dv.qoff[s], Qs, n, in_cum,
+                           in_pairs);
+    flare::launch_envs_struc_gather(ctx->stream, Ms, n, Qs, FD, d_own0, d_ownc, in_cum,
+                                    in_pairs, Ks, n_out);
+    ctx->sync();  // host vectors are reused by the next species
+  }
+}
+
+// self_kernel_struc (normalized_dot_product.cpp:446-578) into d_out (1+3n+6)
+static void self_kernel(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p,
+                        const flare_nlist *nl, const DVDev &dv, double *d_out) {
+  const int n = nl->n, F = p.F;
+  const int64_t n_out = 1 + 3 * (int64_t)n + 6;
+  FLARE_HIP_CHECK(hipMemsetAsync(d_out, 0, n_out * sizeof(double), ctx->stream));
+  const double *B = (const double *)dv.fw.B;
+  const double vol_inv = 1.0 / nl->wh.volume;
+  double *sums = ctx->buf_t<double>("sk.sums", (size_t)p.S * 7);
+  FLARE_HIP_CHECK(hipMemsetAsync(sums, 0, (size_t)p.S * 7 * sizeof(double), ctx->stream));
+  const int combos[6][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 2}};
+  for (int s = 0; s < p.S; s++) {
+    const int ns = dv.off[s + 1] - dv.off[s];
+    if (!ns) continue;
+    const double *Bs = B + (size_t)dv.off[s] * F;
+    const double *n2 = dv.fw.norm2 + dv.off[s];
+    double *T = ctx->buf_t<double>("sk.T", (size_t)ns * ns);
+    double *U = ctx->buf_t<double>("sk.U", (size_t)ns * ns);
+    double *V = ctx->buf_t<double>("sk.V", (size_t)ns * ns);
+    double *G = ctx->buf_t<double>("sk.G", (size_t)ns * F);
+    double *g = ctx->buf_t<double>("sk.g", ns);
+    double *rows = ctx->buf_t<double>("sk.rows", ns);
+    gemm_rm(ctx, false, true, ns, ns, F, 1.0, Bs, F, Bs, F, 0.0, T, ns);
+    flare::launch_self_rows(ctx->stream, ns, T, nullptr, nullptr, nullptr, n2, gp->sig2,
+                            gp->power, 1, rows);
+    flare::launch_colsum(ctx->stream, ns, 1, rows, sums + s * 7);
+    for (int q = 0; q < 6; q++) {
+      flare::launch_self_g(ctx->stream, ns, F, combos[q][0], combos[q][1],
+                           dv.q0 + dv.off[s], dv.cnt + dv.off[s], dv.dB, dv.fd, dv.co, G,
+                           g);
+      gemm_rm(ctx, false, true, ns, ns, F, 1.0, G, F, Bs, F, 0.0, U, ns);
+      gemm_rm(ctx, false, true, ns, ns, F, 1.0, G, F, G, F, 0.0, V, ns);
+      flare::launch_self_rows(ctx->stream, ns, T, U, V, g, n2, gp->sig2, gp->power, 0,
+                              rows);
+      flare::launch_colsum(ctx->stream, ns, 1, rows, sums + s * 7 + 1 + q);
+    }
+  }
+  // force entries
+  int *in_cum = ctx->buf_t<int>("sk.in_cum", (size_t)n + 1);
+  int *in_pairs = ctx->buf_t<int>("sk.in_pairs", std::max<int64_t>(dv.Q, 1));
+  flare::index_by_target(ctx->stream, *ctx->ws_buf("sk.w1"), *ctx->ws_buf("sk.w2"),
+                         *ctx->ws_buf("sk.w3"), dv.ni, dv.Q, n, in_cum, in_pairs);
+  int *ovf = ctx->buf_t<int>("sk.ovf", 1);
+  FLARE_HIP_CHECK(hipMemsetAsync(ovf, 0, sizeof(int), ctx->stream));
+  flare::launch_self_force(ctx->stream, n, F, nl->b.row_of.as<int>(), dv.q0, dv.cnt,
+                           in_cum, in_pairs, dv.pair_rowg,
+                           nl->b.species_of_row.as<int>(), B, dv.fw.norm2, dv.dB, dv.fd,
+                           gp->sig2, gp->power, d_out, ovf);
+  std::vector<double> hs((size_t)p.S * 7);
+  int hovf = 0;
+  FLARE_HIP_CHECK(hipMemcpyAsync(hs.data(), sums, hs.size() * sizeof(double),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+  FLARE_HIP_CHECK(hipMemcpyAsync(&hovf, ovf, sizeof(int), hipMemcpyDeviceToHost,
+                                 ctx->stream));
+  ctx->sync();
+  REQUIRE(hovf == 0, "an atom touches more pairs than the self-kernel workgroup holds (" +
+                         std::to_string(flare::kSelfMaxPairs) + ")");
+  double tail[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int s = 0; s < p.S; s++)
+    for (int q = 0; q < 7; q++) tail[q] += hs[(size_t)s * 7 + q];
+  for (int q = 1; q < 7; q++) tail[q] *= vol_inv * vol_inv;
+  FLARE_HIP_CHECK(hipMemcpyAsync(d_out, tail, sizeof(double), hipMemcpyHostToDevice,
+                                 ctx->stream));
+  FLARE_HIP_CHECK(hipMemcpyAsync(d_out + 1 + 3 * (int64_t)n, tail + 1, 6 * sizeof(double),
+                                 hipMemcpyHostToDevice, ctx->stream));
+  ctx->sync();
+}
+
+enum { kVarSOR = 0, kVarDTC = 1 };
+
+static void sgp_predict_var(flare_ctx *ctx, flare_sgp *gp, const flare_b2_params *params,
+                            const flare_nlist *nl, const int *d_species,
+                            double *d_mean_efs, double *d_var, int kind) {
+  flare::B2Dev p = make_b2(params, 0);
+  check_species(nl, p.S);
+  REQUIRE(p.S == gp->S && p.F == gp->F, "descriptor settings differ from the sparse set");
+  REQUIRE(gp->have_alpha, "alpha not set");
+  REQUIRE(gp->have_R_inv, "R_inv not set (flare_sgp_set_R_inv)");
+  const int n = nl->n, M = gp->M;
+  const int64_t n_out = 1 + 3 * (int64_t)n + 6;
+  DVDev dv = dv_build(ctx, p, nl, d_species, "sv");
+  double *K = ctx->buf_t<double>("sv.K", (size_t)std::max(M, 1) * n_out);
+  sgp_envs_struc(ctx, gp, p, nl, dv, K);
+  // mean_efs = K^T alpha
+  if (M)
+    gemm_rm(ctx, false, false, 1, (int)n_out, M, 1.0, gp->alpha.as<double>(), M, K,
+            (int)n_out, 0.0, d_mean_efs, (int)n_out);
+  else
+    FLARE_HIP_CHECK(hipMemsetAsync(d_mean_efs, 0, n_out * sizeof(double), ctx->stream));
+  double *Y = ctx->buf_t<double>("sv.Y", (size_t)std::max(M, 1) * n_out);
+  if (kind == kVarDTC) {
+    self_kernel(ctx, gp, p, nl, dv, d_var);
+    if (M) {
+      // - diag(K^T K_uu^-1 K) = - |L^-1 K|^2 (K_uu^-1 = L^-T L^-1)
+      gemm_rm(ctx, true, false, M, (int)n_out, M, 1.0, gp->Linv_cm.as<double>(), M, K,
+              (int)n_out, 0.0, Y, (int)n_out);
+      flare::launch_col_sqsum(ctx->stream, M, n_out, Y, -1.0, 1, d_var);
+    }
+  } else {
+    FLARE_HIP_CHECK(hipMemsetAsync(d_var, 0, n_out * sizeof(double), ctx->stream));
+  }
+  if (M) {
+    // + diag(K^T Sigma K) = |R_inv^T K|^2 (Sigma = R_inv R_inv^T)
+    gemm_rm(ctx, true, false, M, (int)n_out, M, 1.0, gp->R_inv.as<double>(), M, K,
+            (int)n_out, 0.0, Y, (int)n_out);
+    flare::launch_col_sqsum(ctx->stream, M, n_out, Y, 1.0, 1, d_var);
+  }
+}
+
+extern "C" {
+
+int flare_sgp_envs_struc(flare_ctx *ctx, flare_sgp *gp, const flare_b2_params *params,
+                         const flare_nlist *nl, const int *d_species, double *d_K) {
+  return guarded([&] {
+    REQUIRE(ctx && gp && d_K, "null argument");
+    flare::B2Dev p = make_b2(params, 0);
+    check_species(nl, p.S);
+    REQUIRE(p.S == gp->S && p.F == gp->F, "descriptor settings differ from the sparse set");
+    ctx->use();
+    DVDev dv = dv_build(ctx, p, nl, d_species, "sv");
+    sgp_envs_struc(ctx, gp, p, nl, dv, d_K);
+    ctx->sync();
+  });
+}
+
+int flare_sgp_self_kernel_struc(flare_ctx *ctx, flare_sgp *gp,
+                                const flare_b2_params *params, const flare_nlist *nl,
+                                const int *d_species, double *d_out) {
+  return guarded([&] {
+    REQUIRE(ctx && gp && d_out, "null argument");
+    flare::B2Dev p = make_b2(params, 0);
+    check_species(nl, p.S);
+    REQUIRE(p.S == gp->S && p.F == gp->F, "descriptor settings differ from the sparse set");
+    ctx->use();
+    DVDev dv = dv_build(ctx, p, nl, d_species, "sv");
+    self_kernel(ctx, gp, p, nl, dv, d_out);
+  });
+}
+
+int flare_sgp_set_R_inv(flare_ctx *ctx, flare_sgp *gp, const double *d_R_inv) {
+  return guarded([&] {
+    REQUIRE(ctx && gp, "null argument");
+    REQUIRE(gp->M == 0 || d_R_inv, "null R_inv");
+    ctx->use();
+    const size_t bytes = (size_t)std::max(gp->M, 1) * std::max(gp->M, 1) * sizeof(double);
+    gp->R_inv.ensure(bytes);
+    if (gp->M)
+      FLARE_HIP_CHECK(hipMemcpyAsync(gp->R_inv.ptr, d_R_inv,
+                                     (size_t)gp->M * gp->M * sizeof(double),
+                                     hipMemcpyDeviceToDevice, ctx->stream));
+    ctx->sync();
+    gp->have_R_inv = true;
+  });
+}
+
+int flare_sgp_predict_SOR(flare_ctx *ctx, flare_sgp *gp, const flare_b2_params *params,
+                          const flare_nlist *nl, const int *d_species,
+                          double *d_mean_efs, double *d_variance_efs) {
+  return guarded([&] {
+    REQUIRE(ctx && gp && d_mean_efs && d_variance_efs, "null argument");
+    ctx->use();
+    sgp_predict_var(ctx, gp, params, nl, d_species, d_mean_efs, d_variance_efs, kVarSOR);
+    ctx->sync();
+  });
+}
+
+int flare_sgp_predict_DTC(flare_ctx *ctx, flare_sgp *gp, const flare_b2_params *params,
+                          const flare_nlist *nl, const int *d_species,
+                          double *d_mean_efs, double *d_variance_efs) {
+  return guarded([&] {
+    REQUIRE(ctx && gp && d_mean_efs && d_variance_efs, "null argument");
+    ctx->use();
+    sgp_predict_var(ctx, gp, params, nl, d_species, d_mean_efs, d_variance_efs, kVarDTC);
+    ctx->sync();
   });
 }
 

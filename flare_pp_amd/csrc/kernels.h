@@ -103,6 +103,11 @@ int64_t nlist_from_external(hipStream_t st, NListBuffers &b, const ExtList &e,
 // and the pair indices sorted by neighbor (stable, ascending pair index).
 void ext_incoming(hipStream_t st, NListBuffers &b, int n_all, int64_t P,
                   int *in_cum, int *in_pairs);
+// Pairs grouped by a key in [0, n_all) (stable, ascending pair index):
+// in_cum (n_all + 1) and in_pairs (P).  work, cnt_buf, scan_tmp: scratch.
+void index_by_target(hipStream_t st, DevBuf &work, DevBuf &cnt_buf, DevBuf &scan_tmp,
+                     const int *keys, int64_t P, int n_all, int *in_cum,
+                     int *in_pairs);
 // forces[a] -= sum of pf over pairs whose neighbor is a (a < n_all), then
 // forces[ilist[c]] += own[c] (c < inum); eatom[ilist[c]] += E_row[row_of[c]]
 // when eatom is non-null.
@@ -212,6 +217,37 @@ void launch_std_kvec(hipStream_t st, int rows, int M, double *T,
 void launch_std_linv(hipStream_t st, int rows, int M, const double *Q,
                      const double *norm2, double sig2, const int *atom_of_row,
                      double *stds);
+// gp_struc.hip: explicit kernel matrices of a structure (envs_struc,
+// self_kernel_struc) for the SOR / DTC variances.
+// Per-pair values from FD (in place) plus the energy and stress entries of K
+// rows (one species; rows u < Ms; K row stride n_out).
+void launch_envs_struc_epilogue(hipStream_t st, int Ms, int ns, int64_t Q,
+                                const double *DOT, double *FD, const double *pnorm,
+                                const double *rnorm2, const double *fdots,
+                                const double *coords, const int *pair_row,
+                                double sig2, double power, double vol_inv, double *K,
+                                int64_t n_out, int n_atoms);
+// K[u][1 + 3a + c] += own-pair sums - incoming-pair sums of VAL
+void launch_envs_struc_gather(hipStream_t st, int Ms, int n_atoms, int64_t Q,
+                              const double *VAL, const int *own0, const int *ownc,
+                              const int *in_cum, const int *in_pairs, double *K,
+                              int64_t n_out);
+void launch_self_g(hipStream_t st, int ns, int F, int m, int nn, const int64_t *q0,
+                   const int *cnt, const double *dB, const double *fdots,
+                   const double *coords, double *G, double *g);
+void launch_self_rows(hipStream_t st, int ns, const double *T, const double *U,
+                      const double *V, const double *g, const double *rnorm2,
+                      double sig2, double power, int energy, double *row_out);
+// entries per atom are limited to this many touching pairs
+constexpr int kSelfMaxPairs = 512;
+void launch_self_force(hipStream_t st, int n_atoms, int F, const int *row_of,
+                       const int64_t *q0, const int *cnt, const int *in_cum,
+                       const int *in_pairs, const int *pair_rowg,
+                       const int *row_species, const double *D, const double *rnorm2,
+                       const double *dB, const double *fdots, double sig2,
+                       double power, double *out, int *overflow);
+void launch_col_sqsum(hipStream_t st, int rows, int64_t cols, const double *X,
+                      double sign, int accumulate, double *out);
 // out[i][i] += v for i < F (row-major F x F)
 void launch_add_diag_rm(hipStream_t st, int F, double *out, double v);
 // Low-precision GP forms: epilogue from t (FP32, rows x M) into kval (FP64),

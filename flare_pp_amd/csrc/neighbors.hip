@@ -982,35 +982,39 @@ __global__ void iota_kernel(int64_t P, int *__restrict__ out) {
   if (k < P) out[k] = (int)k;
 }
 
-void ext_incoming(hipStream_t st, NListBuffers &b, int n_all, int64_t P,
-                  int *in_cum, int *in_pairs) {
+void index_by_target(hipStream_t st, DevBuf &work, DevBuf &cnt_buf, DevBuf &scan_tmp,
+                     const int *keys, int64_t P, int n_all, int *in_cum,
+                     int *in_pairs) {
   // counts (n_all + 1, last zero) -> in_cum; stable radix sort of the pair
-  // indices by neighbor index
-  b.bin_count.ensure((size_t)(n_all + 1) * sizeof(int));
-  int *cnt = b.bin_count.as<int>();
+  // indices by key
+  cnt_buf.ensure((size_t)(n_all + 1) * sizeof(int));
+  int *cnt = cnt_buf.as<int>();
   FLARE_HIP_CHECK(hipMemsetAsync(cnt, 0, (size_t)(n_all + 1) * sizeof(int), st));
   const size_t Pa = (size_t)std::max<int64_t>(P, 1);
-  b.keys_tmp.ensure(Pa * sizeof(int64_t));  // iota values, then sorted keys
-  int *vals = b.keys_tmp.as<int>();
+  work.ensure(Pa * 2 * sizeof(int));  // iota values, then sorted keys
+  int *vals = work.as<int>();
   int *keys_sorted = vals + Pa;
   if (P > 0) {
     const int g = (int)((P + 255) / 256);
-    hipLaunchKernelGGL(ext_hist_kernel, dim3(g), dim3(256), 0, st, P,
-                       b.idx.as<int>(), cnt);
+    hipLaunchKernelGGL(ext_hist_kernel, dim3(g), dim3(256), 0, st, P, keys, cnt);
     hipLaunchKernelGGL(iota_kernel, dim3(g), dim3(256), 0, st, P, vals);
     int bits = 1;
     while (bits < 31 && (1 << bits) < n_all) bits++;
     size_t bytes = 0;
     FLARE_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(
-        nullptr, bytes, b.idx.as<int>(), keys_sorted, vals, in_pairs, (int)P, 0, bits,
-        st));
-    b.scan_tmp.ensure(bytes);
+        nullptr, bytes, keys, keys_sorted, vals, in_pairs, (int)P, 0, bits, st));
+    scan_tmp.ensure(bytes);
     FLARE_HIP_CHECK(hipcub::DeviceRadixSort::SortPairs(
-        b.scan_tmp.ptr, bytes, b.idx.as<int>(), keys_sorted, vals, in_pairs, (int)P,
-        0, bits, st));
+        scan_tmp.ptr, bytes, keys, keys_sorted, vals, in_pairs, (int)P, 0, bits, st));
   }
-  exclusive_scan(st, b.scan_tmp, cnt, in_cum, n_all);
+  exclusive_scan(st, scan_tmp, cnt, in_cum, n_all);
   FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void ext_incoming(hipStream_t st, NListBuffers &b, int n_all, int64_t P,
+                  int *in_cum, int *in_pairs) {
+  index_by_target(st, b.keys_tmp, b.bin_count, b.scan_tmp, b.idx.as<int>(), P, n_all,
+                  in_cum, in_pairs);
 }
 
 __global__ void __launch_bounds__(256)

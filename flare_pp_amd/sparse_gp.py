@@ -212,6 +212,69 @@ class SparseGP:
         structure.local_uncertainties = [var[:structure.noa].cpu().numpy()]
         return structure.mean_efs, structure.local_uncertainties
 
+    # -- SOR / DTC (sparse_gp.cpp:630-680) --------------------------------------
+    def set_R_inv(self, R_inv):
+        """R^-1 of the training QR (sparse_gp.cpp:600-607); Sigma = R^-1 R^-T."""
+        R = np.ascontiguousarray(np.asarray(R_inv, dtype=np.float64))
+        if R.shape != (self.n_sparse, self.n_sparse):
+            raise ValueError("R_inv must be n_sparse x n_sparse")
+        self.R_inv = R
+        self._build()
+        ctx = _capi.context(self.device)
+        t = torch.as_tensor(R, device=f"cuda:{self.device}")
+        check(lib().flare_sgp_set_R_inv(ctx.handle, self._h, dptr(t)))
+
+    @property
+    def Sigma(self):
+        return self.R_inv @ self.R_inv.T
+
+    def _struc_call(self, fn, structure, rows):
+        self._build()
+        ctx = _capi.context(self.device)
+        p = structure.descriptor_calculators[0].params()
+        n_out = 1 + 3 * structure.noa + 6
+        out = torch.zeros((max(rows, 1), n_out), dtype=torch.float64,
+                          device=f"cuda:{self.device}")
+        check(fn(ctx.handle, self._h, ctypes.byref(p), structure._nlist,
+                 dptr(structure.d_species), dptr(out)))
+        return out[:rows].cpu().numpy()
+
+    def envs_struc(self, structure):
+        """Kernel matrix of the sparse set against the structure's outputs,
+        n_sparse x (1+3n+6) (normalized_dot_product.cpp:115-210)."""
+        return self._struc_call(lib().flare_sgp_envs_struc, structure, self.n_sparse)
+
+    def self_kernel_struc(self, structure):
+        """k(o, o) of the structure's outputs (normalized_dot_product.cpp:
+        446-578)."""
+        return self._struc_call(lib().flare_sgp_self_kernel_struc, structure, 1)[0]
+
+    def _predict_var(self, fn, structure):
+        self._check_ready(structure)
+        if getattr(self, "R_inv", None) is None:
+            raise RuntimeError("R_inv is not set")
+        ctx = _capi.context(self.device)
+        p = structure.descriptor_calculators[0].params()
+        n_out = 1 + 3 * structure.noa + 6
+        dev = f"cuda:{self.device}"
+        mean = torch.empty(n_out, dtype=torch.float64, device=dev)
+        var = torch.empty(n_out, dtype=torch.float64, device=dev)
+        check(fn(ctx.handle, self._h, ctypes.byref(p), structure._nlist,
+                 dptr(structure.d_species), dptr(mean), dptr(var)))
+        structure.mean_efs = mean.cpu().numpy()
+        structure.variance_efs = var.cpu().numpy()
+        return structure.mean_efs, structure.variance_efs
+
+    def predict_SOR(self, structure):
+        """mean_efs and variance_efs = diag(K^T Sigma K) (sparse_gp.cpp:
+        630-650)."""
+        return self._predict_var(lib().flare_sgp_predict_SOR, structure)
+
+    def predict_DTC(self, structure):
+        """mean_efs and variance_efs = k(o, o) - diag(K^T K_uu^-1 K) +
+        diag(K^T Sigma K) (sparse_gp.cpp:652-680)."""
+        return self._predict_var(lib().flare_sgp_predict_DTC, structure)
+
     def compute_cluster_uncertainties(self, structure):
         return self.predict_local_uncertainties(structure)[1]
 

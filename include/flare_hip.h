@@ -254,6 +254,33 @@ int flare_sgp_predict_local_uncertainties(flare_ctx *ctx, flare_sgp *gp,
                                           const int *d_species, int form,
                                           double *d_mean_efs,
                                           double *d_variances);
+/* envs_struc (normalized_dot_product.cpp:115-210): the kernel matrix of the
+ * sparse set against every output [E, F (3n), S (6)] of a structure, d_K
+ * M x (1+3n+6), row = sparse environment.  Materialises the structure's dB/dr
+ * (3 P' x F) and an M_s x 3P'_s block per species. */
+int flare_sgp_envs_struc(flare_ctx *ctx, flare_sgp *gp,
+                         const flare_b2_params *p, const flare_nlist *nl,
+                         const int *d_species, double *d_K);
+/* self_kernel_struc (normalized_dot_product.cpp:446-578): k(o, o) of every
+ * output o of a structure, d_out (1+3n+6). */
+int flare_sgp_self_kernel_struc(flare_ctx *ctx, flare_sgp *gp,
+                                const flare_b2_params *p, const flare_nlist *nl,
+                                const int *d_species, double *d_out);
+/* R^-1 of the training QR (sparse_gp.cpp:600-607), M x M row-major, device.
+ * Training is not part of this library: the caller supplies it, as alpha. */
+int flare_sgp_set_R_inv(flare_ctx *ctx, flare_sgp *gp, const double *d_R_inv);
+/* predict_SOR (sparse_gp.cpp:630-650): mean_efs = K^T alpha, variance_efs =
+ * diag(K^T R^-1 R^-T K).  predict_DTC (:652-680): variance_efs = self kernel
+ * - diag(K^T K_uu^-1 K) + diag(K^T Sigma K), Sigma = R^-1 R^-T.  Both
+ * (1+3n+6); they need alpha and R_inv. */
+int flare_sgp_predict_SOR(flare_ctx *ctx, flare_sgp *gp,
+                          const flare_b2_params *p, const flare_nlist *nl,
+                          const int *d_species, double *d_mean_efs,
+                          double *d_variance_efs);
+int flare_sgp_predict_DTC(flare_ctx *ctx, flare_sgp *gp,
+                          const flare_b2_params *p, const flare_nlist *nl,
+                          const int *d_species, double *d_mean_efs,
+                          double *d_variance_efs);
 /* envs_envs (normalized_dot_product.cpp:27-87) of two descriptor sets
  * (species-grouped, n1 x F and n2 x F) into d_out n1 x n2. */
 int flare_envs_envs(flare_ctx *ctx, int n_species, int n_descriptors,

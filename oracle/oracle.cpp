@@ -939,6 +939,101 @@ extern "C" void orc_envs_struc(const orc_b2 *struc, const int *m_by_type,
 }
 
 // ---------------------------------------------------------------------------
+// self_kernel_struc (normalized_dot_product.cpp:446-578): k(o, o) of every
+// output o = [E, F (3n), S (6)] of a structure with itself, summed over atom
+// pairs i <= j of one species (factor 2 when i != j) and their neighbor
+// pairs.  Serial restatement of the reference loops.
+
+extern "C" void orc_self_kernel_struc(const orc_b2 *struc, double sigma,
+                                      double power, double volume,
+                                      double *out) {
+  const int S = struc->S, F = struc->F, n = struc->n;
+  Grouped g;
+  grouped(struc, g, true);
+  const double sig_sq = sigma * sigma;
+  const double vol_inv = 1 / volume;
+  const double vol_inv_sq = vol_inv * vol_inv;
+  const double empty_thresh = 1e-8;
+  const int n_out = 1 + 3 * n + 6;
+  std::fill(out, out + n_out, 0.0);
+  int row0 = 0;
+  int64_t pair0 = 0;
+  for (int s = 0; s < S; s++) {
+    const int n_struc = g.ns[s];
+    const int64_t np3 = 3 * (int64_t)g.nn[s];
+    // dot_vals, force_dot (3P_s x n_s) and force_force (3P_s x 3P_s)
+    auto D = [&](int i) { return &g.d[(size_t)(row0 + i) * F]; };
+    auto dB = [&](int64_t f) { return &g.fd[(size_t)(3 * pair0 + f) * F]; };
+    std::vector<double> force_dot((size_t)np3 * n_struc);
+    for (int64_t f = 0; f < np3; f++)
+      for (int j = 0; j < n_struc; j++)
+        force_dot[(size_t)f * n_struc + j] = dotn(dB(f), D(j), F);
+    std::vector<double> par((size_t)n_struc * n_out, 0.0);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < n_struc; i++) {
+      double norm_i = g.norms[row0 + i];
+      if (norm_i < empty_thresh) continue;
+      double norm_i2 = norm_i * norm_i;
+      double norm_i3 = norm_i2 * norm_i;
+      double *pm = &par[(size_t)i * n_out];
+      for (int j = i; j < n_struc; j++) {
+        double norm_j = g.norms[row0 + j];
+        if (norm_j < empty_thresh) continue;
+        double mult_fac = i == j ? 1 : 2;
+        double norm_j2 = norm_j * norm_j;
+        double norm_j3 = norm_j2 * norm_j;
+        double norm_ij = norm_i * norm_j;
+        double norm_dot = dotn(D(i), D(j), F) / norm_ij;
+        double c1 = (power - 1) * power * std::pow(norm_dot, power - 2);
+        double c2 = power * std::pow(norm_dot, power - 1);
+        pm[0] += sig_sq * mult_fac * std::pow(norm_dot, power);
+        int n_neigh_1 = g.counts[row0 + i], c_neigh_1 = g.cumc[row0 + i];
+        int c_ind_1 = g.atoms[row0 + i];
+        int n_neigh_2 = g.counts[row0 + j], c_neigh_2 = g.cumc[row0 + j];
+        int c_ind_2 = g.atoms[row0 + j];
+        for (int k = 0; k < n_neigh_1; k++) {
+          int ind1 = c_neigh_1 + k;
+          int n_ind_1 = g.nidx[pair0 + ind1];
+          for (int l = 0; l < n_neigh_2; l++) {
+            int ind2 = c_neigh_2 + l;
+            int n_ind_2 = g.nidx[pair0 + ind2];
+            int stress_counter = 0;
+            for (int m = 0; m < 3; m++) {
+              int64_t f1 = 3 * (int64_t)ind1 + m, f2 = 3 * (int64_t)ind2 + m;
+              double sfd1 = g.fdots[3 * pair0 + f1], sfd2 = g.fdots[3 * pair0 + f2];
+              double fdj = force_dot[(size_t)f1 * n_struc + j];
+              double fdi = force_dot[(size_t)f2 * n_struc + i];
+              double v1 = fdj / norm_ij - norm_dot * sfd1 / norm_i2;
+              double v2 = fdi / norm_ij - norm_dot * sfd2 / norm_j2;
+              double v3 = dotn(dB(f1), dB(f2), F) / norm_ij;
+              double v4 = sfd1 * fdi / (norm_i3 * norm_j);
+              double v5 = sfd2 * fdj / (norm_i * norm_j3);
+              double v6 = sfd1 * sfd2 * norm_dot / (norm_i2 * norm_j2);
+              double kern_val =
+                  sig_sq * mult_fac * (c1 * v1 * v2 + c2 * (v3 - v4 - v5 + v6));
+              if (c_ind_1 == c_ind_2) pm[1 + c_ind_1 * 3 + m] += kern_val;
+              if (c_ind_1 == n_ind_2) pm[1 + c_ind_1 * 3 + m] -= kern_val;
+              if (n_ind_1 == c_ind_2) pm[1 + n_ind_1 * 3 + m] -= kern_val;
+              if (n_ind_1 == n_ind_2) pm[1 + n_ind_1 * 3 + m] += kern_val;
+              for (int nn = m; nn < 3; nn++) {
+                double coord1 = g.coords[(pair0 + ind1) * 3 + nn];
+                double coord2 = g.coords[(pair0 + ind2) * 3 + nn];
+                pm[1 + 3 * n + stress_counter] += kern_val * coord1 * coord2 * vol_inv_sq;
+                stress_counter++;
+              }
+            }
+          }
+        }
+      }
+    }
+    for (int i = 0; i < n_struc; i++)
+      for (int c = 0; c < n_out; c++) out[c] += par[(size_t)i * n_out + c];
+    row0 += n_struc;
+    pair0 += g.nn[s];
+  }
+}
+
+// ---------------------------------------------------------------------------
 // SparseGP pieces (sparse_gp.cpp:93-133, 576-628, 682-702)
 
 extern "C" int orc_cholesky_inverse(int M, const double *K, double jitter,

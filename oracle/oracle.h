@@ -114,6 +114,10 @@ void orc_envs_struc(const orc_b2 *struc, const int *m_by_type,
                     double sigma, double power, double volume,
                     double *out /* M x (1+3n+6) */);
 
+/* self_kernel_struc (normalized_dot_product.cpp:446-578): out[1+3n+6] */
+void orc_self_kernel_struc(const orc_b2 *struc, double sigma, double power,
+                           double volume, double *out);
+
 /* ---- SparseGP: sparse_gp.cpp:93-133, 576-628, 682-702 ------------------ */
 /* L = chol(K + jitter I), Linv = L^-1 (lower); returns 0 on success */
 int orc_cholesky_inverse(int M, const double *K, double jitter, double *Linv);

@@ -105,6 +105,8 @@ def lib():
                                             _i, _i, _i, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_int, ctypes.c_int, _d,
                                             ctypes.c_int, _d, _d, _d, _d, _d]),
+            "orc_self_kernel_struc": (None, [_vp, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_double, _d]),
             "orc_varmap_coefficients": (ctypes.c_int, [ctypes.c_int, ctypes.c_int,
                                                        _i, _d, _d,
                                                        ctypes.c_double, _d, _d]),
@@ -436,3 +438,12 @@ def pair_compute_lmp(x, type_, ilist, numneigh, offsets, neigh, S, N, L, power,
                                cutoff_code(cutoff), _pd(_f64(beta_full)), _pd(e),
                                _pd(f), _pd(v), _pd(ea))
     return e[0], f, v, ea
+
+
+def self_kernel_struc(b2, sigma, power, volume_):
+    """k(o, o) for o = [E, F (3n), S (6)] (normalized_dot_product.cpp:446-578);
+    b2 must carry derivatives."""
+    n = b2.n
+    out = np.zeros(1 + 3 * n + 6)
+    lib().orc_self_kernel_struc(b2.handle, sigma, power, volume_, _pd(out))
+    return out
