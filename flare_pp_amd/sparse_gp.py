@@ -74,6 +74,7 @@ class SparseGP:
         self.sparse_indices = []
         self.descriptor_calculators = []
         self.alpha = None
+        self.R_inv = None
         self._h = None
         self._dirty = True
 
@@ -219,10 +220,15 @@ class SparseGP:
         if R.shape != (self.n_sparse, self.n_sparse):
             raise ValueError("R_inv must be n_sparse x n_sparse")
         self.R_inv = R
-        self._build()
+        self._R_handle = None  # uploaded on the next SOR / DTC prediction
+
+    def _upload_R_inv(self):
+        if getattr(self, "_R_handle", None) is self._h and self._h is not None:
+            return
         ctx = _capi.context(self.device)
-        t = torch.as_tensor(R, device=f"cuda:{self.device}")
+        t = torch.as_tensor(self.R_inv, device=f"cuda:{self.device}")
         check(lib().flare_sgp_set_R_inv(ctx.handle, self._h, dptr(t)))
+        self._R_handle = self._h
 
     @property
     def Sigma(self):
@@ -253,6 +259,7 @@ class SparseGP:
         self._check_ready(structure)
         if getattr(self, "R_inv", None) is None:
             raise RuntimeError("R_inv is not set")
+        self._upload_R_inv()
         ctx = _capi.context(self.device)
         p = structure.descriptor_calculators[0].params()
         n_out = 1 + 3 * structure.noa + 6
@@ -274,6 +281,17 @@ class SparseGP:
         """mean_efs and variance_efs = k(o, o) - diag(K^T K_uu^-1 K) +
         diag(K^T Sigma K) (sparse_gp.cpp:652-680)."""
         return self._predict_var(lib().flare_sgp_predict_DTC, structure)
+
+    # -- JSON model files (sparse_gp.h:101-111) ---------------------------------
+    @staticmethod
+    def to_json(file_name, sgp):
+        from .model_json import write_sparse_gp
+        write_sparse_gp(file_name, sgp)
+
+    @staticmethod
+    def from_json(file_name, device=0):
+        from .model_json import read_sparse_gp
+        return read_sparse_gp(file_name, device)
 
     def compute_cluster_uncertainties(self, structure):
         return self.predict_local_uncertainties(structure)[1]
