@@ -527,18 +527,11 @@ static void run_gather_reduce(flare_ctx *c, const flare_nlist *nl, const double 
                               double *virial6, double *stress6,
                               const std::string &tag) {
   const int n = nl->n;
-  double *vir = c->buf_t<double>(tag + ".vir", (size_t)std::max(n, 1) * 9);
   c->mark_begin("gather_reduce");
-  flare::launch_gather(c->stream, nl->dev(), pf, forces, vir);
-  double *sums = c->buf_t<double>(tag + ".sums", 10);
-  if (n > 0) {
-    double *part = c->buf_t<double>(tag + ".part", 128 * 10);
-    flare::launch_energy_virial_sums(c->stream, n, wi.E_row, vir, part, sums);
-  } else {
-    FLARE_HIP_CHECK(hipMemsetAsync(sums, 0, 10 * sizeof(double), c->stream));
-  }
-  flare::launch_finish(c->stream, sums, 1.0 / nl->wh.volume, energy, virial6,
-                       stress6);
+  const int nparts = flare::gather_parts(n);
+  double *part = c->buf_t<double>(tag + ".part", (size_t)std::max(nparts, 1) * 10);
+  flare::launch_gather(c->stream, nl->dev(), pf, forces, nl->b.row_of.as<int>(),
+                       wi.E_row, part, 1.0 / nl->wh.volume, energy, virial6, stress6);
   c->mark_end();
 }
 

@@ -584,7 +584,9 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
 __global__ void __launch_bounds__(256)
 gather_kernel(int n, const int *__restrict__ cum, const int *__restrict__ rev,
               const double *__restrict__ rel, const double *__restrict__ pf,
-              double *__restrict__ forces, double *__restrict__ vir) {
+              double *__restrict__ forces, const int *__restrict__ row_of,
+              const double *__restrict__ e_row, double *__restrict__ part) {
+  __shared__ double sh[16][10];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int a = t >> 4, sub = t & 15;
   double f0 = 0, f1 = 0, f2 = 0;
@@ -637,52 +639,76 @@ gather_kernel(int n, const int *__restrict__ cum, const int *__restrict__ rev,
 #pragma unroll
     for (int q = 0; q < 9; q++) v[q] += __shfl_xor(v[q], off, 16);
   }
-  if (a < n && sub == 0) {
-    if (forces) {
-      forces[3 * (int64_t)a] = f0;
-      forces[3 * (int64_t)a + 1] = f1;
-      forces[3 * (int64_t)a + 2] = f2;
+  if (a < n && sub == 0 && forces) {
+    forces[3 * (int64_t)a] = f0;
+    forces[3 * (int64_t)a + 1] = f1;
+    forces[3 * (int64_t)a + 2] = f2;
+  }
+  // per-block sums of the atoms' energies and virial rows, fixed order
+  if (sub == 0) {
+    const int la = threadIdx.x >> 4;
+    sh[la][0] = a < n ? e_row[row_of[a]] : 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) sh[la][1 + q] = a < n ? v[q] : 0.0;
+  }
+  __syncthreads();
+  if (threadIdx.x < 10) {
+    double s = 0;
+    for (int la = 0; la < 16; la++) s += sh[la][threadIdx.x];
+    part[blockIdx.x * 10 + threadIdx.x] = s;
+  }
+}
+
+// Column q of the gather partials summed by workgroup q (fixed thread
+// partition and tree, so the result is deterministic), written straight to
+// its output slots: q = 0 the energy, q = 1 + 3a + b the virial component
+// V[a][b] = -sum r_a f_b (LAMMPS order xx yy zz xy xz yz) and the GP stress
+// (xx xy xz yy yz zz of -(1/V) sum f_c r_c2, c2 >= c,
+// normalized_dot_product.cpp:193-202).
+__global__ void __launch_bounds__(256)
+sums_cols_kernel(int nparts, const double *__restrict__ part, double inv_volume,
+                 double *energy, double *virial6, double *stress6) {
+  __shared__ double sh[256];
+  const int q = blockIdx.x, t = threadIdx.x;
+  double s = 0;
+  constexpr int kU = 8;
+  for (int b0 = t; b0 < nparts; b0 += 256 * kU) {
+    double x[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int b = b0 + 256 * u;
+      x[u] = b < nparts ? part[(int64_t)b * 10 + q] : 0.0;
     }
 #pragma unroll
-    for (int q = 0; q < 9; q++) vir[9 * (int64_t)a + q] = v[q];
+    for (int u = 0; u < kU; u++) s += x[u];
   }
-}
-
-// Deterministic two-stage sum of the per-row energies and per-atom virial
-// rows: stage 1 gives kPartials partial sums over fixed contiguous ranges.
-constexpr int kPartials = 128;
-
-__global__ void __launch_bounds__(256)
-partial_sums_kernel(int n, const double *__restrict__ e_row,
-                    const double *__restrict__ vir, double *__restrict__ part) {
-  __shared__ double sh[256];
-  const int t = threadIdx.x;
-  const int per = (n + gridDim.x - 1) / gridDim.x;
-  const int lo = blockIdx.x * per, hi = min(n, lo + per);
-  for (int q = 0; q < 10; q++) {
-    double s = 0;
-    for (int r = lo + t; r < hi; r += blockDim.x)
-      s += q == 0 ? e_row[r] : vir[(int64_t)r * 9 + (q - 1)];
-    sh[t] = s;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-      if (t < off) sh[t] += sh[t + off];
-      __syncthreads();
-    }
-    if (t == 0) part[blockIdx.x * 10 + q] = sh[0];
+  sh[t] = s;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (t < off) sh[t] += sh[t + off];
     __syncthreads();
   }
-}
-
-// one wavefront per column, fixed lane partition
-__global__ void __launch_bounds__(640)
-partial_final_kernel(int nparts, const double *__restrict__ part,
-                     double *__restrict__ sums) {
-  const int q = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  double s = 0;
-  for (int b = lane; b < nparts; b += 64) s += part[b * 10 + q];
-  s = wave_sum(s);
-  if (lane == 0) sums[q] = s;
+  if (t != 0) return;
+  const double v = sh[0];
+  if (q == 0) {
+    if (energy) energy[0] = v;
+    return;
+  }
+  // virial slot of V[ab] (LAMMPS order) and stress slot (c2 >= c only)
+  int vs = -1, ss = -1;
+  switch (q - 1) {
+    case 0: vs = 0; ss = 0; break;
+    case 1: vs = 3; break;
+    case 2: vs = 4; break;
+    case 3: ss = 1; break;
+    case 4: vs = 1; ss = 3; break;
+    case 5: vs = 5; break;
+    case 6: ss = 2; break;
+    case 7: ss = 4; break;
+    case 8: vs = 2; ss = 5; break;
+  }
+  if (virial6 && vs >= 0) virial6[vs] = v;
+  if (stress6 && ss >= 0) stress6[ss] = v * inv_volume;
 }
 
 // one workgroup per column; fixed partition => deterministic
@@ -703,31 +729,6 @@ colsum_kernel(int n, int k, const double *__restrict__ x,
   if (t == 0) out[col] = part[0];
 }
 
-__global__ void finish_kernel(const double *__restrict__ sums,
-                              double inv_volume, double *energy,
-                              double *virial6, double *stress6) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const double *V = sums + 1;  // V[a*3+b] = -sum r_a f_b
-  if (energy) energy[0] = sums[0];
-  if (virial6) {
-    virial6[0] = V[0];
-    virial6[1] = V[4];
-    virial6[2] = V[8];
-    virial6[3] = V[1];
-    virial6[4] = V[2];
-    virial6[5] = V[5];
-  }
-  if (stress6) {
-    // (xx, xy, xz, yy, yz, zz) of -(1/V) sum f_c r_c2, c2 >= c
-    // (normalized_dot_product.cpp:193-202)
-    stress6[0] = V[0] * inv_volume;
-    stress6[1] = V[3] * inv_volume;
-    stress6[2] = V[6] * inv_volume;
-    stress6[3] = V[4] * inv_volume;
-    stress6[4] = V[7] * inv_volume;
-    stress6[5] = V[8] * inv_volume;
-  }
-}
 
 // ---------------------------------------------------------------------------
 template <typename T, int MODE, int WM>
@@ -803,20 +804,28 @@ void launch_forces(hipStream_t st, int prec, int mode, const B2Dev &p,
   FLARE_HIP_CHECK(hipGetLastError());
 }
 
-void launch_gather(hipStream_t st, const NListDev &nl, const double *pair_f,
-                   double *forces, double *vir_atom) {
-  if (nl.n == 0) return;
-  hipLaunchKernelGGL(gather_kernel, dim3((nl.n * 16 + 255) / 256), dim3(256), 0,
-                     st, nl.n, nl.cum, nl.rev, nl.rel, pair_f, forces, vir_atom);
-  FLARE_HIP_CHECK(hipGetLastError());
+int gather_parts(int n) { return (n * 16 + 255) / 256; }
+
+__global__ void zero_results_kernel(double *energy, double *virial6, double *stress6) {
+  const int t = threadIdx.x;
+  if (energy && t == 0) energy[0] = 0.0;
+  if (virial6 && t < 6) virial6[t] = 0.0;
+  if (stress6 && t < 6) stress6[t] = 0.0;
 }
 
-void launch_energy_virial_sums(hipStream_t st, int n, const double *e_row,
-                               const double *vir, double *part, double *sums) {
-  hipLaunchKernelGGL(partial_sums_kernel, dim3(kPartials), dim3(256), 0, st, n,
-                     e_row, vir, part);
-  hipLaunchKernelGGL(partial_final_kernel, dim3(1), dim3(640), 0, st, kPartials,
-                     part, sums);
+void launch_gather(hipStream_t st, const NListDev &nl, const double *pair_f,
+                   double *forces, const int *row_of, const double *e_row,
+                   double *part, double inv_volume, double *energy, double *virial6,
+                   double *stress6) {
+  if (nl.n == 0) {
+    hipLaunchKernelGGL(zero_results_kernel, dim3(1), dim3(64), 0, st, energy, virial6,
+                       stress6);
+  } else {
+    hipLaunchKernelGGL(gather_kernel, dim3(gather_parts(nl.n)), dim3(256), 0, st, nl.n,
+                       nl.cum, nl.rev, nl.rel, pair_f, forces, row_of, e_row, part);
+    hipLaunchKernelGGL(sums_cols_kernel, dim3(10), dim3(256), 0, st,
+                       gather_parts(nl.n), part, inv_volume, energy, virial6, stress6);
+  }
   FLARE_HIP_CHECK(hipGetLastError());
 }
 
@@ -825,11 +834,5 @@ void launch_colsum(hipStream_t st, int n, int k, const double *x, double *out) {
   FLARE_HIP_CHECK(hipGetLastError());
 }
 
-void launch_finish(hipStream_t st, const double *sums, double inv_volume,
-                   double *energy, double *virial6, double *stress6) {
-  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(64), 0, st, sums, inv_volume,
-                     energy, virial6, stress6);
-  FLARE_HIP_CHECK(hipGetLastError());
-}
 
 }  // namespace flare

@@ -158,20 +158,20 @@ void launch_forces(hipStream_t st, int prec, int mode, const B2Dev &p,
                    const double *ynorm, const float *ynorm_f, const void *A,
                    const WInput &wi, const double *dA, int64_t dA_first_pair,
                    int a_begin, int a_end, double *pair_f);
-// F_a = sum_k (f_ak - f_rev(ak)) into forces (stride 3; may be NULL) and
-// per-atom virial rows (9 values, V[a][b] = -sum_k r_a f_b)
+// F_a = sum_k (f_ak - f_rev(ak)) into forces (stride 3; may be NULL; pairs
+// without a reverse contribute only their own term), and the energy (sum of
+// e_row[row_of[a]]), virial (xx yy zz xy xz yz of V[a][b] = -sum_k r_a f_b)
+// and GP stress (xx xy xz yy yz zz, times inv_volume); any of the three may
+// be NULL.  Per workgroup of 16 atoms a partial row in part (gather_parts(n)
+// x 10), then one workgroup per column sums them in a fixed order, so the
+// result is deterministic.
+int gather_parts(int n);
 void launch_gather(hipStream_t st, const NListDev &nl, const double *pair_f,
-                   double *forces, double *vir_atom);
-// sums[0] = sum e_row, sums[1..9] = column sums of vir (n x 9); part holds
-// 128 x 10 partial sums; deterministic
-void launch_energy_virial_sums(hipStream_t st, int n, const double *e_row,
-                               const double *vir, double *part, double *sums);
+                   double *forces, const int *row_of, const double *e_row,
+                   double *part, double inv_volume, double *energy, double *virial6,
+                   double *stress6);
 // column sums of an (n x k) row-major matrix into out[k], deterministic
 void launch_colsum(hipStream_t st, int n, int k, const double *x, double *out);
-// final assembly of energy, virial (LAMMPS order) and GP stress
-void launch_finish(hipStream_t st, const double *sums /* e, V9 */,
-                   double inv_volume, double *energy, double *virial6,
-                   double *stress6);
 
 // gp.hip
 // GP kernel epilogue on raw dot products K = X P^T (rows x M), per row:
