@@ -57,7 +57,13 @@ struct GemmBatch {
 };
 typedef __attribute__((ext_vector_type(4))) float floatx4;
 
-constexpr int kBM = 128, kBN = 256, kBK = 32, kLdk = kBK + 8;
+constexpr int kBM = 128, kBN = 256, kBK = 32, kLdk = kBK;
+// LDS rows of 64 bytes (four 16-byte chunks, no padding); chunk c of row r
+// is stored at chunk c ^ ((r >> 2) & 3).  Sixteen consecutive lanes then
+// touch 16 distinct 16-byte bank groups both when a tile is stored (four
+// rows written whole) and when MFMA operands are read (sixteen rows, one
+// chunk each).
+__device__ inline int swz(int r, int chunk) { return (chunk ^ ((r >> 2) & 3)) * 8; }
 constexpr int kChunksA = kBM * kBK / 8 / 256;  // 16-byte chunks per thread
 constexpr int kChunksB = kBN * kBK / 8 / 256;
 
@@ -111,15 +117,15 @@ __device__ inline void store_tile(const Tile &t, E (*sAh)[kLdk],
   for (int h = 0; h < kChunksA; h++) {
     const int c = tid + 256 * h;
     const int r = c >> 2, kc = (c & 3) * 8;
-    *reinterpret_cast<uint4 *>(&sAh[r][kc]) = t.a[h];
-    *reinterpret_cast<uint4 *>(&sAl[r][kc]) = t.a[kChunksA + h];
+    *reinterpret_cast<uint4 *>(&sAh[r][swz(r, kc >> 3)]) = t.a[h];
+    *reinterpret_cast<uint4 *>(&sAl[r][swz(r, kc >> 3)]) = t.a[kChunksA + h];
   }
 #pragma unroll
   for (int h = 0; h < kChunksB; h++) {
     const int c = tid + 256 * h;
     const int r = c >> 2, kc = (c & 3) * 8;
-    *reinterpret_cast<uint4 *>(&sBh[r][kc]) = t.b[h];
-    *reinterpret_cast<uint4 *>(&sBl[r][kc]) = t.b[kChunksB + h];
+    *reinterpret_cast<uint4 *>(&sBh[r][swz(r, kc >> 3)]) = t.b[h];
+    *reinterpret_cast<uint4 *>(&sBl[r][swz(r, kc >> 3)]) = t.b[kChunksB + h];
   }
 }
 
@@ -176,7 +182,8 @@ gemm_x3_kernel(GemmBatch bt, const E *__restrict__ Ah0,
 #pragma unroll
     for (int j = 0; j < 8; j++) acc[i][j] = floatx4{0, 0, 0, 0};
   const int fr = lane & 15;
-  const int fk = (lane >> 4) * 8;
+  // operand rows are wm + 16 i + fr (and wn + 16 j + fr): swizzle by fr
+  const int fk = swz(fr, lane >> 4);
   const int steps = Kp / kBK;
   Tile t;
   load_tile(t, Ah, Al, Bh, Bl, M, N, Kp, m0, n0, 0);
