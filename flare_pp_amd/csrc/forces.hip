@@ -163,6 +163,62 @@ __device__ inline void u_mfma(const B2Dev &p, const T *A, const T *w,
   }
 }
 
+// u_mfma for l_max = L, n_max = 8 and NR (8 or 16) radial channels: the
+// adjoint forms' fixed settings with one or two species.  Harmonic
+// normalisation applied, u2 layout as u_mfma's.  Each lane's matrix operand
+// W_l[c][c2] for its (c, c2 = 4 ks + g) and every l is read once into
+// registers (the L + 1 values of a channel pair are adjacent in w), so the
+// MFMA loop only loads the A operand, and all bounds are compile-time.
+template <typename T, int L, int NR>
+__device__ inline void u_mfma_fixed(const T *A, const T *w, const T *nrm_lm, T *u2) {
+  static_assert(NR == 8 || NR == 16, "one M tile of channels");
+  typedef typename MmaF<T>::V V;
+  constexpr int L1 = L + 1, H = L1 * L1, N = 8, KS = NR / 4;
+  const int lane = threadIdx.x;
+  const int col = lane & 15, g = lane >> 4;
+  const int c = col;
+  const bool crow = NR == 16 || c < NR;
+  T wv[KS][L1];
+#pragma unroll
+  for (int ks = 0; ks < KS; ks++) {
+    const int c2 = 4 * ks + g;
+    const int off = crow ? b2_pair_index(c, c2, NR) * L1 : 0;
+    const T fac = c == c2 ? T(2) : T(1);
+#pragma unroll
+    for (int l = 0; l < L1; l++) wv[ks][l] = crow ? fac * w[off + l] : T(0);
+  }
+  // this lane's four result rows cr = row(g, q) < NR: channel (s, n)
+  int dst[4];
+  bool drow[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int cr = MmaF<T>::row(g, q);
+    drow[q] = NR == 16 || cr < NR;
+    dst[q] = (cr >> 3) * H * N + (cr & 7);
+  }
+#pragma unroll
+  for (int l = 0; l < L1; l++) {
+    constexpr int kMaxJ = 16;
+    const int nl = 2 * l + 1, l2 = l * l;
+    static_assert(2 * L + 1 <= kMaxJ, "one N tile of harmonics");
+    const int j = col;
+    const bool jok = j < nl;
+    V acc = V{0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < KS; ks++) {
+      const int c2 = 4 * ks + g;
+      const T b = jok ? A[c2 * H + l2 + j] : T(0);
+      acc = MmaF<T>::run(wv[ks][l], b, acc);
+    }
+    if (jok) {
+      const T sc = nrm_lm[l2 + j];
+#pragma unroll
+      for (int q = 0; q < 4; q++)
+        if (drow[q]) u2[dst[q] + (l2 + j) * N] = acc[q] * sc;
+    }
+  }
+}
+
 namespace {
 
 // Adjoint contraction for one pair, everything recomputed in registers.
@@ -327,8 +383,13 @@ __device__ inline void adjoint_pair_fixed(const B2Dev &p, T r, T x, T y, T z,
 //   2  mapped power 1 from X and beta_s: E = X.beta/|X|,
 //      w = (beta - E B/|B|) / |B|    (lammps_descriptor.cpp:262-270).
 // Environments with |B|^2 < 1e-8 carry no energy (pair_flare.cpp:135).
+// FP32 instances up to l_max 4 are held to 128 registers (four wavefronts
+// per SIMD): the latency of the prologue's loads is then hidden by more
+// resident atoms, which outweighs the few spilled values (88.7 vs 93.5 us
+// per launch at 32,000 atoms, B2 {2,8,4}).  l_max 6 would spill ~50.
 template <typename T, int MODE, int NMAX, int WM, int LFIX, bool EXACT>
 __global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && LFIX <= 4 ? 4 : 1)))
 force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
              const int *__restrict__ row_of, const T *__restrict__ ynorm,
              const T *__restrict__ A_in, WSource<T> src,
@@ -356,31 +417,19 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
   }
   // Every global read of the prologue is issued in batches before its
   // values are stored, so that a wavefront waits for memory a few times
-  // rather than once per element.
-  if (kFixed && lane < H) {
-    int l = 0;
-    while ((l + 1) * (l + 1) <= lane) l++;
-    const int m = lane - l * l - l;
-    nrm_lm[lane] = ynorm[l * (l + 1) / 2 + (m < 0 ? -m : m)];
-  }
-  for (int q = lane; q < p.nr * p.nr; q += kWave) {
-    const int c = q / p.nr, c2 = q - c * p.nr;
-    ptab[q] = b2_pair_index(c, c2, p.nr) * (p.L + 1);
-  }
-  stage_lds_vec<4>(W, Ai, A);
-  if (WM == 0) {
-    const T *wi = src.w + (int64_t)row * F;
-    stage_lds_vec<4>(F, wi, w);
-  } else if (WM == 3 && (F & 3) == 0 && F <= 16 * kWave) {
-    // mixed form, rows of up to 1,024 features: each lane keeps its 4 x 4
-    // features of x (= hi + lo) and z in registers through E and w
-    const double n2 = src.norm2[row];
+  // rather than once per element.  Waits on loads are in issue order, so in
+  // the mixed form the x / z rows and the harmonic normalisation are issued
+  // ahead of the A row and are in registers when its stores wait.
+  const bool mixed_fast = WM == 3 && (F & 3) == 0 && F <= 16 * kWave;
+  uint2 th[4], tl[4];
+  float4 zv[4];
+  double n2_row = 0;
+  if (WM == 3 && mixed_fast) {
+    n2_row = src.norm2[row];
     const uint2 *xh = reinterpret_cast<const uint2 *>(src.xh + (int64_t)row * src.kp);
     const uint2 *xl = reinterpret_cast<const uint2 *>(src.xl + (int64_t)row * src.kp);
     const float4 *z4 = reinterpret_cast<const float4 *>(src.Z + (int64_t)row * F);
     const int F4 = F >> 2;
-    uint2 th[4], tl[4];
-    float4 zv[4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const int q = lane + j * kWave;
@@ -393,6 +442,55 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
         zv[j] = z4[q];
       }
     }
+  }
+  T ynv = 0;
+  if (kFixed && lane < H) {
+    int l = 0;
+    while ((l + 1) * (l + 1) <= lane) l++;
+    const int m = lane - l * l - l;
+    ynv = ynorm[l * (l + 1) / 2 + (m < 0 ? -m : m)];
+  }
+  // the A row in one unrolled batch of up to four 16-byte loads per lane
+  const int nva = W * (int)sizeof(T) / 16;
+  const bool a_batch = kFixed && ((W * (int)sizeof(T)) & 15) == 0 && nva <= 4 * kWave &&
+                       ((reinterpret_cast<uintptr_t>(Ai) | reinterpret_cast<uintptr_t>(A)) & 15) == 0;
+  uint4 ta[4];
+  if (kFixed && a_batch) {
+    const uint4 *s4 = reinterpret_cast<const uint4 *>(Ai);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int o = lane + j * kWave;
+      ta[j] = uint4{0, 0, 0, 0};
+      if (o < nva) ta[j] = s4[o];
+    }
+  }
+  // nr <= 32 here (checked at launch); the fixed-size path with 8 or 16
+  // channels computes its channel-pair offsets in registers
+  constexpr bool kUFixed = kFixed;
+  if (!kUFixed || (p.nr != 8 && p.nr != 16))
+    for (int q = lane; q < p.nr * p.nr; q += kWave) {
+      const int c = q / p.nr, c2 = q - c * p.nr;
+      ptab[q] = b2_pair_index(c, c2, p.nr) * (p.L + 1);
+    }
+  if (kFixed && a_batch) {
+    uint4 *d4 = reinterpret_cast<uint4 *>(A);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int o = lane + j * kWave;
+      if (o < nva) d4[o] = ta[j];
+    }
+  } else {
+    stage_lds_vec<4>(W, Ai, A);
+  }
+  if (kFixed && lane < H) nrm_lm[lane] = ynv;
+  if (WM == 0) {
+    const T *wi = src.w + (int64_t)row * F;
+    stage_lds_vec<4>(F, wi, w);
+  } else if (WM == 3 && mixed_fast) {
+    // mixed form, rows of up to 1,024 features: each lane keeps its 4 x 4
+    // features of x (= hi + lo) and z in registers through E and w
+    const double n2 = n2_row;
+    const int F4 = F >> 2;
     float xv[4][4];
 #pragma unroll
     for (int j = 0; j < 4; j++) {
@@ -491,7 +589,17 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
     }
   }
   __syncthreads();
-  u_mfma<T, LFIX, kFixed, kFixed ? 8 : -1, MODE == 0>(p, A, w, nrm_lm, ptab, u);
+  bool done = false;
+  if constexpr (kUFixed) {
+    if (p.nr == 16) {
+      u_mfma_fixed<T, LFIX, 16>(A, w, nrm_lm, u);
+      done = true;
+    } else if (p.nr == 8) {
+      u_mfma_fixed<T, LFIX, 8>(A, w, nrm_lm, u);
+      done = true;
+    }
+  }
+  if (!done) u_mfma<T, LFIX, kFixed, kFixed ? 8 : -1, MODE == 0>(p, A, w, nrm_lm, ptab, u);
   __syncthreads();
   const int si = species[i];
   if (MODE == 0) {
