@@ -13,6 +13,8 @@
 // axes, all) neighboring bins cooperatively.
 #include <hipcub/hipcub.hpp>
 
+#include <cmath>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -339,25 +341,20 @@ constexpr int kNbrCap = 128;
 // (stat[0] becomes the largest such count; it stays 0 otherwise).
 // THIN: some axis has fewer than three bins (general combination lists);
 // otherwise the 27 neighboring bins with fixed per-axis lists of three.
+// One wavefront's pass for atom i; prefix (kWave ints), skey and srel
+// (kNbrCap entries) are the wavefront's LDS scratch.
 template <bool THIN>
-__global__ void __launch_bounds__(256)
-nbr_enum_sort_kernel(int n, WrapMats m, const double *__restrict__ wrapped,
-                     const int *__restrict__ bin_of,
-                     const int *__restrict__ bin_start,
-                     const int *__restrict__ bin_atoms,
-                     const double *__restrict__ bin_pos, int *__restrict__ count,
-                     int64_t *__restrict__ keys_pad,
-                     double *__restrict__ rel_pad, int *__restrict__ stat) {
-  __shared__ int prefix_sh[4][kWave];
-  __shared__ int64_t skey_sh[4][kNbrCap];
-  __shared__ double srel_sh[4][kNbrCap][4];
+__device__ inline void enum_sort_atom(int i, const WrapMats &m,
+                                      const double *__restrict__ wrapped,
+                                      const int *__restrict__ bin_of,
+                                      const int *__restrict__ bin_start,
+                                      const int *__restrict__ bin_atoms,
+                                      const double *__restrict__ bin_pos,
+                                      int *__restrict__ count,
+                                      int64_t *__restrict__ keys_pad,
+                                      double *__restrict__ rel_pad, int *__restrict__ stat,
+                                      int *prefix, int64_t *skey, double (*srel)[4]) {
   const int lane = threadIdx.x & (kWave - 1);
-  const int wv = threadIdx.x / kWave;
-  const int i = blockIdx.x * 4 + wv;
-  if (i >= n) return;  // whole wave exits together
-  int *prefix = prefix_sh[wv];
-  int64_t *skey = skey_sh[wv];
-  double(*srel)[4] = srel_sh[wv];
   const int sw = m.sweep;
   const int hb = bin_of[i];
   const int hx = hb / (m.nb[1] * m.nb[2]);
@@ -465,6 +462,234 @@ nbr_enum_sort_kernel(int n, WrapMats m, const double *__restrict__ wrapped,
   if (lane + kWave < found) {
     keys_pad[row + r1] = k1;
     for (int c = 0; c < 4; c++) rel_pad[4 * (row + r1) + c] = srel[lane + kWave][c];
+  }
+}
+
+template <bool THIN>
+__global__ void __launch_bounds__(256)
+nbr_enum_sort_kernel(int n, WrapMats m, const double *__restrict__ wrapped,
+                     const int *__restrict__ bin_of,
+                     const int *__restrict__ bin_start,
+                     const int *__restrict__ bin_atoms,
+                     const double *__restrict__ bin_pos, int *__restrict__ count,
+                     int64_t *__restrict__ keys_pad,
+                     double *__restrict__ rel_pad, int *__restrict__ stat) {
+  __shared__ int prefix_sh[4][kWave];
+  __shared__ int64_t skey_sh[4][kNbrCap];
+  __shared__ double srel_sh[4][kNbrCap][4];
+  const int wv = threadIdx.x / kWave;
+  const int i = blockIdx.x * 4 + wv;
+  if (i >= n) return;  // whole wave exits together
+  enum_sort_atom<THIN>(i, m, wrapped, bin_of, bin_start, bin_atoms, bin_pos, count,
+                       keys_pad, rel_pad, stat, prefix_sh[wv], skey_sh[wv], srel_sh[wv]);
+}
+
+// Candidates of one home bin's 27 neighboring bins staged in LDS: FP32
+// shifted position and the combination (bin, image shift) index.
+constexpr int kBinCap = 512;
+struct __align__(16) BinCand {
+  float x, y, z;
+  int k;
+};
+
+// The same rows as nbr_enum_sort_kernel<false> (at least three bins along
+// every axis), one workgroup per home bin.  The candidates of the 27
+// neighboring bins (one image shift each) are staged in LDS once for all
+// of the bin's atoms, with FP32 copies of their shifted positions.  Each
+// atom's wavefront keeps the candidates within cut_f of an FP32 distance
+// test, then runs the exact FP64 test of structure.cpp:68-77 on those only.
+// cut_f exceeds the cutoff by more than the FP32 rounding of the test
+// (positions rounded to FP32, differences and squares in FP32), so no
+// neighbor is lost; the kept set, its rel vectors and its order are those of
+// the exact test alone.  Rows are rank-sorted by a 32-bit key (j << 6 | the
+// three shifts + 1 in two bits each, the (j, s1, s2, s3) order) when
+// key32 is set (n <= 2^26), else by the 64-bit key.  A home bin with more
+// than kBinCap candidates falls back to enum_sort_atom for its atoms.
+__global__ void __launch_bounds__(256)
+nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
+                         const double *__restrict__ wrapped,
+                         const int *__restrict__ bin_of,
+                         const int *__restrict__ bin_start,
+                         const int *__restrict__ bin_atoms,
+                         const double *__restrict__ bin_pos, int *__restrict__ count,
+                         int64_t *__restrict__ keys_pad, double *__restrict__ rel_pad,
+                         int *__restrict__ stat) {
+  __shared__ int c_pre[28], c_start[27], c_shift[27];
+  __shared__ int cand_slot[kBinCap];
+  __shared__ BinCand cand_pf[kBinCap];
+  __shared__ unsigned short queue_sh[4][kBinCap];
+  __shared__ unsigned skey32_sh[4][kNbrCap];
+  __shared__ int prefix_sh[4][kWave];
+  __shared__ int64_t skey_sh[4][kNbrCap];
+  __shared__ double srel_sh[4][kNbrCap][4];
+  const int b = blockIdx.x;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
+  const int a0 = bin_start[b], na = bin_start[b + 1] - a0;
+  if (na == 0) return;  // the whole workgroup
+  const int nb1 = m.nb[1], nb2 = m.nb[2];
+  if (wv == 0) {
+    const int hx = b / (nb1 * nb2), hy = (b / nb2) % nb1, hz = b % nb2;
+    int cs = 0, st = 0, sh = 0;
+    if (lane < 27) {
+      int bx, by, bz, s1, s2, s3;
+      axis_entry(m.nb[0], hx, m.sweep, lane / 9, bx, s1);
+      axis_entry(nb1, hy, m.sweep, (lane / 3) % 3, by, s2);
+      axis_entry(nb2, hz, m.sweep, lane % 3, bz, s3);
+      const int cb = (bx * nb1 + by) * nb2 + bz;
+      st = bin_start[cb];
+      cs = bin_start[cb + 1] - st;
+      sh = ((s1 + 1) << 4) | ((s2 + 1) << 2) | (s3 + 1);
+    }
+    int incl = cs;
+#pragma unroll
+    for (int off = 1; off < 32; off <<= 1) {
+      const int v = __shfl_up(incl, off, kWave);
+      if (lane >= off) incl += v;
+    }
+    if (lane < 27) {
+      c_pre[lane + 1] = incl;
+      c_start[lane] = st;
+      c_shift[lane] = sh;
+    }
+    if (lane == 0) c_pre[0] = 0;
+  }
+  __syncthreads();
+  const int total = c_pre[27];
+  if (total > kBinCap) {
+    for (int a = wv; a < na; a += 4)
+      enum_sort_atom<false>(bin_atoms[a0 + a], m, wrapped, bin_of, bin_start, bin_atoms,
+                            bin_pos, count, keys_pad, rel_pad, stat, prefix_sh[wv],
+                            skey_sh[wv], srel_sh[wv]);
+    return;
+  }
+  for (int t = tid; t < total; t += 256) {
+    // combo k: the last with c_pre[k] <= t
+    int lo = 0, hi = 26;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (c_pre[mid] <= t)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const int slot = c_start[lo] + (t - c_pre[lo]);
+    const int sh = c_shift[lo];
+    const double s1 = (sh >> 4) - 1, s2 = ((sh >> 2) & 3) - 1, s3 = (sh & 3) - 1;
+    BinCand pf;
+    pf.x = (float)(bin_pos[3 * slot] + s1 * m.cell[0] + s2 * m.cell[3] + s3 * m.cell[6]);
+    pf.y = (float)(bin_pos[3 * slot + 1] + s1 * m.cell[1] + s2 * m.cell[4] + s3 * m.cell[7]);
+    pf.z = (float)(bin_pos[3 * slot + 2] + s1 * m.cell[2] + s2 * m.cell[5] + s3 * m.cell[8]);
+    pf.k = lo;
+    cand_slot[t] = slot;
+    cand_pf[t] = pf;
+  }
+  __syncthreads();
+  unsigned short *queue = queue_sh[wv];
+  unsigned *skey32 = skey32_sh[wv];
+  int64_t *skey = skey_sh[wv];
+  double(*srel)[4] = srel_sh[wv];
+  const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
+  for (int a = wv; a < na; a += 4) {
+    const int hs = a0 + a;
+    const int i = bin_atoms[hs];
+    const double wi[3] = {bin_pos[3 * hs], bin_pos[3 * hs + 1], bin_pos[3 * hs + 2]};
+    const float wx = (float)wi[0], wy = (float)wi[1], wz = (float)wi[2];
+    int nq = 0;
+    for (int t0 = 0; t0 < total; t0 += kWave) {
+      const int t = t0 + lane;
+      bool keep = false;
+      if (t < total) {
+        const BinCand c = cand_pf[t];
+        const float dx = c.x - wx, dy = c.y - wy, dz = c.z - wz;
+        keep = dx * dx + dy * dy + dz * dz < cut_f2;
+      }
+      const uint64_t mk = __ballot(keep);
+      if (keep) queue[nq + __popcll(mk & lt_mask)] = (unsigned short)t;
+      nq += __popcll(mk);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    int found = 0;
+    for (int q0 = 0; q0 < nq; q0 += kWave) {
+      const int q = q0 + lane;
+      bool pass = false;
+      double o[4];
+      int64_t key = 0;
+      unsigned k32 = 0;
+      if (q < nq) {
+        const int t = queue[q];
+        const int slot = cand_slot[t];
+        const int sh = c_shift[cand_pf[t].k];
+        const int s1 = (sh >> 4) - 1, s2 = ((sh >> 2) & 3) - 1, s3 = (sh & 3) - 1;
+        const int j = bin_atoms[slot];
+        const double wj[3] = {bin_pos[3 * slot], bin_pos[3 * slot + 1], bin_pos[3 * slot + 2]};
+        pass = image_test(wi, wj, m.cell, s1, s2, s3, m.cutoff, o);
+        key = pair_key(j, s1, s2, s3, m.sweep);
+        k32 = ((unsigned)j << 6) | (unsigned)sh;
+      }
+      const uint64_t mk = __ballot(pass);
+      if (pass) {
+        const int sl = found + __popcll(mk & lt_mask);
+        if (sl < kNbrCap) {
+          skey[sl] = key;
+          skey32[sl] = k32;
+          srel[sl][0] = o[0];
+          srel[sl][1] = o[1];
+          srel[sl][2] = o[2];
+          srel[sl][3] = o[3];
+        }
+      }
+      found += __popcll(mk);
+    }
+    if (lane == 0) {
+      count[i] = found;
+      if (found > kNbrCap) atomicMax(stat, found);  // rare: no contention
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (found <= kNbrCap) {
+      // rank = number of smaller keys (keys are distinct), LDS broadcasts
+      int r0 = 0, r1 = 0;
+      if (key32) {
+        const unsigned k0 = lane < found ? skey32[lane] : 0xffffffffu;
+        const unsigned k1 = lane + kWave < found ? skey32[lane + kWave] : 0xffffffffu;
+        if (found <= kWave) {
+          for (int q = 0; q < found; q++) r0 += skey32[q] < k0;
+        } else {
+          for (int q = 0; q < found; q++) {
+            const unsigned kq = skey32[q];
+            r0 += kq < k0;
+            r1 += kq < k1;
+          }
+        }
+      } else {
+        const int64_t k0 = lane < found ? skey[lane] : INT64_MAX;
+        const int64_t k1 = lane + kWave < found ? skey[lane + kWave] : INT64_MAX;
+        for (int q = 0; q < found; q++) {
+          const int64_t kq = skey[q];
+          r0 += kq < k0;
+          r1 += kq < k1;
+        }
+      }
+      const int64_t row = (int64_t)i * kNbrCap;
+      if (lane < found) {
+        keys_pad[row + r0] = skey[lane];
+        reinterpret_cast<double4 *>(rel_pad)[row + r0] =
+            double4{srel[lane][0], srel[lane][1], srel[lane][2], srel[lane][3]};
+      }
+      if (lane + kWave < found) {
+        keys_pad[row + r1] = skey[lane + kWave];
+        reinterpret_cast<double4 *>(rel_pad)[row + r1] =
+            double4{srel[lane + kWave][0], srel[lane + kWave][1], srel[lane + kWave][2],
+                    srel[lane + kWave][3]};
+      }
+    }
+    // the next atom reuses the wavefront's scratch
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 
@@ -642,6 +867,23 @@ static WrapMats wrap_mats(const WrapHost &wh) {
   return m;
 }
 
+// Squared radius of nbr_bin_enum_sort_kernel's FP32 distance test.  Wrapped
+// positions lie in the cell, |w| <= e = |a1| + |a2| + |a3|, and the images
+// tested within 2e.  Rounding positions to FP32 moves a component by at most
+// 2^-24 (2e + e), the FP32 difference adds 2^-24 |d|, and the FP32 sum of
+// squares a relative 3 * 2^-24: a neighbor at distance below rc tests below
+// (rc + 3.1e-7 e + 1.1e-7 rc)^2 (1 + 1.8e-7).  The radius used,
+// rc (1 + 1e-5) + 1e-6 (2e + rc), exceeds that several times over.
+static float prefilter_cut2(const WrapHost &wh) {
+  double e = 0;
+  for (int k = 0; k < 3; k++)
+    e += std::sqrt(wh.cell[3 * k] * wh.cell[3 * k] + wh.cell[3 * k + 1] * wh.cell[3 * k + 1] +
+                   wh.cell[3 * k + 2] * wh.cell[3 * k + 2]);
+  const double rc = wh.cutoff;
+  const double r = rc * (1 + 1e-5) + 1e-6 * (2 * e + rc);
+  return std::nextafter((float)(r * r), INFINITY);
+}
+
 void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_pos,
                        const int *d_species, const WrapHost &wh, int *stats_host,
                        hipEvent_t stats_ev) {
@@ -710,8 +952,9 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
                          b.keys_pad.as<int64_t>(), b.rel_pad.as<double>(),
                          overflow);
     else
-      hipLaunchKernelGGL(nbr_enum_sort_kernel<false>, dim3(gw), dim3(256), 0, st,
-                         n, m, b.wrapped.as<double>(), b.bin_of.as<int>(),
+      hipLaunchKernelGGL(nbr_bin_enum_sort_kernel, dim3(nbins), dim3(256), 0, st, m,
+                         prefilter_cut2(wh), n <= (1 << 26) ? 1 : 0,
+                         b.wrapped.as<double>(), b.bin_of.as<int>(),
                          b.bin_start.as<int>(), b.bin_atoms.as<int>(),
                          b.bin_pos.as<double>(), b.count.as<int>(),
                          b.keys_pad.as<int64_t>(), b.rel_pad.as<double>(),

@@ -87,3 +87,42 @@ def test_fused_path_reverse_pairs(orc, cfg1):
     assert rev.min() >= 0
     np.testing.assert_array_equal(o.rel[rev, 1:], -o.rel[:, 1:])
     np.testing.assert_array_equal(rev[rev], np.arange(len(rev)))
+
+
+def random_box(seed, rho, L, cell=None):
+    rng = np.random.default_rng(seed)
+    cell = np.eye(3) * L if cell is None else np.asarray(cell, dtype=np.float64)
+    n = int(round(rho * abs(np.linalg.det(cell))))
+    pos = rng.random((n, 3)) @ cell
+    species = rng.integers(0, 2, n)
+    return cell, species, pos
+
+
+def test_bin_staging_limit(orc):
+    # six bins of 6 A per axis, ~0.09 atoms/A^3: the 27 bins around a home
+    # bin hold ~525 candidates on average, so home bins fall on both sides of
+    # the 512 staged in LDS (above it, the per-atom pass from global memory
+    # runs); ~81 neighbors per atom, below the 128 the rows keep
+    cell, species, pos = random_box(21, 0.09, 36.0)
+    s, o = compare(orc, cell, species, pos, 6.0)
+    assert s.neighbor_count.max() <= 128
+    from flare_pp_amd.structure import d2h
+    rev = d2h(s._view.d_reverse_pair, s.n_neighbors, np.int32)
+    np.testing.assert_array_equal(rev[rev], np.arange(len(rev)))
+    np.testing.assert_array_equal(o.rel[rev, 1:], -o.rel[:, 1:])
+
+
+def test_bins_over_capacity(orc):
+    # ~138 neighbors per atom with at least three bins per axis: the staged
+    # kernel's fallback flags the atoms above 128 and the two-pass path
+    # rebuilds the list
+    cell, species, pos = random_box(22, 0.12, 39.0)
+    s, o = compare(orc, cell, species, pos, 6.5)
+    assert s.neighbor_count.max() > 128
+
+
+def test_triclinic_bins(orc):
+    # sheared cell with three or more bins along every axis
+    cell = np.array([[24.0, 0.0, 0.0], [5.0, 22.0, 0.0], [-3.0, 4.0, 23.0]])
+    cell, species, pos = random_box(23, 0.05, None, cell)
+    compare(orc, cell, species, pos, 5.0)
