@@ -491,20 +491,37 @@ struct __align__(16) BinCand {
   float x, y, z;
   int k;
 };
+// LDS of nbr_bin_enum_sort_kernel, in bytes.  Staged path: candidate FP64
+// positions [3][kBinCap], BinCand [kBinCap], atom ids [kBinCap], then per
+// wavefront the found keys (64-bit [kNbrCap], 32-bit [kNbrCap]), the
+// prefilter queue (u16 [kBinCap]) and the found candidates (u16 [kNbrCap]).
+// Fallback path (enum_sort_atom): prefix [4][kWave], skey [4][kNbrCap],
+// srel [4][kNbrCap][4], in the same bytes.
+constexpr int kBinOffPf = 3 * kBinCap * 8;
+constexpr int kBinOffJ = kBinOffPf + kBinCap * 16;
+constexpr int kBinOffWave = kBinOffJ + kBinCap * 4;
+constexpr int kBinWaveBytes = kNbrCap * 8 + kNbrCap * 4 + kBinCap * 2 + kNbrCap * 2;
+constexpr int kBinStaged = kBinOffWave + 4 * kBinWaveBytes;
+constexpr int kBinFallback = 4 * kWave * 4 + 4 * kNbrCap * 8 + 4 * kNbrCap * 32;
+constexpr int kBinLds = kBinStaged > kBinFallback ? kBinStaged : kBinFallback;
+static_assert(kBinOffWave % 16 == 0 && kBinWaveBytes % 16 == 0, "LDS alignment");
 
 // The same rows as nbr_enum_sort_kernel<false> (at least three bins along
 // every axis), one workgroup per home bin.  The candidates of the 27
 // neighboring bins (one image shift each) are staged in LDS once for all
-// of the bin's atoms, with FP32 copies of their shifted positions.  Each
-// atom's wavefront keeps the candidates within cut_f of an FP32 distance
-// test, then runs the exact FP64 test of structure.cpp:68-77 on those only.
-// cut_f exceeds the cutoff by more than the FP32 rounding of the test
-// (positions rounded to FP32, differences and squares in FP32), so no
-// neighbor is lost; the kept set, its rel vectors and its order are those of
-// the exact test alone.  Rows are rank-sorted by a 32-bit key (j << 6 | the
-// three shifts + 1 in two bits each, the (j, s1, s2, s3) order) when
-// key32 is set (n <= 2^26), else by the 64-bit key.  A home bin with more
-// than kBinCap candidates falls back to enum_sort_atom for its atoms.
+// of the bin's atoms (which are the candidates of the home bin itself, the
+// combination 13 = (0, 0, 0) shift): ids, FP64 wrapped positions and FP32
+// copies of the shifted positions.  Each atom's wavefront keeps the
+// candidates within cut_f of an FP32 distance test, then runs the exact
+// FP64 test of structure.cpp:68-77 on those only.  cut_f exceeds the cutoff
+// by more than the FP32 rounding of the test (positions rounded to FP32,
+// differences and squares in FP32), so no neighbor is lost; the kept set,
+// its rel vectors and its order are those of the exact test alone.  Rows are
+// rank-sorted by a 32-bit key (j << 6 | the three shifts + 1 in two bits
+// each, the (j, s1, s2, s3) order) when key32 is set (n <= 2^26), else by
+// the 64-bit key; the rel vectors are recomputed by the same test when the
+// sorted row is written.  A home bin with more than kBinCap candidates
+// falls back to enum_sort_atom for its atoms.
 __global__ void __launch_bounds__(256)
 nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
                          const double *__restrict__ wrapped,
@@ -515,13 +532,7 @@ nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
                          int64_t *__restrict__ keys_pad, double *__restrict__ rel_pad,
                          int *__restrict__ stat) {
   __shared__ int c_pre[28], c_start[27], c_shift[27];
-  __shared__ int cand_slot[kBinCap];
-  __shared__ BinCand cand_pf[kBinCap];
-  __shared__ unsigned short queue_sh[4][kBinCap];
-  __shared__ unsigned skey32_sh[4][kNbrCap];
-  __shared__ int prefix_sh[4][kWave];
-  __shared__ int64_t skey_sh[4][kNbrCap];
-  __shared__ double srel_sh[4][kNbrCap][4];
+  __shared__ __align__(16) unsigned char lds[kBinLds];
   const int b = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
   const int a0 = bin_start[b], na = bin_start[b + 1] - a0;
@@ -556,12 +567,18 @@ nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
   __syncthreads();
   const int total = c_pre[27];
   if (total > kBinCap) {
+    int *prefix = reinterpret_cast<int *>(lds) + wv * kWave;
+    int64_t *skey = reinterpret_cast<int64_t *>(lds + 4 * kWave * 4) + wv * kNbrCap;
+    double(*srel)[4] = reinterpret_cast<double(*)[4]>(
+        lds + 4 * kWave * 4 + 4 * kNbrCap * 8 + wv * kNbrCap * 32);
     for (int a = wv; a < na; a += 4)
       enum_sort_atom<false>(bin_atoms[a0 + a], m, wrapped, bin_of, bin_start, bin_atoms,
-                            bin_pos, count, keys_pad, rel_pad, stat, prefix_sh[wv],
-                            skey_sh[wv], srel_sh[wv]);
+                            bin_pos, count, keys_pad, rel_pad, stat, prefix, skey, srel);
     return;
   }
+  double(*cw)[kBinCap] = reinterpret_cast<double(*)[kBinCap]>(lds);
+  BinCand *cpf = reinterpret_cast<BinCand *>(lds + kBinOffPf);
+  int *cj = reinterpret_cast<int *>(lds + kBinOffJ);
   for (int t = tid; t < total; t += 256) {
     // combo k: the last with c_pre[k] <= t
     int lo = 0, hi = 26;
@@ -574,32 +591,39 @@ nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
     }
     const int slot = c_start[lo] + (t - c_pre[lo]);
     const int sh = c_shift[lo];
+    const int j = bin_atoms[slot];
+    const double x = bin_pos[3 * slot], y = bin_pos[3 * slot + 1], z = bin_pos[3 * slot + 2];
     const double s1 = (sh >> 4) - 1, s2 = ((sh >> 2) & 3) - 1, s3 = (sh & 3) - 1;
     BinCand pf;
-    pf.x = (float)(bin_pos[3 * slot] + s1 * m.cell[0] + s2 * m.cell[3] + s3 * m.cell[6]);
-    pf.y = (float)(bin_pos[3 * slot + 1] + s1 * m.cell[1] + s2 * m.cell[4] + s3 * m.cell[7]);
-    pf.z = (float)(bin_pos[3 * slot + 2] + s1 * m.cell[2] + s2 * m.cell[5] + s3 * m.cell[8]);
+    pf.x = (float)(x + s1 * m.cell[0] + s2 * m.cell[3] + s3 * m.cell[6]);
+    pf.y = (float)(y + s1 * m.cell[1] + s2 * m.cell[4] + s3 * m.cell[7]);
+    pf.z = (float)(z + s1 * m.cell[2] + s2 * m.cell[5] + s3 * m.cell[8]);
     pf.k = lo;
-    cand_slot[t] = slot;
-    cand_pf[t] = pf;
+    cw[0][t] = x;
+    cw[1][t] = y;
+    cw[2][t] = z;
+    cj[t] = j;
+    cpf[t] = pf;
   }
   __syncthreads();
-  unsigned short *queue = queue_sh[wv];
-  unsigned *skey32 = skey32_sh[wv];
-  int64_t *skey = skey_sh[wv];
-  double(*srel)[4] = srel_sh[wv];
+  unsigned char *wb = lds + kBinOffWave + wv * kBinWaveBytes;
+  int64_t *skey = reinterpret_cast<int64_t *>(wb);
+  unsigned *skey32 = reinterpret_cast<unsigned *>(wb + kNbrCap * 8);
+  unsigned short *queue = reinterpret_cast<unsigned short *>(wb + kNbrCap * 12);
+  unsigned short *sidx = queue + kBinCap;
   const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const int home0 = c_pre[13];
   for (int a = wv; a < na; a += 4) {
-    const int hs = a0 + a;
-    const int i = bin_atoms[hs];
-    const double wi[3] = {bin_pos[3 * hs], bin_pos[3 * hs + 1], bin_pos[3 * hs + 2]};
-    const float wx = (float)wi[0], wy = (float)wi[1], wz = (float)wi[2];
+    const int hc = home0 + a;
+    const int i = cj[hc];
+    const double wi[3] = {cw[0][hc], cw[1][hc], cw[2][hc]};
+    const float wx = cpf[hc].x, wy = cpf[hc].y, wz = cpf[hc].z;
     int nq = 0;
     for (int t0 = 0; t0 < total; t0 += kWave) {
       const int t = t0 + lane;
       bool keep = false;
       if (t < total) {
-        const BinCand c = cand_pf[t];
+        const BinCand c = cpf[t];
         const float dx = c.x - wx, dy = c.y - wy, dz = c.z - wz;
         keep = dx * dx + dy * dy + dz * dz < cut_f2;
       }
@@ -614,16 +638,16 @@ nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
     for (int q0 = 0; q0 < nq; q0 += kWave) {
       const int q = q0 + lane;
       bool pass = false;
-      double o[4];
+      int t = 0;
       int64_t key = 0;
       unsigned k32 = 0;
       if (q < nq) {
-        const int t = queue[q];
-        const int slot = cand_slot[t];
-        const int sh = c_shift[cand_pf[t].k];
+        t = queue[q];
+        const int sh = c_shift[cpf[t].k];
         const int s1 = (sh >> 4) - 1, s2 = ((sh >> 2) & 3) - 1, s3 = (sh & 3) - 1;
-        const int j = bin_atoms[slot];
-        const double wj[3] = {bin_pos[3 * slot], bin_pos[3 * slot + 1], bin_pos[3 * slot + 2]};
+        const int j = cj[t];
+        const double wj[3] = {cw[0][t], cw[1][t], cw[2][t]};
+        double o[4];
         pass = image_test(wi, wj, m.cell, s1, s2, s3, m.cutoff, o);
         key = pair_key(j, s1, s2, s3, m.sweep);
         k32 = ((unsigned)j << 6) | (unsigned)sh;
@@ -634,10 +658,7 @@ nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
         if (sl < kNbrCap) {
           skey[sl] = key;
           skey32[sl] = k32;
-          srel[sl][0] = o[0];
-          srel[sl][1] = o[1];
-          srel[sl][2] = o[2];
-          srel[sl][3] = o[3];
+          sidx[sl] = (unsigned short)t;
         }
       }
       found += __popcll(mk);
@@ -674,16 +695,20 @@ nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
         }
       }
       const int64_t row = (int64_t)i * kNbrCap;
-      if (lane < found) {
-        keys_pad[row + r0] = skey[lane];
-        reinterpret_cast<double4 *>(rel_pad)[row + r0] =
-            double4{srel[lane][0], srel[lane][1], srel[lane][2], srel[lane][3]};
-      }
-      if (lane + kWave < found) {
-        keys_pad[row + r1] = skey[lane + kWave];
-        reinterpret_cast<double4 *>(rel_pad)[row + r1] =
-            double4{srel[lane + kWave][0], srel[lane + kWave][1], srel[lane + kWave][2],
-                    srel[lane + kWave][3]};
+#pragma unroll
+      for (int h = 0; h < 2; h++) {
+        const int e = lane + h * kWave;
+        if (e < found) {
+          const int t = sidx[e];
+          const int sh = c_shift[cpf[t].k];
+          const int s1 = (sh >> 4) - 1, s2 = ((sh >> 2) & 3) - 1, s3 = (sh & 3) - 1;
+          const double wj[3] = {cw[0][t], cw[1][t], cw[2][t]};
+          double o[4];
+          image_test(wi, wj, m.cell, s1, s2, s3, m.cutoff, o);
+          const int64_t r = row + (h ? r1 : r0);
+          keys_pad[r] = skey[e];
+          reinterpret_cast<double4 *>(rel_pad)[r] = double4{o[0], o[1], o[2], o[3]};
+        }
       }
     }
     // the next atom reuses the wavefront's scratch
