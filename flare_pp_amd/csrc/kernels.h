@@ -54,7 +54,7 @@ struct WrapHost {
 
 struct NListBuffers {
   DevBuf wrapped, bin_of, bin_count, bin_start, bin_fill, bin_atoms, bin_pos, count,
-      cum, keys_tmp, rel_tmp, keys_pad, rel_pad, keys, rel, idx, nspec, center,
+      cum, keys_tmp, rel_tmp, keys_pad, keys, rel, idx, nspec, center,
       rev;
   DevBuf row_of, atom_of_row, species_of_row, species_count, species_blk,
       scan_tmp;

@@ -334,15 +334,16 @@ nbr_enum_kernel(int n, WrapMats m, const double *__restrict__ wrapped,
 // Candidates per atom held in LDS by the fused enumerate + sort pass.
 constexpr int kNbrCap = 128;
 
-// Fused pass: enumerate one atom's neighbors (as nbr_enum_kernel), keep them
-// in LDS, rank-sort them by key and write them to the atom's padded slot
-// row (kNbrCap entries) of keys_pad / rel_pad.  count[i] is always written;
+// Fused pass: enumerate one atom's neighbors (as nbr_enum_kernel), keep their
+// keys in LDS, rank-sort them and write them to the atom's padded slot row
+// (kNbrCap entries) of keys_pad; the rel vectors are recomputed from the
+// keys by nbr_compact_kernel.  count[i] is always written;
 // atoms with more than kNbrCap neighbors are left to the two-pass path
 // (stat[0] becomes the largest such count; it stays 0 otherwise).
 // THIN: some axis has fewer than three bins (general combination lists);
 // otherwise the 27 neighboring bins with fixed per-axis lists of three.
-// One wavefront's pass for atom i; prefix (kWave ints), skey and srel
-// (kNbrCap entries) are the wavefront's LDS scratch.
+// One wavefront's pass for atom i; prefix (kWave ints) and skey (kNbrCap
+// entries) are the wavefront's LDS scratch.
 template <bool THIN>
 __device__ inline void enum_sort_atom(int i, const WrapMats &m,
                                       const double *__restrict__ wrapped,
@@ -351,9 +352,8 @@ __device__ inline void enum_sort_atom(int i, const WrapMats &m,
                                       const int *__restrict__ bin_atoms,
                                       const double *__restrict__ bin_pos,
                                       int *__restrict__ count,
-                                      int64_t *__restrict__ keys_pad,
-                                      double *__restrict__ rel_pad, int *__restrict__ stat,
-                                      int *prefix, int64_t *skey, double (*srel)[4]) {
+                                      int64_t *__restrict__ keys_pad, int *__restrict__ stat,
+                                      int *prefix, int64_t *skey) {
   const int lane = threadIdx.x & (kWave - 1);
   const int sw = m.sweep;
   const int hb = bin_of[i];
@@ -424,13 +424,7 @@ __device__ inline void enum_sort_atom(int i, const WrapMats &m,
       const uint64_t mask = __ballot(pass);
       if (pass) {
         const int slot = found + __popcll(mask & lt_mask);
-        if (slot < kNbrCap) {
-          skey[slot] = key;
-          srel[slot][0] = o[0];
-          srel[slot][1] = o[1];
-          srel[slot][2] = o[2];
-          srel[slot][3] = o[3];
-        }
+        if (slot < kNbrCap) skey[slot] = key;
       }
       found += __popcll(mask);
     }
@@ -455,14 +449,8 @@ __device__ inline void enum_sort_atom(int i, const WrapMats &m,
     r1 += kq < k1;
   }
   const int64_t row = (int64_t)i * kNbrCap;
-  if (lane < found) {
-    keys_pad[row + r0] = k0;
-    for (int c = 0; c < 4; c++) rel_pad[4 * (row + r0) + c] = srel[lane][c];
-  }
-  if (lane + kWave < found) {
-    keys_pad[row + r1] = k1;
-    for (int c = 0; c < 4; c++) rel_pad[4 * (row + r1) + c] = srel[lane + kWave][c];
-  }
+  if (lane < found) keys_pad[row + r0] = k0;
+  if (lane + kWave < found) keys_pad[row + r1] = k1;
 }
 
 template <bool THIN>
@@ -472,37 +460,35 @@ nbr_enum_sort_kernel(int n, WrapMats m, const double *__restrict__ wrapped,
                      const int *__restrict__ bin_start,
                      const int *__restrict__ bin_atoms,
                      const double *__restrict__ bin_pos, int *__restrict__ count,
-                     int64_t *__restrict__ keys_pad,
-                     double *__restrict__ rel_pad, int *__restrict__ stat) {
+                     int64_t *__restrict__ keys_pad, int *__restrict__ stat) {
   __shared__ int prefix_sh[4][kWave];
   __shared__ int64_t skey_sh[4][kNbrCap];
-  __shared__ double srel_sh[4][kNbrCap][4];
   const int wv = threadIdx.x / kWave;
   const int i = blockIdx.x * 4 + wv;
   if (i >= n) return;  // whole wave exits together
   enum_sort_atom<THIN>(i, m, wrapped, bin_of, bin_start, bin_atoms, bin_pos, count,
-                       keys_pad, rel_pad, stat, prefix_sh[wv], skey_sh[wv], srel_sh[wv]);
+                       keys_pad, stat, prefix_sh[wv], skey_sh[wv]);
 }
 
 // Candidates of one home bin's 27 neighboring bins staged in LDS: FP32
 // shifted position and the combination (bin, image shift) index.
-constexpr int kBinCap = 512;
+constexpr int kBinCap = 480;  // keeps the workgroup's LDS under 32 KiB: five per CU
 struct __align__(16) BinCand {
   float x, y, z;
   int k;
 };
 // LDS of nbr_bin_enum_sort_kernel, in bytes.  Staged path: candidate FP64
 // positions [3][kBinCap], BinCand [kBinCap], atom ids [kBinCap], then per
-// wavefront the found keys (64-bit [kNbrCap], 32-bit [kNbrCap]), the
-// prefilter queue (u16 [kBinCap]) and the found candidates (u16 [kNbrCap]).
-// Fallback path (enum_sort_atom): prefix [4][kWave], skey [4][kNbrCap],
-// srel [4][kNbrCap][4], in the same bytes.
+// wavefront the found keys (64-bit [kNbrCap], 32-bit [kNbrCap]) and the
+// prefilter queue (u16 [kBinCap]).
+// Fallback path (enum_sort_atom): prefix [4][kWave], skey [4][kNbrCap], in
+// the same bytes.
 constexpr int kBinOffPf = 3 * kBinCap * 8;
 constexpr int kBinOffJ = kBinOffPf + kBinCap * 16;
 constexpr int kBinOffWave = kBinOffJ + kBinCap * 4;
-constexpr int kBinWaveBytes = kNbrCap * 8 + kNbrCap * 4 + kBinCap * 2 + kNbrCap * 2;
+constexpr int kBinWaveBytes = kNbrCap * 8 + kNbrCap * 4 + kBinCap * 2;
 constexpr int kBinStaged = kBinOffWave + 4 * kBinWaveBytes;
-constexpr int kBinFallback = 4 * kWave * 4 + 4 * kNbrCap * 8 + 4 * kNbrCap * 32;
+constexpr int kBinFallback = 4 * kWave * 4 + 4 * kNbrCap * 8;
 constexpr int kBinLds = kBinStaged > kBinFallback ? kBinStaged : kBinFallback;
 static_assert(kBinOffWave % 16 == 0 && kBinWaveBytes % 16 == 0, "LDS alignment");
 
@@ -519,9 +505,9 @@ static_assert(kBinOffWave % 16 == 0 && kBinWaveBytes % 16 == 0, "LDS alignment")
 // its rel vectors and its order are those of the exact test alone.  Rows are
 // rank-sorted by a 32-bit key (j << 6 | the three shifts + 1 in two bits
 // each, the (j, s1, s2, s3) order) when key32 is set (n <= 2^26), else by
-// the 64-bit key; the rel vectors are recomputed by the same test when the
-// sorted row is written.  A home bin with more than kBinCap candidates
-// falls back to enum_sort_atom for its atoms.
+// the 64-bit key; nbr_compact_kernel recomputes the rel vectors from the
+// keys.  A home bin with more than kBinCap candidates falls back to
+// enum_sort_atom for its atoms.
 __global__ void __launch_bounds__(256)
 nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
                          const double *__restrict__ wrapped,
@@ -529,8 +515,7 @@ nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
                          const int *__restrict__ bin_start,
                          const int *__restrict__ bin_atoms,
                          const double *__restrict__ bin_pos, int *__restrict__ count,
-                         int64_t *__restrict__ keys_pad, double *__restrict__ rel_pad,
-                         int *__restrict__ stat) {
+                         int64_t *__restrict__ keys_pad, int *__restrict__ stat) {
   __shared__ int c_pre[28], c_start[27], c_shift[27];
   __shared__ __align__(16) unsigned char lds[kBinLds];
   const int b = blockIdx.x;
@@ -569,11 +554,9 @@ nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
   if (total > kBinCap) {
     int *prefix = reinterpret_cast<int *>(lds) + wv * kWave;
     int64_t *skey = reinterpret_cast<int64_t *>(lds + 4 * kWave * 4) + wv * kNbrCap;
-    double(*srel)[4] = reinterpret_cast<double(*)[4]>(
-        lds + 4 * kWave * 4 + 4 * kNbrCap * 8 + wv * kNbrCap * 32);
     for (int a = wv; a < na; a += 4)
       enum_sort_atom<false>(bin_atoms[a0 + a], m, wrapped, bin_of, bin_start, bin_atoms,
-                            bin_pos, count, keys_pad, rel_pad, stat, prefix, skey, srel);
+                            bin_pos, count, keys_pad, stat, prefix, skey);
     return;
   }
   double(*cw)[kBinCap] = reinterpret_cast<double(*)[kBinCap]>(lds);
@@ -610,7 +593,6 @@ nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
   int64_t *skey = reinterpret_cast<int64_t *>(wb);
   unsigned *skey32 = reinterpret_cast<unsigned *>(wb + kNbrCap * 8);
   unsigned short *queue = reinterpret_cast<unsigned short *>(wb + kNbrCap * 12);
-  unsigned short *sidx = queue + kBinCap;
   const uint64_t lt_mask = lane ? (~0ull >> (64 - lane)) : 0ull;
   const int home0 = c_pre[13];
   for (int a = wv; a < na; a += 4) {
@@ -658,7 +640,6 @@ nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
         if (sl < kNbrCap) {
           skey[sl] = key;
           skey32[sl] = k32;
-          sidx[sl] = (unsigned short)t;
         }
       }
       found += __popcll(mk);
@@ -695,21 +676,8 @@ nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
         }
       }
       const int64_t row = (int64_t)i * kNbrCap;
-#pragma unroll
-      for (int h = 0; h < 2; h++) {
-        const int e = lane + h * kWave;
-        if (e < found) {
-          const int t = sidx[e];
-          const int sh = c_shift[cpf[t].k];
-          const int s1 = (sh >> 4) - 1, s2 = ((sh >> 2) & 3) - 1, s3 = (sh & 3) - 1;
-          const double wj[3] = {cw[0][t], cw[1][t], cw[2][t]};
-          double o[4];
-          image_test(wi, wj, m.cell, s1, s2, s3, m.cutoff, o);
-          const int64_t r = row + (h ? r1 : r0);
-          keys_pad[r] = skey[e];
-          reinterpret_cast<double4 *>(rel_pad)[r] = double4{o[0], o[1], o[2], o[3]};
-        }
-      }
+      if (lane < found) keys_pad[row + r0] = skey[lane];
+      if (lane + kWave < found) keys_pad[row + r1] = skey[lane + kWave];
     }
     // the next atom reuses the wavefront's scratch
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -718,15 +686,17 @@ nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
   }
 }
 
-// Padded sorted rows -> final CSR arrays, with the reverse-pair index found
-// by binary search in the neighbor's sorted row.  One wavefront per atom;
-// does nothing when *overflow is set.
+// Padded sorted key rows -> final CSR arrays, with the reverse-pair index
+// found by binary search in the neighbor's sorted row.  The rel vector of
+// each entry is recomputed from its key by the enumeration's own test
+// (image_test on the same wrapped positions and shift: the same bits).  One
+// wavefront per atom; does nothing when *overflow is set.
 __global__ void __launch_bounds__(256)
-nbr_compact_kernel(int n, int sweep, const int *__restrict__ overflow,
+nbr_compact_kernel(int n, WrapMats m, const int *__restrict__ overflow,
                    const int *__restrict__ cum,
                    const int *__restrict__ count,
                    const int64_t *__restrict__ keys_pad,
-                   const double *__restrict__ rel_pad,
+                   const double *__restrict__ wrapped,
                    const int *__restrict__ species,
                    int64_t *__restrict__ keys_out, double *__restrict__ rel_out,
                    int *__restrict__ idx_out, int *__restrict__ nspec_out,
@@ -736,13 +706,17 @@ nbr_compact_kernel(int n, int sweep, const int *__restrict__ overflow,
   // some atom has more than kNbrCap neighbors: the two-pass path rebuilds
   // the arrays (the output capacity n * kNbrCap may then be too small)
   if (i >= n || *overflow) return;
+  const int sweep = m.sweep;
   const int b = cum[i], cnt = cum[i + 1] - b;
   const int64_t row = (int64_t)i * kNbrCap;
+  const double wi[3] = {wrapped[3 * i], wrapped[3 * i + 1], wrapped[3 * i + 2]};
   for (int q = lane; q < cnt; q += kWave) {
     const int64_t k = keys_pad[row + q];
-    const double4 rv = reinterpret_cast<const double4 *>(rel_pad)[row + q];
     int j, s1, s2, s3;
     key_parts(k, sweep, j, s1, s2, s3);
+    const double wj[3] = {wrapped[3 * j], wrapped[3 * j + 1], wrapped[3 * j + 2]};
+    double o[4];
+    image_test(wi, wj, m.cell, s1, s2, s3, m.cutoff, o);
     const int64_t want = pair_key(i, -s1, -s2, -s3, sweep);
     const int dst = b + q;
     // the entry of the two partners with the smaller key looks the other
@@ -767,7 +741,7 @@ nbr_compact_kernel(int n, int sweep, const int *__restrict__ overflow,
       }
     }
     keys_out[dst] = k;
-    reinterpret_cast<double4 *>(rel_out)[dst] = rv;
+    reinterpret_cast<double4 *>(rel_out)[dst] = double4{o[0], o[1], o[2], o[3]};
     idx_out[dst] = j;
     nspec_out[dst] = species[j];
     center_out[dst] = i;
@@ -927,7 +901,6 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
   b.count.ensure((size_t)(n + 1) * sizeof(int));
   b.cum.ensure((size_t)(n + 1) * sizeof(int));
   b.keys_pad.ensure(cap * sizeof(int64_t));
-  b.rel_pad.ensure(cap * 4 * sizeof(double));
   b.keys.ensure(cap * sizeof(int64_t));
   b.rel.ensure(cap * 4 * sizeof(double));
   b.idx.ensure(cap * sizeof(int));
@@ -974,16 +947,14 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
                          n, m, b.wrapped.as<double>(), b.bin_of.as<int>(),
                          b.bin_start.as<int>(), b.bin_atoms.as<int>(),
                          b.bin_pos.as<double>(), b.count.as<int>(),
-                         b.keys_pad.as<int64_t>(), b.rel_pad.as<double>(),
-                         overflow);
+                         b.keys_pad.as<int64_t>(), overflow);
     else
       hipLaunchKernelGGL(nbr_bin_enum_sort_kernel, dim3(nbins), dim3(256), 0, st, m,
                          prefilter_cut2(wh), n <= (1 << 26) ? 1 : 0,
                          b.wrapped.as<double>(), b.bin_of.as<int>(),
                          b.bin_start.as<int>(), b.bin_atoms.as<int>(),
                          b.bin_pos.as<double>(), b.count.as<int>(),
-                         b.keys_pad.as<int64_t>(), b.rel_pad.as<double>(),
-                         overflow);
+                         b.keys_pad.as<int64_t>(), overflow);
   }
   FLARE_HIP_CHECK(hipMemsetAsync(b.count.as<int>() + n, 0, sizeof(int), st));
   exclusive_scan(st, b.scan_tmp, b.count.as<int>(), b.cum.as<int>(), n);
@@ -996,9 +967,9 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
   if (n > 0) {
     // entries without a partner keep -1 (the partners write each other)
     FLARE_HIP_CHECK(hipMemsetAsync(b.rev.ptr, 0xff, cap * sizeof(int), st));
-    hipLaunchKernelGGL(nbr_compact_kernel, dim3(gw), dim3(256), 0, st, n,
-                       wh.sweep, overflow, b.cum.as<int>(), b.count.as<int>(),
-                       b.keys_pad.as<int64_t>(), b.rel_pad.as<double>(),
+    hipLaunchKernelGGL(nbr_compact_kernel, dim3(gw), dim3(256), 0, st, n, m,
+                       overflow, b.cum.as<int>(), b.count.as<int>(),
+                       b.keys_pad.as<int64_t>(), b.wrapped.as<double>(),
                        d_species, b.keys.as<int64_t>(), b.rel.as<double>(),
                        b.idx.as<int>(), b.nspec.as<int>(), b.center.as<int>(),
                        b.rev.as<int>());

@@ -99,11 +99,11 @@ def random_box(seed, rho, L, cell=None):
 
 
 def test_bin_staging_limit(orc):
-    # six bins of 6 A per axis, ~0.09 atoms/A^3: the 27 bins around a home
-    # bin hold ~525 candidates on average, so home bins fall on both sides of
-    # the 512 staged in LDS (above it, the per-atom pass from global memory
-    # runs); ~81 neighbors per atom, below the 128 the rows keep
-    cell, species, pos = random_box(21, 0.09, 36.0)
+    # six bins of 6 A per axis, ~0.082 atoms/A^3: the 27 bins around a home
+    # bin hold ~478 candidates on average, so home bins fall on both sides of
+    # the 480 staged in LDS (above it, the per-atom pass from global memory
+    # runs); ~74 neighbors per atom, below the 128 the rows keep
+    cell, species, pos = random_box(21, 0.082, 36.0)
     s, o = compare(orc, cell, species, pos, 6.0)
     assert s.neighbor_count.max() <= 128
     from flare_pp_amd.structure import d2h
