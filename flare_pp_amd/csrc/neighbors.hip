@@ -80,49 +80,115 @@ wrap_bin_kernel(int n, const double *__restrict__ pos, WrapMats m,
   if (t < kMaxSpecies) bcount[blockIdx.x * kMaxSpecies + t] = cnt[t];
 }
 
-// Zero the build's counters in one launch: a[0..na), b[0..nb), c[0..nc).
-__global__ void clear3_kernel(int *__restrict__ a, int na, int *__restrict__ b,
-                              int nb, int *__restrict__ c, int nc) {
+// Zero the build's counters in one launch: a[0..na), b[0..nb), c[0..nc),
+// d[0..nd).
+__global__ void clear4_kernel(int *__restrict__ a, int na, int *__restrict__ b,
+                              int nb, int *__restrict__ c, int nc, int *__restrict__ d,
+                              int nd) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q < na) a[q] = 0;
   if (q < nb) b[q] = 0;
   if (q < nc) c[q] = 0;
+  if (q < nd) d[q] = 0;
 }
 
-// Single-workgroup exclusive scan: out[0..n] with out[n] = total, each of
-// the 1024 threads summing a contiguous segment.  Used for small arrays
-// (bins, per-atom counts) where one workgroup beats a device-wide scan.
+// Single-workgroup exclusive scan: out[0..n] with out[n] = total, for
+// n <= C * kScanChunk.  Chunk c holds elements [c kScanChunk, (c + 1)
+// kScanChunk), thread t its four elements 4t..4t+3, so every load and store
+// instruction covers 1 KiB per wavefront contiguously (16-byte vectors where
+// in / out are 16-byte aligned).  All chunks are loaded first; per chunk a
+// wavefront scan of the four-element sums (shuffles), then one wavefront
+// scans the C x 16 wavefront totals in element order.  Two barriers in all.
+// Used for the bin counts and the per-atom counts, where one workgroup beats
+// a device-wide scan's two launches.
 constexpr int kScanThreads = 1024;
+constexpr int kScanChunk = 4 * kScanThreads;
+constexpr int kScanMaxC = 8;  // 16 would spill at 1,024 threads
 
-__device__ inline void block_exclusive_scan(int n, const int *__restrict__ in,
-                                            int *__restrict__ out, int *sh) {
-  const int t = threadIdx.x;
-  const int per = (n + kScanThreads - 1) / kScanThreads;
-  const int lo = min(n, t * per), hi = min(n, lo + per);
-  int s = 0;
-  for (int q = lo; q < hi; q++) s += in[q];
-  sh[t] = s;
-  __syncthreads();
-  for (int off = 1; off < kScanThreads; off <<= 1) {  // Hillis-Steele
-    int v = t >= off ? sh[t - off] : 0;
-    __syncthreads();
-    sh[t] += v;
-    __syncthreads();
-  }
-  int run = sh[t] - s;  // exclusive prefix of this segment
-  for (int q = lo; q < hi; q++) {
-    const int v = in[q];
-    out[q] = run;
-    run += v;
-  }
-  if (t == kScanThreads - 1) out[n] = sh[t];
-  __syncthreads();
-}
-
+template <int C>
 __global__ void __launch_bounds__(kScanThreads)
-scan_small_kernel(int n, const int *__restrict__ in, int *__restrict__ out) {
-  __shared__ int sh[kScanThreads];
-  block_exclusive_scan(n, in, out, sh);
+scan_block_kernel(int n, const int *__restrict__ in, int *__restrict__ out) {
+  constexpr int NW = kScanThreads / kWave;
+  static_assert(C * NW <= 4 * kWave, "one wavefront scans the totals");
+  __shared__ int wtot[C * NW];
+  __shared__ int total_sh;
+  const int t = threadIdx.x, lane = t & (kWave - 1), w = t / kWave;
+  const bool vin = (reinterpret_cast<uintptr_t>(in) & 15) == 0;
+  const bool vout = (reinterpret_cast<uintptr_t>(out) & 15) == 0;
+  int4 v[C];
+#pragma unroll
+  for (int c = 0; c < C; c++) {
+    const int e = c * kScanChunk + 4 * t;
+    if (vin && e + 3 < n) {
+      v[c] = *reinterpret_cast<const int4 *>(in + e);
+    } else {
+      v[c].x = e < n ? in[e] : 0;
+      v[c].y = e + 1 < n ? in[e + 1] : 0;
+      v[c].z = e + 2 < n ? in[e + 2] : 0;
+      v[c].w = e + 3 < n ? in[e + 3] : 0;
+    }
+  }
+  int s[C], incl[C];
+#pragma unroll
+  for (int c = 0; c < C; c++) {
+    s[c] = v[c].x + v[c].y + v[c].z + v[c].w;
+    incl[c] = s[c];
+  }
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+      const int y = __shfl_up(incl[c], off, kWave);
+      if (lane >= off) incl[c] += y;
+    }
+  }
+  if (lane == kWave - 1)
+#pragma unroll
+    for (int c = 0; c < C; c++) wtot[c * NW + w] = incl[c];
+  __syncthreads();
+  if (w == 0) {
+    // exclusive offsets of the C * NW wavefront totals, four per lane
+    int x[4], ls = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int e = 4 * lane + k;
+      x[k] = e < C * NW ? wtot[e] : 0;
+      ls += x[k];
+    }
+    int li = ls;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+      const int y = __shfl_up(li, off, kWave);
+      if (lane >= off) li += y;
+    }
+    int run = li - ls;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const int e = 4 * lane + k;
+      if (e < C * NW) wtot[e] = run;
+      run += x[k];
+    }
+    if (lane == kWave - 1) total_sh = li;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < C; c++) {
+    const int e = c * kScanChunk + 4 * t;
+    int4 o;
+    o.x = wtot[c * NW + w] + incl[c] - s[c];
+    o.y = o.x + v[c].x;
+    o.z = o.y + v[c].y;
+    o.w = o.z + v[c].z;
+    if (vout && e + 3 < n) {
+      *reinterpret_cast<int4 *>(out + e) = o;
+    } else {
+      if (e < n) out[e] = o.x;
+      if (e + 1 < n) out[e + 1] = o.y;
+      if (e + 2 < n) out[e + 2] = o.z;
+      if (e + 3 < n) out[e + 3] = o.w;
+    }
+  }
+  if (t == 0) out[n] = total_sh;
 }
 
 // After the count scan: the species scan of the per-block counts (boff,
@@ -837,6 +903,22 @@ species_rows_kernel(int n, const int *__restrict__ species,
 // must be readable and zero
 static void exclusive_scan(hipStream_t st, DevBuf &tmp, const int *in, int *out,
                            int n) {
+  if (n <= kScanMaxC * kScanChunk) {
+    const int chunks = (n + kScanChunk - 1) / kScanChunk;
+#define FLARE_SCAN(C) \
+  hipLaunchKernelGGL(scan_block_kernel<C>, dim3(1), dim3(kScanThreads), 0, st, n, in, out)
+    if (chunks <= 1)
+      FLARE_SCAN(1);
+    else if (chunks <= 2)
+      FLARE_SCAN(2);
+    else if (chunks <= 4)
+      FLARE_SCAN(4);
+    else
+      FLARE_SCAN(8);
+#undef FLARE_SCAN
+    FLARE_HIP_CHECK(hipGetLastError());
+    return;
+  }
   size_t bytes = 0;
   FLARE_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n + 1, st));
   tmp.ensure(bytes);
@@ -919,9 +1001,10 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
   int *overflow = counts + kMaxSpecies + 1;
 
   const int nclear = std::max(nbins + 2, kNlistStats);
-  hipLaunchKernelGGL(clear3_kernel, dim3((nclear + 255) / 256), dim3(256), 0, st,
+  // count[n] is the zero the count scan reads past the end
+  hipLaunchKernelGGL(clear4_kernel, dim3((nclear + 255) / 256), dim3(256), 0, st,
                      counts, kNlistStats, b.bin_count.as<int>(), nbins + 2,
-                     b.bin_fill.as<int>(), nbins + 1);
+                     b.bin_fill.as<int>(), nbins + 1, b.count.as<int>() + n, 1);
   const int tb = 256;
   const int g1 = (n + tb - 1) / tb;
   const int gw = (n + 3) / 4;
@@ -930,12 +1013,7 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
                        m, d_species, b.wrapped.as<double>(), b.bin_of.as<int>(),
                        b.bin_count.as<int>(), bcount, counts + kMaxSpecies);
   // bin_count[1..nbins] holds the counts; bin_count has one spare zero entry
-  if (nbins <= 16 * kScanThreads)
-    hipLaunchKernelGGL(scan_small_kernel, dim3(1), dim3(kScanThreads), 0, st, nbins,
-                       b.bin_count.as<int>() + 1, b.bin_start.as<int>());
-  else
-    exclusive_scan(st, b.scan_tmp, b.bin_count.as<int>() + 1, b.bin_start.as<int>(),
-                   nbins);
+  exclusive_scan(st, b.scan_tmp, b.bin_count.as<int>() + 1, b.bin_start.as<int>(), nbins);
   if (n > 0) {
     hipLaunchKernelGGL(bin_fill_kernel, dim3(g1), dim3(tb), 0, st, n,
                        b.bin_of.as<int>(), b.bin_start.as<int>(),
@@ -956,7 +1034,6 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
                          b.bin_pos.as<double>(), b.count.as<int>(),
                          b.keys_pad.as<int64_t>(), overflow);
   }
-  FLARE_HIP_CHECK(hipMemsetAsync(b.count.as<int>() + n, 0, sizeof(int), st));
   exclusive_scan(st, b.scan_tmp, b.count.as<int>(), b.cum.as<int>(), n);
   hipLaunchKernelGGL(nbr_stats_kernel, dim3(1), dim3(64 * kMaxSpecies), 0, st, n,
                      b.cum.as<int>(), n > 0 ? nblk : 0, bcount, boff, counts,
@@ -1168,8 +1245,8 @@ int64_t nlist_from_external(hipStream_t st, NListBuffers &b, const ExtList &e,
   int *bcount = b.species_blk.as<int>(), *boff = bcount + nblk * kMaxSpecies;
   int *counts = b.species_count.as<int>();
   int *bad = counts + kMaxSpecies;
-  hipLaunchKernelGGL(clear3_kernel, dim3(1), dim3(256), 0, st, counts, kNlistStats,
-                     b.count.as<int>() + n, 1, (int *)nullptr, 0);
+  hipLaunchKernelGGL(clear4_kernel, dim3(1), dim3(256), 0, st, counts, kNlistStats,
+                     b.count.as<int>() + n, 1, (int *)nullptr, 0, (int *)nullptr, 0);
   const int gw = (n + 3) / 4;
   if (n > 0) {
     hipLaunchKernelGGL(ext_count_kernel, dim3(gw), dim3(256), 0, st, e, c,
