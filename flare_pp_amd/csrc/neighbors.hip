@@ -106,8 +106,8 @@ constexpr int kScanChunk = 4 * kScanThreads;
 constexpr int kScanMaxC = 8;  // 16 would spill at 1,024 threads
 
 template <int C>
-__global__ void __launch_bounds__(kScanThreads)
-scan_block_kernel(int n, const int *__restrict__ in, int *__restrict__ out) {
+__device__ inline int block_scan_body(int n, const int *__restrict__ in,
+                                      int *__restrict__ out) {
   constexpr int NW = kScanThreads / kWave;
   static_assert(C * NW <= 4 * kWave, "one wavefront scans the totals");
   __shared__ int wtot[C * NW];
@@ -189,17 +189,22 @@ scan_block_kernel(int n, const int *__restrict__ in, int *__restrict__ out) {
     }
   }
   if (t == 0) out[n] = total_sh;
+  return total_sh;
+}
+
+template <int C>
+__global__ void __launch_bounds__(kScanThreads)
+scan_block_kernel(int n, const int *__restrict__ in, int *__restrict__ out) {
+  block_scan_body<C>(n, in, out);
 }
 
 // After the count scan: the species scan of the per-block counts (boff,
 // counts[s]) and the pair total cum[n] into stat_P, so that every number
 // the host needs sits in one small buffer.
-__global__ void __launch_bounds__(64 * kMaxSpecies)
-nbr_stats_kernel(int n, const int *__restrict__ cum, int nblk,
-                 const int *__restrict__ bcount, int *__restrict__ boff,
-                 int *__restrict__ counts, int *__restrict__ stat_P) {
+// One wavefront per species slot sp (threads 64 sp .. 64 sp + 63).
+__device__ inline void species_block_scan(int nblk, const int *__restrict__ bcount,
+                                          int *__restrict__ boff, int *__restrict__ counts) {
   const int lane = threadIdx.x & 63, sp = threadIdx.x >> 6;
-  if (threadIdx.x == 0) *stat_P = cum[n];
   int run = 0;
   for (int b0 = 0; b0 < nblk; b0 += 64) {
     const int b = b0 + lane;
@@ -216,21 +221,49 @@ nbr_stats_kernel(int n, const int *__restrict__ cum, int nblk,
   if (lane == 0) counts[sp] = run;
 }
 
-// bin_atoms / bin_pos: atoms and their wrapped positions grouped by bin
-// (order within a bin is arbitrary; the neighbor rows are sorted later)
-__global__ void bin_fill_kernel(int n, const int *__restrict__ bin_of,
-                                const int *__restrict__ bin_start,
-                                const double *__restrict__ wrapped,
-                                int *__restrict__ fillp,
-                                int *__restrict__ bin_atoms,
-                                double *__restrict__ bin_pos) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int b = bin_of[i];
-  int slot = bin_start[b] + atomicAdd(&fillp[b], 1);
-  bin_atoms[slot] = i;
-  for (int c = 0; c < 3; c++) bin_pos[3 * slot + c] = wrapped[3 * i + c];
+__global__ void __launch_bounds__(64 * kMaxSpecies)
+nbr_stats_kernel(int n, const int *__restrict__ cum, int nblk,
+                 const int *__restrict__ bcount, int *__restrict__ boff,
+                 int *__restrict__ counts, int *__restrict__ stat_P) {
+  if (threadIdx.x == 0) *stat_P = cum[n];
+  if (bcount) species_block_scan(nblk, bcount, boff, counts);
 }
+
+// The species scan alone (its counts are the bin scan's when that runs on
+// the device-wide path).
+__global__ void __launch_bounds__(64 * kMaxSpecies)
+species_scan_kernel(int nblk, const int *__restrict__ bcount, int *__restrict__ boff,
+                    int *__restrict__ counts) {
+  species_block_scan(nblk, bcount, boff, counts);
+}
+
+// The count scan (n <= kScanMaxC kScanChunk) with nbr_stats_kernel's work as
+// its epilogue: one launch instead of two.  bcount null: the pair total only
+// (the species scan ran with the bin scan).
+template <int C>
+__global__ void __launch_bounds__(kScanThreads)
+count_scan_stats_kernel(int n, const int *__restrict__ count, int *__restrict__ cum,
+                        int nblk, const int *__restrict__ bcount, int *__restrict__ boff,
+                        int *__restrict__ counts, int *__restrict__ stat_P) {
+  static_assert(kScanThreads >= 64 * kMaxSpecies, "a wavefront per species slot");
+  const int total = block_scan_body<C>(n, count, cum);
+  if (threadIdx.x == 0) *stat_P = total;
+  if (bcount && threadIdx.x < 64 * kMaxSpecies) species_block_scan(nblk, bcount, boff, counts);
+}
+
+// The bin-count scan (nbins <= kScanMaxC kScanChunk) with the species scan of
+// the per-block counts as its epilogue, so that the species rows can be
+// written by bin_fill_rows_kernel.
+template <int C>
+__global__ void __launch_bounds__(kScanThreads)
+bin_scan_species_kernel(int nbins, const int *__restrict__ bin_count1,
+                        int *__restrict__ bin_start, int nblk,
+                        const int *__restrict__ bcount, int *__restrict__ boff,
+                        int *__restrict__ counts) {
+  block_scan_body<C>(nbins, bin_count1, bin_start);
+  if (threadIdx.x < 64 * kMaxSpecies) species_block_scan(nblk, bcount, boff, counts);
+}
+
 
 namespace {
 
@@ -869,13 +902,15 @@ __global__ void nbr_reverse_kernel(int64_t P, int sweep,
 
 // Species-grouped row map (b2.cpp:104-176 regroup: atoms of species 0 in
 // original order, then species 1, ...): per-block species counts (in
-// wrap_bin_kernel), their scan (nbr_stats_kernel), and per-block ranks from
-// wavefront ballots.
-__global__ void __launch_bounds__(kRowBlock)
-species_rows_kernel(int n, const int *__restrict__ species,
-                    const int *__restrict__ boff, const int *__restrict__ counts,
-                    int *__restrict__ row_of, int *__restrict__ atom_of_row,
-                    int *__restrict__ species_of_row) {
+// wrap_bin_kernel or ext_species_kernel), their scan (species_block_scan),
+// and per-block ranks from wavefront ballots.
+// Block body (blockDim kRowBlock, every thread reaches it).
+__device__ inline void species_rows_body(int n, const int *__restrict__ species,
+                                         const int *__restrict__ boff,
+                                         const int *__restrict__ counts,
+                                         int *__restrict__ row_of,
+                                         int *__restrict__ atom_of_row,
+                                         int *__restrict__ species_of_row) {
   __shared__ int wcnt[kRowBlock / 64][kMaxSpecies];
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int a = blockIdx.x * kRowBlock + t;
@@ -897,6 +932,36 @@ species_rows_kernel(int n, const int *__restrict__ species,
     atom_of_row[row] = a;
     species_of_row[row] = s;
   }
+}
+
+__global__ void __launch_bounds__(kRowBlock)
+species_rows_kernel(int n, const int *__restrict__ species,
+                    const int *__restrict__ boff, const int *__restrict__ counts,
+                    int *__restrict__ row_of, int *__restrict__ atom_of_row,
+                    int *__restrict__ species_of_row) {
+  species_rows_body(n, species, boff, counts, row_of, atom_of_row, species_of_row);
+}
+
+// bin_atoms / bin_pos: atoms and their wrapped positions grouped by bin
+// (order within a bin is arbitrary; the neighbor rows are sorted later),
+// with species_rows_kernel's rows in the same launch (both one thread per
+// atom in blocks of kRowBlock).
+__global__ void __launch_bounds__(kRowBlock)
+bin_fill_rows_kernel(int n, const int *__restrict__ bin_of,
+                     const int *__restrict__ bin_start, const double *__restrict__ wrapped,
+                     int *__restrict__ fillp, int *__restrict__ bin_atoms,
+                     double *__restrict__ bin_pos, const int *__restrict__ species,
+                     const int *__restrict__ boff, const int *__restrict__ counts,
+                     int *__restrict__ row_of, int *__restrict__ atom_of_row,
+                     int *__restrict__ species_of_row) {
+  const int i = blockIdx.x * kRowBlock + threadIdx.x;
+  if (i < n) {
+    const int b = bin_of[i];
+    const int slot = bin_start[b] + atomicAdd(&fillp[b], 1);
+    bin_atoms[slot] = i;
+    for (int c = 0; c < 3; c++) bin_pos[3 * slot + c] = wrapped[3 * i + c];
+  }
+  species_rows_body(n, species, boff, counts, row_of, atom_of_row, species_of_row);
 }
 
 // exclusive prefix sum of in[0..n) into out[0..n], out[n] = total; in[n]
@@ -923,6 +988,33 @@ static void exclusive_scan(hipStream_t st, DevBuf &tmp, const int *in, int *out,
   FLARE_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n + 1, st));
   tmp.ensure(bytes);
   FLARE_HIP_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp.ptr, bytes, in, out, n + 1, st));
+}
+
+// cum = exclusive scan of count[0..n), then the species scan of the per-block
+// counts (boff, counts[s]) and the pair total into counts[kMaxSpecies + 2]
+static void count_scan_stats(hipStream_t st, NListBuffers &b, int n, int nblk,
+                             const int *bcount, int *boff, int *counts) {
+  if (n <= kScanMaxC * kScanChunk) {
+    const int chunks = (n + kScanChunk - 1) / kScanChunk;
+#define FLARE_SCAN(C)                                                                    \
+  hipLaunchKernelGGL(count_scan_stats_kernel<C>, dim3(1), dim3(kScanThreads), 0, st, n,   \
+                     b.count.as<int>(), b.cum.as<int>(), nblk, bcount, boff, counts,     \
+                     counts + kMaxSpecies + 2)
+    if (chunks <= 1)
+      FLARE_SCAN(1);
+    else if (chunks <= 2)
+      FLARE_SCAN(2);
+    else if (chunks <= 4)
+      FLARE_SCAN(4);
+    else
+      FLARE_SCAN(8);
+#undef FLARE_SCAN
+    FLARE_HIP_CHECK(hipGetLastError());
+    return;
+  }
+  exclusive_scan(st, b.scan_tmp, b.count.as<int>(), b.cum.as<int>(), n);
+  hipLaunchKernelGGL(nbr_stats_kernel, dim3(1), dim3(64 * kMaxSpecies), 0, st, n,
+                     b.cum.as<int>(), nblk, bcount, boff, counts, counts + kMaxSpecies + 2);
 }
 
 // ---------------------------------------------------------------------------
@@ -1012,13 +1104,37 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
     hipLaunchKernelGGL(wrap_bin_kernel, dim3(g1), dim3(kRowBlock), 0, st, n, d_pos,
                        m, d_species, b.wrapped.as<double>(), b.bin_of.as<int>(),
                        b.bin_count.as<int>(), bcount, counts + kMaxSpecies);
-  // bin_count[1..nbins] holds the counts; bin_count has one spare zero entry
-  exclusive_scan(st, b.scan_tmp, b.bin_count.as<int>() + 1, b.bin_start.as<int>(), nbins);
+  // bin_count[1..nbins] holds the counts; bin_count has one spare zero entry.
+  // The species scan of the per-block counts rides on the bin scan, so the
+  // species rows are written with the bin fill.
+  const int nblk_used = n > 0 ? nblk : 0;
+  if (nbins <= kScanMaxC * kScanChunk) {
+    const int chunks = (nbins + kScanChunk - 1) / kScanChunk;
+#define FLARE_SCAN(C)                                                                    \
+  hipLaunchKernelGGL(bin_scan_species_kernel<C>, dim3(1), dim3(kScanThreads), 0, st,      \
+                     nbins, b.bin_count.as<int>() + 1, b.bin_start.as<int>(), nblk_used, \
+                     bcount, boff, counts)
+    if (chunks <= 1)
+      FLARE_SCAN(1);
+    else if (chunks <= 2)
+      FLARE_SCAN(2);
+    else if (chunks <= 4)
+      FLARE_SCAN(4);
+    else
+      FLARE_SCAN(8);
+#undef FLARE_SCAN
+  } else {
+    exclusive_scan(st, b.scan_tmp, b.bin_count.as<int>() + 1, b.bin_start.as<int>(), nbins);
+    hipLaunchKernelGGL(species_scan_kernel, dim3(1), dim3(64 * kMaxSpecies), 0, st,
+                       nblk_used, bcount, boff, counts);
+  }
   if (n > 0) {
-    hipLaunchKernelGGL(bin_fill_kernel, dim3(g1), dim3(tb), 0, st, n,
+    hipLaunchKernelGGL(bin_fill_rows_kernel, dim3(g1), dim3(kRowBlock), 0, st, n,
                        b.bin_of.as<int>(), b.bin_start.as<int>(),
                        b.wrapped.as<double>(), b.bin_fill.as<int>(),
-                       b.bin_atoms.as<int>(), b.bin_pos.as<double>());
+                       b.bin_atoms.as<int>(), b.bin_pos.as<double>(), d_species, boff,
+                       counts, b.row_of.as<int>(), b.atom_of_row.as<int>(),
+                       b.species_of_row.as<int>());
     const bool thin = wh.nb[0] < 3 || wh.nb[1] < 3 || wh.nb[2] < 3;
     if (thin)
       hipLaunchKernelGGL(nbr_enum_sort_kernel<true>, dim3(gw), dim3(256), 0, st,
@@ -1034,10 +1150,7 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
                          b.bin_pos.as<double>(), b.count.as<int>(),
                          b.keys_pad.as<int64_t>(), overflow);
   }
-  exclusive_scan(st, b.scan_tmp, b.count.as<int>(), b.cum.as<int>(), n);
-  hipLaunchKernelGGL(nbr_stats_kernel, dim3(1), dim3(64 * kMaxSpecies), 0, st, n,
-                     b.cum.as<int>(), n > 0 ? nblk : 0, bcount, boff, counts,
-                     counts + kMaxSpecies + 2);
+  count_scan_stats(st, b, n, 0, nullptr, boff, counts);
   FLARE_HIP_CHECK(hipMemcpyAsync(stats_host, counts, kNlistStats * sizeof(int),
                                  hipMemcpyDeviceToHost, st));
   FLARE_HIP_CHECK(hipEventRecord(stats_ev, st));
@@ -1050,9 +1163,6 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
                        d_species, b.keys.as<int64_t>(), b.rel.as<double>(),
                        b.idx.as<int>(), b.nspec.as<int>(), b.center.as<int>(),
                        b.rev.as<int>());
-    hipLaunchKernelGGL(species_rows_kernel, dim3(nblk), dim3(kRowBlock), 0, st, n,
-                       d_species, boff, counts, b.row_of.as<int>(),
-                       b.atom_of_row.as<int>(), b.species_of_row.as<int>());
   }
   FLARE_HIP_CHECK(hipGetLastError());
 }
@@ -1254,10 +1364,7 @@ int64_t nlist_from_external(hipStream_t st, NListBuffers &b, const ExtList &e,
     hipLaunchKernelGGL(ext_species_kernel, dim3(nblk), dim3(kRowBlock), 0, st, n,
                        cspec, bcount);
   }
-  exclusive_scan(st, b.scan_tmp, b.count.as<int>(), b.cum.as<int>(), n);
-  hipLaunchKernelGGL(nbr_stats_kernel, dim3(1), dim3(64 * kMaxSpecies), 0, st, n,
-                     b.cum.as<int>(), n > 0 ? nblk : 0, bcount, boff, counts,
-                     counts + kMaxSpecies + 2);
+  count_scan_stats(st, b, n, n > 0 ? nblk : 0, bcount, boff, counts);
   int stats[kNlistStats];
   FLARE_HIP_CHECK(hipMemcpyAsync(stats, counts, kNlistStats * sizeof(int),
                                  hipMemcpyDeviceToHost, st));
