@@ -297,6 +297,20 @@ __device__ inline void b2_store_rows(const B2Dev &p, const T *Bs, double nrm,
   }
 }
 
+// Pair staging of b2_forward_mfma_kernel, in T values.  General layout: a
+// G row of MT*16 channels and a Y row of NT*16 harmonics per pair (each
+// padded by one).  Compact layout (n_max == NMAX and l_max fixed): the
+// NMAX radial values and the neighbor's species per pair (the channel mask
+// s = c / NMAX is applied when the operand is read) and an unpadded row of
+// the (L+1)^2 harmonics (bounds applied when read).
+template <int MT, int NT, int NMAX, int LFIX, bool EXACT>
+struct FwdStage {
+  static constexpr bool kCompact = EXACT && LFIX >= 0;
+  static constexpr int kGP = kCompact ? NMAX + 1 : MT * 16 + 1;
+  static constexpr int kYP = kCompact ? (LFIX + 1) * (LFIX + 1) : NT * 16 + 1;
+  static constexpr int kStage = kWave * (kGP + kYP);
+};
+
 // LFIX >= 0: l_max fixed at compile time; EXACT: n_max == NMAX.
 template <typename T, int MT, int NT, bool KAHAN, int NMAX, int LFIX, bool EXACT>
 __global__ void __launch_bounds__(64)
@@ -306,14 +320,16 @@ b2_forward_mfma_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
                        T *__restrict__ A_out, T *__restrict__ B_out,
                        double *__restrict__ norm2_out) {
   typedef typename Mma<T>::V V;
-  constexpr int KP = MT * 16 + 1;  // padded row of G
-  constexpr int YP = NT * 16 + 1;  // padded row of Y
+  typedef FwdStage<MT, NT, NMAX, LFIX, EXACT> St;
+  constexpr bool kCompact = St::kCompact;
+  constexpr int KP = St::kGP;  // staged row of G
+  constexpr int YP = St::kYP;  // staged row of Y
   extern __shared__ __align__(16) unsigned char smem[];
   // B (F values) reuses the pair staging rows, dead once A is formed
   T *Gs = reinterpret_cast<T *>(smem);
   T *Ys = Gs + kWave * KP;
   T *Bs = Gs;
-  T *Al = Gs + max(kWave * (KP + YP), p.F);  // nr x H
+  T *Al = Gs + max(St::kStage, p.F);  // nr x H
   const int i = blockIdx.x;
   const int lane = threadIdx.x;
   const int N = EXACT ? NMAX : p.N;
@@ -351,7 +367,11 @@ b2_forward_mfma_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
         T g[NMAX];
         radial_basis<T, NMAX, EXACT>(r, (T)p.cut[si * S + sj], (T)p.r1, N,
                                      p.cutoff_kind, g, (T *)nullptr);
-        if (EXACT) {
+        if (kCompact) {
+#pragma unroll
+          for (int n = 0; n < NMAX; n++) gr[n] = g[n];
+          gr[NMAX] = (T)sj;
+        } else if (EXACT) {
           // channel c = s NMAX + n holds g_n for the neighbor's species only
 #pragma unroll
           for (int c = 0; c < MT * 16; c++)
@@ -367,10 +387,16 @@ b2_forward_mfma_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
         ylm_visit<T, false, LFIX>((T)rv[1] * rinv, (T)rv[2] * rinv,
                                   (T)rv[3] * rinv, rinv, p.L, ynorm,
                                   [&](int idx, T y, T, T, T) { yr[idx] = y; });
-        if (LFIX >= 0) {
+        if (LFIX >= 0 && !kCompact) {
 #pragma unroll
           for (int c = (LFIX + 1) * (LFIX + 1); c < NT * 16; c++) yr[c] = 0;
         }
+      } else if (kCompact) {
+        // no species matches -1: the G operand is zero; Y zero keeps the
+        // products finite
+        gr[NMAX] = T(-1);
+#pragma unroll
+        for (int c = 0; c < YP; c++) yr[c] = 0;
       } else {
 #pragma unroll
         for (int c = 0; c < MT * 16; c++) gr[c] = 0;
@@ -379,37 +405,50 @@ b2_forward_mfma_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
       }
     }
     __syncthreads();
-    // K steps of 4 pairs; FP32 sums two steps (8 pairs) in the matrix core
-    // and Kahan-adds them into the running (sum, compensation) pair
+    // K steps of 4 pairs.  FP32 sums KG steps (4 KG pairs) in the matrix
+    // core and Kahan-adds them into the running (sum, compensation) pair.
+    constexpr int KG = KAHAN ? 4 : 2;
     const int steps = (cnt + 3) >> 2;
-    for (int st = 0; st < steps; st += 2) {
-      const int kk = 4 * st + krow;
-      const bool two = st + 1 < steps;  // uniform
-      T a[MT], b[NT], a2[MT], b2[NT];
+    for (int st = 0; st < steps; st += KG) {
+      T a[KG][MT], b[KG][NT];
 #pragma unroll
-      for (int m = 0; m < MT; m++) {
-        a[m] = Gs[kk * KP + m * 16 + col];
-        a2[m] = two ? Gs[(kk + 4) * KP + m * 16 + col] : T(0);
-      }
+      for (int u = 0; u < KG; u++) {
+        const int kk = 4 * (st + u) + krow;
+        const bool on = st + u < steps;  // uniform
+        if (kCompact) {
+          const T sk = on ? Gs[kk * KP + NMAX] : T(-1);
 #pragma unroll
-      for (int n = 0; n < NT; n++) {
-        b[n] = Ys[kk * YP + n * 16 + col];
-        b2[n] = two ? Ys[(kk + 4) * YP + n * 16 + col] : T(0);
+          for (int m = 0; m < MT; m++) {
+            const int c = m * 16 + col;
+            a[u][m] = sk == (T)(c / NMAX) ? Gs[kk * KP + (c % NMAX)] : T(0);
+          }
+#pragma unroll
+          for (int n = 0; n < NT; n++) {
+            const int lm = n * 16 + col;
+            b[u][n] = (on && lm < YP) ? Ys[kk * YP + lm] : T(0);
+          }
+        } else {
+#pragma unroll
+          for (int m = 0; m < MT; m++) a[u][m] = on ? Gs[kk * KP + m * 16 + col] : T(0);
+#pragma unroll
+          for (int n = 0; n < NT; n++) b[u][n] = on ? Ys[kk * YP + n * 16 + col] : T(0);
+        }
       }
 #pragma unroll
       for (int m = 0; m < MT; m++)
 #pragma unroll
         for (int n = 0; n < NT; n++) {
           if (KAHAN) {
-            V part = Mma<T>::run(a[m], b[n], V{0, 0, 0, 0});
-            part = Mma<T>::run(a2[m], b2[n], part);
+            V part = Mma<T>::run(a[0][m], b[0][n], V{0, 0, 0, 0});
+#pragma unroll
+            for (int u = 1; u < KG; u++) part = Mma<T>::run(a[u][m], b[u][n], part);
             V y = part - cmp[m][n];
             V t = acc[m][n] + y;
             cmp[m][n] = (t - acc[m][n]) - y;
             acc[m][n] = t;
           } else {
-            acc[m][n] = Mma<T>::run(a[m], b[n], acc[m][n]);
-            acc[m][n] = Mma<T>::run(a2[m], b2[n], acc[m][n]);
+#pragma unroll
+            for (int u = 0; u < KG; u++) acc[m][n] = Mma<T>::run(a[u][m], b[u][n], acc[m][n]);
           }
         }
     }
@@ -441,7 +480,11 @@ template <typename T, bool KAHAN, int MT, int NT, int NMAX, int LFIX, bool EXACT
 static void launch_mfma_k(hipStream_t st, const B2Dev &p, const NListDev &nl,
                           const int *species, const int *row_of, const T *ynorm,
                           const FwdOut &out, void *A_out, void *B_out,
-                          double *norm2_out, size_t lds) {
+                          double *norm2_out) {
+  typedef FwdStage<MT, NT, NMAX, LFIX, EXACT> St;
+  const size_t lds = sizeof(T) * (std::max<size_t>((size_t)St::kStage, (size_t)p.F) +
+                                  (size_t)p.nr * p.H);
+  FLARE_REQUIRE(lds <= 64 * 1024, "descriptor settings too large");
   hipLaunchKernelGGL((b2_forward_mfma_kernel<T, MT, NT, KAHAN, NMAX, LFIX, EXACT>),
                      dim3(nl.n), dim3(kWave), lds, st, p, nl, species, row_of,
                      ynorm, out, (T *)A_out, (T *)B_out, norm2_out);
@@ -452,27 +495,23 @@ static void launch_mfma_t(hipStream_t st, const B2Dev &p, const NListDev &nl,
                           const int *species, const int *row_of, const T *ynorm,
                           const FwdOut &out, void *A_out, void *B_out,
                           double *norm2_out) {
-  size_t lds = sizeof(T) * (std::max<size_t>((size_t)kWave * (MT * 16 + 1 + NT * 16 + 1),
-                                             (size_t)p.F) +
-                            (size_t)p.nr * p.H);
-  FLARE_REQUIRE(lds <= 64 * 1024, "descriptor settings too large");
   // compile-time (n_max, l_max) variants of the common settings
   if (p.N == 8 && p.L == 3 && NT == 1) {
     launch_mfma_k<T, KAHAN, MT, NT, 8, 3, true>(st, p, nl, species, row_of, ynorm,
-                                                out, A_out, B_out, norm2_out, lds);
+                                                out, A_out, B_out, norm2_out);
   } else if (p.N == 8 && p.L == 4 && NT == 2) {
     launch_mfma_k<T, KAHAN, MT, NT, 8, 4, true>(st, p, nl, species, row_of, ynorm,
-                                                out, A_out, B_out, norm2_out, lds);
+                                                out, A_out, B_out, norm2_out);
   } else if (p.N == 8 && p.L == 6 && NT == 4) {
     launch_mfma_k<T, KAHAN, MT, NT, 8, 6, true>(st, p, nl, species, row_of, ynorm,
-                                                out, A_out, B_out, norm2_out, lds);
+                                                out, A_out, B_out, norm2_out);
   } else if (p.N <= 8) {
     launch_mfma_k<T, KAHAN, MT, NT, 8, -1, false>(st, p, nl, species, row_of, ynorm,
-                                                  out, A_out, B_out, norm2_out, lds);
+                                                  out, A_out, B_out, norm2_out);
   } else {
     launch_mfma_k<T, KAHAN, MT, NT, kMaxN, -1, false>(st, p, nl, species, row_of,
                                                       ynorm, out, A_out, B_out,
-                                                      norm2_out, lds);
+                                                      norm2_out);
   }
 }
 
