@@ -384,9 +384,9 @@ __device__ inline void adjoint_pair_fixed(const B2Dev &p, T r, T x, T y, T z,
 //      w = (beta - E B/|B|) / |B|    (lammps_descriptor.cpp:262-270).
 // Environments with |B|^2 < 1e-8 carry no energy (pair_flare.cpp:135).
 // FP32 instances up to l_max 4 are held to 128 registers (four wavefronts
-// per SIMD): the latency of the prologue's loads is then hidden by more
-// resident atoms, which outweighs the few spilled values (88.7 vs 93.5 us
-// per launch at 32,000 atoms, B2 {2,8,4}).  l_max 6 would spill ~50.
+// per SIMD, 124 used at l_max 4 without spills): the latency of the
+// prologue's loads is then hidden by more resident atoms (79 vs 94 us per
+// launch at 32,000 atoms, B2 {2,8,4}).  l_max 6 would spill ~50.
 template <typename T, int MODE, int NMAX, int WM, int LFIX, bool EXACT>
 __global__ void __launch_bounds__(64)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && LFIX <= 4 ? 4 : 1)))
@@ -409,12 +409,6 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
   const int row = row_of[i];
   constexpr bool kFixed = MODE == 1 && LFIX >= 0 && EXACT && NMAX == 8;
   const int b = nl.cum[i], e = nl.cum[i + 1];
-  double4 rv_next = {0, 0, 0, 0};
-  int sj_next = 0;
-  if (MODE == 1 && b + lane < e) {
-    rv_next = reinterpret_cast<const double4 *>(nl.rel)[b + lane];
-    sj_next = nl.nspec[b + lane];
-  }
   // Every global read of the prologue is issued in batches before its
   // values are stored, so that a wavefront waits for memory a few times
   // rather than once per element.  Waits on loads are in issue order, so in
@@ -450,15 +444,19 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
     const int m = lane - l * l - l;
     ynv = ynorm[l * (l + 1) / 2 + (m < 0 ? -m : m)];
   }
-  // the A row in one unrolled batch of up to four 16-byte loads per lane
+  // the A row in one unrolled batch of KA 16-byte loads per lane, KA sized
+  // for 16 channels (registers are what bounds this kernel's occupancy)
+  constexpr int KA = kFixed ? (16 * (LFIX + 1) * (LFIX + 1) * (int)sizeof(T) + 16 * kWave - 1) /
+                                  (16 * kWave)
+                            : 1;
   const int nva = W * (int)sizeof(T) / 16;
-  const bool a_batch = kFixed && ((W * (int)sizeof(T)) & 15) == 0 && nva <= 4 * kWave &&
+  const bool a_batch = kFixed && ((W * (int)sizeof(T)) & 15) == 0 && nva <= KA * kWave &&
                        ((reinterpret_cast<uintptr_t>(Ai) | reinterpret_cast<uintptr_t>(A)) & 15) == 0;
-  uint4 ta[4];
+  uint4 ta[KA];
   if (kFixed && a_batch) {
     const uint4 *s4 = reinterpret_cast<const uint4 *>(Ai);
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < KA; j++) {
       const int o = lane + j * kWave;
       ta[j] = uint4{0, 0, 0, 0};
       if (o < nva) ta[j] = s4[o];
@@ -475,7 +473,7 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
   if (kFixed && a_batch) {
     uint4 *d4 = reinterpret_cast<uint4 *>(A);
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
+    for (int j = 0; j < KA; j++) {
       const int o = lane + j * kWave;
       if (o < nva) d4[o] = ta[j];
     }
@@ -589,6 +587,15 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
     }
   }
   __syncthreads();
+  // the first chunk's pair data, loaded here so that the u contraction's
+  // matrix-core work covers its latency (loaded with the prologue's batch it
+  // would hold nine registers through it, past the four-wavefront budget)
+  double4 rv_next = {0, 0, 0, 0};
+  int sj_next = 0;
+  if (MODE == 1 && b + lane < e) {
+    rv_next = reinterpret_cast<const double4 *>(nl.rel)[b + lane];
+    sj_next = nl.nspec[b + lane];
+  }
   bool done = false;
   if constexpr (kUFixed) {
     if (p.nr == 16) {
@@ -655,7 +662,7 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
       }
     }
   } else {
-    // the first chunk's pair data was loaded before the prologue
+    // the first chunk's pair data was loaded before the u contraction
     for (int k = b + lane; k < e; k += kWave) {
       double4 rv = rv_next;
       int sj = sj_next;
