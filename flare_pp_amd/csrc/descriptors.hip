@@ -311,6 +311,12 @@ struct FwdStage {
   static constexpr int kStage = kWave * (kGP + kYP);
 };
 
+// LDS values of b2_forward_mfma_kernel: the staging rows, with B (F) and A
+// (nr x H) placed inside them when both fit, else after them.
+__host__ __device__ inline size_t fwd_lds_values(int stage, int F, int A) {
+  return F + A <= stage ? (size_t)stage : (size_t)(stage > F ? stage : F) + A;
+}
+
 // LFIX >= 0: l_max fixed at compile time; EXACT: n_max == NMAX.
 template <typename T, int MT, int NT, bool KAHAN, int NMAX, int LFIX, bool EXACT>
 __global__ void __launch_bounds__(64)
@@ -325,11 +331,12 @@ b2_forward_mfma_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
   constexpr int KP = St::kGP;  // staged row of G
   constexpr int YP = St::kYP;  // staged row of Y
   extern __shared__ __align__(16) unsigned char smem[];
-  // B (F values) reuses the pair staging rows, dead once A is formed
+  // B (F values) and A (nr x H) reuse the pair staging rows, dead once A is
+  // formed, when they fit there (fwd_lds_values)
   T *Gs = reinterpret_cast<T *>(smem);
   T *Ys = Gs + kWave * KP;
   T *Bs = Gs;
-  T *Al = Gs + max(St::kStage, p.F);  // nr x H
+  T *Al = Gs + (p.F + p.nr * p.H <= St::kStage ? p.F : max(St::kStage, p.F));
   const int i = blockIdx.x;
   const int lane = threadIdx.x;
   const int N = EXACT ? NMAX : p.N;
@@ -482,8 +489,7 @@ static void launch_mfma_k(hipStream_t st, const B2Dev &p, const NListDev &nl,
                           const FwdOut &out, void *A_out, void *B_out,
                           double *norm2_out) {
   typedef FwdStage<MT, NT, NMAX, LFIX, EXACT> St;
-  const size_t lds = sizeof(T) * (std::max<size_t>((size_t)St::kStage, (size_t)p.F) +
-                                  (size_t)p.nr * p.H);
+  const size_t lds = sizeof(T) * fwd_lds_values(St::kStage, p.F, p.nr * p.H);
   FLARE_REQUIRE(lds <= 64 * 1024, "descriptor settings too large");
   hipLaunchKernelGGL((b2_forward_mfma_kernel<T, MT, NT, KAHAN, NMAX, LFIX, EXACT>),
                      dim3(nl.n), dim3(kWave), lds, st, p, nl, species, row_of,
