@@ -246,11 +246,40 @@ def cpu_baseline(name, c, threads):
         times.append(time.perf_counter() - t0)
     med = float(np.median(times))
     n = len(c["species"])
+    # serial figure (pair_flare runs one thread per MPI rank): up to three
+    # full evaluations on one thread within about 12 s
+    t1 = []
+    t_end = time.time() + 12.0
+    while len(t1) < 3 and (time.time() < t_end or not t1):
+        t0 = time.perf_counter()
+        pyoracle.pair_compute(*args, packed=False, nthreads=1)
+        t1.append(time.perf_counter() - t0)
     return {"value": n / med, "unit": "atoms/s", "cores": nthreads,
             "kind": "port",
             "sample": f"full {n:,}-atom force evaluation (E, F, virial), "
                       f"median of {len(times)} after 1 warm-up, neighbor list "
-                      f"prebuilt, FP64, OpenMP over atoms"}
+                      f"prebuilt, FP64, OpenMP over atoms",
+            "single_thread": {"value": n / float(np.median(t1)), "unit": "atoms/s",
+                              "sample": f"median of {len(t1)} full evaluations on 1 thread"},
+            "host": host_cpu()}
+
+
+def host_cpu():
+    """CPU model, logical CPUs and the cores this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable}
 
 
 def main():
