@@ -785,8 +785,9 @@ nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
   }
 }
 
-// Padded sorted key rows -> final CSR arrays, with the reverse-pair index
-// found by binary search in the neighbor's sorted row.  The rel vector of
+// Padded sorted key rows -> final CSR arrays (rel, idx, nspec, center; the
+// keys are not needed past this point), with the reverse-pair index found
+// by binary search in the neighbor's sorted row.  The rel vector of
 // each entry is recomputed from its key by the enumeration's own test
 // (image_test on the same wrapped positions and shift: the same bits).  One
 // wavefront per atom; does nothing when *overflow is set.
@@ -796,8 +797,7 @@ nbr_compact_kernel(int n, WrapMats m, const int *__restrict__ overflow,
                    const int *__restrict__ count,
                    const int64_t *__restrict__ keys_pad,
                    const double *__restrict__ wrapped,
-                   const int *__restrict__ species,
-                   int64_t *__restrict__ keys_out, double *__restrict__ rel_out,
+                   const int *__restrict__ species, double *__restrict__ rel_out,
                    int *__restrict__ idx_out, int *__restrict__ nspec_out,
                    int *__restrict__ center_out, int *__restrict__ rev_out) {
   const int lane = threadIdx.x & (kWave - 1);
@@ -839,7 +839,6 @@ nbr_compact_kernel(int n, WrapMats m, const int *__restrict__ overflow,
         rev_out[dst] = -1;
       }
     }
-    keys_out[dst] = k;
     reinterpret_cast<double4 *>(rel_out)[dst] = double4{o[0], o[1], o[2], o[3]};
     idx_out[dst] = j;
     nspec_out[dst] = species[j];
@@ -1075,7 +1074,6 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
   b.count.ensure((size_t)(n + 1) * sizeof(int));
   b.cum.ensure((size_t)(n + 1) * sizeof(int));
   b.keys_pad.ensure(cap * sizeof(int64_t));
-  b.keys.ensure(cap * sizeof(int64_t));
   b.rel.ensure(cap * 4 * sizeof(double));
   b.idx.ensure(cap * sizeof(int));
   b.nspec.ensure(cap * sizeof(int));
@@ -1160,9 +1158,8 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
     hipLaunchKernelGGL(nbr_compact_kernel, dim3(gw), dim3(256), 0, st, n, m,
                        overflow, b.cum.as<int>(), b.count.as<int>(),
                        b.keys_pad.as<int64_t>(), b.wrapped.as<double>(),
-                       d_species, b.keys.as<int64_t>(), b.rel.as<double>(),
-                       b.idx.as<int>(), b.nspec.as<int>(), b.center.as<int>(),
-                       b.rev.as<int>());
+                       d_species, b.rel.as<double>(), b.idx.as<int>(),
+                       b.nspec.as<int>(), b.center.as<int>(), b.rev.as<int>());
   }
   FLARE_HIP_CHECK(hipGetLastError());
 }

@@ -126,3 +126,25 @@ def test_triclinic_bins(orc):
     cell = np.array([[24.0, 0.0, 0.0], [5.0, 22.0, 0.0], [-3.0, 4.0, 23.0]])
     cell, species, pos = random_box(23, 0.05, None, cell)
     compare(orc, cell, species, pos, 5.0)
+
+
+def test_prefilter_at_the_cutoff(orc):
+    # pairs at distances within 1e-15 relative of the cutoff, on both sides,
+    # in a cell whose coordinates reach ~1,000 A (FP32 positions there are
+    # rounded by ~6e-5 A): the FP32 prefilter must keep every pair the exact
+    # FP64 test keeps, so the list equals the oracle's bit for bit
+    rng = np.random.default_rng(31)
+    L, rc = 1000.0, 5.0
+    cell = np.diag([L, 40.0, 37.0])
+    pos = []
+    for t in range(150):
+        c = np.array([rng.uniform(0, L), rng.uniform(0, 40.0), rng.uniform(0, 37.0)])
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        for f in (1 - 2e-16, 1 - 1e-15, 1.0, 1 + 1e-15, 1 - 1e-7, 1 + 1e-7):
+            pos.append(c)
+            pos.append(c + rc * f * u)
+    pos = np.array(pos)
+    species = rng.integers(0, 2, len(pos))
+    s, o = compare(orc, cell, species, pos, rc)
+    assert s.neighbor_count.sum() > 0
