@@ -589,7 +589,8 @@ constexpr int kBinWaveBytes = kNbrCap * 8 + kNbrCap * 4 + kBinCap * 2;
 constexpr int kBinStaged = kBinOffWave + 4 * kBinWaveBytes;
 constexpr int kBinFallback = 4 * kWave * 4 + 4 * kNbrCap * 8;
 constexpr int kBinLds = kBinStaged > kBinFallback ? kBinStaged : kBinFallback;
-static_assert(kBinOffWave % 16 == 0 && kBinWaveBytes % 16 == 0, "LDS alignment");
+static_assert(kBinOffWave % 16 == 0 && kBinWaveBytes % 16 == 0 && (kNbrCap * 8) % 16 == 0,
+              "LDS alignment (the 32-bit key rows are read as 16-byte vectors)");
 
 // The same rows as nbr_enum_sort_kernel<false> (at least three bins along
 // every axis), one workgroup per home bin.  The candidates of the 27
@@ -747,22 +748,31 @@ nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
       count[i] = found;
       if (found > kNbrCap) atomicMax(stat, found);  // rare: no contention
     }
+    // pad the 32-bit keys to a multiple of four with a key no entry is below
+    // (j < 2^26, so every key is at most 0xffffffff)
+    if (lane < 3 && found + lane < ((found + 3) & ~3) && found + lane < kNbrCap)
+      skey32[found + lane] = 0xffffffffu;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     if (found <= kNbrCap) {
-      // rank = number of smaller keys (keys are distinct), LDS broadcasts
+      // rank = number of smaller keys (keys are distinct), four keys per
+      // LDS broadcast read
       int r0 = 0, r1 = 0;
       if (key32) {
         const unsigned k0 = lane < found ? skey32[lane] : 0xffffffffu;
         const unsigned k1 = lane + kWave < found ? skey32[lane + kWave] : 0xffffffffu;
+        const uint4 *sk4 = reinterpret_cast<const uint4 *>(skey32);
         if (found <= kWave) {
-          for (int q = 0; q < found; q++) r0 += skey32[q] < k0;
+          for (int q = 0; q < found; q += 4) {
+            const uint4 kk = sk4[q >> 2];
+            r0 += (kk.x < k0) + (kk.y < k0) + (kk.z < k0) + (kk.w < k0);
+          }
         } else {
-          for (int q = 0; q < found; q++) {
-            const unsigned kq = skey32[q];
-            r0 += kq < k0;
-            r1 += kq < k1;
+          for (int q = 0; q < found; q += 4) {
+            const uint4 kk = sk4[q >> 2];
+            r0 += (kk.x < k0) + (kk.y < k0) + (kk.z < k0) + (kk.w < k0);
+            r1 += (kk.x < k1) + (kk.y < k1) + (kk.z < k1) + (kk.w < k1);
           }
         }
       } else {

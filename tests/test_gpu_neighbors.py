@@ -148,3 +148,13 @@ def test_prefilter_at_the_cutoff(orc):
     species = rng.integers(0, 2, len(pos))
     s, o = compare(orc, cell, species, pos, rc)
     assert s.neighbor_count.sum() > 0
+
+
+@pytest.mark.parametrize("shape", [(3, 3, 3), (3, 7, 4)])
+def test_three_bins_per_axis(orc, shape):
+    # exactly three bins along an axis: the bins on both sides of a home bin
+    # are the other two, reached with opposite image shifts
+    rc = 5.0
+    cell = np.diag([3 * rc + 0.4 + b for b in (0.0, 0.1, 0.2)]) * np.array(shape) / 3.0
+    cell, species, pos = random_box(40 + shape[1], 0.04, None, cell)
+    compare(orc, cell, species, pos, rc)
