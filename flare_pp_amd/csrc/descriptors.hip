@@ -302,13 +302,17 @@ __device__ inline void b2_store_rows(const B2Dev &p, const T *Bs, double nrm,
 // padded by one).  Compact layout (n_max == NMAX and l_max fixed): the
 // NMAX radial values and the neighbor's species per pair (the channel mask
 // s = c / NMAX is applied when the operand is read) and an unpadded row of
-// the (L+1)^2 harmonics (bounds applied when read).
+// the (L+1)^2 harmonics (bounds applied when read), for chunks of 48 pairs:
+// most atoms of dense solids and liquids at rc 5-5.5 A have 30-45 pairs, and
+// the smaller LDS footprint admits a third more wavefronts per CU.
 template <int MT, int NT, int NMAX, int LFIX, bool EXACT>
 struct FwdStage {
   static constexpr bool kCompact = EXACT && LFIX >= 0;
+  static constexpr int kRows = kCompact ? 48 : kWave;  // pairs per chunk
   static constexpr int kGP = kCompact ? NMAX + 1 : MT * 16 + 1;
   static constexpr int kYP = kCompact ? (LFIX + 1) * (LFIX + 1) : NT * 16 + 1;
-  static constexpr int kStage = kWave * (kGP + kYP);
+  static constexpr int kStage = kRows * (kGP + kYP);
+  static_assert(kRows % 4 == 0 && kRows <= kWave, "whole K steps, one pair per lane");
 };
 
 // LDS values of b2_forward_mfma_kernel: the staging rows, with B (F) and A
@@ -330,11 +334,12 @@ b2_forward_mfma_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
   constexpr bool kCompact = St::kCompact;
   constexpr int KP = St::kGP;  // staged row of G
   constexpr int YP = St::kYP;  // staged row of Y
+  constexpr int CH = St::kRows;  // pairs per chunk
   extern __shared__ __align__(16) unsigned char smem[];
   // B (F values) and A (nr x H) reuse the pair staging rows, dead once A is
   // formed, when they fit there (fwd_lds_values)
   T *Gs = reinterpret_cast<T *>(smem);
-  T *Ys = Gs + kWave * KP;
+  T *Ys = Gs + CH * KP;
   T *Bs = Gs;
   T *Al = Gs + (p.F + p.nr * p.H <= St::kStage ? p.F : max(St::kStage, p.F));
   const int i = blockIdx.x;
@@ -353,10 +358,10 @@ b2_forward_mfma_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
   const int si = species[i];
   const int beg = nl.cum[i], end = nl.cum[i + 1];
   const int col = lane & 15, krow = lane >> 4;
-  for (int c0 = beg; c0 < end; c0 += kWave) {
-    const int cnt = min(kWave, end - c0);
+  for (int c0 = beg; c0 < end; c0 += CH) {
+    const int cnt = min(CH, end - c0);
     __syncthreads();
-    {
+    if (lane < CH) {
       T *gr = Gs + lane * KP;
       T *yr = Ys + lane * YP;
       bool ok = false;
