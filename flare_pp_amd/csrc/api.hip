@@ -462,13 +462,16 @@ static Forward run_forward(flare_ctx *c, const flare::B2Dev &p,
 
 // Per-pair forces f_ik (P x 3) of every centre of nl, shared by the mapped
 // and GP paths.
-static double *run_pair_forces(flare_ctx *c, const flare::B2Dev &p,
-                               const flare_nlist *nl, const int *d_species,
-                               int prec, int mode, const Forward &fw,
-                               const flare::WInput &wi, const std::string &tag) {
+static flare::PairForces run_pair_forces(flare_ctx *c, const flare::B2Dev &p,
+                                         const flare_nlist *nl, const int *d_species,
+                                         int prec, int mode, const Forward &fw,
+                                         const flare::WInput &wi, const std::string &tag) {
   const int n = nl->n;
   const int64_t P = nl->b.P;
-  double *pf = c->buf_t<double>(tag + ".pf", (size_t)std::max<int64_t>(P, 1) * 3);
+  flare::PairForces pf;
+  pf.f32 = prec != flare::kPrecF64;
+  pf.p = c->buf(tag + (pf.f32 ? ".pf32" : ".pf"),
+                (size_t)std::max<int64_t>(P, 1) * 3 * (pf.f32 ? sizeof(float) : sizeof(double)));
   if (mode == 0) {
     // plain form: dA/dr materialised for a range of atoms at a time, at most
     // c->plain_chunk_bytes of derivatives (one range when everything fits;
@@ -522,7 +525,7 @@ static double *run_pair_forces(flare_ctx *c, const flare::B2Dev &p,
 // Forces (gather over own and reverse pairs), per-centre virial rows, the
 // energy and virial sums.  On a caller's list (no reverse pairs) `forces`
 // receives each centre's own-pair sums.
-static void run_gather_reduce(flare_ctx *c, const flare_nlist *nl, const double *pf,
+static void run_gather_reduce(flare_ctx *c, const flare_nlist *nl, const flare::PairForces &pf,
                               const flare::WInput &wi, double *forces, double *energy,
                               double *virial6, double *stress6,
                               const std::string &tag) {
@@ -541,7 +544,7 @@ static void run_forces(flare_ctx *c, const flare::B2Dev &p,
                        int mode, const Forward &fw, const flare::WInput &wi,
                        double *forces, double *energy, double *virial6,
                        double *stress6, const std::string &tag) {
-  const double *pf = run_pair_forces(c, p, nl, d_species, prec, mode, fw, wi, tag);
+  const flare::PairForces pf = run_pair_forces(c, p, nl, d_species, prec, mode, fw, wi, tag);
   run_gather_reduce(c, nl, pf, wi, forces, energy, virial6, stress6, tag);
 }
 
@@ -1598,8 +1601,8 @@ int flare_pair_compute_neigh(flare_ctx *ctx, flare_pair *pr, int n_all, int inum
     nl->pending = false;
     nl->built = true;
     PairEval ev = pair_eval(ctx, pr, nl, cspec, form, nullptr, 2);
-    const double *pf = run_pair_forces(ctx, ev.p, nl, cspec, ev.prec, ev.mode, ev.fw,
-                                       ev.wi, "pair");
+    const flare::PairForces pf = run_pair_forces(ctx, ev.p, nl, cspec, ev.prec, ev.mode,
+                                                 ev.fw, ev.wi, "pair");
     double *own = ctx->buf_t<double>("ext.own", (size_t)inum * 3);
     run_gather_reduce(ctx, nl, pf, ev.wi, own, d_energy, d_virial, nullptr, "pair");
     // F_j -= f_ij over the pairs whose neighbor is j (ghosts included), then

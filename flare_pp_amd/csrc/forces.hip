@@ -394,7 +394,7 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
              const int *__restrict__ row_of, const T *__restrict__ ynorm,
              const T *__restrict__ A_in, WSource<T> src,
              const double *__restrict__ dA, int64_t dA_p0, int a0,
-             double *__restrict__ pair_f) {
+             T *__restrict__ pair_f) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int i = a0 + blockIdx.x;
   const int lane = threadIdx.x;
@@ -671,7 +671,7 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
         sj = nl.nspec[k];
       }
       const double rva[4] = {rv.x, rv.y, rv.z, rv.w};
-      double f[3] = {0, 0, 0};
+      T f[3] = {0, 0, 0};
       if (pair_inside(p, si, sj, rva)) {
         T f0, f1, f2;
         if constexpr (kFixed)
@@ -696,9 +696,10 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
 // 16 lanes per atom, four atoms per wavefront.  Also the per-atom virial
 // rows V_a[b][c] = -sum_k r_ak,b f_ak,c over the atom's own pairs (the
 // pair_flare tally of (x_i - x_j) f_ij, pair_flare.cpp:160-165).
+template <typename PF>
 __global__ void __launch_bounds__(256)
 gather_kernel(int n, const int *__restrict__ cum, const int *__restrict__ rev,
-              const double *__restrict__ rel, const double *__restrict__ pf,
+              const double *__restrict__ rel, const PF *__restrict__ pf,
               double *__restrict__ forces, const int *__restrict__ row_of,
               const double *__restrict__ e_row, double *__restrict__ part) {
   __shared__ double sh[16][10];
@@ -727,11 +728,11 @@ gather_kernel(int n, const int *__restrict__ cum, const int *__restrict__ rev,
         }
         rv[j] = double4{0, 0, 0, 0};
         if (k < e) {
-          for (int c = 0; c < 3; c++) g[j][c] = pf[3 * (int64_t)k + c];
+          for (int c = 0; c < 3; c++) g[j][c] = (double)pf[3 * (int64_t)k + c];
           rv[j] = reinterpret_cast<const double4 *>(rel)[k];
         }
         if (r[j] >= 0)
-          for (int c = 0; c < 3; c++) h[j][c] = pf[3 * (int64_t)r[j] + c];
+          for (int c = 0; c < 3; c++) h[j][c] = (double)pf[3 * (int64_t)r[j] + c];
       }
 #pragma unroll
       for (int j = 0; j < 4; j++) {
@@ -851,7 +852,7 @@ static void launch_forces_t(hipStream_t st, const B2Dev &p, const NListDev &nl,
                             const int *species, const int *row_of,
                             const T *ynorm, const void *A, const WSource<T> &src,
                             const double *dA, int64_t dA_p0, int a0, int a1,
-                            double *pair_f) {
+                            T *pair_f) {
   size_t lds = sizeof(T) * (2 * (size_t)p.W + 2 * (size_t)p.F + (size_t)p.H) +
                sizeof(int) * (size_t)p.nr * p.nr;
   FLARE_REQUIRE(lds <= 64 * 1024, "descriptor settings too large for the force kernel");
@@ -879,7 +880,7 @@ static void launch_forces_w(hipStream_t st, const B2Dev &p, const NListDev &nl,
                             const int *species, const int *row_of,
                             const T *ynorm, const void *A, const WInput &wi,
                             const double *dA, int64_t dA_p0, int a0, int a1,
-                            double *pair_f) {
+                            T *pair_f) {
   WSource<T> src{wi.normalized, (const T *)wi.X, (const T *)wi.Z,
                  (const T *)wi.beta1, (const T *)wi.w,
                  (const __bf16 *)wi.xh, (const __bf16 *)wi.xl, wi.kp,
@@ -902,19 +903,21 @@ void launch_forces(hipStream_t st, int prec, int mode, const B2Dev &p,
                    const NListDev &nl, const int *species, const int *row_of,
                    const double *ynorm, const float *ynorm_f, const void *A,
                    const WInput &wi, const double *dA, int64_t dA_p0,
-                   int a0, int a1, double *pair_f) {
+                   int a0, int a1, const PairForces &pair_f) {
   if (a1 <= a0) return;
+  FLARE_REQUIRE(pair_f.f32 == (prec != kPrecF64), "pair forces in the form's precision");
   if (prec == kPrecF64) {
+    double *pf = (double *)pair_f.p;
     if (mode == 0)
       launch_forces_w<double, 0>(st, p, nl, species, row_of, ynorm, A, wi, dA,
-                                 dA_p0, a0, a1, pair_f);
+                                 dA_p0, a0, a1, pf);
     else
       launch_forces_w<double, 1>(st, p, nl, species, row_of, ynorm, A, wi, dA,
-                                 dA_p0, a0, a1, pair_f);
+                                 dA_p0, a0, a1, pf);
   } else {
     FLARE_REQUIRE(mode == 1, "the plain (materialised) form runs in FP64 only");
     launch_forces_w<float, 1>(st, p, nl, species, row_of, ynorm_f, A, wi, dA,
-                              dA_p0, a0, a1, pair_f);
+                              dA_p0, a0, a1, (float *)pair_f.p);
   }
   FLARE_HIP_CHECK(hipGetLastError());
 }
@@ -928,7 +931,7 @@ __global__ void zero_results_kernel(double *energy, double *virial6, double *str
   if (stress6 && t < 6) stress6[t] = 0.0;
 }
 
-void launch_gather(hipStream_t st, const NListDev &nl, const double *pair_f,
+void launch_gather(hipStream_t st, const NListDev &nl, const PairForces &pair_f,
                    double *forces, const int *row_of, const double *e_row,
                    double *part, double inv_volume, double *energy, double *virial6,
                    double *stress6) {
@@ -936,8 +939,14 @@ void launch_gather(hipStream_t st, const NListDev &nl, const double *pair_f,
     hipLaunchKernelGGL(zero_results_kernel, dim3(1), dim3(64), 0, st, energy, virial6,
                        stress6);
   } else {
-    hipLaunchKernelGGL(gather_kernel, dim3(gather_parts(nl.n)), dim3(256), 0, st, nl.n,
-                       nl.cum, nl.rev, nl.rel, pair_f, forces, row_of, e_row, part);
+    if (pair_f.f32)
+      hipLaunchKernelGGL(gather_kernel<float>, dim3(gather_parts(nl.n)), dim3(256), 0, st,
+                         nl.n, nl.cum, nl.rev, nl.rel, (const float *)pair_f.p, forces,
+                         row_of, e_row, part);
+    else
+      hipLaunchKernelGGL(gather_kernel<double>, dim3(gather_parts(nl.n)), dim3(256), 0, st,
+                         nl.n, nl.cum, nl.rev, nl.rel, (const double *)pair_f.p, forces,
+                         row_of, e_row, part);
     hipLaunchKernelGGL(sums_cols_kernel, dim3(10), dim3(256), 0, st,
                        gather_parts(nl.n), part, inv_volume, energy, virial6, stress6);
   }

@@ -45,6 +45,14 @@ struct DevBuf {
 };
 
 // Cell matrices of the wrap (structure.cpp:120-133) and the cell-list grid.
+// Per-pair forces f_ik (P x 3): FP64, or FP32 in the FP32 and mixed forms,
+// whose pair forces are computed in FP32 (the values are the same either
+// way; FP32 halves their traffic).
+struct PairForces {
+  void *p = nullptr;
+  bool f32 = false;
+};
+
 struct WrapHost {
   double ct[9], cdi[9], cd[9], cti[9], cell[9];
   int nb[3];
@@ -112,7 +120,7 @@ void index_by_target(hipStream_t st, DevBuf &work, DevBuf &cnt_buf, DevBuf &scan
 // forces[ilist[c]] += own[c] (c < inum); eatom[ilist[c]] += E_row[row_of[c]]
 // when eatom is non-null.
 void ext_assemble(hipStream_t st, int n_all, int inum, const int *ilist,
-                  const int *in_cum, const int *in_pairs, const double *pf,
+                  const int *in_cum, const int *in_pairs, const PairForces &pf,
                   const double *own, const int *row_of, const double *E_row,
                   double *forces, double *eatom);
 
@@ -157,7 +165,7 @@ void launch_forces(hipStream_t st, int prec, int mode, const B2Dev &p,
                    const NListDev &nl, const int *species, const int *row_of,
                    const double *ynorm, const float *ynorm_f, const void *A,
                    const WInput &wi, const double *dA, int64_t dA_first_pair,
-                   int a_begin, int a_end, double *pair_f);
+                   int a_begin, int a_end, const PairForces &pair_f);
 // F_a = sum_k (f_ak - f_rev(ak)) into forces (stride 3; may be NULL; pairs
 // without a reverse contribute only their own term), and the energy (sum of
 // e_row[row_of[a]]), virial (xx yy zz xy xz yz of V[a][b] = -sum_k r_a f_b)
@@ -166,7 +174,7 @@ void launch_forces(hipStream_t st, int prec, int mode, const B2Dev &p,
 // x 10), then one workgroup per column sums them in a fixed order, so the
 // result is deterministic.
 int gather_parts(int n);
-void launch_gather(hipStream_t st, const NListDev &nl, const double *pair_f,
+void launch_gather(hipStream_t st, const NListDev &nl, const PairForces &pair_f,
                    double *forces, const int *row_of, const double *e_row,
                    double *part, double inv_volume, double *energy, double *virial6,
                    double *stress6);

@@ -1447,18 +1447,19 @@ void ext_incoming(hipStream_t st, NListBuffers &b, int n_all, int64_t P,
                   in_cum, in_pairs);
 }
 
+template <typename PF>
 __global__ void __launch_bounds__(256)
 ext_incoming_kernel(int n_all, const int *__restrict__ in_cum,
-                    const int *__restrict__ in_pairs, const double *__restrict__ pf,
+                    const int *__restrict__ in_pairs, const PF *__restrict__ pf,
                     double *__restrict__ forces) {
   const int a = blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= n_all) return;
   double f0 = 0, f1 = 0, f2 = 0;
   for (int q = in_cum[a]; q < in_cum[a + 1]; q++) {
     const int64_t k = in_pairs[q];
-    f0 += pf[3 * k];
-    f1 += pf[3 * k + 1];
-    f2 += pf[3 * k + 2];
+    f0 += (double)pf[3 * k];
+    f1 += (double)pf[3 * k + 1];
+    f2 += (double)pf[3 * k + 2];
   }
   forces[3 * (int64_t)a] -= f0;
   forces[3 * (int64_t)a + 1] -= f1;
@@ -1479,12 +1480,17 @@ ext_own_kernel(int inum, const int *__restrict__ ilist, const double *__restrict
 }
 
 void ext_assemble(hipStream_t st, int n_all, int inum, const int *ilist,
-                  const int *in_cum, const int *in_pairs, const double *pf,
+                  const int *in_cum, const int *in_pairs, const PairForces &pf,
                   const double *own, const int *row_of, const double *E_row,
                   double *forces, double *eatom) {
-  if (n_all > 0)
-    hipLaunchKernelGGL(ext_incoming_kernel, dim3((n_all + 255) / 256), dim3(256), 0,
-                       st, n_all, in_cum, in_pairs, pf, forces);
+  if (n_all > 0) {
+    if (pf.f32)
+      hipLaunchKernelGGL(ext_incoming_kernel<float>, dim3((n_all + 255) / 256), dim3(256),
+                         0, st, n_all, in_cum, in_pairs, (const float *)pf.p, forces);
+    else
+      hipLaunchKernelGGL(ext_incoming_kernel<double>, dim3((n_all + 255) / 256), dim3(256),
+                         0, st, n_all, in_cum, in_pairs, (const double *)pf.p, forces);
+  }
   if (inum > 0)
     hipLaunchKernelGGL(ext_own_kernel, dim3((inum + 255) / 256), dim3(256), 0, st,
                        inum, ilist, own, row_of, E_row, forces, eatom);
