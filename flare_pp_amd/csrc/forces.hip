@@ -18,6 +18,7 @@
 //                    deterministic gather over the full list.
 #include "common.h"
 #include "kernels.h"
+#include "small_mma.h"
 
 namespace flare {
 
@@ -29,29 +30,6 @@ struct WSource {
   int kp;
   const double *norm2;
   double *E_row;
-};
-
-typedef __attribute__((ext_vector_type(4))) float f32x4;
-typedef __attribute__((ext_vector_type(4))) double f64x4;
-
-template <typename T>
-struct MmaF;
-template <>
-struct MmaF<float> {
-  typedef f32x4 V;
-  static __device__ inline V run(float a, float b, V c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  static __device__ inline int row(int g, int j) { return 4 * g + j; }
-};
-template <>
-struct MmaF<double> {
-  typedef f64x4 V;
-  static __device__ inline V run(double a, double b, V c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  // the FP64 instruction interleaves result rows across elements (gfx950)
-  static __device__ inline int row(int g, int j) { return g + 4 * j; }
 };
 
 // dst[o] = load(o) for o < n (dst in LDS); each lane issues KB loads
@@ -119,7 +97,7 @@ __device__ inline void stage_lds_vec(int n, const T *__restrict__ src, T *dst) {
 template <typename T, int LFIX, bool SCALE, int NFIX = -1, bool CHANNEL_MAJOR = false>
 __device__ inline void u_mfma(const B2Dev &p, const T *A, const T *w,
                               const T *nrm_lm, const int *ptab, T *u2) {
-  typedef typename MmaF<T>::V V;
+  typedef typename Mma<T>::V V;
   const int lane = threadIdx.x;
   const int col = lane & 15, g = lane >> 4;
   const int H = LFIX >= 0 ? (LFIX + 1) * (LFIX + 1) : p.H;
@@ -132,7 +110,7 @@ __device__ inline void u_mfma(const B2Dev &p, const T *A, const T *w,
     int dst[4];
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      const int cr = mt * 16 + MmaF<T>::row(g, q);
+      const int cr = mt * 16 + Mma<T>::row(g, q);
       const int s = cr / N, n = cr - s * N;
       dst[q] = cr < nr ? (CHANNEL_MAJOR ? cr * H : s * H * N + n) : -1;
     }
@@ -150,7 +128,7 @@ __device__ inline void u_mfma(const B2Dev &p, const T *A, const T *w,
             a = c == c2 ? 2 * wv : wv;
           }
           if (c2 < nr && j < nl) b = A[c2 * H + l2 + j];
-          acc = MmaF<T>::run(a, b, acc);
+          acc = Mma<T>::run(a, b, acc);
         }
         if (j < nl) {
           const T sc = SCALE ? nrm_lm[l2 + j] : T(1);
@@ -172,7 +150,7 @@ __device__ inline void u_mfma(const B2Dev &p, const T *A, const T *w,
 template <typename T, int L, int NR>
 __device__ inline void u_mfma_fixed(const T *A, const T *w, const T *nrm_lm, T *u2) {
   static_assert(NR == 8 || NR == 16, "one M tile of channels");
-  typedef typename MmaF<T>::V V;
+  typedef typename Mma<T>::V V;
   constexpr int L1 = L + 1, H = L1 * L1, N = 8, KS = NR / 4;
   const int lane = threadIdx.x;
   const int col = lane & 15, g = lane >> 4;
@@ -192,7 +170,7 @@ __device__ inline void u_mfma_fixed(const T *A, const T *w, const T *nrm_lm, T *
   bool drow[4];
 #pragma unroll
   for (int q = 0; q < 4; q++) {
-    const int cr = MmaF<T>::row(g, q);
+    const int cr = Mma<T>::row(g, q);
     drow[q] = NR == 16 || cr < NR;
     dst[q] = (cr >> 3) * H * N + (cr & 7);
   }
@@ -208,7 +186,7 @@ __device__ inline void u_mfma_fixed(const T *A, const T *w, const T *nrm_lm, T *
     for (int ks = 0; ks < KS; ks++) {
       const int c2 = 4 * ks + g;
       const T b = jok ? A[c2 * H + l2 + j] : T(0);
-      acc = MmaF<T>::run(wv[ks][l], b, acc);
+      acc = Mma<T>::run(wv[ks][l], b, acc);
     }
     if (jok) {
       const T sc = nrm_lm[l2 + j];
