@@ -74,6 +74,10 @@ def parse():
                     help="skip the rocprofv3 FETCH_SIZE / WRITE_SIZE passes")
     ap.add_argument("--copy-calibration", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--traffic-child", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--gp", action="store_true",
+                    help="time SparseGP predict_mean (32,000 atoms, 500 sparse) and "
+                         "predict_local_uncertainties (55,296 atoms, 2,000 sparse) "
+                         "instead of the mapped force evaluation")
     return ap.parse_args()
 
 
@@ -282,6 +286,130 @@ def host_cpu():
     return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable}
 
 
+def gp_model(c, cutoff):
+    """Structure + SparseGP of a GP config (the sparse set and alpha it names)."""
+    from flare_pp_amd import B2, NormalizedDotProduct, SparseGP, Structure
+    b2 = B2("chebyshev", "quadratic", [0, cutoff], [], c["settings"])
+    s = Structure(c["cell"], c["species"], c["positions"], cutoff, [b2])
+    gp = SparseGP([NormalizedDotProduct(c["sigma"], c["power"])], 0.1, 0.1, 0.1)
+    gp.add_specific_environments(s, c["sparse_atoms"])
+    gp.set_alpha(c["alpha"])
+    return s, gp
+
+
+def gp_time(gp, s, form, variances, steps, warmup):
+    """Milliseconds per predict call (C-ABI on device buffers, HIP events;
+    the neighbor list is the structure's)."""
+    import ctypes
+    import torch
+    from flare_pp_amd import _capi
+    from flare_pp_amd.sparse_gp import _form
+    gp._check_ready(s)
+    ctx = _capi.context(0)
+    p = s.descriptor_calculators[0].params()
+    out = torch.empty(1 + 3 * s.noa + 6, dtype=torch.float64, device="cuda:0")
+    var = torch.empty(max(s.noa, 1), dtype=torch.float64, device="cuda:0")
+    lib = _capi.lib()
+
+    def call():
+        if variances:
+            _capi.check(lib.flare_sgp_predict_local_uncertainties(
+                ctx.handle, gp._h, ctypes.byref(p), s._nlist, _capi.dptr(s.d_species),
+                _form(form), _capi.dptr(out), _capi.dptr(var)))
+        else:
+            _capi.check(lib.flare_sgp_predict_mean(
+                ctx.handle, gp._h, ctypes.byref(p), s._nlist, _capi.dptr(s.d_species),
+                _form(form), _capi.dptr(out)))
+    for _ in range(warmup):
+        call()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        call()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def gp_cpu_baseline(threads):
+    """The oracle's predict_mean and cluster variances (the reference's
+    per-pair dB/dr formulation) on a 4,000-atom sample of config 3's
+    material: fcc a = 4.05, 2 species, B2 {2, 8, 4}, 500 sparse atoms."""
+    import numpy as np
+    from oracle import pyoracle as orc
+    from flare_pp_amd import synth
+    nthreads = threads or min(16, os.cpu_count() or 1)
+    cell, species, pos = synth.fcc(10, 4.05, 0.08, [2000, 2000], 3)
+    rng = np.random.default_rng(3)
+    atoms = np.sort(rng.choice(len(species), size=500, replace=False)).tolist()
+    alpha = rng.normal(0.0, 1.0, 500)
+    sigma, power, rc, S, N, L = 2.0, 2, 5.0, 2, 8, 4
+    nl = orc.NeighborList(cell, species, pos, rc, nthreads=nthreads)
+    t0 = time.perf_counter()
+    o = orc.B2Result(nl, species, S, N, L, np.full((S, S), rc), dervs=True)
+    g = o.grouped()
+    off = np.concatenate([[0], np.cumsum(g["n_by_type"])[:-1]])
+    sel = [[k for k, a in enumerate(g["atom_indices"][o0:o0 + cnt]) if a in set(atoms)]
+           for o0, cnt in zip(off, g["n_by_type"])]
+    # sparse rows: the selected atoms' descriptor rows, species by species
+    rows = np.concatenate([o0 + np.asarray(k, dtype=int) for o0, k in zip(off, sel)])
+    P, Pn, m = g["descriptors"][rows], g["norms"][rows], [len(k) for k in sel]
+    orc.predict_mean(o, m, P, Pn, sigma, power, orc.volume(cell), alpha)
+    t_mean = time.perf_counter() - t0
+    Li = orc.cholesky_inverse(orc.envs_envs(S, m, P, Pn, m, P, Pn, sigma, power), 1e-8)
+    t1 = time.perf_counter()
+    orc.cluster_uncertainties(o, m, P, Pn, sigma, power, Li)
+    t_var = time.perf_counter() - t1
+    n = len(species)
+    return {"predict_mean": {"value": n / t_mean, "unit": "atoms/s"},
+            "variances": {"value": n / t_var, "unit": "atoms/s"},
+            "cores": nthreads, "kind": "port",
+            "sample": f"{n:,}-atom fcc sample, B2 {{2,8,4}} rc 5, 500 sparse atoms: one "
+                      f"predict_mean (descriptors with per-pair dB/dr, as the "
+                      f"reference) and one cluster-variance pass, FP64, OpenMP",
+            "host": host_cpu()}
+
+
+def gp_main(args):
+    import numpy as np
+    import torch
+    from flare_pp_amd import synth
+    steps = max(1, min(args.steps, 10))
+    res = {}
+    c3 = synth.config3()
+    s3, gp3 = gp_model(c3, 5.0)
+    n3 = s3.noa
+    for form in FORMS:
+        ms = gp_time(gp3, s3, form, False, steps, 1)
+        res[f"predict_mean/{form}"] = {"ms": ms, "atoms_per_s": n3 / (ms * 1e-3)}
+        print(f"# predict_mean {form}: {ms:.3f} ms", file=sys.stderr, flush=True)
+    del s3, gp3
+    c4 = synth.config4()
+    s4, gp4 = gp_model(c4, c4["cutoff"])
+    n4 = s4.noa
+    for form in FORMS:
+        ms = gp_time(gp4, s4, form, True, steps, 1)
+        res[f"predict_local_uncertainties/{form}"] = {"ms": ms,
+                                                     "atoms_per_s": n4 / (ms * 1e-3)}
+        print(f"# predict_local_uncertainties {form}: {ms:.3f} ms", file=sys.stderr,
+              flush=True)
+    line = {"metric": "SparseGP predict_mean atoms/s at 32k atoms (500 sparse), mixed form",
+            "value": res["predict_mean/adjoint_mixed"]["atoms_per_s"], "unit": "atoms/s",
+            "n_gpus": 1, "steps": steps, "warmup": 1, "higher_is_better": True,
+            "data": "synthetic (seeded structures, sparse sets and alpha)",
+            "config": {"workload": "predict_mean: 32,000-atom 2-species fcc, B2 {2,8,4} "
+                                   "rc 5, 500 sparse, NormalizedDotProduct(2, 2); "
+                                   "predict_local_uncertainties: 55,296-atom 3-species "
+                                   "fcc, B2 {3,8,3} rc 4.5, 2,000 sparse",
+                       "includes_neighbor_build": False},
+            "forms": res}
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = gp_cpu_baseline(args.cpu_threads)
+    print(json.dumps(line))
+
+
 def main():
     args = parse()
     if args.copy_calibration:
@@ -292,6 +420,8 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU "
                           "sharding is described in DESIGN.md, not run"}))
         sys.exit(2)
+    if args.gp:
+        return gp_main(args)
     import numpy as np
     import torch
 
