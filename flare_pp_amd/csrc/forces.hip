@@ -141,58 +141,61 @@ __device__ inline void u_mfma(const B2Dev &p, const T *A, const T *w,
   }
 }
 
-// u_mfma for l_max = L, n_max = 8 and NR (8 or 16) radial channels: the
-// adjoint forms' fixed settings with one or two species.  Harmonic
-// normalisation applied, u2 layout as u_mfma's.  Each lane's matrix operand
-// W_l[c][c2] for its (c, c2 = 4 ks + g) and every l is read once into
+// u_mfma for l_max = L, n_max = 8 and NR (8, 16 or 24) radial channels:
+// the adjoint forms' fixed settings with one to three species.
+// Harmonic normalisation applied, u2 layout as u_mfma's.  Each lane's matrix
+// operand W_l[c][c2] for its (c, c2 = 4 ks + g) and every l is read once into
 // registers (the L + 1 values of a channel pair are adjacent in w), so the
 // MFMA loop only loads the A operand, and all bounds are compile-time.
 template <typename T, int L, int NR>
 __device__ inline void u_mfma_fixed(const T *A, const T *w, const T *nrm_lm, T *u2) {
-  static_assert(NR == 8 || NR == 16, "one M tile of channels");
+  static_assert(NR % 8 == 0 && NR <= 24, "one or two M tiles of channels");
   typedef typename Mma<T>::V V;
-  constexpr int L1 = L + 1, H = L1 * L1, N = 8, KS = NR / 4;
+  constexpr int L1 = L + 1, H = L1 * L1, N = 8, KS = NR / 4, MT = (NR + 15) / 16;
   const int lane = threadIdx.x;
   const int col = lane & 15, g = lane >> 4;
-  const int c = col;
-  const bool crow = NR == 16 || c < NR;
-  T wv[KS][L1];
 #pragma unroll
-  for (int ks = 0; ks < KS; ks++) {
-    const int c2 = 4 * ks + g;
-    const int off = crow ? b2_pair_index(c, c2, NR) * L1 : 0;
-    const T fac = c == c2 ? T(2) : T(1);
-#pragma unroll
-    for (int l = 0; l < L1; l++) wv[ks][l] = crow ? fac * w[off + l] : T(0);
-  }
-  // this lane's four result rows cr = row(g, q) < NR: channel (s, n)
-  int dst[4];
-  bool drow[4];
-#pragma unroll
-  for (int q = 0; q < 4; q++) {
-    const int cr = Mma<T>::row(g, q);
-    drow[q] = NR == 16 || cr < NR;
-    dst[q] = (cr >> 3) * H * N + (cr & 7);
-  }
-#pragma unroll
-  for (int l = 0; l < L1; l++) {
-    constexpr int kMaxJ = 16;
-    const int nl = 2 * l + 1, l2 = l * l;
-    static_assert(2 * L + 1 <= kMaxJ, "one N tile of harmonics");
-    const int j = col;
-    const bool jok = j < nl;
-    V acc = V{0, 0, 0, 0};
+  for (int mt = 0; mt < MT; mt++) {
+    const int c = mt * 16 + col;
+    const bool crow = c < NR;
+    T wv[KS][L1];
 #pragma unroll
     for (int ks = 0; ks < KS; ks++) {
       const int c2 = 4 * ks + g;
-      const T b = jok ? A[c2 * H + l2 + j] : T(0);
-      acc = Mma<T>::run(wv[ks][l], b, acc);
-    }
-    if (jok) {
-      const T sc = nrm_lm[l2 + j];
+      const int off = crow ? b2_pair_index(c, c2, NR) * L1 : 0;
+      const T fac = c == c2 ? T(2) : T(1);
 #pragma unroll
-      for (int q = 0; q < 4; q++)
-        if (drow[q]) u2[dst[q] + (l2 + j) * N] = acc[q] * sc;
+      for (int l = 0; l < L1; l++) wv[ks][l] = crow ? fac * w[off + l] : T(0);
+    }
+    // this lane's four result rows cr = mt 16 + row(g, q) < NR: channel (s, n)
+    int dst[4];
+    bool drow[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int cr = mt * 16 + Mma<T>::row(g, q);
+      drow[q] = cr < NR;
+      dst[q] = (cr >> 3) * H * N + (cr & 7);
+    }
+#pragma unroll
+    for (int l = 0; l < L1; l++) {
+      constexpr int kMaxJ = 16;
+      const int nl = 2 * l + 1, l2 = l * l;
+      static_assert(2 * L + 1 <= kMaxJ, "one N tile of harmonics");
+      const int j = col;
+      const bool jok = j < nl;
+      V acc = V{0, 0, 0, 0};
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++) {
+        const int c2 = 4 * ks + g;
+        const T b = jok ? A[c2 * H + l2 + j] : T(0);
+        acc = Mma<T>::run(wv[ks][l], b, acc);
+      }
+      if (jok) {
+        const T sc = nrm_lm[l2 + j];
+#pragma unroll
+        for (int q = 0; q < 4; q++)
+          if (drow[q]) u2[dst[q] + (l2 + j) * N] = acc[q] * sc;
+      }
     }
   }
 }
@@ -440,10 +443,10 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
       if (o < nva) ta[j] = s4[o];
     }
   }
-  // nr <= 32 here (checked at launch); the fixed-size path with 8 or 16
-  // channels computes its channel-pair offsets in registers
+  // nr <= 32 here (checked at launch); the fixed-size path (8, 16 or 24
+  // channels) computes its channel-pair offsets in registers
   constexpr bool kUFixed = kFixed;
-  if (!kUFixed || (p.nr != 8 && p.nr != 16))
+  if (!kUFixed || (p.nr != 8 && p.nr != 16 && p.nr != 24))
     for (int q = lane; q < p.nr * p.nr; q += kWave) {
       const int c = q / p.nr, c2 = q - c * p.nr;
       ptab[q] = b2_pair_index(c, c2, p.nr) * (p.L + 1);
@@ -576,13 +579,15 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
   }
   bool done = false;
   if constexpr (kUFixed) {
-    if (p.nr == 16) {
+    done = true;
+    if (p.nr == 16)
       u_mfma_fixed<T, LFIX, 16>(A, w, nrm_lm, u);
-      done = true;
-    } else if (p.nr == 8) {
+    else if (p.nr == 8)
       u_mfma_fixed<T, LFIX, 8>(A, w, nrm_lm, u);
-      done = true;
-    }
+    else if (p.nr == 24)
+      u_mfma_fixed<T, LFIX, 24>(A, w, nrm_lm, u);
+    else
+      done = false;
   }
   if (!done) u_mfma<T, LFIX, kFixed, kFixed ? 8 : -1, MODE == 0>(p, A, w, nrm_lm, ptab, u);
   __syncthreads();
