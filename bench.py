@@ -417,8 +417,8 @@ def main():
     if args.traffic_child:
         return traffic_child(args.traffic_child, args.workload, args.workload_file)
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU "
-                          "sharding is described in DESIGN.md, not run"}))
+        print(json.dumps({"error": "the force evaluation runs on one GPU; it "
+                          "is not sharded across GPUs (--gpus 1 only)"}))
         sys.exit(2)
     if args.gp:
         return gp_main(args)
