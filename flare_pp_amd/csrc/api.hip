@@ -1511,7 +1511,19 @@ static PairEval pair_eval(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
       flare::gemm_bf16x3_batched(ctx->stream, p.S, g, fw.xh, fw.xl, bh, bl,
                                  (float *)Z);
     }
-    for (int s = 0; s < p.S && !mixed; s++) {
+    // FP64: the hand-written matrix-core GEMM where it was measured faster
+    // than rocBLAS (F = 680: 0.575 vs 0.636 ms at 32,000 atoms); rocBLAS
+    // for wide rows (F = 2,100: 35.3 vs 39.7 ms at 262,144 atoms)
+    const bool fp64_mfma = prec == flare::kPrecF64 && (F % 2) == 0 && F <= 1024;
+    if (fp64_mfma) {  // all species in one launch on the FP64 matrix cores
+      flare::GemmProduct g[flare::kMaxSpecies];
+      for (int s = 0; s < p.S; s++)
+        g[s] = flare::GemmProduct{off[s + 1] - off[s], F, F, F, (int64_t)off[s] * F,
+                                  (int64_t)s * (int64_t)per, (int64_t)off[s] * F};
+      flare::dgemm_batched(ctx->stream, p.S, g, (const double *)X, (const double *)beta,
+                           (double *)Z);
+    }
+    for (int s = 0; s < p.S && !mixed && !fp64_mfma; s++) {
       int rows = off[s + 1] - off[s];
       if (!rows) continue;
       if (prec == flare::kPrecF64) {

@@ -243,6 +243,158 @@ gemm_x3_kernel(GemmBatch bt, const E *__restrict__ Ah0,
   }
 }
 
+// FP64 Z = A B^T on the FP64 matrix cores (v_mfma_f64_16x16x4f64): the
+// mapped contraction Z = X beta of the FP64 forms (beta is symmetric, so
+// its rows are its columns).  A: M rows of K values, B: N rows of K values
+// (row stride K, K even), Z: M x N with leading dimension ldz.
+// Tile 128 x 128 per workgroup of four wavefronts (64 x 64 each, 4 x 4
+// MFMA tiles), K steps of 16 double-buffered in LDS with one barrier per
+// step.  Inside a step the K order is permuted the same way for both
+// operands (lane group g = lane / 16 supplies k = 4 g + t in sub-step t),
+// so a lane's four values of a row are contiguous: two 16-byte LDS reads.
+// LDS rows are 128 bytes (eight 16-byte units); unit u of row r is stored
+// at u ^ ((r / 2) % 8), so the sixteen rows a lane group reads fall in
+// sixteen distinct bank groups.
+typedef __attribute__((ext_vector_type(4))) double doublex4;
+constexpr int kDM = 128, kDN = 128, kDK = 16;
+
+__device__ inline int dswz(int r, int unit) { return (unit ^ ((r >> 1) & 7)) * 2; }
+
+struct DTile {
+  double2 a[4], b[4];  // four 16-byte units of A and of B per thread
+};
+
+__device__ inline void dgemm_load(DTile &t, const double *__restrict__ A,
+                                  const double *__restrict__ B, int M, int N, int K,
+                                  int m0, int n0, int k0) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int h = 0; h < 4; h++) {
+    const int c = tid + 256 * h;
+    const int r = c >> 3, k = k0 + 2 * (c & 7);
+    const double2 z = {0.0, 0.0};
+    t.a[h] = z;
+    t.b[h] = z;
+    if (k < K) {
+      if (m0 + r < M) t.a[h] = *reinterpret_cast<const double2 *>(A + (int64_t)(m0 + r) * K + k);
+      if (n0 + r < N) t.b[h] = *reinterpret_cast<const double2 *>(B + (int64_t)(n0 + r) * K + k);
+    }
+  }
+}
+
+__device__ inline void dgemm_store(const DTile &t, double (*sA)[kDK], double (*sB)[kDK]) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int h = 0; h < 4; h++) {
+    const int c = tid + 256 * h;
+    const int r = c >> 3, u = c & 7;
+    *reinterpret_cast<double2 *>(&sA[r][dswz(r, u)]) = t.a[h];
+    *reinterpret_cast<double2 *>(&sB[r][dswz(r, u)]) = t.b[h];
+  }
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+dgemm_kernel(GemmBatch bt, const double *__restrict__ A0, const double *__restrict__ B0,
+             double *__restrict__ Z0) {
+  __shared__ __align__(16) double sA[2][kDM][kDK], sB[2][kDN][kDK];
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+  const int wm = (wv >> 1) * 64, wn = (wv & 1) * 64;
+  int T, dummy;
+  tile_of_block(bt.tile0[bt.nb], 1, T, dummy);
+  if (T >= bt.tile0[bt.nb]) return;
+  int q = 0;
+  while (T >= bt.tile0[q + 1]) q++;
+  const GemmProduct &gp = bt.g[q];
+  const int M = gp.M, N = gp.N, K = gp.K, ldz = gp.ldz;
+  const int nct = (N + kDN - 1) / kDN;
+  const int local = T - bt.tile0[q];
+  const int rt = local / nct, ct = local - rt * nct;
+  const double *A = A0 + gp.a_off, *B = B0 + gp.b_off;
+  double *Z = Z0 + gp.z_off;
+  const int m0 = rt * kDM, n0 = ct * kDN;
+  doublex4 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[i][j] = doublex4{0, 0, 0, 0};
+  const int fr = lane & 15, g = lane >> 4;
+  const int steps = (K + kDK - 1) / kDK;
+  DTile t;
+  dgemm_load(t, A, B, M, N, K, m0, n0, 0);
+  dgemm_store(t, sA[0], sB[0]);
+  if (steps > 1) dgemm_load(t, A, B, M, N, K, m0, n0, kDK);
+  __syncthreads();
+  for (int s = 0; s < steps; s++) {
+    const int cur = s & 1;
+    if (s + 1 < steps) {
+      // step s + 1 into the other buffer (its last reader, step s - 1,
+      // ended with a barrier); step s + 2 into registers
+      dgemm_store(t, sA[cur ^ 1], sB[cur ^ 1]);
+      if (s + 2 < steps) dgemm_load(t, A, B, M, N, K, m0, n0, (s + 2) * kDK);
+    }
+    double a[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int r = wm + 16 * i + fr;
+      const double2 lo = *reinterpret_cast<const double2 *>(&sA[cur][r][dswz(r, 2 * g)]);
+      const double2 hi = *reinterpret_cast<const double2 *>(&sA[cur][r][dswz(r, 2 * g + 1)]);
+      a[i][0] = lo.x;
+      a[i][1] = lo.y;
+      a[i][2] = hi.x;
+      a[i][3] = hi.y;
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int r = wn + 16 * j + fr;
+      const double2 lo = *reinterpret_cast<const double2 *>(&sB[cur][r][dswz(r, 2 * g)]);
+      const double2 hi = *reinterpret_cast<const double2 *>(&sB[cur][r][dswz(r, 2 * g + 1)]);
+      const double b[4] = {lo.x, lo.y, hi.x, hi.y};
+#pragma unroll
+      for (int tt = 0; tt < 4; tt++)
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][tt], b[tt], acc[i][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  // FP64 result layout (gfx950): element e of lane (col, g) is row g + 4 e
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int col = n0 + wn + 16 * j + fr;
+      if (col >= N) continue;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int row = m0 + wm + 16 * i + g + 4 * e;
+        if (row < M) Z[(int64_t)row * ldz + col] = acc[i][j][e];
+      }
+    }
+}
+
+void dgemm_batched(hipStream_t st, int nb, const GemmProduct *prods, const double *A,
+                   const double *B, double *Z) {
+  FLARE_REQUIRE(nb >= 0 && nb <= kGemmMaxBatch, "dgemm: at most 8 products");
+  GemmBatch bt;
+  bt.nb = 0;
+  int tiles = 0;
+  for (int q = 0; q < nb; q++) {
+    const GemmProduct &g = prods[q];
+    FLARE_REQUIRE(g.K % 2 == 0, "dgemm: K must be even");
+    if (g.M <= 0 || g.N <= 0) continue;
+    bt.g[bt.nb] = g;
+    bt.tile0[bt.nb] = tiles;
+    tiles += (g.M + kDM - 1) / kDM * ((g.N + kDN - 1) / kDN);
+    bt.nb++;
+  }
+  bt.tile0[bt.nb] = tiles;
+  if (tiles == 0) return;
+  const int grid = (tiles + 7) / 8 * 8;
+  hipLaunchKernelGGL(dgemm_kernel, dim3(grid), dim3(256), 0, st, bt, A, B, Z);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
 // hi/lo split of a (rows x cols) FP64 matrix into rows of kp BF16 (zero
 // padded); hi = bf16(x), lo = bf16(x - hi)
 __global__ void split_bf16_kernel(int rows, int cols, int kp,
