@@ -1511,19 +1511,25 @@ static PairEval pair_eval(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
       flare::gemm_bf16x3_batched(ctx->stream, p.S, g, fw.xh, fw.xl, bh, bl,
                                  (float *)Z);
     }
-    // FP64: the hand-written matrix-core GEMM where it was measured faster
-    // than rocBLAS (F = 680: 0.575 vs 0.636 ms at 32,000 atoms); rocBLAS
-    // for wide rows (F = 2,100: 35.3 vs 39.7 ms at 262,144 atoms)
-    const bool fp64_mfma = prec == flare::kPrecF64 && (F % 2) == 0 && F <= 1024;
-    if (fp64_mfma) {  // all species in one launch on the FP64 matrix cores
+    // FP64 and FP32: the hand-written matrix-core GEMM where it was measured
+    // faster than rocBLAS (F = 680 at 32,000 atoms: FP64 0.575 vs 0.636 ms,
+    // FP32 0.296 vs 0.413 ms); rocBLAS for wide rows (F = 2,100 at 262,144
+    // atoms: FP64 35.3 vs 39.7 ms, FP32 18.6 vs 19.9 ms)
+    const bool mcore = !mixed && F <= 1024 &&
+                       (prec == flare::kPrecF64 ? F % 2 == 0 : F % 4 == 0);
+    if (mcore) {  // all species in one launch
       flare::GemmProduct g[flare::kMaxSpecies];
       for (int s = 0; s < p.S; s++)
         g[s] = flare::GemmProduct{off[s + 1] - off[s], F, F, F, (int64_t)off[s] * F,
                                   (int64_t)s * (int64_t)per, (int64_t)off[s] * F};
-      flare::dgemm_batched(ctx->stream, p.S, g, (const double *)X, (const double *)beta,
-                           (double *)Z);
+      if (prec == flare::kPrecF64)
+        flare::dgemm_batched(ctx->stream, p.S, g, (const double *)X, (const double *)beta,
+                             (double *)Z);
+      else
+        flare::sgemm_batched(ctx->stream, p.S, g, (const float *)X, (const float *)beta,
+                             (float *)Z);
     }
-    for (int s = 0; s < p.S && !mixed && !fp64_mfma; s++) {
+    for (int s = 0; s < p.S && !mixed && !mcore; s++) {
       int rows = off[s + 1] - off[s];
       if (!rows) continue;
       if (prec == flare::kPrecF64) {

@@ -31,10 +31,12 @@ struct GemmProduct {
 void gemm_bf16x3_batched(hipStream_t st, int nb, const GemmProduct *prods,
                          const void *Ah, const void *Al, const void *Bh,
                          const void *Bl, float *Z);
-// FP64 products Z = A B^T on the FP64 matrix cores (row stride of A and B
-// = K, K even; no hi/lo parts).
+// FP64 / FP32 products Z = A B^T on the matrix cores (row stride of A and
+// B = K, rows a multiple of 16 bytes; no hi/lo parts).
 void dgemm_batched(hipStream_t st, int nb, const GemmProduct *prods, const double *A,
                    const double *B, double *Z);
+void sgemm_batched(hipStream_t st, int nb, const GemmProduct *prods, const float *A,
+                   const float *B, float *Z);
 // The same on FP16 MFMA (operands from split_f16_*).
 void gemm_f16x3_batched(hipStream_t st, int nb, const GemmProduct *prods,
                         const void *Ah, const void *Al, const void *Bh,

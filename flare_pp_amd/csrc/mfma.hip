@@ -243,122 +243,163 @@ gemm_x3_kernel(GemmBatch bt, const E *__restrict__ Ah0,
   }
 }
 
-// FP64 Z = A B^T on the FP64 matrix cores (v_mfma_f64_16x16x4f64): the
-// mapped contraction Z = X beta of the FP64 forms (beta is symmetric, so
-// its rows are its columns).  A: M rows of K values, B: N rows of K values
-// (row stride K, K even), Z: M x N with leading dimension ldz.
+// FP64 and FP32 Z = A B^T on the matrix cores (v_mfma_f64_16x16x4f64,
+// v_mfma_f32_16x16x4f32): the mapped contraction Z = X beta of the FP64 and
+// FP32 forms (beta is symmetric, so its rows are its columns).  A: M rows
+// of K values, B: N rows of K values (row stride K, K a multiple of 16 bytes
+// / element), Z: M x N with leading dimension ldz.
 // Tile 128 x 128 per workgroup of four wavefronts (64 x 64 each, 4 x 4
 // MFMA tiles), K steps of 16 double-buffered in LDS with one barrier per
 // step.  Inside a step the K order is permuted the same way for both
 // operands (lane group g = lane / 16 supplies k = 4 g + t in sub-step t),
-// so a lane's four values of a row are contiguous: two 16-byte LDS reads.
-// LDS rows are 128 bytes (eight 16-byte units); unit u of row r is stored
-// at u ^ ((r / 2) % 8), so the sixteen rows a lane group reads fall in
-// sixteen distinct bank groups.
+// so a lane's four values of a row are contiguous: 16-byte LDS reads.
+// LDS rows of 16 values are cut into 16-byte units, XOR-swizzled by row so
+// that the sixteen rows a lane group reads fall in distinct bank groups.
 typedef __attribute__((ext_vector_type(4))) double doublex4;
 constexpr int kDM = 128, kDN = 128, kDK = 16;
 
-__device__ inline int dswz(int r, int unit) { return (unit ^ ((r >> 1) & 7)) * 2; }
-
-struct DTile {
-  double2 a[4], b[4];  // four 16-byte units of A and of B per thread
+template <typename T>
+struct DMma;
+template <>
+struct DMma<double> {
+  typedef doublex4 V;
+  typedef double2 U;                           // one 16-byte unit
+  static constexpr int kUnits = kDK / 2;       // units per LDS row
+  static __device__ inline V run(double a, double b, V c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int g, int e) { return g + 4 * e; }  // gfx950 FP64
+  static __device__ inline int swz(int r, int u) { return (u ^ ((r >> 1) & 7)) * 2; }
+};
+template <>
+struct DMma<float> {
+  typedef floatx4 V;
+  typedef float4 U;
+  static constexpr int kUnits = kDK / 4;
+  static __device__ inline V run(float a, float b, V c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ inline int row(int g, int e) { return 4 * g + e; }
+  static __device__ inline int swz(int r, int u) { return (u ^ ((r >> 2) & 3)) * 4; }
 };
 
-__device__ inline void dgemm_load(DTile &t, const double *__restrict__ A,
-                                  const double *__restrict__ B, int M, int N, int K,
+template <typename T>
+struct DTile {
+  static constexpr int kPer = kDM * DMma<T>::kUnits / 256;  // units per thread
+  typename DMma<T>::U a[kPer], b[kPer];
+};
+
+template <typename T>
+__device__ inline void dgemm_load(DTile<T> &t, const T *__restrict__ A,
+                                  const T *__restrict__ B, int M, int N, int K,
                                   int m0, int n0, int k0) {
+  typedef typename DMma<T>::U U;
+  constexpr int NU = DMma<T>::kUnits, PER = DTile<T>::kPer, EU = 16 / sizeof(T);
   const int tid = threadIdx.x;
 #pragma unroll
-  for (int h = 0; h < 4; h++) {
+  for (int h = 0; h < PER; h++) {
     const int c = tid + 256 * h;
-    const int r = c >> 3, k = k0 + 2 * (c & 7);
-    const double2 z = {0.0, 0.0};
+    const int r = c / NU, k = k0 + EU * (c % NU);
+    U z;
+    memset(&z, 0, sizeof(U));
     t.a[h] = z;
     t.b[h] = z;
     if (k < K) {
-      if (m0 + r < M) t.a[h] = *reinterpret_cast<const double2 *>(A + (int64_t)(m0 + r) * K + k);
-      if (n0 + r < N) t.b[h] = *reinterpret_cast<const double2 *>(B + (int64_t)(n0 + r) * K + k);
+      if (m0 + r < M) t.a[h] = *reinterpret_cast<const U *>(A + (int64_t)(m0 + r) * K + k);
+      if (n0 + r < N) t.b[h] = *reinterpret_cast<const U *>(B + (int64_t)(n0 + r) * K + k);
     }
   }
 }
 
-__device__ inline void dgemm_store(const DTile &t, double (*sA)[kDK], double (*sB)[kDK]) {
+template <typename T>
+__device__ inline void dgemm_store(const DTile<T> &t, T (*sA)[kDK], T (*sB)[kDK]) {
+  typedef typename DMma<T>::U U;
+  constexpr int NU = DMma<T>::kUnits;
   const int tid = threadIdx.x;
 #pragma unroll
-  for (int h = 0; h < 4; h++) {
+  for (int h = 0; h < DTile<T>::kPer; h++) {
     const int c = tid + 256 * h;
-    const int r = c >> 3, u = c & 7;
-    *reinterpret_cast<double2 *>(&sA[r][dswz(r, u)]) = t.a[h];
-    *reinterpret_cast<double2 *>(&sB[r][dswz(r, u)]) = t.b[h];
+    const int r = c / NU, u = c % NU;
+    *reinterpret_cast<U *>(&sA[r][DMma<T>::swz(r, u)]) = t.a[h];
+    *reinterpret_cast<U *>(&sB[r][DMma<T>::swz(r, u)]) = t.b[h];
   }
 }
 
+// the lane's four values k = 4 g .. 4 g + 3 of LDS row r
+template <typename T>
+__device__ inline void dgemm_row4(const T (*s)[kDK], int r, int g, T *o) {
+  if constexpr (sizeof(T) == 8) {
+    const double2 lo = *reinterpret_cast<const double2 *>(&s[r][DMma<T>::swz(r, 2 * g)]);
+    const double2 hi = *reinterpret_cast<const double2 *>(&s[r][DMma<T>::swz(r, 2 * g + 1)]);
+    o[0] = lo.x;
+    o[1] = lo.y;
+    o[2] = hi.x;
+    o[3] = hi.y;
+  } else {
+    const float4 v = *reinterpret_cast<const float4 *>(&s[r][DMma<T>::swz(r, g)]);
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+    o[3] = v.w;
+  }
+}
+
+template <typename T>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
-dgemm_kernel(GemmBatch bt, const double *__restrict__ A0, const double *__restrict__ B0,
-             double *__restrict__ Z0) {
-  __shared__ __align__(16) double sA[2][kDM][kDK], sB[2][kDN][kDK];
+dgemm_kernel(GemmBatch bt, const T *__restrict__ A0, const T *__restrict__ B0,
+             T *__restrict__ Z0) {
+  typedef typename DMma<T>::V V;
+  __shared__ __align__(16) T sA[2][kDM][kDK], sB[2][kDN][kDK];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
   const int wm = (wv >> 1) * 64, wn = (wv & 1) * 64;
-  int T, dummy;
-  tile_of_block(bt.tile0[bt.nb], 1, T, dummy);
-  if (T >= bt.tile0[bt.nb]) return;
+  int Tt, dummy;
+  tile_of_block(bt.tile0[bt.nb], 1, Tt, dummy);
+  if (Tt >= bt.tile0[bt.nb]) return;
   int q = 0;
-  while (T >= bt.tile0[q + 1]) q++;
+  while (Tt >= bt.tile0[q + 1]) q++;
   const GemmProduct &gp = bt.g[q];
   const int M = gp.M, N = gp.N, K = gp.K, ldz = gp.ldz;
   const int nct = (N + kDN - 1) / kDN;
-  const int local = T - bt.tile0[q];
+  const int local = Tt - bt.tile0[q];
   const int rt = local / nct, ct = local - rt * nct;
-  const double *A = A0 + gp.a_off, *B = B0 + gp.b_off;
-  double *Z = Z0 + gp.z_off;
+  const T *A = A0 + gp.a_off, *B = B0 + gp.b_off;
+  T *Z = Z0 + gp.z_off;
   const int m0 = rt * kDM, n0 = ct * kDN;
-  doublex4 acc[4][4];
+  V acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
-    for (int j = 0; j < 4; j++) acc[i][j] = doublex4{0, 0, 0, 0};
+    for (int j = 0; j < 4; j++) acc[i][j] = V{0, 0, 0, 0};
   const int fr = lane & 15, g = lane >> 4;
   const int steps = (K + kDK - 1) / kDK;
-  DTile t;
-  dgemm_load(t, A, B, M, N, K, m0, n0, 0);
-  dgemm_store(t, sA[0], sB[0]);
-  if (steps > 1) dgemm_load(t, A, B, M, N, K, m0, n0, kDK);
+  DTile<T> t;
+  dgemm_load<T>(t, A, B, M, N, K, m0, n0, 0);
+  dgemm_store<T>(t, sA[0], sB[0]);
+  if (steps > 1) dgemm_load<T>(t, A, B, M, N, K, m0, n0, kDK);
   __syncthreads();
   for (int s = 0; s < steps; s++) {
     const int cur = s & 1;
     if (s + 1 < steps) {
       // step s + 1 into the other buffer (its last reader, step s - 1,
       // ended with a barrier); step s + 2 into registers
-      dgemm_store(t, sA[cur ^ 1], sB[cur ^ 1]);
-      if (s + 2 < steps) dgemm_load(t, A, B, M, N, K, m0, n0, (s + 2) * kDK);
+      dgemm_store<T>(t, sA[cur ^ 1], sB[cur ^ 1]);
+      if (s + 2 < steps) dgemm_load<T>(t, A, B, M, N, K, m0, n0, (s + 2) * kDK);
     }
-    double a[4][4];
+    T a[4][4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int r = wm + 16 * i + fr;
-      const double2 lo = *reinterpret_cast<const double2 *>(&sA[cur][r][dswz(r, 2 * g)]);
-      const double2 hi = *reinterpret_cast<const double2 *>(&sA[cur][r][dswz(r, 2 * g + 1)]);
-      a[i][0] = lo.x;
-      a[i][1] = lo.y;
-      a[i][2] = hi.x;
-      a[i][3] = hi.y;
-    }
+    for (int i = 0; i < 4; i++) dgemm_row4<T>(sA[cur], wm + 16 * i + fr, g, a[i]);
 #pragma unroll
     for (int j = 0; j < 4; j++) {
-      const int r = wn + 16 * j + fr;
-      const double2 lo = *reinterpret_cast<const double2 *>(&sB[cur][r][dswz(r, 2 * g)]);
-      const double2 hi = *reinterpret_cast<const double2 *>(&sB[cur][r][dswz(r, 2 * g + 1)]);
-      const double b[4] = {lo.x, lo.y, hi.x, hi.y};
+      T b[4];
+      dgemm_row4<T>(sB[cur], wn + 16 * j + fr, g, b);
 #pragma unroll
       for (int tt = 0; tt < 4; tt++)
 #pragma unroll
-        for (int i = 0; i < 4; i++)
-          acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i][tt], b[tt], acc[i][j], 0, 0, 0);
+        for (int i = 0; i < 4; i++) acc[i][j] = DMma<T>::run(a[i][tt], b[tt], acc[i][j]);
     }
     __syncthreads();
   }
-  // FP64 result layout (gfx950): element e of lane (col, g) is row g + 4 e
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -367,21 +408,22 @@ dgemm_kernel(GemmBatch bt, const double *__restrict__ A0, const double *__restri
       if (col >= N) continue;
 #pragma unroll
       for (int e = 0; e < 4; e++) {
-        const int row = m0 + wm + 16 * i + g + 4 * e;
+        const int row = m0 + wm + 16 * i + DMma<T>::row(g, e);
         if (row < M) Z[(int64_t)row * ldz + col] = acc[i][j][e];
       }
     }
 }
 
-void dgemm_batched(hipStream_t st, int nb, const GemmProduct *prods, const double *A,
-                   const double *B, double *Z) {
-  FLARE_REQUIRE(nb >= 0 && nb <= kGemmMaxBatch, "dgemm: at most 8 products");
+template <typename T>
+static void dgemm_batched_t(hipStream_t st, int nb, const GemmProduct *prods, const T *A,
+                            const T *B, T *Z) {
+  FLARE_REQUIRE(nb >= 0 && nb <= kGemmMaxBatch, "gemm: at most 8 products");
   GemmBatch bt;
   bt.nb = 0;
   int tiles = 0;
   for (int q = 0; q < nb; q++) {
     const GemmProduct &g = prods[q];
-    FLARE_REQUIRE(g.K % 2 == 0, "dgemm: K must be even");
+    FLARE_REQUIRE((g.K * (int)sizeof(T)) % 16 == 0, "gemm: rows must be 16-byte multiples");
     if (g.M <= 0 || g.N <= 0) continue;
     bt.g[bt.nb] = g;
     bt.tile0[bt.nb] = tiles;
@@ -391,8 +433,18 @@ void dgemm_batched(hipStream_t st, int nb, const GemmProduct *prods, const doubl
   bt.tile0[bt.nb] = tiles;
   if (tiles == 0) return;
   const int grid = (tiles + 7) / 8 * 8;
-  hipLaunchKernelGGL(dgemm_kernel, dim3(grid), dim3(256), 0, st, bt, A, B, Z);
+  hipLaunchKernelGGL(dgemm_kernel<T>, dim3(grid), dim3(256), 0, st, bt, A, B, Z);
   FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void dgemm_batched(hipStream_t st, int nb, const GemmProduct *prods, const double *A,
+                   const double *B, double *Z) {
+  dgemm_batched_t<double>(st, nb, prods, A, B, Z);
+}
+
+void sgemm_batched(hipStream_t st, int nb, const GemmProduct *prods, const float *A,
+                   const float *B, float *Z) {
+  dgemm_batched_t<float>(st, nb, prods, A, B, Z);
 }
 
 // hi/lo split of a (rows x cols) FP64 matrix into rows of kp BF16 (zero
