@@ -92,6 +92,23 @@ __global__ void clear4_kernel(int *__restrict__ a, int na, int *__restrict__ b,
   if (q < nd) d[q] = 0;
 }
 
+// rev[0..P) = -1 ahead of the compaction (partners that find each other
+// overwrite both entries), P = the pair total the count scan left on the
+// device; nothing to do when an over-capacity atom sends the build to the
+// two-pass path, which writes every entry.
+__global__ void __launch_bounds__(256)
+fill_rev_kernel(const int *__restrict__ total, const int *__restrict__ overflow,
+                int *__restrict__ rev) {
+  if (*overflow) return;
+  const int P = *total, P4 = P >> 2;
+  const int stride = gridDim.x * blockDim.x;
+  const int4 m1 = {-1, -1, -1, -1};
+  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < P4; k += stride)
+    reinterpret_cast<int4 *>(rev)[k] = m1;
+  for (int k = 4 * P4 + blockIdx.x * blockDim.x + threadIdx.x; k < P; k += stride)
+    rev[k] = -1;
+}
+
 // Single-workgroup exclusive scan: out[0..n] with out[n] = total, for
 // n <= C * kScanChunk.  Chunk c holds elements [c kScanChunk, (c + 1)
 // kScanChunk), thread t its four elements 4t..4t+3, so every load and store
@@ -1164,7 +1181,9 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
   FLARE_HIP_CHECK(hipEventRecord(stats_ev, st));
   if (n > 0) {
     // entries without a partner keep -1 (the partners write each other)
-    FLARE_HIP_CHECK(hipMemsetAsync(b.rev.ptr, 0xff, cap * sizeof(int), st));
+    const int fill_blocks = (int)std::min<size_t>(1024, (cap / 4 + 255) / 256);
+    hipLaunchKernelGGL(fill_rev_kernel, dim3(fill_blocks), dim3(256), 0, st,
+                       counts + kMaxSpecies + 2, overflow, b.rev.as<int>());
     hipLaunchKernelGGL(nbr_compact_kernel, dim3(gw), dim3(256), 0, st, n, m,
                        overflow, b.cum.as<int>(), b.count.as<int>(),
                        b.keys_pad.as<int64_t>(), b.wrapped.as<double>(),
