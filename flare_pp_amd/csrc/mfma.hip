@@ -212,7 +212,37 @@ gemm_x3_kernel(GemmBatch bt, const E *__restrict__ Ah0,
   }
   const int dcol = lane & 15;
   const int drow = (lane >> 4) * 4;
-  if (!gp.rs && !gp.cs) {
+  const bool vec4 = !gp.rs && !gp.cs && (N & 3) == 0 && (ldz & 3) == 0 && (gp.z_off & 3) == 0;
+  if (vec4) {
+    // 16 rows at a time through LDS (8 KB per wavefront in the B tile
+    // buffers, free after the last step), then 16-byte stores: 512
+    // contiguous bytes per row instead of 64
+    __syncthreads();
+    float *stage = reinterpret_cast<float *>(wv < 2 ? &sBh[0][0] : &sBl[0][0]) +
+                   (wv & 1) * (16 * 128);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+#pragma unroll
+      for (int j = 0; j < 8; j++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) stage[(drow + q) * 128 + 16 * j + dcol] = acc[i][j][q];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+      for (int t = 0; t < 8; t++) {
+        const int f = lane + 64 * t;
+        const int r = f >> 5, c = (f & 31) * 4;
+        const int row = m0 + wm + 16 * i + r, col = n0 + wn + c;
+        if (row < M && col < N)
+          *reinterpret_cast<float4 *>(Z + (int64_t)row * ldz + col) =
+              reinterpret_cast<const float4 *>(stage)[f];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  } else if (!gp.rs && !gp.cs) {
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
