@@ -43,7 +43,8 @@ extern "C" {
  * ADJOINT_FP32  adjoint, per-pair work and contractions in FP32 (Kahan
  *               single-bond sums), norms and reductions in FP64.
  * ADJOINT_MIXED adjoint; dense contractions on BF16 MFMA with FP32
- *               accumulation and hi+lo splitting; the rest as ADJOINT_FP32. */
+ *               accumulation and hi+lo splitting; the rest as ADJOINT_FP32,
+ *               with uncompensated single-bond sums. */
 #define FLARE_FORM_PLAIN_FP64 0
 #define FLARE_FORM_ADJOINT_FP64 1
 #define FLARE_FORM_ADJOINT_FP32 2
