@@ -17,12 +17,18 @@ class MappedEnergy(torch.autograd.Function):
     positions (n, 3) float64 on the GPU, beta the unpacked coefficients
     (S, F, F) float64, pair a :class:`PairFLARE`, species (n,) int32 on the
     GPU.  Gradients flow to positions and beta; the virial (xx yy zz xy xz
-    yz) is returned as a non-differentiable output."""
+    yz) is returned as a non-differentiable output.  Only the symmetric part
+    of beta enters E = d^T beta d, and the force kernel assumes a symmetric
+    beta, so (beta + beta^T) / 2 is evaluated; dE/dbeta = d d^T holds for
+    any beta."""
 
     @staticmethod
     def forward(ctx, positions, beta, pair, cell, species, form):
         pos = positions.detach().contiguous()
-        b = beta.detach().contiguous()
+        b = beta.detach()
+        if b.dim() == 3:
+            b = 0.5 * (b + b.transpose(1, 2))
+        b = b.contiguous()
         n = pos.shape[0]
         dev = pos.device
         pair.build_neighbors(cell, pos, species)

@@ -9,6 +9,7 @@
 #include <cstring>
 #include <ctime>
 #include <fstream>
+#include <atomic>
 #include <map>
 #include <memory>
 #include <sstream>
@@ -55,6 +56,10 @@ static int guarded(Fn &&fn) {
     return FLARE_EINVAL;
   }
 }
+
+namespace {
+std::atomic<uint64_t> g_eval_gen{0};  // mapped evaluations, numbered
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // context
@@ -107,6 +112,9 @@ struct flare_ctx {
   DevBuf ynorm_d, ynorm_f;
   std::map<std::string, std::unique_ptr<DevBuf>> ws;
   std::map<int64_t, std::unique_ptr<DevBuf>> b2idx;
+  // Generation of the last mapped evaluation that wrote this context's
+  // "pair.*" workspace (0: none); flare_pair_beta_grad reads those rows.
+  uint64_t pair_ws_gen = 0;
 
   void *buf(const std::string &key, size_t bytes) {
     auto &p = ws[key];
@@ -559,9 +567,17 @@ struct flare_pair {
   std::string radial = "chebyshev", cutoff_name = "quadratic";
   std::vector<double> cutoffs;
   DevBuf beta, beta_f32, beta_h, beta_l;  // FP64, FP32, BF16 hi/lo (K padded)
+  // The last evaluation: its descriptor rows (B, or B/|B| in FP32, or the
+  // BF16 hi + lo split) stay in the evaluating context's "pair.*" workspace
+  // until a later evaluation overwrites them; last_gen numbers the
+  // evaluation so flare_pair_beta_grad can tell.
   bool have_last = false, last_mixed = false;
   int last_prec = 0, last_n = 0;
   int last_off[flare::kMaxSpecies + 1] = {0};
+  const flare_ctx *last_ctx = nullptr;
+  uint64_t last_gen = 0;
+  const void *last_B = nullptr, *last_xh = nullptr, *last_xl = nullptr;
+  const double *last_norm2 = nullptr;
   std::unique_ptr<flare_nlist> ext;  // list built from a caller's neighbor list
 };
 
@@ -1447,6 +1463,9 @@ static PairEval pair_eval(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
                           int pair_pred) {
   REQUIRE(form >= FLARE_FORM_PLAIN_FP64 && form <= FLARE_FORM_ADJOINT_MIXED,
           "unknown form");
+  // the "pair.*" rows are about to be overwritten: no earlier evaluation
+  // can be differentiated from them any more
+  ctx->pair_ws_gen = ++g_eval_gen;
   PairEval ev;
   flare_b2_params bp;
   bp.n_species = pr->S;
@@ -1559,12 +1578,20 @@ static PairEval pair_eval(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
   return ev;
 }
 
-static void pair_remember(flare_pair *pr, const PairEval &ev, int n) {
+static void pair_remember(flare_ctx *ctx, flare_pair *pr, const PairEval &ev, int n) {
+  const uint64_t gen = ++g_eval_gen;
   pr->have_last = true;
   pr->last_mixed = ev.mixed;
   pr->last_prec = ev.prec;
   pr->last_n = n;
   for (int s = 0; s <= ev.p.S; s++) pr->last_off[s] = ev.off[s];
+  pr->last_ctx = ctx;
+  pr->last_gen = gen;
+  pr->last_B = ev.fw.B;
+  pr->last_xh = ev.fw.xh;
+  pr->last_xl = ev.fw.xl;
+  pr->last_norm2 = ev.fw.norm2;
+  ctx->pair_ws_gen = gen;
 }
 
 extern "C" {
@@ -1583,7 +1610,7 @@ int flare_pair_compute(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
     if (d_eatom)
       flare::launch_rows_to_atoms(ctx->stream, n, nl->b.atom_of_row.as<int>(),
                                   ev.wi.E_row, d_eatom);
-    pair_remember(pr, ev, n);
+    pair_remember(ctx, pr, ev, n);
     ctx->finish();
   });
 }
@@ -1632,7 +1659,7 @@ int flare_pair_compute_neigh(flare_ctx *ctx, flare_pair *pr, int n_all, int inum
     flare::ext_assemble(ctx->stream, n_all, inum, d_ilist, in_cum, in_pairs, pf, own,
                         nl->b.row_of.as<int>(), ev.wi.E_row, d_forces, d_eatom);
     ctx->mark_end();
-    pair_remember(pr, ev, inum);
+    pair_remember(ctx, pr, ev, inum);
     ctx->finish();
   });
 }
@@ -1641,21 +1668,26 @@ int flare_pair_beta_grad(flare_ctx *ctx, flare_pair *pr, double *d_grad) {
   return guarded([&] {
     REQUIRE(ctx && pr && d_grad, "null argument");
     REQUIRE(pr->have_last, "no force evaluation to differentiate");
+    // the rows of this pair's last evaluation must still be where it left
+    // them: no later evaluation (of any pair) may have reused that memory
+    const bool current = ctx == pr->last_ctx && ctx->pair_ws_gen == pr->last_gen;
+    REQUIRE(current, "dE/dbeta needs the descriptor rows of this pair's last force "
+                     "evaluation, and a later evaluation (of this or another pair, on this "
+                     "context) has overwritten "
+                     "them; evaluate this pair again first");
     ctx->use();
     const int n = pr->last_n, F = pr->F;
     const int *off = pr->last_off;
-    const double *norm2 = ctx->ws["pair.norm2"]->as<double>();
+    const double *norm2 = pr->last_norm2;
     double *dhat = ctx->buf_t<double>("pair.dhat", (size_t)n * F);
     if (pr->last_prec == flare::kPrecF64)
-      flare::launch_normalize_rows_f64(ctx->stream, n, F,
-                                       ctx->ws["pair.B"]->as<double>(), norm2, dhat);
+      flare::launch_normalize_rows_f64(ctx->stream, n, F, (const double *)pr->last_B,
+                                       norm2, dhat);
     else if (pr->last_mixed)  // B/|B| as the GEMM's BF16 hi + lo rows
       flare::launch_bf16_pair_to_f64(ctx->stream, n, F, flare::bf16_padded_k(F),
-                                     ctx->ws["pair.xh"]->ptr, ctx->ws["pair.xl"]->ptr,
-                                     dhat);
+                                     pr->last_xh, pr->last_xl, dhat);
     else  // the FP32 forms keep the normalised rows B/|B|
-      flare::launch_to_f64(ctx->stream, (int64_t)n * F,
-                           ctx->ws["pair.B"]->as<float>(), dhat);
+      flare::launch_to_f64(ctx->stream, (int64_t)n * F, (const float *)pr->last_B, dhat);
     // empty environments (|B|^2 < 1e-8) carry no energy
     flare::launch_mask_rows(ctx->stream, n, F, norm2, 1e-8, dhat);
     for (int s = 0; s < pr->S; s++) {

@@ -183,7 +183,9 @@ int flare_pair_get_beta(flare_ctx *ctx, const flare_pair *pr, double *beta);
 /* PairFLARE::compute (pair_flare.cpp:68-184): energy (1), forces n x 3,
  * virial (6: xx yy zz xy xz yz, sum of (x_i - x_j) f_ij) and optionally
  * per-atom energies (n).  d_beta: optional device override of the unpacked
- * beta (S x F x F or S x F); NULL uses the loaded coefficients. */
+ * beta (S x F x F or S x F); NULL uses the loaded coefficients.  A power-2
+ * override must be symmetric, as the file's beta is by construction: the
+ * forces use w = 2 (beta B - E B) / |B|^2. */
 int flare_pair_compute(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
                        const int *d_species, int form, const double *d_beta,
                        double *d_energy, double *d_forces, double *d_virial,
