@@ -152,6 +152,8 @@ SIGNATURES = {
     "flare_pair_get_cutoffs": (_i, [_vp, _pd]),
     "flare_pair_get_beta": (_i, [_vp, _vp, _pd]),
     "flare_pair_compute": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp]),
+    "flare_pair_compute_noalloc": (_i, [_vp, _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp,
+                                        _vp]),
     "flare_pair_compute_neigh": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp, _i64, _i64,
                                       _vp, _vp, _i, _vp, _vp, _vp, _vp]),
     "flare_pair_beta_grad": (_i, [_vp, _vp, _vp]),

@@ -12,6 +12,7 @@
 #include <atomic>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -57,8 +58,38 @@ static int guarded(Fn &&fn) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Static scratch of flare_pair_compute_noalloc.  Each GPU holds its own copy
+// of the array at its own address; host code uses only the address that
+// hipGetSymbolAddress gives for the device (the array's name is a host-side
+// stand-in).  The address is looked up once per device, with that device
+// current, when the first context on it is created.
+
+static __device__ __align__(256) unsigned char scratch[FLARE_NOALLOC_SCRATCH_BYTES];
+
 namespace {
-std::atomic<uint64_t> g_eval_gen{0};  // mapped evaluations, numbered
+std::mutex g_scratch_mu;
+std::map<int, char *> g_scratch_ptr;       // device -> address of its array
+std::map<int, uint64_t> g_scratch_gen;     // device -> last evaluation in it
+std::atomic<uint64_t> g_eval_gen{0};       // mapped evaluations, numbered
+
+void scratch_setup(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr.count(device)) return;
+  char *ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    throw Error{FLARE_EHIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                hipGetErrorString(e)};
+  g_scratch_ptr[device] = ptr;
+  g_scratch_gen[device] = 0;
+}
+
+char *scratch_of(int device) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(device);
+  return it == g_scratch_ptr.end() ? nullptr : it->second;
+}
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -116,7 +147,32 @@ struct flare_ctx {
   // "pair.*" workspace (0: none); flare_pair_beta_grad reads those rows.
   uint64_t pair_ws_gen = 0;
 
+  // While an arena is active (the no-allocation entry point), workspace
+  // requests are carved out of the device's static scratch array instead of
+  // the growable buffers: a key keeps its slot for the rest of the call.
+  struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+    std::map<std::string, std::pair<size_t, size_t>> slots;  // offset, bytes
+  };
+  Arena *arena = nullptr;
+
   void *buf(const std::string &key, size_t bytes) {
+    if (arena) {
+      auto it = arena->slots.find(key);
+      if (it != arena->slots.end()) {
+        REQUIRE(bytes <= it->second.second, "scratch slot " + key + " reused with a larger size");
+        return arena->base + it->second.first;
+      }
+      const size_t off = (arena->used + 255) & ~size_t(255);
+      REQUIRE(off <= arena->cap && bytes <= arena->cap - off,
+              "input too large for the no-allocation entry point: its scratch holds " +
+                  std::to_string(arena->cap) + " bytes and this evaluation needs more (at " +
+                  key + ")");
+      arena->slots[key] = {off, bytes};
+      arena->used = off + bytes;
+      return arena->base + off;
+    }
     auto &p = ws[key];
     if (!p) p.reset(new DevBuf);
     p->ensure(bytes);
@@ -139,6 +195,12 @@ struct flare_ctx {
     if (blocking) sync();
   }
   const int *b2_index(int S, int N, int L) {
+    if (arena) {  // no upload into a new buffer: built in the scratch array
+      const int F = flare::b2_n_descriptors(S, N, L);
+      int *code = buf_t<int>("b2idx", F);
+      flare::launch_b2_index(stream, S * N, L, code);
+      return code;
+    }
     int64_t key = ((int64_t)S << 32) | (N << 16) | L;
     auto &p = b2idx[key];
     if (!p) {
@@ -485,7 +547,13 @@ static flare::PairForces run_pair_forces(flare_ctx *c, const flare::B2Dev &p,
     // c->plain_chunk_bytes of derivatives (one range when everything fits;
     // an atom with more pairs than the budget still gets a range of its own)
     const int64_t per_pair = 3 * (int64_t)p.NH * (int64_t)sizeof(double);
-    const int64_t max_pairs = std::max<int64_t>(1, c->plain_chunk_bytes / per_pair);
+    int64_t budget = c->plain_chunk_bytes;
+    if (c->arena) {  // what the scratch has left once the gather's partials are set aside
+      const int64_t left = (int64_t)c->arena->cap - (int64_t)c->arena->used -
+                           (int64_t)flare::gather_parts(n) * 10 * (int64_t)sizeof(double) - 1024;
+      budget = std::min<int64_t>(budget, std::max<int64_t>(left, 0));
+    }
+    const int64_t max_pairs = std::max<int64_t>(1, budget / per_pair);
     std::vector<int> bounds{0, n};
     std::vector<int> cum_host;
     int64_t chunk_pairs = P;
@@ -568,11 +636,11 @@ struct flare_pair {
   std::vector<double> cutoffs;
   DevBuf beta, beta_f32, beta_h, beta_l;  // FP64, FP32, BF16 hi/lo (K padded)
   // The last evaluation: its descriptor rows (B, or B/|B| in FP32, or the
-  // BF16 hi + lo split) stay in the evaluating context's "pair.*" workspace
-  // until a later evaluation overwrites them; last_gen numbers the
-  // evaluation so flare_pair_beta_grad can tell.
-  bool have_last = false, last_mixed = false;
-  int last_prec = 0, last_n = 0;
+  // BF16 hi + lo split) stay in the evaluating context's "pair.*" workspace,
+  // or in the device's static scratch, until a later evaluation overwrites
+  // them; last_gen numbers the evaluation so flare_pair_beta_grad can tell.
+  bool have_last = false, last_mixed = false, last_scratch = false;
+  int last_prec = 0, last_n = 0, last_device = -1;
   int last_off[flare::kMaxSpecies + 1] = {0};
   const flare_ctx *last_ctx = nullptr;
   uint64_t last_gen = 0;
@@ -977,6 +1045,7 @@ int flare_ctx_create(int device, flare_ctx **out) {
     std::unique_ptr<flare_ctx> c(new flare_ctx);
     c->device = device;
     c->use();
+    scratch_setup(device);
     BLAS_CHECK(rocblas_create_handle(&c->blas));
     BLAS_CHECK(rocblas_set_stream(c->blas, c->stream));
     double tab[flare::kNormTableSize];
@@ -1463,9 +1532,14 @@ static PairEval pair_eval(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
                           int pair_pred) {
   REQUIRE(form >= FLARE_FORM_PLAIN_FP64 && form <= FLARE_FORM_ADJOINT_MIXED,
           "unknown form");
-  // the "pair.*" rows are about to be overwritten: no earlier evaluation
-  // can be differentiated from them any more
-  ctx->pair_ws_gen = ++g_eval_gen;
+  // the "pair.*" rows (or the scratch) are about to be overwritten: no
+  // earlier evaluation can be differentiated from them any more
+  if (ctx->arena) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    g_scratch_gen[ctx->device] = ++g_eval_gen;
+  } else {
+    ctx->pair_ws_gen = ++g_eval_gen;
+  }
   PairEval ev;
   flare_b2_params bp;
   bp.n_species = pr->S;
@@ -1586,12 +1660,19 @@ static void pair_remember(flare_ctx *ctx, flare_pair *pr, const PairEval &ev, in
   pr->last_n = n;
   for (int s = 0; s <= ev.p.S; s++) pr->last_off[s] = ev.off[s];
   pr->last_ctx = ctx;
+  pr->last_device = ctx->device;
   pr->last_gen = gen;
+  pr->last_scratch = ctx->arena != nullptr;
   pr->last_B = ev.fw.B;
   pr->last_xh = ev.fw.xh;
   pr->last_xl = ev.fw.xl;
   pr->last_norm2 = ev.fw.norm2;
-  ctx->pair_ws_gen = gen;
+  if (pr->last_scratch) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    g_scratch_gen[ctx->device] = gen;
+  } else {
+    ctx->pair_ws_gen = gen;
+  }
 }
 
 extern "C" {
@@ -1603,6 +1684,62 @@ int flare_pair_compute(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
   return guarded([&] {
     REQUIRE(ctx && pr, "null argument");
     ctx->use();
+    PairEval ev = pair_eval(ctx, pr, nl, d_species, form, d_beta, 1);
+    const int n = nl->n;
+    run_forces(ctx, ev.p, nl, d_species, ev.prec, ev.mode, ev.fw, ev.wi, d_forces,
+               d_energy, d_virial, nullptr, "pair");
+    if (d_eatom)
+      flare::launch_rows_to_atoms(ctx->stream, n, nl->b.atom_of_row.as<int>(),
+                                  ev.wi.E_row, d_eatom);
+    pair_remember(ctx, pr, ev, n);
+    ctx->finish();
+  });
+}
+
+int flare_pair_compute_noalloc(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
+                               const int *d_species, int form, const double *d_beta,
+                               double *d_energy, double *d_forces, double *d_virial,
+                               double *d_eatom) {
+  return guarded([&] {
+    REQUIRE(ctx && pr, "null argument");
+    REQUIRE(nl && nl->built, "neighbor list not built");
+    ctx->use();
+    char *base = scratch_of(ctx->device);
+    REQUIRE(base, "no static scratch on this device");
+    // a pending build is completed here only when that needs no new buffers
+    // (rows over the one-pass capacity are rebuilt into larger ones)
+    if (nl->pending) {
+      FLARE_HIP_CHECK(hipEventSynchronize(nl->stats_ev));
+      REQUIRE(nl->stats_host[flare::kMaxSpecies + 1] <= flare::kNbrCap,
+              "the neighbor list has atoms with more than " +
+                  std::to_string(flare::kNbrCap) +
+                  " neighbors, whose rebuild allocates: complete it first "
+                  "(flare_nlist_get_view or flare_pair_compute)");
+    }
+    // rocBLAS manages device workspace of its own: only the forms whose
+    // contraction is a hand-written kernel are served here
+    if (pr->power == 2 && form != FLARE_FORM_ADJOINT_MIXED) {
+      const bool f64 = form == FLARE_FORM_PLAIN_FP64 || form == FLARE_FORM_ADJOINT_FP64;
+      REQUIRE(pr->F <= 1024 && pr->F % (f64 ? 2 : 4) == 0,
+              "the no-allocation entry point needs F <= 1024 and F a multiple of " +
+                  std::string(f64 ? "2" : "4") + " in this form (F = " +
+                  std::to_string(pr->F) + ")");
+    }
+    flare_ctx::Arena arena;
+    arena.base = base;
+    arena.cap = FLARE_NOALLOC_SCRATCH_BYTES;
+    // stage timing would create HIP events: off for the call
+    struct Scope {
+      flare_ctx *c;
+      int profiling;
+      ~Scope() {
+        c->arena = nullptr;
+        c->profiling = profiling;
+      }
+    } scope{ctx, ctx->profiling};
+    ctx->collect();
+    ctx->profiling = 0;
+    ctx->arena = &arena;
     PairEval ev = pair_eval(ctx, pr, nl, d_species, form, d_beta, 1);
     const int n = nl->n;
     run_forces(ctx, ev.p, nl, d_species, ev.prec, ev.mode, ev.fw, ev.wi, d_forces,
@@ -1670,10 +1807,16 @@ int flare_pair_beta_grad(flare_ctx *ctx, flare_pair *pr, double *d_grad) {
     REQUIRE(pr->have_last, "no force evaluation to differentiate");
     // the rows of this pair's last evaluation must still be where it left
     // them: no later evaluation (of any pair) may have reused that memory
-    const bool current = ctx == pr->last_ctx && ctx->pair_ws_gen == pr->last_gen;
+    bool current;
+    if (pr->last_scratch) {
+      std::lock_guard<std::mutex> lk(g_scratch_mu);
+      current = ctx->device == pr->last_device && g_scratch_gen[pr->last_device] == pr->last_gen;
+    } else {
+      current = ctx == pr->last_ctx && ctx->pair_ws_gen == pr->last_gen;
+    }
     REQUIRE(current, "dE/dbeta needs the descriptor rows of this pair's last force "
                      "evaluation, and a later evaluation (of this or another pair, on this "
-                     "context) has overwritten "
+                     "context or in the device's no-allocation scratch) has overwritten "
                      "them; evaluate this pair again first");
     ctx->use();
     const int n = pr->last_n, F = pr->F;

@@ -73,6 +73,9 @@ struct NListBuffers {
 // Ints of the host-side build summary: species counts (kMaxSpecies), a
 // bad-species flag, the largest over-capacity neighbor count, the pair total.
 constexpr int kNlistStats = kMaxSpecies + 3;
+// Neighbors per atom held in LDS by the fused enumerate + sort pass; rows
+// with more go to the two-pass build.
+constexpr int kNbrCap = 128;
 // Enqueues the neighbor list (count, cum, idx, nspec, rel, center, rev,
 // keys), wrapped positions and the species-grouped row map row_of[i],
 // atom_of_row[row], species_of_row[row] without waiting; the summary is

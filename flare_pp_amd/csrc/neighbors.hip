@@ -447,9 +447,6 @@ nbr_enum_kernel(int n, WrapMats m, const double *__restrict__ wrapped,
   if (MODE == 0 && lane == 0) count[i] = found;
 }
 
-// Candidates per atom held in LDS by the fused enumerate + sort pass.
-constexpr int kNbrCap = 128;
-
 // Fused pass: enumerate one atom's neighbors (as nbr_enum_kernel), keep their
 // keys in LDS, rank-sort them and write them to the atom's padded slot row
 // (kNbrCap entries) of keys_pad; the rel vectors are recomputed from the

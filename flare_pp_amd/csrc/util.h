@@ -6,6 +6,8 @@
 namespace flare {
 
 void launch_sqrt(hipStream_t st, int n, const double *x, double *y);
+// the (c1, c2, l) table of the B2 features, as flare_ctx::b2_index uploads it
+void launch_b2_index(hipStream_t st, int nr, int L, int *code);
 void launch_rows_to_atoms(hipStream_t st, int n, const int *atom_of_row,
                           const double *x, double *y);
 void launch_b2_filter_count(hipStream_t st, const B2Dev &p, const NListDev &nl,

@@ -134,9 +134,29 @@ __global__ void transpose_kernel(int M, const double *__restrict__ a,
   b[(int64_t)j * M + i] = a[q];
 }
 
+// B2 feature table: entry k = (c1 << 20) | (c2 << 8) | l for the packed
+// pair p(c1 <= c2) and l of feature k = p * (L + 1) + l (b2.cpp:209-233)
+__global__ void b2_index_kernel(int nr, int L, int *__restrict__ code) {
+  const int F = nr * (nr + 1) / 2 * (L + 1);
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= F) return;
+  int l = k % (L + 1), p = k / (L + 1), c1 = 0;
+  while (p >= nr - c1) {  // row c1 of the packed upper triangle holds nr - c1 pairs
+    p -= nr - c1;
+    c1++;
+  }
+  code[k] = (c1 << 20) | ((c1 + p) << 8) | l;
+}
+
 void launch_sqrt(hipStream_t st, int n, const double *x, double *y) {
   if (n == 0) return;
   hipLaunchKernelGGL(sqrt_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, n, x, y);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_b2_index(hipStream_t st, int nr, int L, int *code) {
+  const int F = nr * (nr + 1) / 2 * (L + 1);
+  hipLaunchKernelGGL(b2_index_kernel, dim3(nblk(F, 256)), dim3(256), 0, st, nr, L, code);
   FLARE_HIP_CHECK(hipGetLastError());
 }
 

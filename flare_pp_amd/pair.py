@@ -118,13 +118,16 @@ class PairFLARE:
         return self._nlist
 
     def compute_device(self, d_species, form="plain", d_beta=None, energy=None,
-                       forces=None, virial=None, eatom=None, nlist=None):
-        """Force evaluation on device buffers with the last neighbor list."""
+                       forces=None, virial=None, eatom=None, nlist=None,
+                       noalloc=False):
+        """Force evaluation on device buffers with the last neighbor list.
+        noalloc: through flare_pair_compute_noalloc, whose workspace is the
+        device's static scratch (no device allocation during the call)."""
         ctx = _capi.context(self.device)
         nl = nlist if nlist is not None else self._nlist
-        check(lib().flare_pair_compute(ctx.handle, self._h, nl, dptr(d_species),
-                                       _form(form), dptr(d_beta), dptr(energy),
-                                       dptr(forces), dptr(virial), dptr(eatom)))
+        fn = lib().flare_pair_compute_noalloc if noalloc else lib().flare_pair_compute
+        check(fn(ctx.handle, self._h, nl, dptr(d_species), _form(form), dptr(d_beta),
+                 dptr(energy), dptr(forces), dptr(virial), dptr(eatom)))
 
     def compute_neigh_device(self, n_all, inum, d_ilist, d_numneigh, d_neighbors,
                              stride_i, stride_j, d_x, d_type, form="plain",

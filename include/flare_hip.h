@@ -190,6 +190,30 @@ int flare_pair_compute(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
                        const int *d_species, int form, const double *d_beta,
                        double *d_energy, double *d_forces, double *d_virial,
                        double *d_eatom);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory while they run (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): flare_pair_compute with the same
+ * arguments and results, whose workspace is carved out of a static device
+ * array of FLARE_NOALLOC_SCRATCH_BYTES instead of the context's growable
+ * buffers.  Each GPU the library runs on holds its own array until the
+ * process ends.  The call makes no hipMalloc / hipFree and no rocBLAS call;
+ * it returns FLARE_EINVAL, before any device work, for
+ *  - FP64 / FP32 power-2 forms whose contraction would need rocBLAS
+ *    (F > 1024, or F not a multiple of 2 (FP64) / 4 (FP32));
+ *  - a pending neighbor list with atoms over the one-pass row capacity
+ *    (complete it with flare_nlist_get_view first);
+ * and, once the workspace outgrows the array, for inputs that are too large
+ * (the mixed form at F = 680 needs about 7.5 KB per atom: 32,000 atoms fit;
+ * the plain form's per-pair derivatives are processed in atom ranges that
+ * fit).  Every call on a GPU shares that GPU's array and overwrites it:
+ * calls that overlap on one GPU (other streams, threads or contexts) race.
+ * Stage profiling is off during the call. */
+#define FLARE_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+int flare_pair_compute_noalloc(flare_ctx *ctx, flare_pair *pr,
+                               const flare_nlist *nl, const int *d_species,
+                               int form, const double *d_beta,
+                               double *d_energy, double *d_forces,
+                               double *d_virial, double *d_eatom);
 /* PairFLARE::compute on the caller's own full neighbor list over local and
  * ghost atoms (pair_flare.cpp:68-184; the Kokkos contract of
  * pair_flare_kokkos.cpp:83-124, 633-662): what a LAMMPS pair_style
