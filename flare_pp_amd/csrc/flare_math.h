@@ -210,6 +210,21 @@ FLARE_HD inline int b2_n_descriptors(int S, int N, int L) {
   return nr * (nr + 1) / 2 * (L + 1);
 }
 
+// Power-of-two scale of one row of the FP16 hi + lo operand splits: for the
+// row's largest magnitude m, the exponent e with m 2^e in [0.5, 1) (in
+// [1, 2) for m >= 2^127, whose inverse scale 2^128 a float cannot hold).
+// A row whose m is below the smallest normal float (all zero, or FP64
+// values that are denormal or zero as floats) would need 2^e beyond the
+// float range: it is reported as a zero row (returns false, e = 0) and
+// stored as zeros.
+FLARE_HD inline bool f16_row_scale(float m, int &e) {
+  e = 0;
+  if (!(m >= 1.17549435e-38f)) return false;  // FLT_MIN
+  std::frexp(m, &e);                          // m = f 2^e, f in [0.5, 1)
+  e = e > 127 ? -127 : -e;
+  return true;
+}
+
 // l of harmonic index lm = l*l + l + m
 FLARE_HD inline int lm_to_l(int lm) {
   int l = 0;

@@ -22,6 +22,7 @@
 //   B[k][j]: lane = j + 16 (k / 8), element k % 8
 //   D[i][j]: lane = j + 16 (i / 4), element i % 4
 #include "common.h"
+#include "flare_math.h"
 #include "mfma.h"
 
 namespace flare {
@@ -545,14 +546,11 @@ split_f16_rows_kernel(int rows, int cols, int ldx, int kp, const T *__restrict__
   float m = 0.0f;
   for (int c = lane; c < cols; c += 64) m = fmaxf(m, fabsf((float)xr[c]));
   for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-  int e = 0;
-  if (m > 0.0f) {
-    frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
-    e = -e;
-  }
+  int e;
+  const bool nonzero = f16_row_scale(m, e);
   const float sc = ldexpf(1.0f, e);
   for (int c = lane; c < kp; c += 64) {
-    const float v = c < cols ? (float)xr[c] * sc : 0.0f;
+    const float v = nonzero && c < cols ? (float)xr[c] * sc : 0.0f;
     const _Float16 h = (_Float16)v;
     hi[(int64_t)r * kp + c] = h;
     lo[(int64_t)r * kp + c] = (_Float16)(v - (float)h);
@@ -571,14 +569,11 @@ split_f16_cols_kernel(int rows, int cols, int ldo, const double *__restrict__ x,
   float m = 0.0f;
   for (int r = lane; r < rows; r += 64) m = fmaxf(m, fabsf((float)x[(int64_t)r * cols + c]));
   for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
-  int e = 0;
-  if (m > 0.0f) {
-    frexpf(m, &e);
-    e = -e;
-  }
+  int e;
+  const bool nonzero = f16_row_scale(m, e);
   const float sc = ldexpf(1.0f, e);
   for (int r = lane; r < ldo; r += 64) {
-    const float v = r < rows ? (float)x[(int64_t)r * cols + c] * sc : 0.0f;
+    const float v = nonzero && r < rows ? (float)x[(int64_t)r * cols + c] * sc : 0.0f;
     const _Float16 h = (_Float16)v;
     hi[(int64_t)c * ldo + r] = h;
     lo[(int64_t)c * ldo + r] = (_Float16)(v - (float)h);

@@ -1082,6 +1082,55 @@ extern "C" void orc_predict_mean(const orc_b2 *struc, const int *m_by_type,
   }
 }
 
+// compute_cluster_uncertainties (sparse_gp.cpp:93-133) for the grouped rows
+// rows[0..nrows) (ascending): each variance k(d, d) - |L^-1 k|^2 depends on
+// its own descriptor only, so a subset is evaluated on its own.
+extern "C" void orc_cluster_uncertainties_rows(const orc_b2 *struc,
+                                               const int *m_by_type,
+                                               const double *sparse,
+                                               const double *sparse_norms,
+                                               double sigma, double power,
+                                               const double *Linv, int nrows,
+                                               const int *rows, double *variances) {
+  const int S = struc->S, F = struc->F;
+  Grouped g;
+  grouped(struc, g, false);
+  int M = 0;
+  for (int s = 0; s < S; s++) M += m_by_type[s];
+  // the selected rows and their per-species counts (rows are species-grouped)
+  std::vector<int> off(S + 1, 0), ns(S, 0);
+  for (int s = 0; s < S; s++) off[s + 1] = off[s] + g.ns[s];
+  std::vector<double> d((size_t)nrows * F), nrm(nrows);
+  for (int q = 0; q < nrows; q++) {
+    const int r = rows[q];
+    for (int s = 0; s < S; s++)
+      if (r >= off[s] && r < off[s + 1]) ns[s]++;
+    std::copy(&g.d[(size_t)r * F], &g.d[(size_t)r * F] + F, &d[(size_t)q * F]);
+    nrm[q] = g.norms[r];
+  }
+  // K_self = diag(envs_envs(clusters, clusters))
+  std::vector<double> kself(nrows, 0.0);
+  const double empty_thresh = 1e-8;
+  for (int q = 0; q < nrows; q++) {
+    if (nrm[q] < empty_thresh) continue;
+    double dv = dotn(&d[(size_t)q * F], &d[(size_t)q * F], F);
+    kself[q] = sigma * sigma * std::pow(dv / (nrm[q] * nrm[q]), power);
+  }
+  std::vector<double> kmat((size_t)nrows * M);
+  orc_envs_envs(S, F, ns.data(), d.data(), nrm.data(), m_by_type, sparse,
+                sparse_norms, sigma, power, kmat.data());
+#pragma omp parallel for schedule(static)
+  for (int q = 0; q < nrows; q++) {
+    double qs = 0;
+    for (int u = 0; u < M; u++) {
+      double t = 0;
+      for (int v = 0; v <= u; v++) t += Linv[(size_t)u * M + v] * kmat[(size_t)q * M + v];
+      qs += t * t;
+    }
+    variances[q] = kself[q] - qs;
+  }
+}
+
 extern "C" void orc_cluster_uncertainties(const orc_b2 *struc,
                                           const int *m_by_type,
                                           const double *sparse,
@@ -1089,35 +1138,10 @@ extern "C" void orc_cluster_uncertainties(const orc_b2 *struc,
                                           double sigma, double power,
                                           const double *Linv,
                                           double *variances) {
-  const int S = struc->S, F = struc->F, n = struc->n;
-  Grouped g;
-  grouped(struc, g, false);
-  int M = 0;
-  for (int s = 0; s < S; s++) M += m_by_type[s];
-  // K_self = diag(envs_envs(clusters, clusters))
-  std::vector<double> kself(n, 0.0);
-  {
-    const double empty_thresh = 1e-8;
-    for (int j = 0; j < n; j++) {
-      double nj = g.norms[j];
-      if (nj < empty_thresh) continue;
-      double dv = dotn(&g.d[(size_t)j * F], &g.d[(size_t)j * F], F);
-      kself[j] = sigma * sigma * std::pow(dv / (nj * nj), power);
-    }
-  }
-  std::vector<double> kmat((size_t)n * M);
-  orc_envs_envs(S, F, g.ns.data(), g.d.data(), g.norms.data(), m_by_type,
-                sparse, sparse_norms, sigma, power, kmat.data());
-#pragma omp parallel for schedule(static)
-  for (int j = 0; j < n; j++) {
-    double qs = 0;
-    for (int u = 0; u < M; u++) {
-      double q = 0;
-      for (int v = 0; v <= u; v++) q += Linv[(size_t)u * M + v] * kmat[(size_t)j * M + v];
-      qs += q * q;
-    }
-    variances[j] = kself[j] - qs;
-  }
+  std::vector<int> rows(struc->n);
+  for (int r = 0; r < struc->n; r++) rows[r] = r;
+  orc_cluster_uncertainties_rows(struc, m_by_type, sparse, sparse_norms, sigma, power,
+                                 Linv, struc->n, rows.data(), variances);
 }
 
 // ---------------------------------------------------------------------------

@@ -132,6 +132,13 @@ void orc_cluster_uncertainties(const orc_b2 *struc, const int *m_by_type,
                                const double *sparse_norms, double sigma,
                                double power, const double *Linv,
                                double *variances);
+/* the same for the grouped rows rows[0..nrows) (ascending) only */
+void orc_cluster_uncertainties_rows(const orc_b2 *struc, const int *m_by_type,
+                                    const double *sparse,
+                                    const double *sparse_norms, double sigma,
+                                    double power, const double *Linv,
+                                    int nrows, const int *rows,
+                                    double *variances);
 
 /* ---- mapping coefficients: normalized_dot_product.cpp:643-759 ---------- */
 /* out: S x beta_size, beta_size = F (power 1) or F(F+1)/2 (power 2) */

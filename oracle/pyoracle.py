@@ -87,6 +87,10 @@ def lib():
             "orc_cluster_uncertainties": (None, [_vp, _i, _d, _d,
                                                  ctypes.c_double,
                                                  ctypes.c_double, _d, _d]),
+            "orc_cluster_uncertainties_rows": (None, [_vp, _i, _d, _d,
+                                                      ctypes.c_double,
+                                                      ctypes.c_double, _d,
+                                                      ctypes.c_int, _i, _d]),
             "orc_mapping_coefficients": (ctypes.c_int, [ctypes.c_int,
                                                         ctypes.c_int, _i, _d, _d,
                                                         ctypes.c_double,
@@ -338,11 +342,21 @@ def predict_mean(b2, m_by_type, sparse, sparse_norms, sigma, power, volume_,
 
 
 def cluster_uncertainties(b2, m_by_type, sparse, sparse_norms, sigma, power,
-                          Linv):
-    out = np.zeros(b2.n)
-    lib().orc_cluster_uncertainties(b2.handle, _pi(_i32(m_by_type)),
-                                    _pd(_f64(sparse)), _pd(_f64(sparse_norms)),
-                                    sigma, power, _pd(_f64(Linv)), _pd(out))
+                          Linv, rows=None):
+    """Local variances in cluster (species-grouped) order; with rows, only
+    those grouped rows (ascending), in that order."""
+    if rows is None:
+        out = np.zeros(b2.n)
+        lib().orc_cluster_uncertainties(b2.handle, _pi(_i32(m_by_type)),
+                                        _pd(_f64(sparse)), _pd(_f64(sparse_norms)),
+                                        sigma, power, _pd(_f64(Linv)), _pd(out))
+        return out
+    rows = _i32(np.sort(np.asarray(rows)))
+    out = np.zeros(len(rows))
+    lib().orc_cluster_uncertainties_rows(b2.handle, _pi(_i32(m_by_type)),
+                                         _pd(_f64(sparse)), _pd(_f64(sparse_norms)),
+                                         sigma, power, _pd(_f64(Linv)), len(rows),
+                                         _pi(rows), _pd(out))
     return out
 
 

@@ -25,6 +25,7 @@ void probe_ylm(double x, double y, double z, int L, double *Y, double *Yx,
                                    Yz[i] = gz;
                                  });
 }
+int probe_f16_row_scale(float m, int *e) { return flare::f16_row_scale(m, *e) ? 1 : 0; }
 int probe_pair_index(int c1, int c2, int nr) {
   return flare::b2_pair_index(c1, c2, nr);
 }

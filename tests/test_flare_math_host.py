@@ -23,6 +23,7 @@ def probe(tmp_path_factory):
     L.probe_radial.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int] * 2 + [pd, pd]
     L.probe_ylm.argtypes = [ctypes.c_double] * 3 + [ctypes.c_int] + [pd] * 4
     L.probe_pair_index.argtypes = [ctypes.c_int] * 3
+    L.probe_f16_row_scale.argtypes = [ctypes.c_float, ctypes.POINTER(ctypes.c_int)]
     return L
 
 
@@ -65,3 +66,19 @@ def test_pair_index_layout(probe):
             seen.append(probe.probe_pair_index(c1, c2, nr))
             assert probe.probe_pair_index(c2, c1, nr) == seen[-1]
     assert seen == list(range(nr * (nr + 1) // 2))  # b2.cpp:209-233 counter
+
+
+def test_f16_row_scale_range(probe):
+    """Row scales of the FP16 hi + lo splits stay finite: rows below the
+    smallest normal float (zero, denormal, FP64 underflow) are zero rows
+    with scale 1; others land in [0.5, 1) (in [1, 2) at the top of the float
+    range) with a scale and inverse scale a float holds."""
+    e = ctypes.c_int()
+    for m in (0.0, 1e-45, 1e-40, 1.1e-38):
+        assert probe.probe_f16_row_scale(m, ctypes.byref(e)) == 0 and e.value == 0, m
+    for m in (1.17549435e-38, 2.5e-30, 1e-3, 0.5, 0.75, 1.0, 3.0, 1e30, 3.4e38):
+        assert probe.probe_f16_row_scale(m, ctypes.byref(e)) == 1, m
+        f = np.float32(m) * np.float32(2.0) ** e.value
+        assert (0.5 <= f < 1.0 or (m >= 2.0 ** 127 and 1.0 <= f < 2.0)), m
+        assert np.isfinite(np.float32(2.0) ** -e.value), m
+        assert np.isfinite(np.float32(2.0) ** e.value), m

@@ -135,3 +135,61 @@ def test_mapped_energy_symmetrizes_beta():
         assert torch.allclose(gx, gxs, rtol=0, atol=1e-10 * gxs.abs().max().item()), form
         assert abs(E.item() - Es.item()) < 1e-10 * abs(Es.item()), form
         assert torch.allclose(gb, gb.transpose(1, 2))
+
+
+def test_gp_mean_energy_config3_finite_differences():
+    """Config 3 (32,000 atoms, 2 species, 500 sparse, sigma 2, power 2):
+    GPMeanEnergy through torch.autograd.grad against central differences
+    (h = 1e-4 A) of the FP64 energy on 64 seeded atoms x 3 components, in the
+    adjoint FP64 and FP32 forms, and against the plain form's forces (the
+    reference formulation with per-pair dB/dr, normalized_dot_product.cpp:
+    115-210; FD pattern of ctests/test_kernels.cpp:140-406).
+
+    The differences are taken of the FP64 energy: the FP32 form's energy
+    rounds at ~1e-7 of each E_i (tens of eV here), which over h = 1e-4 A is
+    noise of order 1 eV/A, so its forces are held to the FP64 differences at
+    the FP32 form's force precision instead."""
+    from flare_pp_amd import B2, NormalizedDotProduct, SparseGP, Structure
+    from flare_pp_amd.autograd import gp_mean_energy
+    c = synth.config3()
+    b2 = B2("chebyshev", "quadratic", [0, 5.0], [], c["settings"])
+    s = Structure(c["cell"], c["species"], c["positions"], 5.0, [b2])
+    gp = SparseGP([NormalizedDotProduct(c["sigma"], c["power"])], 0.1, 0.1, 0.1)
+    gp.add_specific_environments(s, c["sparse_atoms"])
+    gp.set_alpha(c["alpha"])
+    n = s.noa
+    f_plain = gp.predict_mean(s, form="plain")[1:1 + 3 * n].reshape(n, 3).copy()
+    x0 = torch.tensor(c["positions"], device="cuda:0")
+    grads = {}
+    for form in ("adjoint_fp64", "adjoint_fp32"):
+        x = x0.clone().requires_grad_(True)
+        E, stress = gp_mean_energy(x, gp, s, form)
+        (g,) = torch.autograd.grad(E, (x,))
+        grads[form] = g.cpu().numpy()
+    gmax = np.abs(grads["adjoint_fp64"]).max()
+    # -dE/dx of the adjoint equals the plain form's forces
+    assert np.abs(grads["adjoint_fp64"] + f_plain).max() < 1e-10 * gmax
+    rng = np.random.default_rng(42)
+    atoms = rng.choice(n, 64, replace=False)
+    h = 1e-4
+    fd = np.zeros((64, 3))
+    with torch.no_grad():
+        for q, a in enumerate(atoms):
+            for k in range(3):
+                xp = x0.clone()
+                xm = x0.clone()
+                xp[a, k] += h
+                xm[a, k] -= h
+                ep = gp_mean_energy(xp, gp, s, "adjoint_fp64")[0].item()
+                em = gp_mean_energy(xm, gp, s, "adjoint_fp64")[0].item()
+                fd[q, k] = (ep - em) / (2 * h)
+    # FP64: the differences' rounding, eps |E| / h, and their h^2 truncation
+    # are both far below 1e-6 of the largest force
+    e64 = np.abs(fd - grads["adjoint_fp64"][atoms]).max()
+    e32 = np.abs(fd - grads["adjoint_fp32"][atoms]).max()
+    print(f"config 3 FD ({64 * 3} components): max |F| {gmax:.3e} eV/A, adjoint FP64 "
+          f"err {e64:.3e}, adjoint FP32 err {e32:.3e} eV/A", flush=True)
+    assert e64 < 1e-6 * gmax
+    # FP32 form: force error 1e-4 of the largest force at most (its
+    # measured force error against FP64 is ~1e-6 of it)
+    assert e32 < 1e-4 * gmax

@@ -14,6 +14,21 @@ def nrel(a, b):
     return np.abs(np.asarray(a) - b).max() / max(np.abs(b).max(), 1e-300)
 
 
+def variance_bound(Kuu, kvec, sigma, jitter=1e-8):
+    """Per-row rounding bound of v = k_self - |L^-1 k|^2 (sparse_gp.cpp:
+    93-133) for two FP64 evaluations: a backward-stable Cholesky gives
+    L L^T = K + jitter I + dK with |dK| <= M eps |K|, which moves k^T (K +
+    jitter I)^-1 k by |w|^2 |dK|, w = (K + jitter I)^-1 k; the rest rounds at
+    eps sigma^2.  The conditioning of K_uu enters through |w|."""
+    from scipy.linalg import cho_factor, cho_solve
+    M = len(Kuu)
+    eps = np.finfo(np.float64).eps
+    Kj = Kuu + jitter * np.eye(M)
+    w = cho_solve(cho_factor(Kj, lower=True), np.asarray(kvec).T)
+    return 2 * (M * eps * np.linalg.norm(Kj, 2) * np.sum(w ** 2, axis=0)
+                + 16 * eps * sigma ** 2)
+
+
 def build(cell, species, pos, settings, rc, sigma, power, atoms, alpha):
     b2 = B2("chebyshev", "quadratic", [0, rc], [], settings)
     s = Structure(cell, species, pos, rc, [b2])
@@ -57,7 +72,15 @@ def test_config1_mean_and_variance(orc, cfg1, power):
                                               power), 1e-8)
     var_o = orc.cluster_uncertainties(o, m, P, Pn, c["sigma"], power, Li_o)
     _, (var,) = gp.predict_local_uncertainties(s)
-    assert np.abs(var - var_o).max() < 1e-6 * c["sigma"] ** 2
+    k_rows = orc.envs_envs(1, g["n_by_type"], g["descriptors"], g["norms"], m, P, Pn,
+                           c["sigma"], power)
+    bound = variance_bound(gp.Kuu, k_rows, c["sigma"])
+    err = np.abs(var - var_o)
+    print(f"power {power}: variance error max {err.max():.2e} sigma^2 = "
+          f"{c['sigma'] ** 2}, bound max {bound.max():.2e} median "
+          f"{np.median(bound):.2e}; cond(K_uu + jitter) "
+          f"{np.linalg.cond(gp.Kuu + 1e-8 * np.eye(len(gp.Kuu))):.2e}", flush=True)
+    assert np.all(err <= bound)
     Kuu = gp.Kuu
     K_o = orc.envs_envs(1, m, P, Pn, m, P, Pn, c["sigma"], power)
     assert nrel(Kuu, K_o) < 1e-12
@@ -116,12 +139,67 @@ def test_envs_envs(orc, cfg1):
 
 
 def test_config3_mean(orc):
+    """Config 3 (32,000 atoms, 500 sparse) at full size against the oracle:
+    for power 2 the GP mean equals the mapped pair energy of
+    beta_s = sigma^2 sum_u alpha_u p_u p_u^T (tests/test_lammps.py:97-102),
+    so the oracle's pair_flare restatement on the GPU-made coefficients
+    (themselves pinned to the oracle's mapping at config 1 size) gives E, F
+    and the stress of the same structure."""
     c = synth.config3()
     s, gp, b2 = build(c["cell"], c["species"], c["positions"], c["settings"], 5.0,
                       c["sigma"], c["power"], c["sparse_atoms"], c["alpha"])
     e_plain = gp.predict_mean(s, form="plain").copy()
     e_adj = gp.predict_mean(s, form="adjoint_fp64")
     assert nrel(e_adj, e_plain) < 1e-11
+    packed = gp.mapping_coefficients()
+    nl = orc.NeighborList(c["cell"], c["species"], c["positions"], 5.0)
+    e0, f0, v0, ea0 = orc.pair_compute(nl, c["species"], 2, 8, 4, 2,
+                                       np.full((2, 2), 5.0), packed, want_eatom=True)
+    n = s.noa
+    st = e_plain[1 + 3 * n:]
+    V = orc.volume(c["cell"])
+    errs = (abs(e_plain[0] - e0) / np.abs(ea0).sum(), nrel(e_plain[1:1 + 3 * n], f0.ravel()),
+            nrel(np.array([st[0], st[3], st[5], st[1], st[2], st[4]]) * V, v0))
+    print("config 3 GP mean vs oracle mapped: energy rel %.2e force rel %.2e "
+          "stress rel %.2e" % errs, flush=True)
+    assert errs[0] < 1e-10 and errs[1] < 1e-10 and errs[2] < 1e-10
+
+
+def test_config4_variances_oracle_subset(orc):
+    """Config 4 at full size (55,296 atoms, 3 species, 2,000 sparse): the
+    FP64 local variances of 256 seeded atoms against the oracle's
+    compute_cluster_uncertainties (sparse_gp.cpp:93-133) on the same rows,
+    within the conditioning bound of each row."""
+    c = synth.config4()
+    s, gp, b2 = build(c["cell"], c["species"], c["positions"], c["settings"],
+                      c["cutoff"], c["sigma"], c["power"], c["sparse_atoms"],
+                      c["alpha"])
+    _, (var,) = gp.predict_local_uncertainties(s, form="plain")
+    var = var.copy()
+    S = c["settings"][0]
+    nl = orc.NeighborList(c["cell"], c["species"], c["positions"], c["cutoff"])
+    o = orc.B2Result(nl, c["species"], *c["settings"], np.full((S, S), c["cutoff"]))
+    g = o.grouped()
+    off = np.concatenate([[0], np.cumsum(g["n_by_type"])])
+    sel = [[k for k, a in enumerate(g["atom_indices"][off[t]:off[t + 1]])
+            if a in set(c["sparse_atoms"])] for t in range(S)]
+    P, Pn, m = grouped_sparse(g, sel)
+    K_o = orc.envs_envs(S, m, P, Pn, m, P, Pn, c["sigma"], c["power"])
+    assert nrel(gp.Kuu, K_o) < 1e-12
+    Li_o = orc.cholesky_inverse(K_o, 1e-8)
+    rows = np.sort(np.random.default_rng(41).choice(s.noa, 256, replace=False))
+    var_o = orc.cluster_uncertainties(o, m, P, Pn, c["sigma"], c["power"], Li_o,
+                                      rows=rows)
+    ns = [int(np.sum((rows >= off[t]) & (rows < off[t + 1]))) for t in range(S)]
+    k_rows = orc.envs_envs(S, ns, g["descriptors"][rows], g["norms"][rows], m, P, Pn,
+                           c["sigma"], c["power"])
+    bound = variance_bound(K_o, k_rows, c["sigma"])
+    err = np.abs(var[rows] - var_o)
+    print(f"config 4: variance error max {err.max():.2e} (sigma^2 = "
+          f"{c['sigma'] ** 2}), bound min {bound.min():.2e} median "
+          f"{np.median(bound):.2e}; variance/sigma^2 median "
+          f"{np.median(var_o) / c['sigma'] ** 2:.2e}", flush=True)
+    assert np.all(err <= bound)
 
 
 # Low-precision forms.  FP32: t = d . p_hat in FP32; mixed: t and G = c P_hat
@@ -190,12 +268,20 @@ def test_config4_variance_mixed():
     efs64, (var64,) = gp.predict_local_uncertainties(s, form="adjoint_fp64")
     efs64 = efs64.copy()
     var64 = var64.copy()
+    s2 = c["sigma"] ** 2
     for form in ("adjoint_fp32", "adjoint_mixed"):
         efs, (var,) = gp.predict_local_uncertainties(s, form=form)
         n = s.noa
         dv = np.abs(var - var64)
         rel = dv / np.maximum(np.abs(var64), 1e-12)
-        print(f"{form}: variance abs max {dv.max():.3e} (sigma^2 = "
-              f"{c['sigma'] ** 2}), rel median {np.median(rel):.3e}, "
-              f"energy {abs(efs[0] - efs64[0]) / n * 1e3:.3e} meV/atom", flush=True)
-        assert dv.max() < LOWP_TOL[form][2] * c["sigma"] ** 2
+        # relative error by variance band: std / sigma in [0.03, 0.1) and
+        # [0.1, 1], where uncertainty thresholds are set
+        small = (var64 >= 1e-3 * s2) & (var64 < 1e-2 * s2)
+        large = var64 >= 1e-2 * s2
+        print(f"{form}: variance abs max {dv.max():.3e} (sigma^2 = {s2}), rel "
+              f"median {np.median(rel):.3e}; std/sigma in [0.03, 0.1): "
+              f"{small.sum()} atoms, rel max {rel[small].max():.3e}; >= 0.1: "
+              f"{large.sum()} atoms, rel max {rel[large].max():.3e}; energy "
+              f"{abs(efs[0] - efs64[0]) / n * 1e3:.3e} meV/atom", flush=True)
+        assert dv.max() < LOWP_TOL[form][2] * s2
+        assert rel[small].max() < 2e-2 and rel[large].max() < 2e-3, form

@@ -42,6 +42,39 @@ def test_b2_contraction_explicit(orc):
     np.testing.assert_allclose(nrm, np.linalg.norm(B, axis=1), rtol=1e-14)
 
 
+def test_single_bond_layout_independent_build(orc):
+    """The single-bond array A (b2.cpp:244-399) built independently in numpy
+    from closed forms: all periodic images by brute force, the pair kept when
+    0 < r <= rc(s_i, s_j) (row = centre species, asymmetric here), g_n =
+    T_n(r / rc) (r - rc)^2 with T_n(cos t) = cos(n t), real Y_lm of l <= 3
+    written out, accumulated into channel s_j N + n and lm = l^2 + m + l.
+    A transposed cutoff matrix, a channel taken from the centre's species or
+    a shifted lm index each fail it."""
+    from test_oracle_math import closed_forms
+    S, N, L = 2, 5, 3
+    cut = np.array([[5.0, 4.4], [4.0, 4.7]])
+    cell, species, pos = small_two_species(seed=21)
+    b = b2_of(orc, cell, species, pos, S, N, L, cutoffs=cut)
+    A_o = b.atom_order()[0]
+    H = (L + 1) ** 2
+    A = np.zeros((len(species), S * N * H))
+    shifts = np.array([[a, c, d] for a in range(-2, 3) for c in range(-2, 3)
+                       for d in range(-2, 3)]) @ cell
+    for i in range(len(species)):
+        for j in range(len(species)):
+            rc = cut[species[i], species[j]]
+            for dv in pos[j] - pos[i] + shifts:
+                r = np.linalg.norm(dv)
+                if not 0 < r <= rc:
+                    continue
+                g = np.cos(np.arange(N) * np.arccos(r / rc)) * (r - rc) ** 2
+                y = closed_forms(*dv)
+                for n in range(N):
+                    for (l, m), v in y.items():
+                        A[i, (species[j] * N + n) * H + l * l + m + l] += g[n] * v
+    assert np.abs(A - A_o).max() < 1e-13 * np.abs(A).max()
+
+
 def test_b2_rotation_translation_permutation(orc):
     rng = np.random.default_rng(4)
     cell, species, pos = small_two_species(seed=1)

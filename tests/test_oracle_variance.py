@@ -99,3 +99,13 @@ def test_varmap_self_term_needs_first_sparse_env(orc):
     # same non-empty environments: the maps differ by exactly sigma^2 I
     np.testing.assert_allclose(out[1][s0] - out[0][s0], sigma ** 2 * np.eye(F),
                                atol=1e-9)
+
+
+def test_cluster_uncertainties_rows_subset(orc):
+    """The row-subset variances (used at full size on the GPU box) are the
+    full result's entries: rows of both species, one row alone."""
+    cell, species, pos, nl, P, Pn, m, Li, var, want = _setup(orc, 2, 1.3)
+    b = orc.B2Result(nl, species, S, N, L, np.full((S, S), 5.0))
+    for rows in ([0, 5, 17, 20, 31], [len(var) - 1], list(range(len(var)))):
+        sub = orc.cluster_uncertainties(b, m, P, Pn, 1.3, 2, Li, rows=rows)
+        np.testing.assert_array_equal(sub, var[rows])
