@@ -2128,7 +2128,7 @@ static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p
   } else {
     cf = ctx->buf_t<float>("gpl.c", std::max<size_t>(ktot, 1));
   }
-  ctx->mark_begin("gemm");
+  ctx->mark_begin("gp_K");
   if (mixed) {
     const int kp = gp->kp;
     void *dh = ctx->buf("gpl.dh", std::max<size_t>((size_t)n * kp, 1) * 2);
@@ -2180,7 +2180,7 @@ static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p
         mixed ? c_is + off[s] : nullptr, e_row + off[s]);
   }
   ctx->mark_end();
-  ctx->mark_begin("gemm");
+  ctx->mark_begin("gp_G");
   if (mixed) {
     flare::GemmProduct g[flare::kMaxSpecies];
     for (int s = 0; s < p.S; s++) {
@@ -2208,8 +2208,10 @@ static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p
   ctx->mark_end();
   float *w = ctx->buf_t<float>("gpl.w", std::max<size_t>((size_t)n * F, 1));
   double *E_row = ctx->buf_t<double>("gpl.E", std::max(n, 1));
+  ctx->mark_begin("gp_w");
   flare::launch_gp_w_lp(ctx->stream, n, F, G, e_row, d, st.fw.norm2, gp->sig2,
                         gp->power, w, E_row);
+  ctx->mark_end();
   flare::WInput wi{0, 1, nullptr, nullptr, nullptr, w, st.fw.norm2, E_row};
   run_forces(ctx, p, nl, d_species, flare::kPrecF32, 1, st.fw, wi, d_mean_efs + 1,
              d_mean_efs, nullptr, d_mean_efs + 1 + 3 * (size_t)n, "gpl");
@@ -2255,19 +2257,27 @@ static GPState gp_mean(flare_ctx *ctx, flare_sgp *gp, const flare_b2_params *par
       continue;
     }
     double *Ks = Kraw + st.koff[s];
+    ctx->mark_begin("gp_K");
     gemm_rm(ctx, false, true, rows, Ms, F, 1.0, B + (size_t)off[s] * F, F,
             gp->P.as<double>() + (size_t)o * F, F, 0.0, Ks, Ms);
+    ctx->mark_end();
+    ctx->mark_begin("gp_epilogue");
     flare::launch_gp_epilogue(ctx->stream, rows, Ms, 0, Ks, st.fw.norm2 + off[s],
                               gp->pnorm.as<double>() + o, gp->alpha.as<double>() + o,
                               gp->sig2, gp->power, st.kval + st.koff[s],
                               cm + st.koff[s], e_row + off[s]);
+    ctx->mark_end();
+    ctx->mark_begin("gp_G");
     gemm_rm(ctx, false, false, rows, F, Ms, 1.0, cm + st.koff[s], Ms,
             gp->Phat.as<double>() + (size_t)o * F, F, 0.0, Gs, F);
+    ctx->mark_end();
   }
   double *w = ctx->buf_t<double>("gp.w", (size_t)n * F);
   double *E_row = ctx->buf_t<double>("gp.E", n);
+  ctx->mark_begin("gp_w");
   flare::launch_gp_w(ctx->stream, n, F, G, e_row, B, st.fw.norm2, gp->sig2,
                      gp->power, w, E_row);
+  ctx->mark_end();
   const int mode = form == FLARE_FORM_PLAIN_FP64 ? 0 : 1;
   flare::WInput wi{0, 0, nullptr, nullptr, nullptr, w, st.fw.norm2, E_row};
   run_forces(ctx, p, nl, d_species, flare::kPrecF64, mode, st.fw, wi,
@@ -2284,6 +2294,7 @@ int flare_sgp_predict_mean(flare_ctx *ctx, flare_sgp *gp,
     REQUIRE(ctx && gp && d_mean_efs, "null argument");
     ctx->use();
     gp_mean(ctx, gp, p, nl, d_species, form, d_mean_efs, false);
+    ctx->collect();
     ctx->sync();
   });
 }
@@ -2302,23 +2313,30 @@ int flare_sgp_predict_local_uncertainties(flare_ctx *ctx, flare_sgp *gp,
     int off[flare::kMaxSpecies + 1];
     nl->row_offsets(gp->S, off);
     double *kself = ctx->buf_t<double>("gp.kself", n);
+    ctx->mark_begin("gp_kself");
     if (st.lowp)
       flare::launch_kself_lp(ctx->stream, n, F, (const float *)st.fw.B, st.fw.norm2,
                              gp->sig2, gp->power, kself);
     else
       flare::launch_kself(ctx->stream, n, F, (const double *)st.fw.B, st.fw.norm2,
                           gp->sig2, gp->power, kself);
+    ctx->mark_end();
     double *Q = ctx->buf_t<double>("gp.Q", st.koff[gp->S]);
     for (int s = 0; s < gp->S; s++) {
       int rows = off[s + 1] - off[s], Ms = gp->m_by_type[s], o = gp->m_off[s];
       if (!rows) continue;
+      ctx->mark_begin("gp_Q");
       if (Ms)
         gemm_rm(ctx, false, false, rows, Ms, Ms, 1.0, st.kval + st.koff[s], Ms,
                 gp->Linv_cm.as<double>() + (size_t)o * M + o, M, 0.0,
                 Q + st.koff[s], Ms);
+      ctx->mark_end();
+      ctx->mark_begin("gp_var");
       flare::launch_row_sqsum_sub(ctx->stream, rows, Ms, Q + st.koff[s],
                                   kself + off[s], d_variances + off[s]);
+      ctx->mark_end();
     }
+    ctx->collect();
     ctx->sync();
   });
 }

@@ -51,7 +51,13 @@ extern "C" int flare_synth_rsa(int n, double box, double d_min, uint64_t seed,
   const double edge = box / nc;
   if (nc < 3) return FLARE_EINVAL;  // neighbor cells must be distinct
   const double d2 = d_min * d_min;
-  std::vector<std::vector<int>> cells((size_t)nc * nc * nc);
+  // cells of edge in [d_min, 2 d_min): at most 14 points fit in one with the
+  // minimum distance, so a fixed 16 slots per cell (flat, no per-cell heap
+  // blocks) holds every cell
+  constexpr int kSlots = 16;
+  const size_t n_cells = (size_t)nc * nc * nc;
+  std::vector<int> slot((size_t)n_cells * kSlots);
+  std::vector<unsigned char> fill(n_cells, 0);
   Xoshiro rng(seed);
   int have = 0;
   int64_t attempts = 0;
@@ -63,13 +69,15 @@ extern "C" int flare_synth_rsa(int n, double box, double d_min, uint64_t seed,
     int ci[3];
     for (int a = 0; a < 3; a++) ci[a] = std::min(nc - 1, (int)(c[a] / edge));
     bool ok = true;
-    for (int dx = -1; dx <= 1 && ok; dx++)
-      for (int dy = -1; dy <= 1 && ok; dy++)
+    for (int dx = -1; dx <= 1 && ok; dx++) {
+      const int x = (ci[0] + dx + nc) % nc;
+      for (int dy = -1; dy <= 1 && ok; dy++) {
+        const int y = (ci[1] + dy + nc) % nc;
         for (int dz = -1; dz <= 1 && ok; dz++) {
-          const int x = (ci[0] + dx + nc) % nc, y = (ci[1] + dy + nc) % nc,
-                    z = (ci[2] + dz + nc) % nc;
-          for (int j : cells[((size_t)x * nc + y) * nc + z]) {
-            const double *p = positions + 3 * (size_t)j;
+          const size_t cell = ((size_t)x * nc + y) * nc + (ci[2] + dz + nc) % nc;
+          const int *js = &slot[cell * kSlots];
+          for (int q = 0; q < fill[cell]; q++) {
+            const double *p = positions + 3 * (size_t)js[q];
             const double u = wrap(c[0] - p[0]), v = wrap(c[1] - p[1]),
                          w = wrap(c[2] - p[2]);
             if (u * u + v * v + w * w < d2) {
@@ -78,9 +86,13 @@ extern "C" int flare_synth_rsa(int n, double box, double d_min, uint64_t seed,
             }
           }
         }
+      }
+    }
     if (!ok) continue;
+    const size_t cell = ((size_t)ci[0] * nc + ci[1]) * nc + ci[2];
+    if (fill[cell] == kSlots) return FLARE_EINVAL;  // not reachable with edge < 2 d_min
     std::memcpy(positions + 3 * (size_t)have, c, sizeof(c));
-    cells[((size_t)ci[0] * nc + ci[1]) * nc + ci[2]].push_back(have);
+    slot[cell * kSlots + fill[cell]++] = have;
     have++;
   }
   return FLARE_OK;

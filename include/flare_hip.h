@@ -113,7 +113,8 @@ int flare_ctx_set_blocking(flare_ctx *ctx, int blocking);
  * (pair_flare_kokkos.cpp:117-124); results do not depend on it. */
 int flare_ctx_set_plain_chunk(flare_ctx *ctx, int64_t bytes);
 /* Stage timing: with profiling enabled every stage (nlist, forward, gemm,
- * energy_w, pair_dA, forces_*, gather_reduce) is bracketed by HIP events on
+ * pair_dA, forces_*, gather_reduce; in the sparse GP gp_K, gp_epilogue,
+ * gp_G, gp_w, gp_kself, gp_Q, gp_var) is bracketed by HIP events on
  * the context stream; enabling or disabling resets the totals.  Stage
  * `index` (0-based, in order of first appearance) reports its name, total
  * milliseconds and count.  No counterpart in the reference. */
