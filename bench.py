@@ -118,7 +118,7 @@ KERNELS = {
     ("forward", "adjoint_fp32"): "b2_forward_mfma_kernel<float",
     ("forward", "adjoint_fp64"): "b2_forward_mfma_kernel<double",
     ("forward", "plain"): "b2_forward_mfma_kernel<double",
-    ("gemm", "adjoint_mixed"): "gemm_x3_kernel",
+    ("gemm", "adjoint_mixed"): "gemm_x3",
     ("forces_adjoint", "adjoint_mixed"): "force_kernel<float, 1,",
     ("forces_adjoint", "adjoint_fp32"): "force_kernel<float, 1,",
     ("forces_adjoint", "adjoint_fp64"): "force_kernel<double, 1,",

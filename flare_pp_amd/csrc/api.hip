@@ -1601,8 +1601,12 @@ static PairEval pair_eval(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
         g[s] = flare::GemmProduct{off[s + 1] - off[s], F, kp, F,
                                   (int64_t)off[s] * kp, (int64_t)s * F * kp,
                                   (int64_t)off[s] * F};
-      flare::gemm_bf16x3_batched(ctx->stream, p.S, g, fw.xh, fw.xl, bh, bl,
-                                 (float *)Z);
+      // wide rows: 256 x 352 tiles (one round of workgroups at F = 680)
+      if (F > 256)
+        flare::gemm_bf16x3_wide(ctx->stream, p.S, g, fw.xh, fw.xl, bh, bl, (float *)Z);
+      else
+        flare::gemm_bf16x3_batched(ctx->stream, p.S, g, fw.xh, fw.xl, bh, bl,
+                                   (float *)Z);
     }
     // FP64 and FP32: the hand-written matrix-core GEMM where it was measured
     // faster than rocBLAS (F = 680 at 32,000 atoms: FP64 0.575 vs 0.636 ms,

@@ -31,6 +31,11 @@ struct GemmProduct {
 void gemm_bf16x3_batched(hipStream_t st, int nb, const GemmProduct *prods,
                          const void *Ah, const void *Al, const void *Bh,
                          const void *Bl, float *Z);
+// The same contraction with 256 x 352 workgroup tiles for wide rows (the
+// mapped Z = D beta, N = F); no row / column scales.
+void gemm_bf16x3_wide(hipStream_t st, int nb, const GemmProduct *prods,
+                      const void *Ah, const void *Al, const void *Bh,
+                      const void *Bl, float *Z);
 // FP64 / FP32 products Z = A B^T on the matrix cores (row stride of A and
 // B = K, rows a multiple of 16 bytes; no hi/lo parts).
 void dgemm_batched(hipStream_t st, int nb, const GemmProduct *prods, const double *A,

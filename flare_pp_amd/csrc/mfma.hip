@@ -533,6 +533,251 @@ void gemm_f16x3_batched(hipStream_t st, int nb, const GemmProduct *prods,
   gemm_x3_batched<_Float16>(st, nb, prods, Ah, Al, Bh, Bl, Z);
 }
 
+// ---------------------------------------------------------------------------
+// Wide-row BF16 x 3 GEMM for the mapped contraction Z = D beta (N = F of a
+// few hundred to a few thousand columns, M = atoms of a species).
+//
+// Tile 256 x 352 x 32 per workgroup of eight wavefronts in 4 x 2 (two per
+// SIMD), each 64 x 176 = 4 x 11 MFMA tiles (176 FP32 accumulators per lane,
+// all in AGPRs), one workgroup per CU (its 152 KB of LDS).  At F = 680 two
+// column tiles cover a row (704 columns, the last 24 discarded) and the
+// 32,000-atom workload is 252 workgroups: a single round on 256 CUs, where
+// 128 x 256 tiles at two per CU need 1.5 rounds.
+//
+// Operands move global -> LDS without registers (global_load_lds_dwordx4:
+// lane l of an instruction fills bytes [16 l, 16 l + 16) of a 1 KB LDS run),
+// into two stages of 76 KB: the loads of step s + 1 are issued before the
+// MFMAs of step s, then one wait and one barrier end the step.  A stage
+// holds Ah, Al (256 rows each), Bh, Bl (352 rows), 64-byte rows whose four
+// 16-byte chunks are stored at chunk c ^ ((r >> 2) & 3), as swz() above.
+// Rows past M or N load a clamped in-range row: they only reach output
+// rows / columns that are not stored.
+constexpr int kWM = 256, kWN = 352, kWK = 32;
+constexpr int kWaves = 8, kWaveM = 64, kWaveN = 176;
+constexpr int kFragM = kWaveM / 16, kFragN = kWaveN / 16;
+constexpr int kWRowBytes = kWK * 2;                            // 64
+constexpr int kWStage = (2 * kWM + 2 * kWN) * kWRowBytes;      // 77,824
+constexpr int kWEpiStride = kWaveN + 4;  // FP32 staging row (banks offset)
+static_assert((kWM * 4) % 64 == 0 && (kWN * 4) % 64 == 0,
+              "each 64-chunk load run lies inside one operand block");
+
+// Per-lane byte offsets of the operand chunks a wavefront loads: the hi
+// and lo runs of a block share them (same rows and chunks, other base), and
+// the K step enters through the wave-uniform base.  Wavefront w loads A
+// runs w + 8 t (t < 2) of each of Ah, Al, and B runs w + 8 t (t < 3, run
+// < 22) of each of Bh, Bl.
+constexpr int kWRunsA = kWM * 4 / 64, kWRunsB = kWN * 4 / 64;  // 16, 22
+constexpr int kWPerA = kWRunsA / kWaves, kWPerB = (kWRunsB + kWaves - 1) / kWaves;
+static_assert(kWRunsA % kWaves == 0, "A runs split evenly over the wavefronts");
+
+struct WideOffsets {
+  uint32_t a[kWPerA], b[kWPerB];
+};
+
+__device__ inline WideOffsets wide_offsets(int M, int N, int Kp, int m0, int n0, int wv,
+                                           int lane) {
+  WideOffsets o;
+#pragma unroll
+  for (int t = 0; t < kWPerA; t++) {
+    const int q = (wv + kWaves * t) * 64 + lane;  // chunk within the block
+    const int row = q >> 2, kc = (q & 3) ^ ((row >> 2) & 3);
+    o.a[t] = (uint32_t)(min(m0 + row, M - 1) * Kp + kc * 8) * 2u;
+  }
+#pragma unroll
+  for (int t = 0; t < kWPerB; t++) {
+    const int run = min(wv + kWaves * t, kWRunsB - 1);
+    const int q = run * 64 + lane;
+    const int row = q >> 2, kc = (q & 3) ^ ((row >> 2) & 3);
+    o.b[t] = (uint32_t)(min(n0 + row, N - 1) * Kp + kc * 8) * 2u;
+  }
+  return o;
+}
+
+__device__ inline void lds_dma16(const unsigned char *src, unsigned char *dst) {
+  __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)src,
+                                   (void __attribute__((address_space(3))) *)dst, 16, 0,
+                                   0);
+}
+
+// One stage (K columns k0 .. k0 + 31) into LDS.  Block bases are uniform
+// byte pointers already advanced to k0.
+__device__ inline void wide_stage_load(unsigned char *stage, const WideOffsets &o,
+                                       const unsigned char *Ah, const unsigned char *Al,
+                                       const unsigned char *Bh, const unsigned char *Bl,
+                                       int wv) {
+  unsigned char *sAh = stage, *sAl = stage + kWM * kWRowBytes;
+  unsigned char *sBh = stage + 2 * kWM * kWRowBytes, *sBl = sBh + kWN * kWRowBytes;
+#pragma unroll
+  for (int t = 0; t < kWPerA; t++) {
+    const int run = wv + kWaves * t;
+    lds_dma16(Ah + o.a[t], sAh + run * 1024);
+    lds_dma16(Al + o.a[t], sAl + run * 1024);
+  }
+#pragma unroll
+  for (int t = 0; t < kWPerB; t++) {
+    const int run = wv + kWaves * t;
+    if (run < kWRunsB) {
+      lds_dma16(Bh + o.b[t], sBh + run * 1024);
+      lds_dma16(Bl + o.b[t], sBl + run * 1024);
+    }
+  }
+}
+
+template <typename E>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2)))
+gemm_x3w_kernel(GemmBatch bt, const E *__restrict__ Ah0, const E *__restrict__ Al0,
+                const E *__restrict__ Bh0, const E *__restrict__ Bl0,
+                float *__restrict__ Z0) {
+  typedef typename Frag<E>::V V8;
+  __shared__ __align__(16) unsigned char smem[2 * kWStage];
+  const int lane = threadIdx.x & 63;
+  // wave-uniform (scalar) so the per-wavefront load runs branch on SGPRs
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wm = (wv >> 1) * kWaveM, wn = (wv & 1) * kWaveN;
+  int T, dummy;
+  tile_of_block(bt.tile0[bt.nb], 1, T, dummy);
+  if (T >= bt.tile0[bt.nb]) return;
+  int q = 0;
+  while (T >= bt.tile0[q + 1]) q++;
+  const GemmProduct &gp = bt.g[q];
+  const int M = gp.M, N = gp.N, Kp = gp.K, ldz = gp.ldz;
+  const int nct = (N + kWN - 1) / kWN;
+  static_assert(kWaves == 2 * (kWM / kWaveM) && 2 * kWaveN == kWN, "4 x 2 wavefronts");
+  const int local = T - bt.tile0[q];
+  const int rt = local / nct, ct = local - rt * nct;
+  const E *Ah = Ah0 + gp.a_off, *Al = Al0 + gp.a_off;
+  const E *Bh = Bh0 + gp.b_off, *Bl = Bl0 + gp.b_off;
+  float *Z = Z0 + gp.z_off;
+  const int m0 = rt * kWM, n0 = ct * kWN;
+  const int steps = Kp / kWK;
+
+  floatx4 acc[kFragM][kFragN];
+#pragma unroll
+  for (int i = 0; i < kFragM; i++)
+#pragma unroll
+    for (int j = 0; j < kFragN; j++) acc[i][j] = floatx4{0, 0, 0, 0};
+
+  // operand rows of this lane: wm + 16 i + fr (A), wn + 16 j + fr (B); the
+  // lane's chunk is lane >> 4 (elements 8 (lane >> 4) .. + 7 of the K step)
+  const int fr = lane & 15, gch = lane >> 4;
+  const int offA = (wm + fr) * kWRowBytes + ((gch ^ (((wm + fr) >> 2) & 3)) << 4);
+  const int offB = (wn + fr) * kWRowBytes + ((gch ^ (((wn + fr) >> 2) & 3)) << 4);
+  // (16 i rows later the swizzle ((r >> 2) & 3) is unchanged: 16 i >> 2 is a
+  // multiple of 4)
+
+  const WideOffsets lo = wide_offsets(M, N, Kp, m0, n0, wv, lane);
+  const unsigned char *gAh = reinterpret_cast<const unsigned char *>(Ah);
+  const unsigned char *gAl = reinterpret_cast<const unsigned char *>(Al);
+  const unsigned char *gBh = reinterpret_cast<const unsigned char *>(Bh);
+  const unsigned char *gBl = reinterpret_cast<const unsigned char *>(Bl);
+  wide_stage_load(smem, lo, gAh, gAl, gBh, gBl, wv);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  for (int s = 0; s < steps; s++) {
+    unsigned char *cur = smem + (s & 1) * kWStage;
+    if (s + 1 < steps) {
+      const int kb = (s + 1) * kWK * 2;  // bytes
+      wide_stage_load(smem + ((s + 1) & 1) * kWStage, lo, gAh + kb, gAl + kb, gBh + kb,
+                      gBl + kb, wv);
+    }
+    const unsigned char *sAh = cur, *sAl = cur + kWM * kWRowBytes;
+    const unsigned char *sBh = cur + 2 * kWM * kWRowBytes;
+    const unsigned char *sBl = sBh + kWN * kWRowBytes;
+    V8 ah[kFragM], al[kFragM];
+#pragma unroll
+    for (int i = 0; i < kFragM; i++) {
+      ah[i] = *reinterpret_cast<const V8 *>(sAh + offA + i * 16 * kWRowBytes);
+      al[i] = *reinterpret_cast<const V8 *>(sAl + offA + i * 16 * kWRowBytes);
+    }
+    // column j's B fragments are read while column j - 1's MFMAs run
+    V8 bh = *reinterpret_cast<const V8 *>(sBh + offB);
+    V8 bl = *reinterpret_cast<const V8 *>(sBl + offB);
+#pragma unroll
+    for (int j = 0; j < kFragN; j++) {
+      V8 nbh = bh, nbl = bl;
+      if (j + 1 < kFragN) {
+        nbh = *reinterpret_cast<const V8 *>(sBh + offB + (j + 1) * 16 * kWRowBytes);
+        nbl = *reinterpret_cast<const V8 *>(sBl + offB + (j + 1) * 16 * kWRowBytes);
+      }
+      // three independent passes over the accumulators of column j
+#pragma unroll
+      for (int i = 0; i < kFragM; i++) acc[i][j] = mma32(al[i], bh, acc[i][j]);
+#pragma unroll
+      for (int i = 0; i < kFragM; i++) acc[i][j] = mma32(ah[i], bl, acc[i][j]);
+#pragma unroll
+      for (int i = 0; i < kFragM; i++) acc[i][j] = mma32(ah[i], bh, acc[i][j]);
+      bh = nbh;
+      bl = nbl;
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+  }
+
+  // Z through LDS, 16 rows of the wavefront tile at a time, 16-byte stores
+  float *stage = reinterpret_cast<float *>(smem) + wv * 16 * kWEpiStride;
+  const int dcol = lane & 15, drow = (lane >> 4) * 4;
+  const bool vec4 = (N & 3) == 0 && (ldz & 3) == 0 && (gp.z_off & 3) == 0;
+#pragma unroll
+  for (int i = 0; i < kFragM; i++) {
+#pragma unroll
+    for (int j = 0; j < kFragN; j++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) stage[(drow + e) * kWEpiStride + 16 * j + dcol] = acc[i][j][e];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    constexpr int kVec = kWaveN / 4;  // float4 per staged row
+#pragma unroll
+    for (int t = 0; t < 16 * kVec / 64; t++) {
+      const int f = lane + 64 * t;
+      const int r = f / kVec, c = (f - r * kVec) * 4;
+      const int row = m0 + wm + 16 * i + r, col = n0 + wn + c;
+      const float4 v = *reinterpret_cast<const float4 *>(&stage[r * kWEpiStride + c]);
+      if (row < M) {
+        float *dst = Z + (int64_t)row * ldz + col;
+        if (vec4) {
+          if (col < N) *reinterpret_cast<float4 *>(dst) = v;
+        } else {
+          if (col < N) dst[0] = v.x;
+          if (col + 1 < N) dst[1] = v.y;
+          if (col + 2 < N) dst[2] = v.z;
+          if (col + 3 < N) dst[3] = v.w;
+        }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+static_assert(16 * (kWaveN / 4) % 64 == 0, "epilogue rows split evenly over lanes");
+static_assert(kWaves * 16 * kWEpiStride * 4 <= 2 * kWStage, "epilogue staging fits the LDS");
+
+void gemm_bf16x3_wide(hipStream_t st, int nb, const GemmProduct *prods, const void *Ah,
+                      const void *Al, const void *Bh, const void *Bl, float *Z) {
+  FLARE_REQUIRE(nb >= 0 && nb <= kGemmMaxBatch, "gemm_x3w: at most 8 products");
+  GemmBatch bt;
+  bt.nb = 0;
+  int tiles = 0;
+  for (int q = 0; q < nb; q++) {
+    const GemmProduct &g = prods[q];
+    FLARE_REQUIRE(g.K % kWK == 0, "gemm_x3w: K must be padded to 32");
+    FLARE_REQUIRE(!g.rs && !g.cs, "gemm_x3w: no row / column scales");
+    if (g.M <= 0 || g.N <= 0) continue;
+    bt.g[bt.nb] = g;
+    bt.tile0[bt.nb] = tiles;
+    tiles += (g.M + kWM - 1) / kWM * ((g.N + kWN - 1) / kWN);
+    bt.nb++;
+  }
+  bt.tile0[bt.nb] = tiles;
+  if (tiles == 0) return;
+  const int grid = (tiles + 7) / 8 * 8;
+  hipLaunchKernelGGL(gemm_x3w_kernel<__bf16>, dim3(grid), dim3(64 * kWaves), 0, st, bt,
+                     (const __bf16 *)Ah, (const __bf16 *)Al, (const __bf16 *)Bh,
+                     (const __bf16 *)Bl, Z);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
 // FP16 hi/lo split of rows with a power-of-two scale per row: the row's
 // largest |x| is brought into [0.5, 1); inv_scale[r] = 2^-e.  One wavefront
 // per row; X is FP64 or FP32 (row stride ldx), out rows of kp (zero padded).
