@@ -728,6 +728,13 @@ struct flare_sgp {
   std::vector<int> mp;
   std::vector<int64_t> tpoff;
   DevBuf Phat_f, Phat_h, Phat_l, Phat_is, PhatT_h, PhatT_l, PhatT_is;
+  // matrix-core operands, built on first use: per species P_hat^T (F rows
+  // of ldk[s], at toff[s]; FP64 and FP32) and L^-1 rows (M_s rows of
+  // ldk[s], at loff[s]), ldk[s] = M_s rounded up to 16
+  bool have_pad = false;
+  std::vector<int> ldk;
+  std::vector<int64_t> toff, loff;
+  DevBuf PhatT_d, PhatT_f, Linv_rm;
 };
 
 // ---------------------------------------------------------------------------
@@ -2055,7 +2062,7 @@ int flare_sgp_get_Kuu(flare_ctx *ctx, const flare_sgp *gp, double *d_out) {
 
 struct GPState {
   Forward fw;
-  double *kval = nullptr;  // per species blocks rows_s x M_s
+  double *kval = nullptr;  // per species blocks rows_s x ldk_s
   std::vector<size_t> koff;
   bool lowp = false;
 };
@@ -2092,45 +2099,92 @@ static void sgp_build_lowp(flare_ctx *ctx, flare_sgp *gp) {
   gp->have_lowp = true;
 }
 
+// Operands of the matrix-core GP products, built once per model: per
+// species the transposed normalised sparse rows P_hat^T (F rows of ldk_s,
+// FP64 and FP32) and the species' block of L^-1 as rows of ldk_s (row-major,
+// lower triangular), ldk_s = M_s rounded up to 16 (zero padding).
+static void sgp_build_pad(flare_ctx *ctx, flare_sgp *gp) {
+  if (gp->have_pad) return;
+  const int M = gp->M, F = gp->F, S = gp->S;
+  gp->ldk.assign(S, 16);
+  gp->toff.assign(S + 1, 0);
+  gp->loff.assign(S + 1, 0);
+  for (int s = 0; s < S; s++) {
+    gp->ldk[s] = (std::max(gp->m_by_type[s], 1) + 15) / 16 * 16;
+    gp->toff[s + 1] = gp->toff[s] + (int64_t)F * gp->ldk[s];
+    gp->loff[s + 1] = gp->loff[s] + (int64_t)gp->m_by_type[s] * gp->ldk[s];
+  }
+  gp->PhatT_d.ensure((size_t)std::max<int64_t>(gp->toff[S], 1) * sizeof(double));
+  gp->PhatT_f.ensure((size_t)std::max<int64_t>(gp->toff[S], 1) * sizeof(float));
+  gp->Linv_rm.ensure((size_t)std::max<int64_t>(gp->loff[S], 1) * sizeof(double));
+  for (int s = 0; s < S; s++) {
+    const int Ms = gp->m_by_type[s], o = gp->m_off[s];
+    const double *ph = gp->Phat.as<double>() + (size_t)o * F;
+    // P_hat^T: element (f, u) = P_hat[o + u][f]
+    flare::launch_pad_block(ctx->stream, F, Ms, ph, 1, F,
+                            gp->PhatT_d.as<double>() + gp->toff[s], gp->ldk[s]);
+    flare::launch_pad_block_f32(ctx->stream, F, Ms, ph, 1, F,
+                                gp->PhatT_f.as<float>() + gp->toff[s], gp->ldk[s]);
+    // L^-1 rows: element (u, v) = L^-1[o + u][o + v] (column-major source)
+    flare::launch_pad_block(ctx->stream, Ms, Ms, gp->Linv_cm.as<double>() + (size_t)o * M + o,
+                            1, M, gp->Linv_rm.as<double>() + gp->loff[s], gp->ldk[s]);
+  }
+  gp->have_pad = true;
+}
+
+// Whether the matrix-core products take these rows (16-byte K rows)
+static bool gp_mcore(const flare_sgp *gp, bool f32) { return gp->F % (f32 ? 4 : 2) == 0; }
+
 // FP32 / mixed forms of the GP mean: normalised FP32 rows d, t = d . p_hat
-// in FP32 (sgemm) or on FP16 MFMA with hi+lo splitting and power-of-two row
-// scaling, G = c P_hat the same way, k, c, e, w in FP64 from t, forces from
-// the FP32 adjoint kernel.
+// in FP32 (matrix-core GEMM with the kernel-value epilogue fused) or on FP16
+// MFMA with hi+lo splitting and power-of-two row scaling, G = c P_hat the
+// same way, k, c, e, w in FP64 from t, forces from the FP32 adjoint kernel.
 static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p,
                             const flare_nlist *nl, const int *d_species, int form,
-                            double *d_mean_efs) {
+                            double *d_mean_efs, bool want_k) {
   const bool mixed = form == FLARE_FORM_ADJOINT_MIXED;
   const int n = nl->n, F = p.F;
+  const bool mcore = !mixed && gp_mcore(gp, true);
   sgp_build_lowp(ctx, gp);
+  sgp_build_pad(ctx, gp);
   GPState st;
   st.lowp = true;
   st.fw = run_forward(ctx, p, nl, d_species, flare::kPrecF32, "gpl", nullptr, 1);
   const float *d = (const float *)st.fw.B;
   int off[flare::kMaxSpecies + 1];
   nl->row_offsets(p.S, off);
-  size_t ktot = 0, ctot = 0;
+  size_t ktot = 0, ctot = 0, rtot = 0;
   st.koff.assign(p.S + 1, 0);
-  std::vector<size_t> coff(p.S + 1, 0);
+  std::vector<size_t> coff(p.S + 1, 0), roff(p.S + 1, 0);
   for (int s = 0; s < p.S; s++) {
     st.koff[s] = ktot;
     coff[s] = ctot;
-    ktot += (size_t)(off[s + 1] - off[s]) * gp->m_by_type[s];
+    roff[s] = rtot;
+    ktot += (size_t)(off[s + 1] - off[s]) * gp->ldk[s];
     ctot += (size_t)(off[s + 1] - off[s]) * gp->mp[s];
+    rtot += (size_t)(off[s + 1] - off[s]) * gp->m_by_type[s];
   }
   st.koff[p.S] = ktot;
   coff[p.S] = ctot;
-  float *Kt = ctx->buf_t<float>("gpl.K", std::max<size_t>(ktot, 1));
+  roff[p.S] = rtot;
+  // e partial sums: one per 64 sparse columns (matrix-core path), or the
+  // row's sum itself (FP16 path)
+  int np = 1;
+  for (int s = 0; s < p.S; s++) np = std::max(np, (gp->m_by_type[s] + 63) / 64);
+  if (!mcore) np = 1;
   st.kval = ctx->buf_t<double>("gpl.kval", std::max<size_t>(ktot, 1));
   float *G = ctx->buf_t<float>("gpl.G", std::max<size_t>((size_t)n * F, 1));
-  double *e_row = ctx->buf_t<double>("gpl.e", std::max(n, 1));
-  float *cf = nullptr, *c_is = nullptr;
+  double *e_part = ctx->buf_t<double>("gpl.e", std::max<size_t>((size_t)n * np, 1));
+  float *cf = nullptr, *c_is = nullptr, *Kt = nullptr;
   void *ch = nullptr, *cl = nullptr;
   if (mixed) {
     ch = ctx->buf("gpl.ch", std::max<size_t>(ctot, 1) * 2);
     cl = ctx->buf("gpl.cl", std::max<size_t>(ctot, 1) * 2);
     c_is = ctx->buf_t<float>("gpl.c_is", std::max(n, 1));
+    Kt = ctx->buf_t<float>("gpl.K", std::max<size_t>(rtot, 1));
   } else {
     cf = ctx->buf_t<float>("gpl.c", std::max<size_t>(ktot, 1));
+    if (!mcore) Kt = ctx->buf_t<float>("gpl.K", std::max<size_t>(rtot, 1));
   }
   ctx->mark_begin("gp_K");
   if (mixed) {
@@ -2143,19 +2197,42 @@ static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p
     for (int s = 0; s < p.S; s++) {
       g[s] = flare::GemmProduct{off[s + 1] - off[s], gp->m_by_type[s], kp,
                                 gp->m_by_type[s], (int64_t)off[s] * kp,
-                                (int64_t)gp->m_off[s] * kp, (int64_t)st.koff[s]};
+                                (int64_t)gp->m_off[s] * kp, (int64_t)roff[s]};
       g[s].rs = d_is + off[s];
       g[s].cs = gp->Phat_is.as<float>() + gp->m_off[s];
     }
     flare::gemm_f16x3_batched(ctx->stream, p.S, g, dh, dl, gp->Phat_h.ptr,
                               gp->Phat_l.ptr, Kt);
+  } else if (mcore) {
+    // t = d . p_hat with the kernel values, c = alpha t^(xi-1) and the
+    // partial energies in the GEMM's epilogue
+    flare::GemmProduct g[flare::kMaxSpecies];
+    flare::DEpilogue ep[flare::kMaxSpecies];
+    for (int s = 0; s < p.S; s++) {
+      const int o = gp->m_off[s];
+      g[s] = flare::GemmProduct{off[s + 1] - off[s], gp->m_by_type[s], F, 0,
+                                (int64_t)off[s] * F, (int64_t)o * F, 0};
+      ep[s].mode = flare::DEpilogue::kKernel;
+      ep[s].norm2_rows = st.fw.norm2 + off[s];
+      ep[s].pnorm = gp->pnorm.as<double>() + o;
+      ep[s].alpha = gp->alpha.as<double>() + o;
+      ep[s].sig2 = gp->sig2;
+      ep[s].power = gp->power;
+      ep[s].normalized = 1;
+      ep[s].kval = want_k ? st.kval + st.koff[s] : nullptr;  // the variance's k rows
+      ep[s].cmat = cf + st.koff[s];
+      ep[s].ldk = gp->ldk[s];
+      ep[s].part = e_part + (size_t)off[s] * np;
+      ep[s].nparts = np;
+    }
+    flare::sgemm_epi_batched(ctx->stream, p.S, g, ep, d, gp->Phat_f.as<float>());
   } else {
     for (int s = 0; s < p.S; s++) {
       const int rows = off[s + 1] - off[s], Ms = gp->m_by_type[s];
       if (!rows || !Ms) continue;
       sgemm_rm(ctx, false, true, rows, Ms, F, 1.0f, d + (size_t)off[s] * F, F,
                gp->Phat_f.as<float>() + (size_t)gp->m_off[s] * F, F, 0.0f,
-               Kt + st.koff[s], Ms);
+               Kt + roff[s], Ms);
     }
   }
   ctx->mark_end();
@@ -2164,7 +2241,11 @@ static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p
     const int rows = off[s + 1] - off[s], Ms = gp->m_by_type[s], o = gp->m_off[s];
     if (!rows) continue;
     if (!Ms) {
-      FLARE_HIP_CHECK(hipMemsetAsync(e_row + off[s], 0, rows * sizeof(double), ctx->stream));
+      // no sparse environments of this species: k, c and e are zero
+      FLARE_HIP_CHECK(hipMemsetAsync(e_part + (size_t)off[s] * np, 0,
+                                     (size_t)rows * np * sizeof(double), ctx->stream));
+      FLARE_HIP_CHECK(hipMemsetAsync(st.kval + st.koff[s], 0,
+                                     (size_t)rows * gp->ldk[s] * sizeof(double), ctx->stream));
       if (mixed) {
         FLARE_HIP_CHECK(hipMemsetAsync((_Float16 *)ch + coff[s], 0,
                                        (size_t)rows * gp->mp[s] * 2, ctx->stream));
@@ -2172,16 +2253,22 @@ static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p
                                        (size_t)rows * gp->mp[s] * 2, ctx->stream));
         FLARE_HIP_CHECK(hipMemsetAsync(c_is + off[s], 0, rows * sizeof(float),
                                        ctx->stream));
+      } else {
+        FLARE_HIP_CHECK(hipMemsetAsync(cf + st.koff[s], 0,
+                                       (size_t)rows * gp->ldk[s] * sizeof(float), ctx->stream));
       }
       continue;
     }
+    if (mcore) continue;  // done in the GEMM
+    // kval and (FP32 fallback) cf rows of ldk, or FP16 hi/lo rows of mp (mixed)
     flare::launch_gp_epilogue_lp(
-        ctx->stream, rows, Ms, gp->mp[s], Kt + st.koff[s], st.fw.norm2 + off[s],
+        ctx->stream, rows, Ms, gp->mp[s], Kt + roff[s], st.fw.norm2 + off[s],
         gp->pnorm.as<double>() + o, gp->alpha.as<double>() + o, gp->sig2,
-        gp->power, st.kval + st.koff[s], mixed ? nullptr : cf + st.koff[s],
+        gp->power, want_k ? st.kval + st.koff[s] : nullptr, gp->ldk[s],
+        mixed ? nullptr : cf + st.koff[s],
         mixed ? (void *)((_Float16 *)ch + coff[s]) : nullptr,
         mixed ? (void *)((_Float16 *)cl + coff[s]) : nullptr,
-        mixed ? c_is + off[s] : nullptr, e_row + off[s]);
+        mixed ? c_is + off[s] : nullptr, e_part + off[s]);
   }
   ctx->mark_end();
   ctx->mark_begin("gp_G");
@@ -2196,24 +2283,18 @@ static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p
     flare::gemm_f16x3_batched(ctx->stream, p.S, g, ch, cl, gp->PhatT_h.ptr,
                               gp->PhatT_l.ptr, G);
   } else {
-    for (int s = 0; s < p.S; s++) {
-      const int rows = off[s + 1] - off[s], Ms = gp->m_by_type[s];
-      if (!rows) continue;
-      if (!Ms) {
-        FLARE_HIP_CHECK(hipMemsetAsync(G + (size_t)off[s] * F, 0,
-                                       (size_t)rows * F * sizeof(float), ctx->stream));
-        continue;
-      }
-      sgemm_rm(ctx, false, false, rows, F, Ms, 1.0f, cf + st.koff[s], Ms,
-               gp->Phat_f.as<float>() + (size_t)gp->m_off[s] * F, F, 0.0f,
-               G + (size_t)off[s] * F, F);
-    }
+    // G = c P_hat on the matrix cores: c rows of ldk against P_hat^T rows
+    flare::GemmProduct g[flare::kMaxSpecies];
+    for (int s = 0; s < p.S; s++)
+      g[s] = flare::GemmProduct{off[s + 1] - off[s], F, gp->ldk[s], F,
+                                (int64_t)st.koff[s], gp->toff[s], (int64_t)off[s] * F};
+    flare::sgemm_batched(ctx->stream, p.S, g, cf, gp->PhatT_f.as<float>(), G);
   }
   ctx->mark_end();
   float *w = ctx->buf_t<float>("gpl.w", std::max<size_t>((size_t)n * F, 1));
   double *E_row = ctx->buf_t<double>("gpl.E", std::max(n, 1));
   ctx->mark_begin("gp_w");
-  flare::launch_gp_w_lp(ctx->stream, n, F, G, e_row, d, st.fw.norm2, gp->sig2,
+  flare::launch_gp_w_lp(ctx->stream, n, F, G, e_part, np, d, st.fw.norm2, gp->sig2,
                         gp->power, w, E_row);
   ctx->mark_end();
   flare::WInput wi{0, 1, nullptr, nullptr, nullptr, w, st.fw.norm2, E_row};
@@ -2232,8 +2313,10 @@ static GPState gp_mean(flare_ctx *ctx, flare_sgp *gp, const flare_b2_params *par
   REQUIRE(p.S == gp->S && p.F == gp->F, "descriptor settings differ from the sparse set");
   REQUIRE(gp->have_alpha, "alpha not set");
   if (form == FLARE_FORM_ADJOINT_FP32 || form == FLARE_FORM_ADJOINT_MIXED)
-    return gp_mean_lowp(ctx, gp, p, nl, d_species, form, d_mean_efs);
+    return gp_mean_lowp(ctx, gp, p, nl, d_species, form, d_mean_efs, want_k);
   const int n = nl->n, F = p.F;
+  const bool mcore = gp_mcore(gp, false);
+  sgp_build_pad(ctx, gp);
   GPState st;
   st.fw = run_forward(ctx, p, nl, d_species, flare::kPrecF64, "gp");
   const double *B = (const double *)st.fw.B;
@@ -2241,45 +2324,90 @@ static GPState gp_mean(flare_ctx *ctx, flare_sgp *gp, const flare_b2_params *par
   nl->row_offsets(p.S, off);
   size_t ktot = 0;
   st.koff.assign(p.S + 1, 0);
+  // e partial sums: one per 64 sparse columns on the matrix cores, the
+  // row's sum itself otherwise
+  int np = 1;
   for (int s = 0; s < p.S; s++) {
     st.koff[s] = ktot;
-    ktot += (size_t)(off[s + 1] - off[s]) * gp->m_by_type[s];
+    ktot += (size_t)(off[s + 1] - off[s]) * gp->ldk[s];
+    if (mcore) np = std::max(np, (gp->m_by_type[s] + 63) / 64);
   }
   st.koff[p.S] = ktot;
-  double *Kraw = ctx->buf_t<double>("gp.K", ktot);
-  st.kval = ctx->buf_t<double>("gp.kval", ktot);
-  double *cm = ctx->buf_t<double>("gp.C", ktot);
-  double *e_row = ctx->buf_t<double>("gp.e", n);
-  double *G = ctx->buf_t<double>("gp.G", (size_t)n * F);
+  st.kval = ctx->buf_t<double>("gp.kval", std::max<size_t>(ktot, 1));
+  double *cm = ctx->buf_t<double>("gp.C", std::max<size_t>(ktot, 1));
+  double *e_part = ctx->buf_t<double>("gp.e", std::max<size_t>((size_t)n * np, 1));
+  double *G = ctx->buf_t<double>("gp.G", std::max<size_t>((size_t)n * F, 1));
+  // species without sparse environments: k, c, e and G rows are zero
   for (int s = 0; s < p.S; s++) {
-    int rows = off[s + 1] - off[s], Ms = gp->m_by_type[s], o = gp->m_off[s];
-    if (!rows) continue;
-    double *Gs = G + (size_t)off[s] * F;
-    if (!Ms) {
-      FLARE_HIP_CHECK(hipMemsetAsync(Gs, 0, (size_t)rows * F * sizeof(double), ctx->stream));
-      FLARE_HIP_CHECK(hipMemsetAsync(e_row + off[s], 0, rows * sizeof(double), ctx->stream));
-      continue;
+    const int rows = off[s + 1] - off[s];
+    if (!rows || gp->m_by_type[s]) continue;
+    FLARE_HIP_CHECK(hipMemsetAsync(e_part + (size_t)off[s] * np, 0,
+                                   (size_t)rows * np * sizeof(double), ctx->stream));
+    FLARE_HIP_CHECK(hipMemsetAsync(st.kval + st.koff[s], 0,
+                                   (size_t)rows * gp->ldk[s] * sizeof(double), ctx->stream));
+    FLARE_HIP_CHECK(hipMemsetAsync(cm + st.koff[s], 0,
+                                   (size_t)rows * gp->ldk[s] * sizeof(double), ctx->stream));
+  }
+  ctx->mark_begin("gp_K");
+  if (mcore) {
+    // K = B P^T with k, c = alpha t^(xi-1) and the partial energies in the
+    // GEMM's epilogue (normalized_dot_product.cpp:115-210)
+    flare::GemmProduct g[flare::kMaxSpecies];
+    flare::DEpilogue ep[flare::kMaxSpecies];
+    for (int s = 0; s < p.S; s++) {
+      const int o = gp->m_off[s];
+      g[s] = flare::GemmProduct{off[s + 1] - off[s], gp->m_by_type[s], F, 0,
+                                (int64_t)off[s] * F, (int64_t)o * F, 0};
+      ep[s].mode = flare::DEpilogue::kKernel;
+      ep[s].norm2_rows = st.fw.norm2 + off[s];
+      ep[s].pnorm = gp->pnorm.as<double>() + o;
+      ep[s].alpha = gp->alpha.as<double>() + o;
+      ep[s].sig2 = gp->sig2;
+      ep[s].power = gp->power;
+      ep[s].normalized = 0;
+      ep[s].kval = want_k ? st.kval + st.koff[s] : nullptr;  // the variance's k rows
+      ep[s].cmat = cm + st.koff[s];
+      ep[s].ldk = gp->ldk[s];
+      ep[s].part = e_part + (size_t)off[s] * np;
+      ep[s].nparts = np;
     }
-    double *Ks = Kraw + st.koff[s];
-    ctx->mark_begin("gp_K");
-    gemm_rm(ctx, false, true, rows, Ms, F, 1.0, B + (size_t)off[s] * F, F,
-            gp->P.as<double>() + (size_t)o * F, F, 0.0, Ks, Ms);
+    flare::dgemm_epi_batched(ctx->stream, p.S, g, ep, B, gp->P.as<double>());
+  } else {
+    // rows of odd F: rocBLAS and the separate epilogue
+    double *Kraw = ctx->buf_t<double>("gp.K", std::max<size_t>(ktot, 1));
+    for (int s = 0; s < p.S; s++) {
+      const int rows = off[s + 1] - off[s], Ms = gp->m_by_type[s], o = gp->m_off[s];
+      if (!rows || !Ms) continue;
+      gemm_rm(ctx, false, true, rows, Ms, F, 1.0, B + (size_t)off[s] * F, F,
+              gp->P.as<double>() + (size_t)o * F, F, 0.0, Kraw + st.koff[s], gp->ldk[s]);
+    }
     ctx->mark_end();
     ctx->mark_begin("gp_epilogue");
-    flare::launch_gp_epilogue(ctx->stream, rows, Ms, 0, Ks, st.fw.norm2 + off[s],
-                              gp->pnorm.as<double>() + o, gp->alpha.as<double>() + o,
-                              gp->sig2, gp->power, st.kval + st.koff[s],
-                              cm + st.koff[s], e_row + off[s]);
-    ctx->mark_end();
-    ctx->mark_begin("gp_G");
-    gemm_rm(ctx, false, false, rows, F, Ms, 1.0, cm + st.koff[s], Ms,
-            gp->Phat.as<double>() + (size_t)o * F, F, 0.0, Gs, F);
-    ctx->mark_end();
+    for (int s = 0; s < p.S; s++) {
+      const int rows = off[s + 1] - off[s], Ms = gp->m_by_type[s], o = gp->m_off[s];
+      if (!rows || !Ms) continue;
+      flare::launch_gp_epilogue(ctx->stream, rows, Ms, 0, Kraw + st.koff[s],
+                                st.fw.norm2 + off[s], gp->pnorm.as<double>() + o,
+                                gp->alpha.as<double>() + o, gp->sig2, gp->power,
+                                st.kval + st.koff[s], cm + st.koff[s], e_part + off[s],
+                                gp->ldk[s]);
+    }
   }
-  double *w = ctx->buf_t<double>("gp.w", (size_t)n * F);
-  double *E_row = ctx->buf_t<double>("gp.E", n);
+  ctx->mark_end();
+  ctx->mark_begin("gp_G");
+  {
+    // G = c P_hat: c rows of ldk against the padded P_hat^T rows
+    flare::GemmProduct g[flare::kMaxSpecies];
+    for (int s = 0; s < p.S; s++)
+      g[s] = flare::GemmProduct{off[s + 1] - off[s], F, gp->ldk[s], F,
+                                (int64_t)st.koff[s], gp->toff[s], (int64_t)off[s] * F};
+    flare::dgemm_batched(ctx->stream, p.S, g, cm, gp->PhatT_d.as<double>(), G);
+  }
+  ctx->mark_end();
+  double *w = ctx->buf_t<double>("gp.w", std::max<size_t>((size_t)n * F, 1));
+  double *E_row = ctx->buf_t<double>("gp.E", std::max(n, 1));
   ctx->mark_begin("gp_w");
-  flare::launch_gp_w(ctx->stream, n, F, G, e_row, B, st.fw.norm2, gp->sig2,
+  flare::launch_gp_w(ctx->stream, n, F, G, e_part, np, B, st.fw.norm2, gp->sig2,
                      gp->power, w, E_row);
   ctx->mark_end();
   const int mode = form == FLARE_FORM_PLAIN_FP64 ? 0 : 1;
@@ -2287,7 +2415,6 @@ static GPState gp_mean(flare_ctx *ctx, flare_sgp *gp, const flare_b2_params *par
   run_forces(ctx, p, nl, d_species, flare::kPrecF64, mode, st.fw, wi,
              d_mean_efs + 1, d_mean_efs, nullptr, d_mean_efs + 1 + 3 * (size_t)n,
              "gp");
-  (void)want_k;
   return st;
 }
 
@@ -2313,10 +2440,10 @@ int flare_sgp_predict_local_uncertainties(flare_ctx *ctx, flare_sgp *gp,
     REQUIRE(ctx && gp && d_mean_efs && d_variances, "null argument");
     ctx->use();
     GPState st = gp_mean(ctx, gp, params, nl, d_species, form, d_mean_efs, true);
-    const int n = nl->n, F = gp->F, M = gp->M;
+    const int n = nl->n, F = gp->F;
     int off[flare::kMaxSpecies + 1];
     nl->row_offsets(gp->S, off);
-    double *kself = ctx->buf_t<double>("gp.kself", n);
+    double *kself = ctx->buf_t<double>("gp.kself", std::max(n, 1));
     ctx->mark_begin("gp_kself");
     if (st.lowp)
       flare::launch_kself_lp(ctx->stream, n, F, (const float *)st.fw.B, st.fw.norm2,
@@ -2325,21 +2452,33 @@ int flare_sgp_predict_local_uncertainties(flare_ctx *ctx, flare_sgp *gp,
       flare::launch_kself(ctx->stream, n, F, (const double *)st.fw.B, st.fw.norm2,
                           gp->sig2, gp->power, kself);
     ctx->mark_end();
-    double *Q = ctx->buf_t<double>("gp.Q", st.koff[gp->S]);
+    // |L^-1 k_i|^2 (sparse_gp.cpp:93-133) on the matrix cores: k rows of
+    // ldk against the species' L^-1 rows, squares summed in the epilogue
+    // per 64 columns, the K loop of each tile cut at its last column (L^-1
+    // is lower triangular); FP64 in every form
+    int np = 1;
+    for (int s = 0; s < gp->S; s++) np = std::max(np, (gp->m_by_type[s] + 63) / 64);
+    double *vpart = ctx->buf_t<double>("gp.vpart", std::max<size_t>((size_t)n * np, 1));
+    flare::GemmProduct g[flare::kMaxSpecies];
+    flare::DEpilogue ep[flare::kMaxSpecies];
     for (int s = 0; s < gp->S; s++) {
-      int rows = off[s + 1] - off[s], Ms = gp->m_by_type[s], o = gp->m_off[s];
-      if (!rows) continue;
-      ctx->mark_begin("gp_Q");
-      if (Ms)
-        gemm_rm(ctx, false, false, rows, Ms, Ms, 1.0, st.kval + st.koff[s], Ms,
-                gp->Linv_cm.as<double>() + (size_t)o * M + o, M, 0.0,
-                Q + st.koff[s], Ms);
-      ctx->mark_end();
-      ctx->mark_begin("gp_var");
-      flare::launch_row_sqsum_sub(ctx->stream, rows, Ms, Q + st.koff[s],
-                                  kself + off[s], d_variances + off[s]);
-      ctx->mark_end();
+      const int rows = off[s + 1] - off[s], Ms = gp->m_by_type[s];
+      if (rows && !Ms)
+        FLARE_HIP_CHECK(hipMemsetAsync(vpart + (size_t)off[s] * np, 0,
+                                       (size_t)rows * np * sizeof(double), ctx->stream));
+      g[s] = flare::GemmProduct{rows, Ms, gp->ldk[s], 0, (int64_t)st.koff[s],
+                                gp->loff[s], 0};
+      ep[s].mode = flare::DEpilogue::kSquares;
+      ep[s].part = vpart + (size_t)off[s] * np;
+      ep[s].nparts = np;
+      ep[s].tri = 1;
     }
+    ctx->mark_begin("gp_Q");
+    flare::dgemm_epi_batched(ctx->stream, gp->S, g, ep, st.kval, gp->Linv_rm.as<double>());
+    ctx->mark_end();
+    ctx->mark_begin("gp_var");
+    flare::launch_var_from_parts(ctx->stream, n, np, vpart, kself, d_variances);
+    ctx->mark_end();
     ctx->collect();
     ctx->sync();
   });

@@ -24,7 +24,7 @@ __device__ inline double powi(double t, double power) {
 }
 
 __global__ void __launch_bounds__(64)
-gp_epilogue_kernel(int rows, int M, int normalized, const double *__restrict__ K,
+gp_epilogue_kernel(int rows, int M, int ld, int normalized, const double *__restrict__ K,
                    const double *__restrict__ norm2_rows,
                    const double *__restrict__ pnorm,
                    const double *__restrict__ alpha, double sig2, double power,
@@ -35,19 +35,19 @@ gp_epilogue_kernel(int rows, int M, int normalized, const double *__restrict__ K
   const double nb = sqrt(norm2_rows[r]);
   const bool empty = nb < 1e-8;
   double e = 0;
-  for (int u = lane; u < M; u += kWave) {
-    const double pn = pnorm[u];
+  // rows of ld >= M (K read, kval / cmat written); zero past M
+  for (int u = lane; u < ld; u += kWave) {
     double kv = 0, cv = 0;
-    if (!empty && !(pn < 1e-8)) {
-      const double dot = K[(int64_t)r * M + u];
-      const double t = normalized ? dot : dot / (nb * pn);
+    if (u < M && !empty && !(pnorm[u] < 1e-8)) {
+      const double dot = K[(int64_t)r * ld + u];
+      const double t = normalized ? dot : dot / (nb * pnorm[u]);
       const double tp = powi(t, power - 1.0);
       kv = sig2 * powi(t, power);
       cv = alpha ? alpha[u] * tp : 0.0;
       if (alpha) e += alpha[u] * powi(t, power);
     }
-    if (kval) kval[(int64_t)r * M + u] = kv;
-    if (cmat) cmat[(int64_t)r * M + u] = cv;
+    if (kval) kval[(int64_t)r * ld + u] = kv;
+    if (cmat) cmat[(int64_t)r * ld + u] = cv;
   }
   e = wave_sum(e);
   if (lane == 0 && e_row) e_row[r] = e;
@@ -55,14 +55,15 @@ gp_epilogue_kernel(int rows, int M, int normalized, const double *__restrict__ K
 
 __global__ void __launch_bounds__(64)
 gp_w_kernel(int rows, int F, const double *__restrict__ G,
-            const double *__restrict__ e_row, const double *__restrict__ B,
+            const double *__restrict__ e_part, int nparts, const double *__restrict__ B,
             const double *__restrict__ norm2, double sig2, double power,
             double *__restrict__ w, double *__restrict__ E_row) {
   const int r = blockIdx.x;
   const int lane = threadIdx.x;
   const double nb = sqrt(norm2[r]);
   const bool empty = nb < 1e-8;
-  const double e = e_row[r];
+  double e = 0;
+  for (int q = 0; q < nparts; q++) e += e_part[(int64_t)r * nparts + q];
   const double s = empty ? 0.0 : sig2 * power / nb;
   for (int f = lane; f < F; f += kWave) {
     double v = 0;
@@ -86,6 +87,17 @@ row_sqsum_sub_kernel(int rows, int M, const double *__restrict__ Q,
   }
   s = wave_sum(s);
   if (lane == 0) var[r] = kself[r] - s;
+}
+
+// var = k(i, i) - |L^-1 k_i|^2 from per-row partial sums of squares
+__global__ void var_from_parts_kernel(int rows, int nparts, const double *__restrict__ part,
+                                      const double *__restrict__ kself,
+                                      double *__restrict__ var) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= rows) return;
+  double s = 0;
+  for (int q = 0; q < nparts; q++) s += part[(int64_t)r * nparts + q];
+  var[r] = kself[r] - s;
 }
 
 __global__ void __launch_bounds__(64)
@@ -211,7 +223,7 @@ gp_epilogue_lp_kernel(int rows, int M, int Mp, const float *__restrict__ K,
                       const double *__restrict__ norm2_rows,
                       const double *__restrict__ pnorm,
                       const double *__restrict__ alpha, double sig2,
-                      double power, double *__restrict__ kval,
+                      double power, double *__restrict__ kval, int ldk,
                       float *__restrict__ cf, _Float16 *__restrict__ ch,
                       _Float16 *__restrict__ cl, float *__restrict__ c_is,
                       double *__restrict__ e_row) {
@@ -230,12 +242,13 @@ gp_epilogue_lp_kernel(int rows, int M, int Mp, const float *__restrict__ K,
   };
   double e = 0;
   float cmax = 0.0f;
-  for (int u = lane; u < M; u += kWave) {
+  // kval and cf rows of ldk (>= M), zero past M
+  for (int u = lane; u < max(M, ldk); u += kWave) {
     double kv, cv, ev;
     cval(u, kv, cv, ev);
     e += ev;
-    if (kval) kval[(int64_t)r * M + u] = kv;
-    if (cf) cf[(int64_t)r * M + u] = (float)cv;
+    if (kval) kval[(int64_t)r * ldk + u] = kv;
+    if (cf) cf[(int64_t)r * ldk + u] = (float)cv;
     cmax = fmaxf(cmax, fabsf((float)cv));
   }
   e = wave_sum(e);
@@ -262,14 +275,15 @@ gp_epilogue_lp_kernel(int rows, int M, int Mp, const float *__restrict__ K,
 // w = sig2 xi / |B| (G - e d) with d = B/|B| the FP32 row and G = c P_hat.
 __global__ void __launch_bounds__(64)
 gp_w_lp_kernel(int rows, int F, const float *__restrict__ G,
-               const double *__restrict__ e_row, const float *__restrict__ d,
+               const double *__restrict__ e_part, int nparts, const float *__restrict__ d,
                const double *__restrict__ norm2, double sig2, double power,
                float *__restrict__ w, double *__restrict__ E_row) {
   const int r = blockIdx.x;
   const int lane = threadIdx.x;
   const double nb = sqrt(norm2[r]);
   const bool empty = nb < 1e-8;
-  const double e = e_row[r];
+  double e = 0;
+  for (int q = 0; q < nparts; q++) e += e_part[(int64_t)r * nparts + q];
   const double s = empty ? 0.0 : sig2 * power / nb;
   for (int f = lane; f < F; f += kWave) {
     double v = 0;
@@ -309,20 +323,20 @@ void launch_gp_epilogue(hipStream_t st, int rows, int M, int normalized,
                         const double *K, const double *norm2_rows,
                         const double *pnorm, const double *alpha, double sig2,
                         double power, double *kval, double *cmat,
-                        double *e_row) {
+                        double *e_row, int ld) {
   if (rows == 0) return;
   hipLaunchKernelGGL(gp_epilogue_kernel, dim3(rows), dim3(kWave), 0, st, rows,
-                     M, normalized, K, norm2_rows, pnorm, alpha, sig2, power,
-                     kval, cmat, e_row);
+                     M, ld > 0 ? ld : M, normalized, K, norm2_rows, pnorm, alpha, sig2,
+                     power, kval, cmat, e_row);
   FLARE_HIP_CHECK(hipGetLastError());
 }
 
 void launch_gp_w(hipStream_t st, int rows, int F, const double *G,
-                 const double *e_row, const double *B, const double *norm2,
+                 const double *e_part, int nparts, const double *B, const double *norm2,
                  double sig2, double power, double *w, double *E_row) {
   if (rows == 0) return;
   hipLaunchKernelGGL(gp_w_kernel, dim3(rows), dim3(kWave), 0, st, rows, F, G,
-                     e_row, B, norm2, sig2, power, w, E_row);
+                     e_part, nparts, B, norm2, sig2, power, w, E_row);
   FLARE_HIP_CHECK(hipGetLastError());
 }
 
@@ -426,6 +440,14 @@ void launch_add_diag_rm(hipStream_t st, int F, double *out, double v) {
   FLARE_HIP_CHECK(hipGetLastError());
 }
 
+void launch_var_from_parts(hipStream_t st, int rows, int nparts, const double *part,
+                           const double *kself, double *var) {
+  if (rows == 0) return;
+  hipLaunchKernelGGL(var_from_parts_kernel, dim3((rows + 255) / 256), dim3(256), 0, st, rows,
+                     nparts, part, kself, var);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
 void launch_row_sqsum_sub(hipStream_t st, int rows, int M, const double *Q,
                           const double *kself, double *var) {
   if (rows == 0) return;
@@ -500,21 +522,21 @@ void launch_unpack_beta(hipStream_t st, int F, int power, const double *packed,
 void launch_gp_epilogue_lp(hipStream_t st, int rows, int M, int Mp,
                            const float *K, const double *norm2_rows,
                            const double *pnorm, const double *alpha, double sig2,
-                           double power, double *kval, float *cf, void *ch,
+                           double power, double *kval, int ldk, float *cf, void *ch,
                            void *cl, float *c_is, double *e_row) {
   if (rows == 0) return;
   hipLaunchKernelGGL(gp_epilogue_lp_kernel, dim3(rows), dim3(kWave), 0, st, rows,
-                     M, Mp, K, norm2_rows, pnorm, alpha, sig2, power, kval, cf,
+                     M, Mp, K, norm2_rows, pnorm, alpha, sig2, power, kval, ldk, cf,
                      (_Float16 *)ch, (_Float16 *)cl, c_is, e_row);
   FLARE_HIP_CHECK(hipGetLastError());
 }
 
 void launch_gp_w_lp(hipStream_t st, int rows, int F, const float *G,
-                    const double *e_row, const float *d, const double *norm2,
+                    const double *e_part, int nparts, const float *d, const double *norm2,
                     double sig2, double power, float *w, double *E_row) {
   if (rows == 0) return;
   hipLaunchKernelGGL(gp_w_lp_kernel, dim3(rows), dim3(kWave), 0, st, rows, F, G,
-                     e_row, d, norm2, sig2, power, w, E_row);
+                     e_part, nparts, d, norm2, sig2, power, w, E_row);
   FLARE_HIP_CHECK(hipGetLastError());
 }
 

@@ -192,13 +192,17 @@ void launch_gp_epilogue(hipStream_t st, int rows, int M, int normalized,
                         const double *K, const double *norm2_rows,
                         const double *pnorm, const double *alpha, double sig2,
                         double power, double *kval, double *cmat,
-                        double *e_row);
-// w = sig2 xi / |B| (G - e B / |B|) (Appendix-A chain rule)
+                        double *e_row, int ld = 0);
+// w = sig2 xi / |B| (G - e B / |B|) (Appendix-A chain rule); e of row r is
+// the sum of e_part[r * nparts .. + nparts)
 void launch_gp_w(hipStream_t st, int rows, int F, const double *G,
-                 const double *e_row, const double *B, const double *norm2,
+                 const double *e_part, int nparts, const double *B, const double *norm2,
                  double sig2, double power, double *w, double *E_row);
 void launch_row_sqsum_sub(hipStream_t st, int rows, int M, const double *Q,
                           const double *kself, double *var);
+// var[r] = kself[r] - sum of part[r * nparts .. + nparts)
+void launch_var_from_parts(hipStream_t st, int rows, int nparts, const double *part,
+                           const double *kself, double *var);
 void launch_kself(hipStream_t st, int rows, int F, const double *B,
                   const double *norm2, double sig2, double power,
                   double *kself);
@@ -261,17 +265,18 @@ void launch_col_sqsum(hipStream_t st, int rows, int64_t cols, const double *X,
                       double sign, int accumulate, double *out);
 // out[i][i] += v for i < F (row-major F x F)
 void launch_add_diag_rm(hipStream_t st, int F, double *out, double v);
-// Low-precision GP forms: epilogue from t (FP32, rows x M) into kval (FP64),
-// c as FP32 rows (cf) or as FP16 hi/lo rows of Mp scaled per row (ch, cl,
-// inverse scales c_is), e (FP64)
+// Low-precision GP forms: epilogue from t (FP32, rows x M) into kval (FP64,
+// rows of ldk >= M, zero past M), c as FP32 rows of ldk (cf) or as FP16 hi/lo rows
+// of Mp scaled per row (ch, cl, inverse scales c_is), e (FP64)
 void launch_gp_epilogue_lp(hipStream_t st, int rows, int M, int Mp,
                            const float *K, const double *norm2_rows,
                            const double *pnorm, const double *alpha, double sig2,
-                           double power, double *kval, float *cf, void *ch,
+                           double power, double *kval, int ldk, float *cf, void *ch,
                            void *cl, float *c_is, double *e_row);
-// w (FP32) = sig2 xi / |B| (G - e d), d the FP32 normalised rows
+// w (FP32) = sig2 xi / |B| (G - e d), d the FP32 normalised rows; e as in
+// launch_gp_w
 void launch_gp_w_lp(hipStream_t st, int rows, int F, const float *G,
-                    const double *e_row, const float *d, const double *norm2,
+                    const double *e_part, int nparts, const float *d, const double *norm2,
                     double sig2, double power, float *w, double *E_row);
 void launch_kself_lp(hipStream_t st, int rows, int F, const float *d,
                      const double *norm2, double sig2, double power,

@@ -42,6 +42,39 @@ void dgemm_batched(hipStream_t st, int nb, const GemmProduct *prods, const doubl
                    const double *B, double *Z);
 void sgemm_batched(hipStream_t st, int nb, const GemmProduct *prods, const float *A,
                    const float *B, float *Z);
+// Fused epilogues of the FP64 / FP32 matrix-core product Z = A B^T for the
+// sparse GP (one per product of a batch).
+//  kKernel (normalized_dot_product.cpp:27-87, 115-210): the product is
+//   B_i . P_u (or d_i . p_hat_u when normalized); t = product / (|B_i||P_u|)
+//   (the product itself when normalized), k = sig2 t^xi into kval (FP64;
+//   skipped when null),
+//   c = alpha_u t^(xi-1) into cmat (the GEMM's type), both with leading
+//   dimension ldk and zero in columns N .. ldk-1, and per row the partial
+//   sums of alpha_u t^xi over each 64-column group into
+//   part[row * nparts + group]; k = c = 0 where |B_i| or |P_u| < 1e-8.
+//  kSquares (sparse_gp.cpp:93-133): part[row * nparts + group] = the sum of
+//   the squared products over the group's columns; with tri set, row n of
+//   B is zero past column n (a lower-triangular factor), so a tile ends its
+//   K loop at its last column.
+// nparts >= ceil(N / 64) is the row pitch of part; groups past ceil(N / 64)
+// are written as zero.
+struct DEpilogue {
+  enum Mode { kKernel = 1, kSquares = 2 };
+  int mode = kKernel;
+  const double *norm2_rows = nullptr, *pnorm = nullptr, *alpha = nullptr;
+  double sig2 = 1, power = 2;
+  int normalized = 0;
+  double *kval = nullptr;
+  void *cmat = nullptr;
+  int ldk = 0;
+  double *part = nullptr;
+  int nparts = 0;
+  int tri = 0;
+};
+void dgemm_epi_batched(hipStream_t st, int nb, const GemmProduct *prods,
+                       const DEpilogue *ep, const double *A, const double *B);
+void sgemm_epi_batched(hipStream_t st, int nb, const GemmProduct *prods,
+                       const DEpilogue *ep, const float *A, const float *B);
 // The same on FP16 MFMA (operands from split_f16_*).
 void gemm_f16x3_batched(hipStream_t st, int nb, const GemmProduct *prods,
                         const void *Ah, const void *Al, const void *Bh,

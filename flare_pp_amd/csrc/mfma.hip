@@ -375,11 +375,24 @@ __device__ inline void dgemm_row4(const T (*s)[kDK], int r, int g, T *o) {
   }
 }
 
-template <typename T>
+// A batch of products with their fused epilogues (mode 0: plain store).
+struct DGemmBatch {
+  GemmBatch bt;
+  DEpilogue ep[kGemmMaxBatch];
+};
+
+__device__ inline double powi_d(double t, double power) {
+  if (power == 2.0) return t * t;
+  if (power == 1.0) return t;
+  return pow(t, power);
+}
+
+template <typename T, int MODE>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
-dgemm_kernel(GemmBatch bt, const T *__restrict__ A0, const T *__restrict__ B0,
+dgemm_kernel(DGemmBatch dbt, const T *__restrict__ A0, const T *__restrict__ B0,
              T *__restrict__ Z0) {
   typedef typename DMma<T>::V V;
+  const GemmBatch &bt = dbt.bt;
   __shared__ __align__(16) T sA[2][kDM][kDK], sB[2][kDN][kDK];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
@@ -403,7 +416,9 @@ dgemm_kernel(GemmBatch bt, const T *__restrict__ A0, const T *__restrict__ B0,
 #pragma unroll
     for (int j = 0; j < 4; j++) acc[i][j] = V{0, 0, 0, 0};
   const int fr = lane & 15, g = lane >> 4;
-  const int steps = (K + kDK - 1) / kDK;
+  // a lower-triangular B has nothing past the tile's last column
+  const int Keff = MODE == DEpilogue::kSquares && dbt.ep[q].tri ? min(K, n0 + kDN) : K;
+  const int steps = (Keff + kDK - 1) / kDK;
   DTile<T> t;
   dgemm_load<T>(t, A, B, M, N, K, m0, n0, 0);
   dgemm_store<T>(t, sA[0], sB[0]);
@@ -431,31 +446,120 @@ dgemm_kernel(GemmBatch bt, const T *__restrict__ A0, const T *__restrict__ B0,
     }
     __syncthreads();
   }
+  if constexpr (MODE == 0) {
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int col = n0 + wn + 16 * j + fr;
+        if (col >= N) continue;
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          const int row = m0 + wm + 16 * i + DMma<T>::row(g, e);
+          if (row < M) Z[(int64_t)row * ldz + col] = acc[i][j][e];
+        }
+      }
+  } else {
+    // per row: the sum over this wavefront's 64 columns, reduced across the
+    // 16 lanes (fr) that hold a row's columns
+    const DEpilogue &ep = dbt.ep[q];
+    double rs[4][4];
+    double inv_nb[4][4];
+    bool row_empty[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        rs[i][e] = 0;
+        inv_nb[i][e] = 0;
+        row_empty[i][e] = true;
+        if constexpr (MODE == DEpilogue::kKernel) {
+          const int row = m0 + wm + 16 * i + DMma<T>::row(g, e);
+          if (row < M) {
+            const double nb = sqrt(ep.norm2_rows[row]);
+            row_empty[i][e] = nb < 1e-8;
+            inv_nb[i][e] = nb;
+          }
+        }
+      }
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const int col = n0 + wn + 16 * j + fr;
-      if (col >= N) continue;
+      double pn = 0, al = 0;
+      if constexpr (MODE == DEpilogue::kKernel) {
+        if (col < N) {
+          pn = ep.pnorm[col];
+          al = ep.alpha ? ep.alpha[col] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          const int row = m0 + wm + 16 * i + DMma<T>::row(g, e);
+          const double a = (double)acc[i][j][e];
+          if constexpr (MODE == DEpilogue::kSquares) {
+            if (col < N) rs[i][e] += a * a;
+          } else {
+            double kv = 0, cv = 0;
+            if (col < N && !row_empty[i][e] && !(pn < 1e-8)) {
+              const double tt = ep.normalized ? a : a / (inv_nb[i][e] * pn);
+              const double tx = powi_d(tt, ep.power);
+              kv = ep.sig2 * tx;
+              cv = al * powi_d(tt, ep.power - 1.0);
+              rs[i][e] += al * tx;
+            }
+            if (row < M && col < ep.ldk) {
+              if (ep.kval) ep.kval[(int64_t)row * ep.ldk + col] = kv;
+              reinterpret_cast<T *>(ep.cmat)[(int64_t)row * ep.ldk + col] = (T)cv;
+            }
+          }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++)
 #pragma unroll
       for (int e = 0; e < 4; e++) {
+        double v = rs[i][e];
+#pragma unroll
+        for (int off = 1; off < 16; off <<= 1) v += __shfl_xor(v, off, 16);
         const int row = m0 + wm + 16 * i + DMma<T>::row(g, e);
-        if (row < M) Z[(int64_t)row * ldz + col] = acc[i][j][e];
+        // a 64-column group past N has no partial (its index may be nparts);
+        // the row's last group also clears the groups up to nparts (the row
+        // pitch may exceed this product's group count)
+        if (fr == 0 && row < M && n0 + wn < N) {
+          const int grp = (n0 + wn) / 64;
+          double *pr = ep.part + (int64_t)row * ep.nparts;
+          pr[grp] = v;
+          if (grp == (N - 1) / 64)
+            for (int x = grp + 1; x < ep.nparts; x++) pr[x] = 0.0;
+        }
       }
-    }
+  }
 }
 
 template <typename T>
-static void dgemm_batched_t(hipStream_t st, int nb, const GemmProduct *prods, const T *A,
-                            const T *B, T *Z) {
+static void dgemm_batched_t(hipStream_t st, int nb, const GemmProduct *prods,
+                            const DEpilogue *ep, const T *A, const T *B, T *Z) {
   FLARE_REQUIRE(nb >= 0 && nb <= kGemmMaxBatch, "gemm: at most 8 products");
-  GemmBatch bt;
+  DGemmBatch dbt;
+  GemmBatch &bt = dbt.bt;
   bt.nb = 0;
-  int tiles = 0;
+  int tiles = 0, mode = 0;
   for (int q = 0; q < nb; q++) {
     const GemmProduct &g = prods[q];
     FLARE_REQUIRE((g.K * (int)sizeof(T)) % 16 == 0, "gemm: rows must be 16-byte multiples");
     if (g.M <= 0 || g.N <= 0) continue;
+    if (ep) {
+      const DEpilogue &e = ep[q];
+      FLARE_REQUIRE(mode == 0 || e.mode == mode, "gemm: one epilogue mode per batch");
+      mode = e.mode;
+      FLARE_REQUIRE(e.part && e.nparts >= (g.N + 63) / 64, "gemm: partial-sum columns");
+      FLARE_REQUIRE(e.mode != DEpilogue::kKernel ||
+                        (e.cmat && e.ldk >= g.N && e.norm2_rows && e.pnorm),
+                    "gemm: kernel-value epilogue outputs");
+      dbt.ep[bt.nb] = e;
+    }
     bt.g[bt.nb] = g;
     bt.tile0[bt.nb] = tiles;
     tiles += (g.M + kDM - 1) / kDM * ((g.N + kDN - 1) / kDN);
@@ -464,18 +568,35 @@ static void dgemm_batched_t(hipStream_t st, int nb, const GemmProduct *prods, co
   bt.tile0[bt.nb] = tiles;
   if (tiles == 0) return;
   const int grid = (tiles + 7) / 8 * 8;
-  hipLaunchKernelGGL(dgemm_kernel<T>, dim3(grid), dim3(256), 0, st, bt, A, B, Z);
+  if (mode == 0)
+    hipLaunchKernelGGL((dgemm_kernel<T, 0>), dim3(grid), dim3(256), 0, st, dbt, A, B, Z);
+  else if (mode == DEpilogue::kKernel)
+    hipLaunchKernelGGL((dgemm_kernel<T, DEpilogue::kKernel>), dim3(grid), dim3(256), 0, st,
+                       dbt, A, B, Z);
+  else
+    hipLaunchKernelGGL((dgemm_kernel<T, DEpilogue::kSquares>), dim3(grid), dim3(256), 0, st,
+                       dbt, A, B, Z);
   FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void dgemm_epi_batched(hipStream_t st, int nb, const GemmProduct *prods,
+                       const DEpilogue *ep, const double *A, const double *B) {
+  dgemm_batched_t<double>(st, nb, prods, ep, A, B, nullptr);
+}
+
+void sgemm_epi_batched(hipStream_t st, int nb, const GemmProduct *prods,
+                       const DEpilogue *ep, const float *A, const float *B) {
+  dgemm_batched_t<float>(st, nb, prods, ep, A, B, nullptr);
 }
 
 void dgemm_batched(hipStream_t st, int nb, const GemmProduct *prods, const double *A,
                    const double *B, double *Z) {
-  dgemm_batched_t<double>(st, nb, prods, A, B, Z);
+  dgemm_batched_t<double>(st, nb, prods, nullptr, A, B, Z);
 }
 
 void sgemm_batched(hipStream_t st, int nb, const GemmProduct *prods, const float *A,
                    const float *B, float *Z) {
-  dgemm_batched_t<float>(st, nb, prods, A, B, Z);
+  dgemm_batched_t<float>(st, nb, prods, nullptr, A, B, Z);
 }
 
 // hi/lo split of a (rows x cols) FP64 matrix into rows of kp BF16 (zero

@@ -6,6 +6,11 @@
 namespace flare {
 
 void launch_sqrt(hipStream_t st, int n, const double *x, double *y);
+// dst[r * ldd + c] = src[r * srs + c * scs] for c < cols, zero up to ldd
+void launch_pad_block(hipStream_t st, int rows, int cols, const double *src, int64_t srs,
+                      int64_t scs, double *dst, int ldd);
+void launch_pad_block_f32(hipStream_t st, int rows, int cols, const double *src,
+                          int64_t srs, int64_t scs, float *dst, int ldd);
 // the (c1, c2, l) table of the B2 features, as flare_ctx::b2_index uploads it
 void launch_b2_index(hipStream_t st, int nr, int L, int *code);
 void launch_rows_to_atoms(hipStream_t st, int n, const int *atom_of_row,

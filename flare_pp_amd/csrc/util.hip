@@ -148,6 +148,17 @@ __global__ void b2_index_kernel(int nr, int L, int *__restrict__ code) {
   code[k] = (c1 << 20) | ((c1 + p) << 8) | l;
 }
 
+// dst[r * ldd + c] = src[r * srs + c * scs] for c < cols, 0 for cols <= c <
+// ldd: strided block copies with zero-padded rows (transposes included)
+template <typename D>
+__global__ void pad_block_kernel(int rows, int cols, const double *__restrict__ src,
+                                 int64_t srs, int64_t scs, D *__restrict__ dst, int ldd) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= (int64_t)rows * ldd) return;
+  const int r = (int)(q / ldd), c = (int)(q - (int64_t)r * ldd);
+  dst[q] = c < cols ? (D)src[r * srs + c * scs] : (D)0;
+}
+
 void launch_sqrt(hipStream_t st, int n, const double *x, double *y) {
   if (n == 0) return;
   hipLaunchKernelGGL(sqrt_kernel, dim3(nblk(n, 256)), dim3(256), 0, st, n, x, y);
@@ -157,6 +168,22 @@ void launch_sqrt(hipStream_t st, int n, const double *x, double *y) {
 void launch_b2_index(hipStream_t st, int nr, int L, int *code) {
   const int F = nr * (nr + 1) / 2 * (L + 1);
   hipLaunchKernelGGL(b2_index_kernel, dim3(nblk(F, 256)), dim3(256), 0, st, nr, L, code);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_pad_block(hipStream_t st, int rows, int cols, const double *src, int64_t srs,
+                      int64_t scs, double *dst, int ldd) {
+  if (rows == 0 || ldd == 0) return;
+  hipLaunchKernelGGL(pad_block_kernel<double>, dim3(nblk((int64_t)rows * ldd, 256)), dim3(256),
+                     0, st, rows, cols, src, srs, scs, dst, ldd);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void launch_pad_block_f32(hipStream_t st, int rows, int cols, const double *src,
+                          int64_t srs, int64_t scs, float *dst, int ldd) {
+  if (rows == 0 || ldd == 0) return;
+  hipLaunchKernelGGL(pad_block_kernel<float>, dim3(nblk((int64_t)rows * ldd, 256)), dim3(256),
+                     0, st, rows, cols, src, srs, scs, dst, ldd);
   FLARE_HIP_CHECK(hipGetLastError());
 }
 
