@@ -65,7 +65,7 @@ struct NListBuffers {
       cum, keys_tmp, rel_tmp, keys_pad, keys, rel, idx, nspec, center,
       rev;
   DevBuf row_of, atom_of_row, species_of_row, species_count, species_blk,
-      scan_tmp;
+      scan_tmp, blk_sum;
   int P = 0;
 };
 

@@ -92,21 +92,45 @@ __global__ void clear4_kernel(int *__restrict__ a, int na, int *__restrict__ b,
   if (q < nd) d[q] = 0;
 }
 
-// rev[0..P) = -1 ahead of the compaction (partners that find each other
-// overwrite both entries), P = the pair total the count scan left on the
-// device; nothing to do when an over-capacity atom sends the build to the
-// two-pass path, which writes every entry.
-__global__ void __launch_bounds__(256)
-fill_rev_kernel(const int *__restrict__ total, const int *__restrict__ overflow,
-                int *__restrict__ rev) {
-  if (*overflow) return;
-  const int P = *total, P4 = P >> 2;
-  const int stride = gridDim.x * blockDim.x;
-  const int4 m1 = {-1, -1, -1, -1};
-  for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < P4; k += stride)
-    reinterpret_cast<int4 *>(rev)[k] = m1;
-  for (int k = 4 * P4 + blockIdx.x * blockDim.x + threadIdx.x; k < P; k += stride)
-    rev[k] = -1;
+// cum[i] = sum of count[k], k < i, from the per-block pair sums
+// blk_sum[b kCumStride] (count[i] accumulated into block i / kCumBlock by
+// the enumeration; one 128-byte line per block sum, so that the atomics of
+// different blocks do not queue on one line): workgroup b adds the sums of
+// the blocks before it and scans its own kCumBlock counts; the last
+// workgroup writes cum[n] and the pair total.  Integer sums, so the result
+// does not depend on the order of the enumeration's atomics.  For n <=
+// kCumBlock kCumMaxBlocks (each workgroup reads at most kCumMaxBlocks sums).
+constexpr int kCumBlock = 256, kCumMaxBlocks = 1024, kCumStride = 32;
+__global__ void __launch_bounds__(kCumBlock)
+cum_blocks_kernel(int n, const int *__restrict__ count, const int *__restrict__ blk_sum,
+                  int *__restrict__ cum, int *__restrict__ stat_P) {
+  constexpr int kW = kCumBlock / kWave;
+  __shared__ int before_w[kW], total_w[kW];
+  const int b = blockIdx.x, t = threadIdx.x, lane = t & (kWave - 1), wv = t / kWave;
+  int s = 0;
+  for (int k = t; k < b; k += kCumBlock) s += blk_sum[k * kCumStride];
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kWave);
+  const int i = b * kCumBlock + t;
+  const int c = i < n ? count[i] : 0;
+  int incl = c;
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const int v = __shfl_up(incl, off, kWave);
+    if (lane >= off) incl += v;
+  }
+  if (lane == 0) before_w[wv] = s;
+  if (lane == kWave - 1) total_w[wv] = incl;
+  __syncthreads();
+  int base = 0;
+#pragma unroll
+  for (int w = 0; w < kW; w++) base += before_w[w] + (w < wv ? total_w[w] : 0);
+  const int ex = base + incl - c;
+  if (i < n) cum[i] = ex;
+  if (b == gridDim.x - 1 && t == kCumBlock - 1) {
+    cum[n] = ex + c;
+    *stat_P = ex + c;
+  }
 }
 
 // Single-workgroup exclusive scan: out[0..n] with out[n] = total, for
@@ -466,7 +490,7 @@ __device__ inline void enum_sort_atom(int i, const WrapMats &m,
                                       const double *__restrict__ bin_pos,
                                       int *__restrict__ count,
                                       int64_t *__restrict__ keys_pad, int *__restrict__ stat,
-                                      int *prefix, int64_t *skey) {
+                                      int *__restrict__ blk_sum, int *prefix, int64_t *skey) {
   const int lane = threadIdx.x & (kWave - 1);
   const int sw = m.sweep;
   const int hb = bin_of[i];
@@ -545,6 +569,7 @@ __device__ inline void enum_sort_atom(int i, const WrapMats &m,
   }
   if (lane == 0) {
     count[i] = found;
+    if (blk_sum) atomicAdd(blk_sum + i / kCumBlock * kCumStride, found);
     if (found > kNbrCap) atomicMax(stat, found);  // rare: no contention
   }
   if (found > kNbrCap) return;
@@ -573,14 +598,15 @@ nbr_enum_sort_kernel(int n, WrapMats m, const double *__restrict__ wrapped,
                      const int *__restrict__ bin_start,
                      const int *__restrict__ bin_atoms,
                      const double *__restrict__ bin_pos, int *__restrict__ count,
-                     int64_t *__restrict__ keys_pad, int *__restrict__ stat) {
+                     int64_t *__restrict__ keys_pad, int *__restrict__ stat,
+                     int *__restrict__ blk_sum) {
   __shared__ int prefix_sh[4][kWave];
   __shared__ int64_t skey_sh[4][kNbrCap];
   const int wv = threadIdx.x / kWave;
   const int i = blockIdx.x * 4 + wv;
   if (i >= n) return;  // whole wave exits together
   enum_sort_atom<THIN>(i, m, wrapped, bin_of, bin_start, bin_atoms, bin_pos, count,
-                       keys_pad, stat, prefix_sh[wv], skey_sh[wv]);
+                       keys_pad, stat, blk_sum, prefix_sh[wv], skey_sh[wv]);
 }
 
 // Candidates of one home bin's 27 neighboring bins staged in LDS: FP32
@@ -629,7 +655,8 @@ nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
                          const int *__restrict__ bin_start,
                          const int *__restrict__ bin_atoms,
                          const double *__restrict__ bin_pos, int *__restrict__ count,
-                         int64_t *__restrict__ keys_pad, int *__restrict__ stat) {
+                         int64_t *__restrict__ keys_pad, int *__restrict__ stat,
+                         int *__restrict__ blk_sum) {
   __shared__ int c_pre[28], c_start[27], c_shift[27];
   __shared__ __align__(16) unsigned char lds[kBinLds];
   const int b = blockIdx.x;
@@ -670,7 +697,7 @@ nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
     int64_t *skey = reinterpret_cast<int64_t *>(lds + 4 * kWave * 4) + wv * kNbrCap;
     for (int a = wv; a < na; a += 4)
       enum_sort_atom<false>(bin_atoms[a0 + a], m, wrapped, bin_of, bin_start, bin_atoms,
-                            bin_pos, count, keys_pad, stat, prefix, skey);
+                            bin_pos, count, keys_pad, stat, blk_sum, prefix, skey);
     return;
   }
   double(*cw)[kBinCap] = reinterpret_cast<double(*)[kBinCap]>(lds);
@@ -760,6 +787,7 @@ nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
     }
     if (lane == 0) {
       count[i] = found;
+      if (blk_sum) atomicAdd(blk_sum + i / kCumBlock * kCumStride, found);
       if (found > kNbrCap) atomicMax(stat, found);  // rare: no contention
     }
     // pad the 32-bit keys to a multiple of four with a key no entry is below
@@ -1114,11 +1142,20 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
   int *counts = b.species_count.as<int>();
   int *overflow = counts + kMaxSpecies + 1;
 
-  const int nclear = std::max(nbins + 2, kNlistStats);
-  // count[n] is the zero the count scan reads past the end
+  // cum from per-block pair sums (cum_blocks_kernel) up to kCumMaxBlocks
+  // blocks, else the single-workgroup / device-wide count scan
+  const int ncb = (n + kCumBlock - 1) / kCumBlock;
+  const bool cum_blocks = n > 0 && ncb <= kCumMaxBlocks;
+  b.blk_sum.ensure((size_t)std::max(ncb, 1) * kCumStride * sizeof(int));
+  int *blk_sum = cum_blocks ? b.blk_sum.as<int>() : nullptr;
+  const int nclear = std::max(std::max(nbins + 2, kNlistStats), ncb * kCumStride);
+  // count[n] is the zero the count scan reads past the end; blk_sum the
+  // enumeration's per-block sums
   hipLaunchKernelGGL(clear4_kernel, dim3((nclear + 255) / 256), dim3(256), 0, st,
                      counts, kNlistStats, b.bin_count.as<int>(), nbins + 2,
-                     b.bin_fill.as<int>(), nbins + 1, b.count.as<int>() + n, 1);
+                     b.bin_fill.as<int>(), nbins + 1,
+                     cum_blocks ? blk_sum : b.count.as<int>() + n,
+                     cum_blocks ? ncb * kCumStride : 1);
   const int tb = 256;
   const int g1 = (n + tb - 1) / tb;
   const int gw = (n + 3) / 4;
@@ -1163,24 +1200,27 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
                          n, m, b.wrapped.as<double>(), b.bin_of.as<int>(),
                          b.bin_start.as<int>(), b.bin_atoms.as<int>(),
                          b.bin_pos.as<double>(), b.count.as<int>(),
-                         b.keys_pad.as<int64_t>(), overflow);
+                         b.keys_pad.as<int64_t>(), overflow, blk_sum);
     else
       hipLaunchKernelGGL(nbr_bin_enum_sort_kernel, dim3(nbins), dim3(256), 0, st, m,
                          prefilter_cut2(wh), n <= (1 << 26) ? 1 : 0,
                          b.wrapped.as<double>(), b.bin_of.as<int>(),
                          b.bin_start.as<int>(), b.bin_atoms.as<int>(),
                          b.bin_pos.as<double>(), b.count.as<int>(),
-                         b.keys_pad.as<int64_t>(), overflow);
+                         b.keys_pad.as<int64_t>(), overflow, blk_sum);
   }
-  count_scan_stats(st, b, n, 0, nullptr, boff, counts);
+  if (cum_blocks)
+    hipLaunchKernelGGL(cum_blocks_kernel, dim3(ncb), dim3(kCumBlock), 0, st, n,
+                       b.count.as<int>(), blk_sum, b.cum.as<int>(),
+                       counts + kMaxSpecies + 2);
+  else
+    count_scan_stats(st, b, n, 0, nullptr, boff, counts);
   FLARE_HIP_CHECK(hipMemcpyAsync(stats_host, counts, kNlistStats * sizeof(int),
                                  hipMemcpyDeviceToHost, st));
   FLARE_HIP_CHECK(hipEventRecord(stats_ev, st));
   if (n > 0) {
-    // entries without a partner keep -1 (the partners write each other)
-    const int fill_blocks = (int)std::min<size_t>(1024, (cap / 4 + 255) / 256);
-    hipLaunchKernelGGL(fill_rev_kernel, dim3(fill_blocks), dim3(256), 0, st,
-                       counts + kMaxSpecies + 2, overflow, b.rev.as<int>());
+    // every entry's reverse index is written by the partner with the
+    // smaller key (the predicate is symmetric bitwise, so both exist)
     hipLaunchKernelGGL(nbr_compact_kernel, dim3(gw), dim3(256), 0, st, n, m,
                        overflow, b.cum.as<int>(), b.count.as<int>(),
                        b.keys_pad.as<int64_t>(), b.wrapped.as<double>(),
