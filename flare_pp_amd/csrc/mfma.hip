@@ -314,44 +314,55 @@ struct DMma<float> {
   static __device__ inline int swz(int r, int u) { return (u ^ ((r >> 2) & 3)) * 4; }
 };
 
-template <typename T>
+// Register staging of one K step: A rows (kDM) and B rows (TN, the tile's
+// column count: 128, or 64 for the narrow tiles)
+template <typename T, int TN>
 struct DTile {
-  static constexpr int kPer = kDM * DMma<T>::kUnits / 256;  // units per thread
-  typename DMma<T>::U a[kPer], b[kPer];
+  static constexpr int kPer = kDM * DMma<T>::kUnits / 256;   // A units per thread
+  static constexpr int kPerB = TN * DMma<T>::kUnits / 256;   // B units per thread
+  typename DMma<T>::U a[kPer], b[kPerB];
 };
 
-template <typename T>
-__device__ inline void dgemm_load(DTile<T> &t, const T *__restrict__ A,
+template <typename T, int TN>
+__device__ inline void dgemm_load(DTile<T, TN> &t, const T *__restrict__ A,
                                   const T *__restrict__ B, int M, int N, int K,
                                   int m0, int n0, int k0) {
   typedef typename DMma<T>::U U;
-  constexpr int NU = DMma<T>::kUnits, PER = DTile<T>::kPer, EU = 16 / sizeof(T);
+  constexpr int NU = DMma<T>::kUnits, EU = 16 / sizeof(T);
   const int tid = threadIdx.x;
+  U z;
+  memset(&z, 0, sizeof(U));
 #pragma unroll
-  for (int h = 0; h < PER; h++) {
+  for (int h = 0; h < DTile<T, TN>::kPer; h++) {
     const int c = tid + 256 * h;
     const int r = c / NU, k = k0 + EU * (c % NU);
-    U z;
-    memset(&z, 0, sizeof(U));
     t.a[h] = z;
+    if (k < K && m0 + r < M) t.a[h] = *reinterpret_cast<const U *>(A + (int64_t)(m0 + r) * K + k);
+  }
+#pragma unroll
+  for (int h = 0; h < DTile<T, TN>::kPerB; h++) {
+    const int c = tid + 256 * h;
+    const int r = c / NU, k = k0 + EU * (c % NU);
     t.b[h] = z;
-    if (k < K) {
-      if (m0 + r < M) t.a[h] = *reinterpret_cast<const U *>(A + (int64_t)(m0 + r) * K + k);
-      if (n0 + r < N) t.b[h] = *reinterpret_cast<const U *>(B + (int64_t)(n0 + r) * K + k);
-    }
+    if (k < K && n0 + r < N) t.b[h] = *reinterpret_cast<const U *>(B + (int64_t)(n0 + r) * K + k);
   }
 }
 
-template <typename T>
-__device__ inline void dgemm_store(const DTile<T> &t, T (*sA)[kDK], T (*sB)[kDK]) {
+template <typename T, int TN>
+__device__ inline void dgemm_store(const DTile<T, TN> &t, T (*sA)[kDK], T (*sB)[kDK]) {
   typedef typename DMma<T>::U U;
   constexpr int NU = DMma<T>::kUnits;
   const int tid = threadIdx.x;
 #pragma unroll
-  for (int h = 0; h < DTile<T>::kPer; h++) {
+  for (int h = 0; h < DTile<T, TN>::kPer; h++) {
     const int c = tid + 256 * h;
     const int r = c / NU, u = c % NU;
     *reinterpret_cast<U *>(&sA[r][DMma<T>::swz(r, u)]) = t.a[h];
+  }
+#pragma unroll
+  for (int h = 0; h < DTile<T, TN>::kPerB; h++) {
+    const int c = tid + 256 * h;
+    const int r = c / NU, u = c % NU;
     *reinterpret_cast<U *>(&sB[r][DMma<T>::swz(r, u)]) = t.b[h];
   }
 }
@@ -387,16 +398,20 @@ __device__ inline double powi_d(double t, double power) {
   return pow(t, power);
 }
 
-template <typename T, int MODE>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
-dgemm_kernel(DGemmBatch dbt, const T *__restrict__ A0, const T *__restrict__ B0,
-             T *__restrict__ Z0) {
+// TN = 128: four wavefronts of 64 x 64 (2 x 2); TN = 64 (narrow tiles, for
+// products whose tile count would leave most of a second round of
+// workgroups idle): four wavefronts of 32 x 64 stacked by rows, so every
+// wavefront still covers one whole 64-column group of the epilogue.
+template <typename T, int MODE, int TN>
+__device__ inline void dgemm_body(const DGemmBatch &dbt, const T *__restrict__ A0,
+                                  const T *__restrict__ B0, T *__restrict__ Z0) {
   typedef typename DMma<T>::V V;
+  constexpr int IM = TN == 128 ? 4 : 2;  // 16-row blocks per wavefront
   const GemmBatch &bt = dbt.bt;
-  __shared__ __align__(16) T sA[2][kDM][kDK], sB[2][kDN][kDK];
+  __shared__ __align__(16) T sA[2][kDM][kDK], sB[2][TN][kDK];
   const int lane = threadIdx.x & 63;
   const int wv = threadIdx.x >> 6;
-  const int wm = (wv >> 1) * 64, wn = (wv & 1) * 64;
+  const int wm = TN == 128 ? (wv >> 1) * 64 : wv * 32, wn = TN == 128 ? (wv & 1) * 64 : 0;
   int Tt, dummy;
   tile_of_block(bt.tile0[bt.nb], 1, Tt, dummy);
   if (Tt >= bt.tile0[bt.nb]) return;
@@ -404,37 +419,37 @@ dgemm_kernel(DGemmBatch dbt, const T *__restrict__ A0, const T *__restrict__ B0,
   while (Tt >= bt.tile0[q + 1]) q++;
   const GemmProduct &gp = bt.g[q];
   const int M = gp.M, N = gp.N, K = gp.K, ldz = gp.ldz;
-  const int nct = (N + kDN - 1) / kDN;
+  const int nct = (N + TN - 1) / TN;
   const int local = Tt - bt.tile0[q];
   const int rt = local / nct, ct = local - rt * nct;
   const T *A = A0 + gp.a_off, *B = B0 + gp.b_off;
   T *Z = Z0 + gp.z_off;
-  const int m0 = rt * kDM, n0 = ct * kDN;
-  V acc[4][4];
+  const int m0 = rt * kDM, n0 = ct * TN;
+  V acc[IM][4];
 #pragma unroll
-  for (int i = 0; i < 4; i++)
+  for (int i = 0; i < IM; i++)
 #pragma unroll
     for (int j = 0; j < 4; j++) acc[i][j] = V{0, 0, 0, 0};
   const int fr = lane & 15, g = lane >> 4;
   // a lower-triangular B has nothing past the tile's last column
-  const int Keff = MODE == DEpilogue::kSquares && dbt.ep[q].tri ? min(K, n0 + kDN) : K;
+  const int Keff = MODE == DEpilogue::kSquares && dbt.ep[q].tri ? min(K, n0 + TN) : K;
   const int steps = (Keff + kDK - 1) / kDK;
-  DTile<T> t;
-  dgemm_load<T>(t, A, B, M, N, K, m0, n0, 0);
-  dgemm_store<T>(t, sA[0], sB[0]);
-  if (steps > 1) dgemm_load<T>(t, A, B, M, N, K, m0, n0, kDK);
+  DTile<T, TN> t;
+  dgemm_load<T, TN>(t, A, B, M, N, K, m0, n0, 0);
+  dgemm_store<T, TN>(t, sA[0], sB[0]);
+  if (steps > 1) dgemm_load<T, TN>(t, A, B, M, N, K, m0, n0, kDK);
   __syncthreads();
   for (int s = 0; s < steps; s++) {
     const int cur = s & 1;
     if (s + 1 < steps) {
       // step s + 1 into the other buffer (its last reader, step s - 1,
       // ended with a barrier); step s + 2 into registers
-      dgemm_store<T>(t, sA[cur ^ 1], sB[cur ^ 1]);
-      if (s + 2 < steps) dgemm_load<T>(t, A, B, M, N, K, m0, n0, (s + 2) * kDK);
+      dgemm_store<T, TN>(t, sA[cur ^ 1], sB[cur ^ 1]);
+      if (s + 2 < steps) dgemm_load<T, TN>(t, A, B, M, N, K, m0, n0, (s + 2) * kDK);
     }
-    T a[4][4];
+    T a[IM][4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) dgemm_row4<T>(sA[cur], wm + 16 * i + fr, g, a[i]);
+    for (int i = 0; i < IM; i++) dgemm_row4<T>(sA[cur], wm + 16 * i + fr, g, a[i]);
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       T b[4];
@@ -442,13 +457,13 @@ dgemm_kernel(DGemmBatch dbt, const T *__restrict__ A0, const T *__restrict__ B0,
 #pragma unroll
       for (int tt = 0; tt < 4; tt++)
 #pragma unroll
-        for (int i = 0; i < 4; i++) acc[i][j] = DMma<T>::run(a[i][tt], b[tt], acc[i][j]);
+        for (int i = 0; i < IM; i++) acc[i][j] = DMma<T>::run(a[i][tt], b[tt], acc[i][j]);
     }
     __syncthreads();
   }
   if constexpr (MODE == 0) {
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < IM; i++)
 #pragma unroll
       for (int j = 0; j < 4; j++) {
         const int col = n0 + wn + 16 * j + fr;
@@ -463,11 +478,11 @@ dgemm_kernel(DGemmBatch dbt, const T *__restrict__ A0, const T *__restrict__ B0,
     // per row: the sum over this wavefront's 64 columns, reduced across the
     // 16 lanes (fr) that hold a row's columns
     const DEpilogue &ep = dbt.ep[q];
-    double rs[4][4];
-    double inv_nb[4][4];
-    bool row_empty[4][4];
+    double rs[IM][4];
+    double inv_nb[IM][4];
+    bool row_empty[IM][4];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < IM; i++)
 #pragma unroll
       for (int e = 0; e < 4; e++) {
         rs[i][e] = 0;
@@ -493,7 +508,7 @@ dgemm_kernel(DGemmBatch dbt, const T *__restrict__ A0, const T *__restrict__ B0,
         }
       }
 #pragma unroll
-      for (int i = 0; i < 4; i++)
+      for (int i = 0; i < IM; i++)
 #pragma unroll
         for (int e = 0; e < 4; e++) {
           const int row = m0 + wm + 16 * i + DMma<T>::row(g, e);
@@ -517,7 +532,7 @@ dgemm_kernel(DGemmBatch dbt, const T *__restrict__ A0, const T *__restrict__ B0,
         }
     }
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < IM; i++)
 #pragma unroll
       for (int e = 0; e < 4; e++) {
         double v = rs[i][e];
@@ -538,6 +553,21 @@ dgemm_kernel(DGemmBatch dbt, const T *__restrict__ A0, const T *__restrict__ B0,
   }
 }
 
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+dgemm_kernel(DGemmBatch dbt, const T *__restrict__ A0, const T *__restrict__ B0,
+             T *__restrict__ Z0) {
+  dgemm_body<T, MODE, 128>(dbt, A0, B0, Z0);
+}
+
+// 128 x 64 tiles, three workgroups per CU (48 KiB of LDS each in FP64)
+template <typename T, int MODE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
+dgemm_narrow_kernel(DGemmBatch dbt, const T *__restrict__ A0, const T *__restrict__ B0,
+                    T *__restrict__ Z0) {
+  dgemm_body<T, MODE, 64>(dbt, A0, B0, Z0);
+}
+
 template <typename T>
 static void dgemm_batched_t(hipStream_t st, int nb, const GemmProduct *prods,
                             const DEpilogue *ep, const T *A, const T *B, T *Z) {
@@ -545,7 +575,7 @@ static void dgemm_batched_t(hipStream_t st, int nb, const GemmProduct *prods,
   DGemmBatch dbt;
   GemmBatch &bt = dbt.bt;
   bt.nb = 0;
-  int tiles = 0, mode = 0;
+  int mode = 0;
   for (int q = 0; q < nb; q++) {
     const GemmProduct &g = prods[q];
     FLARE_REQUIRE((g.K * (int)sizeof(T)) % 16 == 0, "gemm: rows must be 16-byte multiples");
@@ -561,21 +591,45 @@ static void dgemm_batched_t(hipStream_t st, int nb, const GemmProduct *prods,
       dbt.ep[bt.nb] = e;
     }
     bt.g[bt.nb] = g;
-    bt.tile0[bt.nb] = tiles;
-    tiles += (g.M + kDM - 1) / kDM * ((g.N + kDN - 1) / kDN);
     bt.nb++;
+  }
+  // Tile width: the 128 x 128 tiles run two workgroups per CU, the
+  // 128 x 64 tiles three.  Rounds of workgroups over 256 CUs, each round of
+  // narrow tiles doing 3/4 of a wide round's work per CU: narrow when that
+  // is at least 1/8 less (products with a few hundred columns, whose wide
+  // tiles leave most of a second round idle).
+  int64_t wide = 0, narrow = 0;
+  for (int q = 0; q < bt.nb; q++) {
+    const int64_t rows = (bt.g[q].M + kDM - 1) / kDM;
+    wide += rows * ((bt.g[q].N + kDN - 1) / kDN);
+    narrow += rows * ((bt.g[q].N + 63) / 64);
+  }
+  const int64_t rounds_w = (wide + 511) / 512, rounds_n = (narrow + 767) / 768;
+  const bool use_narrow = 8 * 3 * rounds_n <= 7 * 4 * rounds_w;
+  const int TN = use_narrow ? 64 : kDN;
+  int tiles = 0;
+  for (int q = 0; q < bt.nb; q++) {
+    bt.tile0[q] = tiles;
+    tiles += (bt.g[q].M + kDM - 1) / kDM * ((bt.g[q].N + TN - 1) / TN);
   }
   bt.tile0[bt.nb] = tiles;
   if (tiles == 0) return;
   const int grid = (tiles + 7) / 8 * 8;
-  if (mode == 0)
-    hipLaunchKernelGGL((dgemm_kernel<T, 0>), dim3(grid), dim3(256), 0, st, dbt, A, B, Z);
-  else if (mode == DEpilogue::kKernel)
-    hipLaunchKernelGGL((dgemm_kernel<T, DEpilogue::kKernel>), dim3(grid), dim3(256), 0, st,
+#define FLARE_DGEMM(KERN)                                                                \
+  if (mode == 0)                                                                        \
+    hipLaunchKernelGGL((KERN<T, 0>), dim3(grid), dim3(256), 0, st, dbt, A, B, Z);        \
+  else if (mode == DEpilogue::kKernel)                                                  \
+    hipLaunchKernelGGL((KERN<T, DEpilogue::kKernel>), dim3(grid), dim3(256), 0, st, dbt, \
+                       A, B, Z);                                                        \
+  else                                                                                  \
+    hipLaunchKernelGGL((KERN<T, DEpilogue::kSquares>), dim3(grid), dim3(256), 0, st,     \
                        dbt, A, B, Z);
-  else
-    hipLaunchKernelGGL((dgemm_kernel<T, DEpilogue::kSquares>), dim3(grid), dim3(256), 0, st,
-                       dbt, A, B, Z);
+  if (use_narrow) {
+    FLARE_DGEMM(dgemm_narrow_kernel)
+  } else {
+    FLARE_DGEMM(dgemm_kernel)
+  }
+#undef FLARE_DGEMM
   FLARE_HIP_CHECK(hipGetLastError());
 }
 
