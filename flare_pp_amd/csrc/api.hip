@@ -508,7 +508,8 @@ static Forward run_forward(flare_ctx *c, const flare::B2Dev &p,
                            const flare_nlist *nl, const int *d_species,
                            int prec, const std::string &tag,
                            void *B_out = nullptr, int out_mode = 0,
-                           bool bf16 = false) {
+                           bool bf16 = false, void *f16_hi = nullptr,
+                           void *f16_lo = nullptr, float *f16_is = nullptr) {
   Forward f;
   const size_t ts = prec == flare::kPrecF64 ? sizeof(double) : sizeof(float);
   const int n = nl->n;
@@ -520,12 +521,16 @@ static Forward run_forward(flare_ctx *c, const flare::B2Dev &p,
   if (bf16) {
     f.xh = c->buf(tag + ".xh", std::max<size_t>(1, (size_t)n * f.kp) * 2);
     f.xl = c->buf(tag + ".xl", std::max<size_t>(1, (size_t)n * f.kp) * 2);
+  } else if (f16_is) {
+    // the caller's FP16 row-scaled split (rows of kp)
+    f.xh = f16_hi;
+    f.xl = f16_lo;
   }
   c->mark_begin("forward");
   flare::launch_b2_forward(c->stream, p, nl->dev(), d_species,
                            nl->b.row_of.as<int>(), c->ynorm_d.as<double>(),
                            c->ynorm_f.as<float>(), c->b2_index(p.S, p.N, p.L),
-                           prec, out_mode, f.kp, f.xh, f.xl, f.A, f.B, f.norm2);
+                           prec, out_mode, f.kp, f.xh, f.xl, f.A, f.B, f.norm2, f16_is);
   c->mark_end();
   return f;
 }
@@ -2149,7 +2154,18 @@ static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p
   sgp_build_pad(ctx, gp);
   GPState st;
   st.lowp = true;
-  st.fw = run_forward(ctx, p, nl, d_species, flare::kPrecF32, "gpl", nullptr, 1);
+  // mixed: the forward also writes the FP16 hi + lo split of the d rows
+  // with their power-of-two row scales (t's A operand)
+  void *dh = nullptr, *dl = nullptr;
+  float *d_is = nullptr;
+  if (mixed) {
+    REQUIRE(gp->kp == flare::bf16_padded_k(F), "padded row length");
+    dh = ctx->buf("gpl.dh", std::max<size_t>((size_t)n * gp->kp, 1) * 2);
+    dl = ctx->buf("gpl.dl", std::max<size_t>((size_t)n * gp->kp, 1) * 2);
+    d_is = ctx->buf_t<float>("gpl.d_is", std::max(n, 1));
+  }
+  st.fw = run_forward(ctx, p, nl, d_species, flare::kPrecF32, "gpl", nullptr, 1, false, dh,
+                      dl, d_is);
   const float *d = (const float *)st.fw.B;
   int off[flare::kMaxSpecies + 1];
   nl->row_offsets(p.S, off);
@@ -2189,10 +2205,6 @@ static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p
   ctx->mark_begin("gp_K");
   if (mixed) {
     const int kp = gp->kp;
-    void *dh = ctx->buf("gpl.dh", std::max<size_t>((size_t)n * kp, 1) * 2);
-    void *dl = ctx->buf("gpl.dl", std::max<size_t>((size_t)n * kp, 1) * 2);
-    float *d_is = ctx->buf_t<float>("gpl.d_is", std::max(n, 1));
-    flare::split_f16_rows_f32(ctx->stream, n, F, F, kp, d, dh, dl, d_is);
     flare::GemmProduct g[flare::kMaxSpecies];
     for (int s = 0; s < p.S; s++) {
       g[s] = flare::GemmProduct{off[s + 1] - off[s], gp->m_by_type[s], kp,

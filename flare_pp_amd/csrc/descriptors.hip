@@ -26,11 +26,48 @@ namespace flare {
 //   empty-environment threshold of pair_flare.cpp:135), plus optionally its
 //   BF16 hi + lo split into rows of out.kp (zero padded);  2: only that
 //   split (the mixed-precision form reads d nowhere else).
+// Rows the forward kernel writes beside A and norm2: mode 0 B, 1 d = B/|B|,
+// 2 none; xh / xl (when set) d as hi + lo rows of kp, BF16 (f16 == 0) or FP16
+// with one power-of-two scale per row (f16 == 1, inv_scale[row] = 2^-e).
 struct FwdOut {
   int mode;
   int kp;
   __bf16 *xh, *xl;
+  int f16;
+  float *inv_scale;
 };
+
+// FP16 hi + lo of the d row x_f = Bs[f] sc, its largest |x| brought into
+// [0.5, 1) by a power of two (split_f16_rows on the FP32 d rows, bit for bit)
+template <typename T>
+__device__ inline void store_f16_split(const T *Bs, T sc, int F, int row, const FwdOut &o) {
+  const int lane = threadIdx.x;
+  float m = 0.0f;
+  for (int f = lane; f < F; f += kWave) m = fmaxf(m, fabsf((float)(Bs[f] * sc)));
+  for (int off = kWave / 2; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  int e;
+  const bool nonzero = f16_row_scale(m, e);
+  const float s2 = ldexpf(1.0f, e);
+  uint2 *xh = reinterpret_cast<uint2 *>(o.xh + (int64_t)row * o.kp);
+  uint2 *xl = reinterpret_cast<uint2 *>(o.xl + (int64_t)row * o.kp);
+  for (int f4 = lane; 4 * f4 < o.kp; f4 += kWave) {
+    union {
+      _Float16 h[4];
+      uint2 u;
+    } H, Lo;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int f = 4 * f4 + j;
+      const float v = nonzero && f < F ? (float)(Bs[f] * sc) * s2 : 0.0f;
+      const _Float16 h = (_Float16)v;
+      H.h[j] = h;
+      Lo.h[j] = (_Float16)(v - (float)h);
+    }
+    xh[f4] = H.u;
+    xl[f4] = Lo.u;
+  }
+  if (lane == 0) o.inv_scale[row] = ldexpf(1.0f, -e);
+}
 
 template <typename T>
 __device__ inline void b2_contract_store(const B2Dev &p, const T *Al, T *Bs,
@@ -64,7 +101,9 @@ __device__ inline void b2_contract_store(const B2Dev &p, const T *Al, T *Bs,
     T *Bi = B_out + (int64_t)row * F;
     for (int f = lane; f < F; f += kWave) Bi[f] = Bs[f] * sc;
   }
-  if (o.xh) {
+  if (o.xh && o.f16) {
+    store_f16_split<T>(Bs, sc, F, row, o);
+  } else if (o.xh) {
     __bf16 *xh = o.xh + (int64_t)row * o.kp, *xl = o.xl + (int64_t)row * o.kp;
     for (int f = lane; f < o.kp; f += kWave) {
       float x = f < F ? (float)(Bs[f] * sc) : 0.0f;
@@ -201,7 +240,9 @@ __device__ inline void b2_store_rows(const B2Dev &p, const T *Bs, double nrm,
     T *Bi = B_out + (int64_t)row * F;
     for (int f = lane; f < F; f += kWave) Bi[f] = Bs[f] * sc;
   }
-  if (o.xh) {
+  if (o.xh && o.f16) {
+    store_f16_split<T>(Bs, sc, F, row, o);
+  } else if (o.xh) {
     uint2 *xh = reinterpret_cast<uint2 *>(o.xh + (int64_t)row * o.kp);
     uint2 *xl = reinterpret_cast<uint2 *>(o.xl + (int64_t)row * o.kp);
     for (int f4 = lane; 4 * f4 < o.kp; f4 += kWave) {
@@ -647,9 +688,11 @@ void launch_b2_forward(hipStream_t st, const B2Dev &p, const NListDev &nl,
                        const double *ynorm, const float *ynorm_f,
                        const int *b2_idx, int prec, int out_mode, int kp,
                        void *xh, void *xl, void *A_out, void *B_out,
-                       double *norm2_out) {
+                       double *norm2_out, float *f16_inv_scale) {
   if (nl.n == 0) return;
-  FwdOut out{out_mode, kp, (__bf16 *)xh, (__bf16 *)xl};
+  FLARE_REQUIRE(!f16_inv_scale || (prec != kPrecF64 && xh && xl),
+                "FP16 row split: FP32 rows with hi / lo outputs");
+  FwdOut out{out_mode, kp, (__bf16 *)xh, (__bf16 *)xl, f16_inv_scale ? 1 : 0, f16_inv_scale};
   // matrix-core path when the channel and harmonic counts fit its tiles.
   // FP32 single-bond sums are Kahan-compensated in the FP32 form; the mixed
   // form (out_mode 2) sums plainly: its error is set by the BF16 hi + lo

@@ -129,14 +129,15 @@ void ext_assemble(hipStream_t st, int n_all, int inum, const int *ilist,
 
 // descriptors.hip
 // A (atom order), GEMM rows (species-grouped: B when out_mode 0, B/|B| when
-// 1, plus the BF16 hi/lo split of the latter into rows of kp when xh/xl are
-// non-null) and |B|^2.
+// 1, plus the hi/lo split of the latter into rows of kp when xh/xl are
+// non-null: BF16, or FP16 with one power-of-two scale per row when
+// f16_inv_scale is non-null, which receives the rows' 2^-e) and |B|^2.
 void launch_b2_forward(hipStream_t st, const B2Dev &p, const NListDev &nl,
                        const int *species, const int *row_of,
                        const double *ynorm, const float *ynorm_f,
                        const int *b2_idx, int prec, int out_mode, int kp,
                        void *xh, void *xl, void *A_out, void *B_out,
-                       double *norm2_out);
+                       double *norm2_out, float *f16_inv_scale = nullptr);
 // dA/dr of pairs [p_begin, p_end) (plain form)
 void launch_pair_dA(hipStream_t st, const B2Dev &p, const NListDev &nl,
                     const int *species, const double *ynorm, int64_t p_begin,

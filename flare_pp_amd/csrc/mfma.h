@@ -83,8 +83,6 @@ void gemm_f16x3_batched(hipStream_t st, int nb, const GemmProduct *prods,
 // [0.5, 1)); inv_scale[r] = 1 / scale.  x rows of stride ldx, out rows of kp.
 void split_f16_rows(hipStream_t st, int rows, int cols, int ldx, int kp,
                     const double *x, void *hi, void *lo, float *inv_scale);
-void split_f16_rows_f32(hipStream_t st, int rows, int cols, int ldx, int kp,
-                        const float *x, void *hi, void *lo, float *inv_scale);
 // the same for the transpose of a rows x cols FP64 block (cols output rows
 // of ldo values, one scale per output row)
 void split_f16_cols(hipStream_t st, int rows, int cols, int ldo, const double *x,

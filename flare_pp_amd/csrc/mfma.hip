@@ -1010,15 +1010,6 @@ void split_f16_rows(hipStream_t st, int rows, int cols, int ldx, int kp,
   FLARE_HIP_CHECK(hipGetLastError());
 }
 
-void split_f16_rows_f32(hipStream_t st, int rows, int cols, int ldx, int kp,
-                        const float *x, void *hi, void *lo, float *inv_scale) {
-  if (rows == 0) return;
-  hipLaunchKernelGGL(split_f16_rows_kernel<float>, dim3(rows), dim3(64), 0, st,
-                     rows, cols, ldx, kp, x, (_Float16 *)hi, (_Float16 *)lo,
-                     inv_scale);
-  FLARE_HIP_CHECK(hipGetLastError());
-}
-
 void split_f16_cols(hipStream_t st, int rows, int cols, int ldo, const double *x,
                     void *hi, void *lo, float *inv_scale) {
   if (cols == 0) return;
