@@ -676,35 +676,38 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
   }
 }
 
-// 16 lanes per atom, four atoms per wavefront.  Also the per-atom virial
+// kGatherLanes lanes per atom (kGatherBatch pairs per lane per batch).  Also
+// the per-atom virial
 // rows V_a[b][c] = -sum_k r_ak,b f_ak,c over the atom's own pairs (the
 // pair_flare tally of (x_i - x_j) f_ij, pair_flare.cpp:160-165).
+constexpr int kGatherLanes = 8, kGatherBatch = kWave / kGatherLanes;
+constexpr int kGatherAtoms = 256 / kGatherLanes;  // atoms per workgroup
 template <typename PF>
 __global__ void __launch_bounds__(256)
 gather_kernel(int n, const int *__restrict__ cum, const int *__restrict__ rev,
               const double *__restrict__ rel, const PF *__restrict__ pf,
               double *__restrict__ forces, const int *__restrict__ row_of,
               const double *__restrict__ e_row, double *__restrict__ part) {
-  __shared__ double sh[16][10];
+  __shared__ double sh[kGatherAtoms][10];
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int a = t >> 4, sub = t & 15;
+  const int a = t / kGatherLanes, sub = t % kGatherLanes;
   double f0 = 0, f1 = 0, f2 = 0;
   double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (a < n) {
     const int e = cum[a + 1];
-    // up to four pairs per lane per batch, all loads issued before use
-    for (int k0 = cum[a] + sub; k0 < e; k0 += 64) {
-      int r[4];
+    // kGatherBatch pairs per lane per batch, all loads issued before use
+    for (int k0 = cum[a] + sub; k0 < e; k0 += kWave) {
+      int r[kGatherBatch];
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int k = k0 + 16 * j;
+      for (int j = 0; j < kGatherBatch; j++) {
+        const int k = k0 + kGatherLanes * j;
         r[j] = k < e ? rev[k] : -1;
       }
-      double g[4][3], h[4][3];
-      double4 rv[4];
+      double g[kGatherBatch][3], h[kGatherBatch][3];
+      double4 rv[kGatherBatch];
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int k = k0 + 16 * j;
+      for (int j = 0; j < kGatherBatch; j++) {
+        const int k = k0 + kGatherLanes * j;
         for (int c = 0; c < 3; c++) {
           g[j][c] = 0;
           h[j][c] = 0;
@@ -718,7 +721,7 @@ gather_kernel(int n, const int *__restrict__ cum, const int *__restrict__ rev,
           for (int c = 0; c < 3; c++) h[j][c] = (double)pf[3 * (int64_t)r[j] + c];
       }
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
+      for (int j = 0; j < kGatherBatch; j++) {
         f0 += g[j][0] - h[j][0];
         f1 += g[j][1] - h[j][1];
         f2 += g[j][2] - h[j][2];
@@ -731,12 +734,12 @@ gather_kernel(int n, const int *__restrict__ cum, const int *__restrict__ rev,
     }
   }
 #pragma unroll
-  for (int off = 8; off > 0; off >>= 1) {
-    f0 += __shfl_xor(f0, off, 16);
-    f1 += __shfl_xor(f1, off, 16);
-    f2 += __shfl_xor(f2, off, 16);
+  for (int off = kGatherLanes / 2; off > 0; off >>= 1) {
+    f0 += __shfl_xor(f0, off, kGatherLanes);
+    f1 += __shfl_xor(f1, off, kGatherLanes);
+    f2 += __shfl_xor(f2, off, kGatherLanes);
 #pragma unroll
-    for (int q = 0; q < 9; q++) v[q] += __shfl_xor(v[q], off, 16);
+    for (int q = 0; q < 9; q++) v[q] += __shfl_xor(v[q], off, kGatherLanes);
   }
   if (a < n && sub == 0 && forces) {
     forces[3 * (int64_t)a] = f0;
@@ -745,7 +748,7 @@ gather_kernel(int n, const int *__restrict__ cum, const int *__restrict__ rev,
   }
   // per-block sums of the atoms' energies and virial rows, fixed order
   if (sub == 0) {
-    const int la = threadIdx.x >> 4;
+    const int la = threadIdx.x / kGatherLanes;
     sh[la][0] = a < n ? e_row[row_of[a]] : 0.0;
 #pragma unroll
     for (int q = 0; q < 9; q++) sh[la][1 + q] = a < n ? v[q] : 0.0;
@@ -753,7 +756,7 @@ gather_kernel(int n, const int *__restrict__ cum, const int *__restrict__ rev,
   __syncthreads();
   if (threadIdx.x < 10) {
     double s = 0;
-    for (int la = 0; la < 16; la++) s += sh[la][threadIdx.x];
+    for (int la = 0; la < kGatherAtoms; la++) s += sh[la][threadIdx.x];
     part[blockIdx.x * 10 + threadIdx.x] = s;
   }
 }
@@ -905,7 +908,7 @@ void launch_forces(hipStream_t st, int prec, int mode, const B2Dev &p,
   FLARE_HIP_CHECK(hipGetLastError());
 }
 
-int gather_parts(int n) { return (n * 16 + 255) / 256; }
+int gather_parts(int n) { return (n + kGatherAtoms - 1) / kGatherAtoms; }
 
 __global__ void zero_results_kernel(double *energy, double *virial6, double *stress6) {
   const int t = threadIdx.x;
