@@ -196,6 +196,7 @@ class NeighborList:
             self._h = L.orc_nlist_cells(c, n, _pd(self.positions),
                                         _pi(self.species), self.cutoff, nthreads)
             rows = n
+        self._free = L.orc_nlist_free
         P = L.orc_nlist_n_pairs(self._h)
         self.n_rows = rows
         self.count = np.zeros(rows, np.int32)
@@ -216,9 +217,12 @@ class NeighborList:
         return self._h
 
     def __del__(self):
+        # the library handle is looked up at construction: module globals may
+        # already be gone when the interpreter shuts down
         h = getattr(self, "_h", None)
-        if h:
-            lib().orc_nlist_free(h)
+        free = getattr(self, "_free", None)
+        if h and free is not None:
+            free(h)
             self._h = None
 
 
@@ -250,6 +254,7 @@ class B2Result:
         self._h = lib().orc_b2_compute(nlist.handle, _pi(self.species), S, N, L,
                                        r1, cutoff_code(cutoff),
                                        _pd(self.cutoffs.ravel()), int(dervs))
+        self._free = lib().orc_b2_free
         self.dervs = dervs
         self.F = lib().orc_b2_n_descriptors(self._h)
         self.H = (L + 1) ** 2
@@ -298,9 +303,12 @@ class B2Result:
         return out
 
     def __del__(self):
+        # the library handle is looked up at construction: module globals may
+        # already be gone when the interpreter shuts down
         h = getattr(self, "_h", None)
-        if h:
-            lib().orc_b2_free(h)
+        free = getattr(self, "_free", None)
+        if h and free is not None:
+            free(h)
             self._h = None
 
 

@@ -57,6 +57,21 @@ def test_config2_full(orc):
     np.testing.assert_array_equal(rev[rev], np.arange(len(rev)))
 
 
+def test_above_block_scan_limit(orc):
+    """275,684 atoms (41^3 fcc cells, a = 3.6 A, rc 4.5 A): above the
+    262,144 atoms whose CSR offsets come from the enumeration's per-block
+    sums, so the device-wide count scan runs; bitwise against the oracle's
+    cell-list build, reverse pairs an involution with opposite vectors."""
+    n = 4 * 41 ** 3
+    cell, species, pos = synth.fcc(41, 3.6, 0.05, [n // 2, n - n // 2], 57)
+    s, o = compare(orc, cell, species, pos, 4.5)
+    from flare_pp_amd.structure import d2h
+    rev = d2h(s._view.d_reverse_pair, s.n_neighbors, np.int32)
+    assert rev.min() >= 0
+    np.testing.assert_array_equal(o.rel[rev, 1:], -o.rel[:, 1:])
+    np.testing.assert_array_equal(rev[rev], np.arange(len(rev)))
+
+
 def test_species_grouping(orc):
     c = synth.config4()
     s = Structure(c["cell"], c["species"], c["positions"], 4.5)
