@@ -354,8 +354,11 @@ __device__ inline void adjoint_pair_fixed(const B2Dev &p, T r, T x, T y, T z,
 
 }  // namespace
 
-// One wavefront per atom.  Dynamic LDS (T): A[W], w[F], u[W], x[F], n[H],
-// then int ptab[nr * nr].
+// One wavefront per atom.  Dynamic LDS (T): A[W], w[F], then u[W] sharing
+// max(W, F) values with the X row x[F] (x is read only before w is formed,
+// u written only after), n[H], then int ptab[nr * nr] unless the u
+// contraction's channel-pair offsets come from registers (force_lds_bytes).
+// At l_max 6 the LDS, not the registers, sets how many wavefronts a CU holds.
 // WM 3 is WM 1 with X given as BF16 hi + lo rows (the GEMM's operand).
 // WM selects where w = dE_i/dB_i comes from:
 //   0  precomputed (sparse-GP path);
@@ -383,8 +386,8 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
   T *A = reinterpret_cast<T *>(smem);
   T *w = A + W;
   T *u = w + F;
-  T *xs = u + W;        // X row (rebuilt from hi + lo when BF16)
-  T *nrm_lm = xs + F;   // harmonic normalisation per lm
+  T *xs = u;                        // X row (rebuilt from hi + lo when BF16)
+  T *nrm_lm = u + (W > F ? W : F);  // harmonic normalisation per lm
   int *ptab = reinterpret_cast<int *>(nrm_lm + H);  // nr x nr feature offsets
   const T *Ai = A_in + (int64_t)i * W;
   const int row = row_of[i];
@@ -839,8 +842,13 @@ static void launch_forces_t(hipStream_t st, const B2Dev &p, const NListDev &nl,
                             const T *ynorm, const void *A, const WSource<T> &src,
                             const double *dA, int64_t dA_p0, int a0, int a1,
                             T *pair_f) {
-  size_t lds = sizeof(T) * (2 * (size_t)p.W + 2 * (size_t)p.F + (size_t)p.H) +
-               sizeof(int) * (size_t)p.nr * p.nr;
+  // the fixed-size instances with 8, 16 or 24 channels keep the u
+  // contraction's channel-pair offsets in registers (no ptab)
+  const bool fixed_u = MODE == 1 && p.N == 8 && (p.L == 3 || p.L == 4 || p.L == 6) &&
+                       (p.nr == 8 || p.nr == 16 || p.nr == 24);
+  const size_t lds = sizeof(T) * ((size_t)p.W + (size_t)p.F + (size_t)std::max(p.W, p.F) +
+                                  (size_t)p.H) +
+                     (fixed_u ? 0 : sizeof(int) * (size_t)p.nr * p.nr);
   FLARE_REQUIRE(lds <= 64 * 1024, "descriptor settings too large for the force kernel");
   FLARE_REQUIRE(p.nr <= 32, "at most 32 radial channels (n_species * n_max)");
 #define FLARE_FORCE_LAUNCH(NM, LF, EX)                                        \
