@@ -383,11 +383,21 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
   const int i = a0 + blockIdx.x;
   const int lane = threadIdx.x;
   const int H = p.H, NH = p.NH, W = p.W, F = p.F;
+  // the fixed-size l_max 6 instances of the mixed and FP64 forms hold the X
+  // row in registers, kXPairs feature pairs per lane, when it fits (F <=
+  // 2,176; the launch's LDS size agrees): their LDS, not their registers,
+  // bounds the wavefronts per CU.  (At l_max 3 and 4 the extra registers
+  // would spill; the FP32 form ran slower with them: 4.2 vs 3.6 ms at
+  // 262,144 atoms.)
+  constexpr bool kFixedShape = MODE == 1 && LFIX == 6 && EXACT && NMAX == 8 &&
+                               (WM == 3 || sizeof(T) == 8);
+  constexpr int kXPairs = 17;
+  const bool x_regs = kFixedShape && F <= 2 * kXPairs * kWave;
   T *A = reinterpret_cast<T *>(smem);
   T *w = A + W;
   T *u = w + F;
-  T *xs = u;                        // X row (rebuilt from hi + lo when BF16)
-  T *nrm_lm = u + (W > F ? W : F);  // harmonic normalisation per lm
+  T *xs = u;  // X row (rebuilt from hi + lo when BF16) when not in registers
+  T *nrm_lm = u + (x_regs ? W : (W > F ? W : F));  // harmonic normalisation per lm
   int *ptab = reinterpret_cast<int *>(nrm_lm + H);  // nr x nr feature offsets
   const T *Ai = A_in + (int64_t)i * W;
   const int row = row_of[i];
@@ -509,7 +519,47 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
     if (lane == 0) src.E_row[row] = E;
   } else {
     const double n2 = src.norm2[row];
-    if (WM == 3) {
+    // X row in registers: features 2 q and 2 q + 1, q = lane + j kWave
+    T xv[kFixedShape ? 2 * kXPairs : 1];
+    if constexpr (kFixedShape) {
+      if (x_regs) {
+        if (WM == 3) {
+          const uint32_t *xh =
+              reinterpret_cast<const uint32_t *>(src.xh + (int64_t)row * src.kp);
+          const uint32_t *xl =
+              reinterpret_cast<const uint32_t *>(src.xl + (int64_t)row * src.kp);
+          const int F2 = (F + 1) >> 1;
+          uint32_t th2[kXPairs], tl2[kXPairs];
+#pragma unroll
+          for (int j = 0; j < kXPairs; j++) {
+            const int q = lane + j * kWave;
+            th2[j] = 0;
+            tl2[j] = 0;
+            if (q < F2) {
+              th2[j] = xh[q];
+              tl2[j] = xl[q];
+            }
+          }
+#pragma unroll
+          for (int j = 0; j < kXPairs; j++) {
+            xv[2 * j] = (T)(__uint_as_float(th2[j] << 16) + __uint_as_float(tl2[j] << 16));
+            xv[2 * j + 1] = (T)(__uint_as_float(th2[j] & 0xffff0000u) +
+                                __uint_as_float(tl2[j] & 0xffff0000u));
+          }
+        } else {
+          const T *x = src.X + (int64_t)row * F;
+#pragma unroll
+          for (int j = 0; j < kXPairs; j++) {
+            const int f = 2 * (lane + j * kWave);
+            xv[2 * j] = f < F ? x[f] : T(0);
+            xv[2 * j + 1] = f + 1 < F ? x[f + 1] : T(0);
+          }
+        }
+      }
+    }
+    if (x_regs) {
+      // (held in xv)
+    } else if (WM == 3) {
       // pairs of BF16 values: rows are padded to an even length (kp)
       const uint32_t *xh = reinterpret_cast<const uint32_t *>(src.xh + (int64_t)row * src.kp);
       const uint32_t *xl = reinterpret_cast<const uint32_t *>(src.xl + (int64_t)row * src.kp);
@@ -553,7 +603,18 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
     } else {
       const double bn = sqrt(n2);
       double d = 0;
-      for (int f = lane; f < F; f += kWave) d += (double)xs[f] * (double)w[f];
+      if constexpr (kFixedShape) {
+        if (x_regs) {
+#pragma unroll
+          for (int j = 0; j < kXPairs; j++) {
+            const int f = 2 * (lane + j * kWave);
+            if (f < F) d += (double)xv[2 * j] * (double)w[f];
+            if (f + 1 < F) d += (double)xv[2 * j + 1] * (double)w[f + 1];
+          }
+        }
+      }
+      if (!x_regs)
+        for (int f = lane; f < F; f += kWave) d += (double)xs[f] * (double)w[f];
       d = wave_sum(d);
       double E, sx, sz;
       if (WM != 2) {
@@ -565,8 +626,19 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
         sz = 1.0 / bn;
         sx = -E / bn * (src.normalized ? 1.0 : 1.0 / bn);
       }
-      for (int f = lane; f < F; f += kWave)
-        w[f] = (T)(sz * (double)w[f] + sx * (double)xs[f]);
+      if constexpr (kFixedShape) {
+        if (x_regs) {
+#pragma unroll
+          for (int j = 0; j < kXPairs; j++) {
+            const int f = 2 * (lane + j * kWave);
+            if (f < F) w[f] = (T)(sz * (double)w[f] + sx * (double)xv[2 * j]);
+            if (f + 1 < F) w[f + 1] = (T)(sz * (double)w[f + 1] + sx * (double)xv[2 * j + 1]);
+          }
+        }
+      }
+      if (!x_regs)
+        for (int f = lane; f < F; f += kWave)
+          w[f] = (T)(sz * (double)w[f] + sx * (double)xs[f]);
       if (lane == 0) src.E_row[row] = E;
     }
   }
@@ -844,10 +916,14 @@ static void launch_forces_t(hipStream_t st, const B2Dev &p, const NListDev &nl,
                             T *pair_f) {
   // the fixed-size instances with 8, 16 or 24 channels keep the u
   // contraction's channel-pair offsets in registers (no ptab)
-  const bool fixed_u = MODE == 1 && p.N == 8 && (p.L == 3 || p.L == 4 || p.L == 6) &&
-                       (p.nr == 8 || p.nr == 16 || p.nr == 24);
-  const size_t lds = sizeof(T) * ((size_t)p.W + (size_t)p.F + (size_t)std::max(p.W, p.F) +
-                                  (size_t)p.H) +
+  const bool fixed_shape = MODE == 1 && p.N == 8 && (p.L == 3 || p.L == 4 || p.L == 6);
+  const bool fixed_u = fixed_shape && (p.nr == 8 || p.nr == 16 || p.nr == 24);
+  // and, at l_max 6 in the mixed and FP64 forms, hold the X row in
+  // registers up to 2,176 features
+  const bool x_regs = MODE == 1 && p.N == 8 && p.L == 6 && p.F <= 2 * 17 * kWave &&
+                      (WM == 3 || sizeof(T) == 8);
+  const size_t lds = sizeof(T) * ((size_t)p.W + (size_t)p.F +
+                                  (size_t)(x_regs ? p.W : std::max(p.W, p.F)) + (size_t)p.H) +
                      (fixed_u ? 0 : sizeof(int) * (size_t)p.nr * p.nr);
   FLARE_REQUIRE(lds <= 64 * 1024, "descriptor settings too large for the force kernel");
   FLARE_REQUIRE(p.nr <= 32, "at most 32 radial channels (n_species * n_max)");
