@@ -283,10 +283,12 @@ struct FwdStage {
   static_assert(kRows % 4 == 0 && kRows <= kWave, "whole K steps, one pair per lane");
 };
 
-// LDS values of b2_forward_mfma_kernel: the staging rows, with B (F) and A
-// (nr x H) placed inside them when both fit, else after them.
+// LDS values of b2_forward_mfma_kernel: the staging rows, and B (F) then A
+// (nr x H) from the start once the staging rows are dead (a barrier ends
+// the single-bond loop before A is written).  At l_max 6 with 24 channels
+// this is what bounds the wavefronts per CU.
 __host__ __device__ inline size_t fwd_lds_values(int stage, int F, int A) {
-  return F + A <= stage ? (size_t)stage : (size_t)(stage > F ? stage : F) + A;
+  return F + A <= stage ? (size_t)stage : (size_t)(F + A);
 }
 
 // LFIX >= 0: l_max fixed at compile time; EXACT: n_max == NMAX.
@@ -305,11 +307,11 @@ b2_forward_mfma_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
   constexpr int CH = St::kRows;  // pairs per chunk
   extern __shared__ __align__(16) unsigned char smem[];
   // B (F values) and A (nr x H) reuse the pair staging rows, dead once A is
-  // formed, when they fit there (fwd_lds_values)
+  // formed (fwd_lds_values)
   T *Gs = reinterpret_cast<T *>(smem);
   T *Ys = Gs + CH * KP;
   T *Bs = Gs;
-  T *Al = Gs + (p.F + p.nr * p.H <= St::kStage ? p.F : max(St::kStage, p.F));
+  T *Al = Gs + p.F;
   const int i = blockIdx.x;
   const int lane = threadIdx.x;
   const int N = EXACT ? NMAX : p.N;
