@@ -383,14 +383,11 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
   const int i = a0 + blockIdx.x;
   const int lane = threadIdx.x;
   const int H = p.H, NH = p.NH, W = p.W, F = p.F;
-  // the fixed-size l_max 6 instances of the mixed and FP64 forms hold the X
-  // row in registers, kXPairs feature pairs per lane, when it fits (F <=
-  // 2,176; the launch's LDS size agrees): their LDS, not their registers,
-  // bounds the wavefronts per CU.  (At l_max 3 and 4 the extra registers
-  // would spill; the FP32 form ran slower with them: 4.2 vs 3.6 ms at
-  // 262,144 atoms.)
-  constexpr bool kFixedShape = MODE == 1 && LFIX == 6 && EXACT && NMAX == 8 &&
-                               (WM == 3 || sizeof(T) == 8);
+  // the fixed-size l_max 6 instances hold the X row in registers, kXPairs
+  // feature pairs per lane, when it fits (F <= 2,176; the launch's LDS size
+  // agrees): their LDS, not their registers, bounds the wavefronts per CU.
+  // (At l_max 3 and 4 the extra registers would spill.)
+  constexpr bool kFixedShape = MODE == 1 && LFIX == 6 && EXACT && NMAX == 8;
   constexpr int kXPairs = 17;
   const bool x_regs = kFixedShape && F <= 2 * kXPairs * kWave;
   T *A = reinterpret_cast<T *>(smem);
@@ -548,11 +545,30 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
           }
         } else {
           const T *x = src.X + (int64_t)row * F;
+          if ((F & 1) == 0) {
+            // even rows: each lane's feature pairs as one 2-vector load
+            typedef typename std::conditional<sizeof(T) == 8, double2, float2>::type T2;
+            const T2 *x2 = reinterpret_cast<const T2 *>(x);
+            T2 t2[kXPairs];
 #pragma unroll
-          for (int j = 0; j < kXPairs; j++) {
-            const int f = 2 * (lane + j * kWave);
-            xv[2 * j] = f < F ? x[f] : T(0);
-            xv[2 * j + 1] = f + 1 < F ? x[f + 1] : T(0);
+            for (int j = 0; j < kXPairs; j++) {
+              const int q = lane + j * kWave;
+              t2[j].x = T(0);
+              t2[j].y = T(0);
+              if (2 * q < F) t2[j] = x2[q];
+            }
+#pragma unroll
+            for (int j = 0; j < kXPairs; j++) {
+              xv[2 * j] = t2[j].x;
+              xv[2 * j + 1] = t2[j].y;
+            }
+          } else {
+#pragma unroll
+            for (int j = 0; j < kXPairs; j++) {
+              const int f = 2 * (lane + j * kWave);
+              xv[2 * j] = f < F ? x[f] : T(0);
+              xv[2 * j + 1] = f + 1 < F ? x[f + 1] : T(0);
+            }
           }
         }
       }
@@ -918,10 +934,8 @@ static void launch_forces_t(hipStream_t st, const B2Dev &p, const NListDev &nl,
   // contraction's channel-pair offsets in registers (no ptab)
   const bool fixed_shape = MODE == 1 && p.N == 8 && (p.L == 3 || p.L == 4 || p.L == 6);
   const bool fixed_u = fixed_shape && (p.nr == 8 || p.nr == 16 || p.nr == 24);
-  // and, at l_max 6 in the mixed and FP64 forms, hold the X row in
-  // registers up to 2,176 features
-  const bool x_regs = MODE == 1 && p.N == 8 && p.L == 6 && p.F <= 2 * 17 * kWave &&
-                      (WM == 3 || sizeof(T) == 8);
+  // and, at l_max 6, hold the X row in registers up to 2,176 features
+  const bool x_regs = MODE == 1 && p.N == 8 && p.L == 6 && p.F <= 2 * 17 * kWave;
   const size_t lds = sizeof(T) * ((size_t)p.W + (size_t)p.F +
                                   (size_t)(x_regs ? p.W : std::max(p.W, p.F)) + (size_t)p.H) +
                      (fixed_u ? 0 : sizeof(int) * (size_t)p.nr * p.nr);
