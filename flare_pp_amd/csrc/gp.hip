@@ -255,16 +255,13 @@ gp_epilogue_lp_kernel(int rows, int M, int Mp, const float *__restrict__ K,
   if (lane == 0) e_row[r] = e;
   if (!ch) return;
   for (int off = 32; off > 0; off >>= 1) cmax = fmaxf(cmax, __shfl_xor(cmax, off));
-  int ex = 0;
-  if (cmax > 0.0f) {
-    frexpf(cmax, &ex);
-    ex = -ex;
-  }
+  int ex;
+  const bool nonzero = f16_row_scale(cmax, ex);
   const float sc = ldexpf(1.0f, ex);
   for (int u = lane; u < Mp; u += kWave) {
     double kv, cv, ev;
     cval(u, kv, cv, ev);
-    const float x = (float)cv * sc;
+    const float x = nonzero ? (float)cv * sc : 0.0f;
     const _Float16 h = (_Float16)x;
     ch[(int64_t)r * Mp + u] = h;
     cl[(int64_t)r * Mp + u] = (_Float16)(x - (float)h);
