@@ -406,22 +406,26 @@ struct flare_nlist {
   int n_species_present = 0;
   bool built = false;
   // a build is enqueued without waiting; its host-side summary (pinned
-  // stats_host, ready at stats_ev) is read on the list's next use
+  // coherent stats_host, stored by the device with sequence number
+  // stats_seq) is read on the list's next use
   bool pending = false;
-  int *stats_host = nullptr;
-  hipEvent_t stats_ev = nullptr;
+  int *stats_host = nullptr, *stats_dev = nullptr;
+  unsigned stats_seq = 0;
+  hipStream_t build_stream = nullptr;
   const int *d_species = nullptr;
 
   ~flare_nlist() {
     if (stats_host) (void)hipHostFree(stats_host);
-    if (stats_ev) (void)hipEventDestroy(stats_ev);
+  }
+  void wait_stats() const {
+    flare::nlist_wait_stats(build_stream, stats_host, stats_seq);
   }
   void resolve() {
     if (!pending) return;
     pending = false;
     built = false;
     ctx->use();
-    FLARE_HIP_CHECK(hipEventSynchronize(stats_ev));
+    wait_stats();
     n_species_present = flare::nlist_resolve(ctx->stream, b, n, d_species, wh,
                                              stats_host, species_count);
     built = true;
@@ -1176,12 +1180,17 @@ int flare_nlist_build(flare_ctx *ctx, flare_nlist *nl, int n_atoms,
     nl->d_species = d_species;
     if (!nl->stats_host) {
       FLARE_HIP_CHECK(hipHostMalloc((void **)&nl->stats_host,
-                                    flare::kNlistStats * sizeof(int), hipHostMallocDefault));
-      FLARE_HIP_CHECK(hipEventCreateWithFlags(&nl->stats_ev, hipEventDisableTiming));
+                                    flare::kNlistHostInts * sizeof(int),
+                                    hipHostMallocMapped | hipHostMallocCoherent));
+      std::memset(nl->stats_host, 0, flare::kNlistHostInts * sizeof(int));
+      FLARE_HIP_CHECK(hipHostGetDevicePointer((void **)&nl->stats_dev, nl->stats_host, 0));
     }
+    nl->stats_seq++;
+    if (nl->stats_seq == 0) nl->stats_seq = 1;  // 0 is the block's initial value
+    nl->build_stream = ctx->stream;
     ctx->mark_begin("nlist");
     flare::nlist_build_async(ctx->stream, nl->b, n_atoms, d_positions, d_species,
-                             nl->wh, nl->stats_host, nl->stats_ev);
+                             nl->wh, nl->stats_dev, nl->stats_seq);
     ctx->mark_end();
     nl->pending = true;
     nl->built = true;
@@ -1725,7 +1734,7 @@ int flare_pair_compute_noalloc(flare_ctx *ctx, flare_pair *pr, const flare_nlist
     // a pending build is completed here only when that needs no new buffers
     // (rows over the one-pass capacity are rebuilt into larger ones)
     if (nl->pending) {
-      FLARE_HIP_CHECK(hipEventSynchronize(nl->stats_ev));
+      nl->wait_stats();
       REQUIRE(nl->stats_host[flare::kMaxSpecies + 1] <= flare::kNbrCap,
               "the neighbor list has atoms with more than " +
                   std::to_string(flare::kNbrCap) +

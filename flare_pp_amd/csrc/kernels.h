@@ -73,17 +73,26 @@ struct NListBuffers {
 // Ints of the host-side build summary: species counts (kMaxSpecies), a
 // bad-species flag, the largest over-capacity neighbor count, the pair total.
 constexpr int kNlistStats = kMaxSpecies + 3;
+// The host-side summary block (pinned, coherent): the kNlistStats ints,
+// then at kNlistSeq the sequence number of the build they belong to, stored
+// last with system scope by the kernel that completes the summary.
+constexpr int kNlistSeq = 15, kNlistHostInts = 16;
+static_assert(kNlistStats <= kNlistSeq, "summary ints before the sequence slot");
 // Neighbors per atom held in LDS by the fused enumerate + sort pass; rows
 // with more go to the two-pass build.
 constexpr int kNbrCap = 128;
 // Enqueues the neighbor list (count, cum, idx, nspec, rel, center, rev,
 // keys), wrapped positions and the species-grouped row map row_of[i],
 // atom_of_row[row], species_of_row[row] without waiting; the summary is
-// copied to stats_host (pinned, kNlistStats ints) before stats_ev.
+// written by the device into the host block at stats_dev (the device address
+// of a kNlistHostInts pinned block) with sequence number seq.
 void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_pos,
-                       const int *d_species, const WrapHost &wh, int *stats_host,
-                       hipEvent_t stats_ev);
-// After stats_ev: validates the species codes, sets b.P, completes the list
+                       const int *d_species, const WrapHost &wh, int *stats_dev,
+                       unsigned seq);
+// Waits on the host until the summary of build seq is in stats_host; an
+// error if stream st (where the build was enqueued) ends or fails first.
+void nlist_wait_stats(hipStream_t st, const volatile int *stats_host, unsigned seq);
+// After nlist_wait_stats: validates the species codes, sets b.P, completes the list
 // with the two-pass path when an atom had more neighbors than the fused
 // path holds, copies the species counts to host_counts and returns the
 // number of species present (max species + 1).

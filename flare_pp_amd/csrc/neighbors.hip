@@ -13,6 +13,8 @@
 // axes, all) neighboring bins cooperatively.
 #include <hipcub/hipcub.hpp>
 
+#include <atomic>
+#include <chrono>
 #include <cmath>
 
 #include "common.h"
@@ -100,10 +102,24 @@ __global__ void clear4_kernel(int *__restrict__ a, int na, int *__restrict__ b,
 // workgroup writes cum[n] and the pair total.  Integer sums, so the result
 // does not depend on the order of the enumeration's atomics.  For n <=
 // kCumBlock kCumMaxBlocks (each workgroup reads at most kCumMaxBlocks sums).
+// Copies the finished summary counts[0..kNlistStats) into the pinned host
+// block, then its sequence number (one thread; system-scope release so the
+// host sees the counts once it sees seq).
+__device__ inline void publish_stats(const int *counts, int *host, unsigned seq) {
+  for (int k = 0; k < kNlistStats; k++) host[k] = counts[k];
+  __threadfence_system();
+  __hip_atomic_store(host + kNlistSeq, (int)seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__global__ void publish_stats_kernel(const int *__restrict__ counts, int *host, unsigned seq) {
+  if (threadIdx.x == 0) publish_stats(counts, host, seq);
+}
+
 constexpr int kCumBlock = 256, kCumMaxBlocks = 1024, kCumStride = 32;
 __global__ void __launch_bounds__(kCumBlock)
 cum_blocks_kernel(int n, const int *__restrict__ count, const int *__restrict__ blk_sum,
-                  int *__restrict__ cum, int *__restrict__ stat_P) {
+                  int *__restrict__ cum, int *__restrict__ counts, int *host,
+                  unsigned seq) {
   constexpr int kW = kCumBlock / kWave;
   __shared__ int before_w[kW], total_w[kW];
   const int b = blockIdx.x, t = threadIdx.x, lane = t & (kWave - 1), wv = t / kWave;
@@ -129,7 +145,8 @@ cum_blocks_kernel(int n, const int *__restrict__ count, const int *__restrict__ 
   if (i < n) cum[i] = ex;
   if (b == gridDim.x - 1 && t == kCumBlock - 1) {
     cum[n] = ex + c;
-    *stat_P = ex + c;
+    counts[kMaxSpecies + 2] = ex + c;
+    publish_stats(counts, host, seq);
   }
 }
 
@@ -1072,9 +1089,11 @@ static void count_scan_stats(hipStream_t st, NListBuffers &b, int n, int nblk,
 // Host driver.  The build is enqueued without waiting: the fused path
 // writes the final arrays into capacity n * kNbrCap, and the numbers the
 // host needs (species counts, bad-code flag, largest over-capacity count,
-// pair total) are copied to stats_host behind an event.  nlist_resolve
-// reads them and, in the rare case of an over-capacity atom, rebuilds the
-// arrays with the two-pass path.
+// pair total) are stored by the kernel that completes them into the pinned
+// host block with the build's sequence number (no copy and no event: an
+// event's system-scope release and the copy kernel cost about 10 us before
+// the compaction).  nlist_resolve reads them and, in the rare case of an
+// over-capacity atom, rebuilds the arrays with the two-pass path.
 
 static WrapMats wrap_mats(const WrapHost &wh) {
   WrapMats m;
@@ -1108,9 +1127,31 @@ static float prefilter_cut2(const WrapHost &wh) {
   return std::nextafter((float)(r * r), INFINITY);
 }
 
+// The stream is queried only after 20 ms without the summary (a build that
+// failed or never ran): a query enqueues a completion marker whose
+// system-scope release stalls the next kernel by about 5 us.
+void nlist_wait_stats(hipStream_t st, const volatile int *stats_host, unsigned seq) {
+  const auto t0 = std::chrono::steady_clock::now();
+  for (unsigned spin = 1;; spin++) {
+    if ((unsigned)stats_host[kNlistSeq] == seq) break;
+    if ((spin & 1023) == 0 &&
+        std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(20)) {
+      const hipError_t q = hipStreamQuery(st);
+      if (q == hipErrorNotReady) continue;
+      FLARE_HIP_CHECK(q);
+      FLARE_HIP_CHECK(hipStreamSynchronize(st));
+      FLARE_REQUIRE((unsigned)stats_host[kNlistSeq] == seq,
+                    "the neighbor-list build ended without publishing its summary");
+      break;
+    }
+    __builtin_ia32_pause();
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+}
+
 void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_pos,
-                       const int *d_species, const WrapHost &wh, int *stats_host,
-                       hipEvent_t stats_ev) {
+                       const int *d_species, const WrapHost &wh, int *stats_dev,
+                       unsigned seq) {
   const WrapMats m = wrap_mats(wh);
   const int nbins = wh.nb[0] * wh.nb[1] * wh.nb[2];
   const int nblk = std::max(1, (n + kRowBlock - 1) / kRowBlock);
@@ -1209,15 +1250,15 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
                          b.bin_pos.as<double>(), b.count.as<int>(),
                          b.keys_pad.as<int64_t>(), overflow, blk_sum);
   }
-  if (cum_blocks)
+  if (cum_blocks) {
     hipLaunchKernelGGL(cum_blocks_kernel, dim3(ncb), dim3(kCumBlock), 0, st, n,
-                       b.count.as<int>(), blk_sum, b.cum.as<int>(),
-                       counts + kMaxSpecies + 2);
-  else
+                       b.count.as<int>(), blk_sum, b.cum.as<int>(), counts, stats_dev,
+                       seq);
+  } else {
     count_scan_stats(st, b, n, 0, nullptr, boff, counts);
-  FLARE_HIP_CHECK(hipMemcpyAsync(stats_host, counts, kNlistStats * sizeof(int),
-                                 hipMemcpyDeviceToHost, st));
-  FLARE_HIP_CHECK(hipEventRecord(stats_ev, st));
+    hipLaunchKernelGGL(publish_stats_kernel, dim3(1), dim3(64), 0, st, counts, stats_dev,
+                       seq);
+  }
   if (n > 0) {
     // every entry's reverse index is written by the partner with the
     // smaller key (the predicate is symmetric bitwise, so both exist)
