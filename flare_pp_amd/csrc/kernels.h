@@ -67,6 +67,12 @@ struct NListBuffers {
   DevBuf row_of, atom_of_row, species_of_row, species_count, species_blk,
       scan_tmp, blk_sum;
   int P = 0;
+  // counters left zeroed by the last build (bin and block counts, buffers)
+  // and the summary block the next build uses
+  bool zero_ready = false;
+  int zr_bins = -1, zr_ncb = -1;
+  const void *zr_ptr[4] = {nullptr, nullptr, nullptr, nullptr};
+  int parity = 0;
 };
 
 // neighbors.hip

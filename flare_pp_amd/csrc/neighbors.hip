@@ -42,16 +42,20 @@ __device__ inline void rowmul3(const double *v, const double *b, double *o) {
 
 // Per atom: wrapped position and bin, counted into bin_count[bin + 1]; per
 // block of kRowBlock atoms the species counts (bcount, bad code -> *bad).
+// next_flags: the two accumulated summary ints (bad code, largest
+// over-capacity count) of the other summary block, zeroed for the next build.
 constexpr int kRowBlock = 256;
 
 __global__ void __launch_bounds__(kRowBlock)
 wrap_bin_kernel(int n, const double *__restrict__ pos, WrapMats m,
                 const int *__restrict__ species, double *__restrict__ wrapped,
                 int *__restrict__ bin_of, int *__restrict__ bin_count,
-                int *__restrict__ bcount, int *__restrict__ bad) {
+                int *__restrict__ bcount, int *__restrict__ bad,
+                int *__restrict__ next_flags) {
 #pragma clang fp contract(off)
   __shared__ int cnt[kMaxSpecies];
   const int t = threadIdx.x;
+  if (blockIdx.x == 0 && t < 2) next_flags[t] = 0;
   if (t < kMaxSpecies) cnt[t] = 0;
   __syncthreads();
   const int i = blockIdx.x * blockDim.x + t;
@@ -859,9 +863,17 @@ nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
 // by binary search in the neighbor's sorted row.  The rel vector of
 // each entry is recomputed from its key by the enumeration's own test
 // (image_test on the same wrapped positions and shift: the same bits).  One
-// wavefront per atom; does nothing when *overflow is set.
+// wavefront per atom; does nothing when *overflow is set.  It also zeroes
+// the build's counters z[0..nz) (the bin counts and fill cursors and the
+// per-block pair sums, whose last readers ran before it), so that the next
+// build of the same size needs no clearing launch.
+struct ZeroSpans {
+  int *p[3];
+  int len[3];
+};
+
 __global__ void __launch_bounds__(256)
-nbr_compact_kernel(int n, WrapMats m, const int *__restrict__ overflow,
+nbr_compact_kernel(int n, WrapMats m, ZeroSpans zs, const int *__restrict__ overflow,
                    const int *__restrict__ cum,
                    const int *__restrict__ count,
                    const int64_t *__restrict__ keys_pad,
@@ -871,6 +883,10 @@ nbr_compact_kernel(int n, WrapMats m, const int *__restrict__ overflow,
                    int *__restrict__ center_out, int *__restrict__ rev_out) {
   const int lane = threadIdx.x & (kWave - 1);
   const int i = blockIdx.x * 4 + threadIdx.x / kWave;
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x, gn = gridDim.x * blockDim.x;
+#pragma unroll
+  for (int z = 0; z < 3; z++)
+    for (int q = gt; q < zs.len[z]; q += gn) zs.p[z][q] = 0;
   // some atom has more than kNbrCap neighbors: the two-pass path rebuilds
   // the arrays (the output capacity n * kNbrCap may then be too small)
   if (i >= n || *overflow) return;
@@ -1176,11 +1192,13 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
   b.atom_of_row.ensure((size_t)std::max(n, 1) * sizeof(int));
   b.species_of_row.ensure((size_t)std::max(n, 1) * sizeof(int));
   // counts[0..8): species counts, [8]: bad species code, [9]: largest
-  // over-capacity neighbor count, [10]: pair total
-  b.species_count.ensure(kNlistStats * sizeof(int));
+  // over-capacity neighbor count, [10]: pair total.  Two blocks used in
+  // turn: a build zeroes the other block's accumulated ints [8], [9].
+  b.species_count.ensure(2 * kNlistStats * sizeof(int));
   b.species_blk.ensure((size_t)2 * nblk * kMaxSpecies * sizeof(int));
   int *bcount = b.species_blk.as<int>(), *boff = bcount + nblk * kMaxSpecies;
-  int *counts = b.species_count.as<int>();
+  int *counts = b.species_count.as<int>() + b.parity * kNlistStats;
+  int *next_counts = b.species_count.as<int>() + (1 - b.parity) * kNlistStats;
   int *overflow = counts + kMaxSpecies + 1;
 
   // cum from per-block pair sums (cum_blocks_kernel) up to kCumMaxBlocks
@@ -1189,21 +1207,31 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
   const bool cum_blocks = n > 0 && ncb <= kCumMaxBlocks;
   b.blk_sum.ensure((size_t)std::max(ncb, 1) * kCumStride * sizeof(int));
   int *blk_sum = cum_blocks ? b.blk_sum.as<int>() : nullptr;
-  const int nclear = std::max(std::max(nbins + 2, kNlistStats), ncb * kCumStride);
-  // count[n] is the zero the count scan reads past the end; blk_sum the
-  // enumeration's per-block sums
-  hipLaunchKernelGGL(clear4_kernel, dim3((nclear + 255) / 256), dim3(256), 0, st,
-                     counts, kNlistStats, b.bin_count.as<int>(), nbins + 2,
-                     b.bin_fill.as<int>(), nbins + 1,
-                     cum_blocks ? blk_sum : b.count.as<int>() + n,
-                     cum_blocks ? ncb * kCumStride : 1);
+  // The counters start at zero: either the previous build's compaction left
+  // them zeroed (same buffers, at least as many bins and blocks) and the
+  // previous build's wrap zeroed this summary block's flags, or one launch
+  // clears them.  count[n] is the zero the count scan reads past the end;
+  // blk_sum the enumeration's per-block sums.
+  const bool ready = b.zero_ready && cum_blocks && n > 0 && b.zr_bins >= nbins &&
+                     b.zr_ncb >= ncb && b.zr_ptr[0] == b.bin_count.ptr &&
+                     b.zr_ptr[1] == b.bin_fill.ptr && b.zr_ptr[2] == b.blk_sum.ptr &&
+                     b.zr_ptr[3] == b.species_count.ptr;
+  if (!ready) {
+    const int nclear = std::max(std::max(nbins + 2, kNlistStats), ncb * kCumStride);
+    hipLaunchKernelGGL(clear4_kernel, dim3((nclear + 255) / 256), dim3(256), 0, st,
+                       counts, kNlistStats, b.bin_count.as<int>(), nbins + 2,
+                       b.bin_fill.as<int>(), nbins + 1,
+                       cum_blocks ? blk_sum : b.count.as<int>() + n,
+                       cum_blocks ? ncb * kCumStride : 1);
+  }
   const int tb = 256;
   const int g1 = (n + tb - 1) / tb;
   const int gw = (n + 3) / 4;
   if (n > 0)
     hipLaunchKernelGGL(wrap_bin_kernel, dim3(g1), dim3(kRowBlock), 0, st, n, d_pos,
                        m, d_species, b.wrapped.as<double>(), b.bin_of.as<int>(),
-                       b.bin_count.as<int>(), bcount, counts + kMaxSpecies);
+                       b.bin_count.as<int>(), bcount, counts + kMaxSpecies,
+                       next_counts + kMaxSpecies);
   // bin_count[1..nbins] holds the counts; bin_count has one spare zero entry.
   // The species scan of the per-block counts rides on the bin scan, so the
   // species rows are written with the bin fill.
@@ -1262,13 +1290,25 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
   if (n > 0) {
     // every entry's reverse index is written by the partner with the
     // smaller key (the predicate is symmetric bitwise, so both exist)
-    hipLaunchKernelGGL(nbr_compact_kernel, dim3(gw), dim3(256), 0, st, n, m,
+    ZeroSpans zs{{b.bin_count.as<int>(), b.bin_fill.as<int>(), blk_sum},
+                 {nbins + 2, nbins + 1, cum_blocks ? ncb * kCumStride : 0}};
+    hipLaunchKernelGGL(nbr_compact_kernel, dim3(gw), dim3(256), 0, st, n, m, zs,
                        overflow, b.cum.as<int>(), b.count.as<int>(),
                        b.keys_pad.as<int64_t>(), b.wrapped.as<double>(),
                        d_species, b.rel.as<double>(), b.idx.as<int>(),
                        b.nspec.as<int>(), b.center.as<int>(), b.rev.as<int>());
   }
   FLARE_HIP_CHECK(hipGetLastError());
+  // n > 0: the wrap kernel zeroed the other summary block's flags; with the
+  // per-block sums the compaction zeroed this build's counters
+  b.zero_ready = cum_blocks && n > 0;
+  b.zr_bins = nbins;
+  b.zr_ncb = ncb;
+  b.zr_ptr[0] = b.bin_count.ptr;
+  b.zr_ptr[1] = b.bin_fill.ptr;
+  b.zr_ptr[2] = b.blk_sum.ptr;
+  b.zr_ptr[3] = b.species_count.ptr;
+  b.parity ^= 1;
 }
 
 int nlist_resolve(hipStream_t st, NListBuffers &b, int n, const int *d_species,
@@ -1454,7 +1494,8 @@ int64_t nlist_from_external(hipStream_t st, NListBuffers &b, const ExtList &e,
   b.row_of.ensure((size_t)std::max(n, 1) * sizeof(int));
   b.atom_of_row.ensure((size_t)std::max(n, 1) * sizeof(int));
   b.species_of_row.ensure((size_t)std::max(n, 1) * sizeof(int));
-  b.species_count.ensure(kNlistStats * sizeof(int));
+  b.species_count.ensure(2 * kNlistStats * sizeof(int));
+  b.zero_ready = false;  // this path uses the counters (and bin_count as scratch)
   b.species_blk.ensure((size_t)2 * nblk * kMaxSpecies * sizeof(int));
   int *bcount = b.species_blk.as<int>(), *boff = bcount + nblk * kMaxSpecies;
   int *counts = b.species_count.as<int>();
