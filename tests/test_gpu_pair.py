@@ -230,3 +230,32 @@ def test_config5_scaleup_forms():
         # relative bounds: config 5's RMS force is ~0.2 meV/A
         assert rms_rel(f2, f0) < b_rms and nrel(f2, f0) < b_max, form
         assert abs(e2 - e0) < b_e * np.abs(ea0).sum() and nrel(v2, v0) < b_v, form
+
+
+def test_rebuild_sequence_one_list(orc):
+    """One neighbor list rebuilt for a run of different structures: larger
+    (its buffers grow), smaller (counters left zeroed by the previous
+    build's compaction, no clearing launch), a thin triclinic cell (the
+    per-atom enumeration), a dense box with more than 128 neighbors per atom
+    (the two-pass rebuild), then the first structure again.  Every build in
+    the adjoint FP64 form against the oracle (1e-10), the mixed form on the
+    last."""
+    pair = make_pair(2, 4, 3, seed=61)
+    rng = np.random.default_rng(62)
+    thin_cell = np.array([[3.1, 0.0, 0.0], [0.9, 2.7, 0.0], [0.4, -0.6, 4.2]])
+    thin_pos = rng.random((7, 3)) @ thin_cell + rng.normal(0, 0.3, (7, 3))
+    cases = [
+        ("fcc 2^3", *small_two_species(seed=63)),
+        ("fcc 5^3", *synth.fcc(5, 4.05, 0.08, [250, 250], 64)),
+        ("fcc 3^3", *synth.fcc(3, 4.05, 0.08, [54, 54], 65)),
+        ("thin triclinic", thin_cell, rng.integers(0, 2, 7), thin_pos),
+        ("dense a = 2.5", *synth.fcc(4, 2.5, 0.03, [128, 128], 66)),
+        ("fcc 2^3 again", *small_two_species(seed=63)),
+    ]
+    for name, cell, species, pos in cases:
+        e0, f0, v0, ea0 = oracle_pair(orc, pair, cell, species, pos)
+        e, f, v, ea = pair.compute(cell, species, pos, form="adjoint_fp64", want_eatom=True)
+        print(f"{name}: n={len(species)} force rel {nrel(f, f0):.2e}", flush=True)
+        assert abs(e - e0) <= 1e-10 * np.abs(ea0).sum(), name
+        assert nrel(f, f0) < 1e-10 and nrel(v, v0) < 1e-10 and nrel(ea, ea0) < 1e-10, name
+    check_low_precision(pair, cell, species, pos, e0, f0, v0, ea0)
