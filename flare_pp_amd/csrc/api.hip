@@ -1157,6 +1157,9 @@ int flare_nlist_destroy(flare_nlist *nl) {
     if (!nl) return;
     nl->ctx->use();
     (void)hipStreamSynchronize(nl->ctx->stream);
+    // a build enqueued on another stream may still store its summary into
+    // the pinned block the destructor frees
+    if (nl->build_stream != nl->ctx->stream) (void)hipStreamSynchronize(nl->build_stream);
     delete nl;
   });
 }
