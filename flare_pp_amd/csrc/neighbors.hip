@@ -1216,6 +1216,9 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
                      b.zr_ncb >= ncb && b.zr_ptr[0] == b.bin_count.ptr &&
                      b.zr_ptr[1] == b.bin_fill.ptr && b.zr_ptr[2] == b.blk_sum.ptr &&
                      b.zr_ptr[3] == b.species_count.ptr;
+  // set again once the whole build is enqueued: a build that stops part way
+  // (an error) leaves the counters in an unknown state
+  b.zero_ready = false;
   if (!ready) {
     const int nclear = std::max(std::max(nbins + 2, kNlistStats), ncb * kCumStride);
     hipLaunchKernelGGL(clear4_kernel, dim3((nclear + 255) / 256), dim3(256), 0, st,
