@@ -426,8 +426,11 @@ struct flare_nlist {
     built = false;
     ctx->use();
     wait_stats();
-    n_species_present = flare::nlist_resolve(ctx->stream, b, n, d_species, wh,
-                                             stats_host, species_count);
+    n_species_present = flare::nlist_resolve(build_stream, b, n, d_species, wh, stats_host,
+                                             stats_dev, &stats_seq, species_count);
+    // the list's last kernels (the compaction) run after the summary is
+    // published: work on another stream must not start before them
+    if (build_stream != ctx->stream) FLARE_HIP_CHECK(hipStreamSynchronize(build_stream));
     built = true;
   }
 

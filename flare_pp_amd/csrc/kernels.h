@@ -73,12 +73,17 @@ struct NListBuffers {
   int zr_bins = -1, zr_ncb = -1;
   const void *zr_ptr[4] = {nullptr, nullptr, nullptr, nullptr};
   int parity = 0;
+  // the last build grouped atoms in fixed-capacity bins (else scanned), and
+  // its summary block on the device
+  bool fcb = false;
+  int *counts_block = nullptr;
 };
 
 // neighbors.hip
 // Ints of the host-side build summary: species counts (kMaxSpecies), a
-// bad-species flag, the largest over-capacity neighbor count, the pair total.
-constexpr int kNlistStats = kMaxSpecies + 3;
+// bad-species flag, the largest over-capacity neighbor count, the pair
+// total, a fixed-capacity bin overflow flag.
+constexpr int kNlistStats = kMaxSpecies + 4;
 // The host-side summary block (pinned, coherent): the kNlistStats ints,
 // then at kNlistSeq the sequence number of the build they belong to, stored
 // last with system scope by the kernel that completes the summary.
@@ -98,12 +103,15 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
 // Waits on the host until the summary of build seq is in stats_host; an
 // error if stream st (where the build was enqueued) ends or fails first.
 void nlist_wait_stats(hipStream_t st, const volatile int *stats_host, unsigned seq);
-// After nlist_wait_stats: validates the species codes, sets b.P, completes the list
-// with the two-pass path when an atom had more neighbors than the fused
-// path holds, copies the species counts to host_counts and returns the
-// number of species present (max species + 1).
+// After nlist_wait_stats: validates the species codes, regroups the atoms
+// in the scanned bin layout and enumerates again when a fixed-capacity bin
+// overflowed (a new summary with sequence number ++*seq), sets b.P,
+// completes the list with the two-pass path when an atom had more neighbors
+// than the fused path holds, copies the species counts to host_counts and
+// returns the number of species present (max species + 1).
 int nlist_resolve(hipStream_t st, NListBuffers &b, int n, const int *d_species,
-                  const WrapHost &wh, const int *stats, int *host_counts);
+                  const WrapHost &wh, const volatile int *stats_host, int *stats_dev,
+                  unsigned *seq, int *host_counts);
 
 // A caller's neighbor list over local + ghost atoms (LAMMPS: x, type,
 // ilist, numneigh, firstneigh as a 2-D array; pair_flare.cpp:80-118,

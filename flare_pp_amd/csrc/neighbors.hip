@@ -42,8 +42,9 @@ __device__ inline void rowmul3(const double *v, const double *b, double *o) {
 
 // Per atom: wrapped position and bin, counted into bin_count[bin + 1]; per
 // block of kRowBlock atoms the species counts (bcount, bad code -> *bad).
-// next_flags: the two accumulated summary ints (bad code, largest
-// over-capacity count) of the other summary block, zeroed for the next build.
+// next_flags: the accumulated summary ints (bad code, largest over-capacity
+// count, pair total, bin overflow) of the other summary block, zeroed for the
+// next build.
 constexpr int kRowBlock = 256;
 
 __global__ void __launch_bounds__(kRowBlock)
@@ -55,7 +56,7 @@ wrap_bin_kernel(int n, const double *__restrict__ pos, WrapMats m,
 #pragma clang fp contract(off)
   __shared__ int cnt[kMaxSpecies];
   const int t = threadIdx.x;
-  if (blockIdx.x == 0 && t < 2) next_flags[t] = 0;
+  if (blockIdx.x == 0 && t < 4) next_flags[t] = 0;
   if (t < kMaxSpecies) cnt[t] = 0;
   __syncthreads();
   const int i = blockIdx.x * blockDim.x + t;
@@ -86,6 +87,73 @@ wrap_bin_kernel(int n, const double *__restrict__ pos, WrapMats m,
   if (t < kMaxSpecies) bcount[blockIdx.x * kMaxSpecies + t] = cnt[t];
 }
 
+// Fixed-capacity bins: as wrap_bin_kernel, and each atom placed into a slot
+// of its bin (bin b's slots [b kBinSlots, (b + 1) kBinSlots) in arrival
+// order; the enumeration sorts every neighbor row, so the order within a bin
+// does not matter).  bin_count[b + 1] ends as the bin's atom count; a bin
+// with more atoms than slots sets *bin_ovf, and the build then groups the
+// atoms again in the scanned layout.
+constexpr int kBinSlots = 32;
+
+__global__ void __launch_bounds__(kRowBlock)
+wrap_place_kernel(int n, const double *__restrict__ pos, WrapMats m,
+                  const int *__restrict__ species, double *__restrict__ wrapped,
+                  int *__restrict__ bin_of, int *__restrict__ bin_count,
+                  int *__restrict__ bcount, int *__restrict__ bad,
+                  int *__restrict__ next_flags, int *__restrict__ bin_atoms,
+                  double *__restrict__ bin_pos, int *__restrict__ bin_ovf) {
+#pragma clang fp contract(off)
+  __shared__ int cnt[kMaxSpecies];
+  const int t = threadIdx.x;
+  if (blockIdx.x == 0 && t < 4) next_flags[t] = 0;
+  if (t < kMaxSpecies) cnt[t] = 0;
+  __syncthreads();
+  const int i = blockIdx.x * blockDim.x + t;
+  if (i < n) {
+    double p[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    double tt[3], rel[3], rw[3], t2[3], w[3];
+    rowmul3(p, m.ct, tt);
+    rowmul3(tt, m.cdi, rel);
+    for (int c = 0; c < 3; c++) rw[c] = rel[c] - floor(rel[c]);
+    rowmul3(rw, m.cd, t2);
+    rowmul3(t2, m.cti, w);
+    for (int c = 0; c < 3; c++) wrapped[3 * i + c] = w[c];
+    int b[3];
+    for (int c = 0; c < 3; c++) {
+      int v = (int)floor(rw[c] * m.nb[c]);
+      b[c] = v < 0 ? 0 : (v >= m.nb[c] ? m.nb[c] - 1 : v);
+    }
+    const int bin = (b[0] * m.nb[1] + b[1]) * m.nb[2] + b[2];
+    bin_of[i] = bin;
+    const int slot = atomicAdd(&bin_count[bin + 1], 1);
+    if (slot < kBinSlots) {
+      const int q = bin * kBinSlots + slot;
+      bin_atoms[q] = i;
+      for (int c = 0; c < 3; c++) bin_pos[3 * q + c] = w[c];
+    } else {
+      atomicOr(bin_ovf, 1);
+    }
+    const int sp = species[i];
+    if (sp < 0 || sp >= kMaxSpecies)
+      atomicOr(bad, 1);
+    else
+      atomicAdd(&cnt[sp], 1);
+  }
+  __syncthreads();
+  if (t < kMaxSpecies) bcount[blockIdx.x * kMaxSpecies + t] = cnt[t];
+}
+
+// bin_count[b + 1] = number of atoms with bin_of == b (bin_count zero on
+// entry): the counts the scanned layout needs when fixed-capacity bins
+// overflowed.
+__global__ void __launch_bounds__(256)
+count_bins_kernel(int n, const int *__restrict__ bin_of, int *__restrict__ bin_count,
+                  int *__restrict__ clear, int nclear) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nclear) clear[i] = 0;
+  if (i < n) atomicAdd(&bin_count[bin_of[i] + 1], 1);
+}
+
 // Zero the build's counters in one launch: a[0..na), b[0..nb), c[0..nc),
 // d[0..nd).
 __global__ void clear4_kernel(int *__restrict__ a, int na, int *__restrict__ b,
@@ -96,6 +164,42 @@ __global__ void clear4_kernel(int *__restrict__ a, int na, int *__restrict__ b,
   if (q < nb) b[q] = 0;
   if (q < nc) c[q] = 0;
   if (q < nd) d[q] = 0;
+}
+
+// Species-grouped row map (b2.cpp:104-176 regroup: atoms of species 0 in
+// original order, then species 1, ...): per-block species counts (in
+// wrap_bin_kernel or ext_species_kernel), their scan (species_block_scan),
+// and per-block ranks from wavefront ballots.
+// Block body (blockDim kRowBlock, every thread reaches it).
+// blk_off: the block's kMaxSpecies offsets (atoms of each species in the
+// blocks before it), totals: the species counts of the whole structure.
+__device__ inline void species_rows_body(int n, const int *__restrict__ species,
+                                         const int *blk_off, const int *totals,
+                                         int *__restrict__ row_of,
+                                         int *__restrict__ atom_of_row,
+                                         int *__restrict__ species_of_row) {
+  __shared__ int wcnt[kRowBlock / 64][kMaxSpecies];
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int a = blockIdx.x * kRowBlock + t;
+  const int s = a < n ? species[a] : -1;
+  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
+  int my_rank = 0;
+  for (int q = 0; q < kMaxSpecies; q++) {
+    const uint64_t m = __ballot(s == q);
+    if (s == q) my_rank = __popcll(m & lt);
+    if (lane == 0) wcnt[wv][q] = __popcll(m);
+  }
+  __syncthreads();
+  // a code outside [0, kMaxSpecies) gets no row (the build reports it)
+  if (s >= 0 && s < kMaxSpecies) {
+    int base = 0;
+    for (int q = 0; q < s; q++) base += totals[q];
+    for (int w = 0; w < wv; w++) my_rank += wcnt[w][s];
+    const int row = base + blk_off[s] + my_rank;
+    row_of[a] = row;
+    atom_of_row[row] = a;
+    species_of_row[row] = s;
+  }
 }
 
 // cum[i] = sum of count[k], k < i, from the per-block pair sums
@@ -109,8 +213,10 @@ __global__ void clear4_kernel(int *__restrict__ a, int na, int *__restrict__ b,
 // Copies the finished summary counts[0..kNlistStats) into the pinned host
 // block, then its sequence number (one thread; system-scope release so the
 // host sees the counts once it sees seq).
-__device__ inline void publish_stats(const int *counts, int *host, unsigned seq) {
-  for (int k = 0; k < kNlistStats; k++) host[k] = counts[k];
+__device__ inline void publish_stats(const int *counts, int *host, unsigned seq,
+                                     const int *species_totals = nullptr) {
+  for (int k = 0; k < kNlistStats; k++)
+    host[k] = species_totals && k < kMaxSpecies ? species_totals[k] : counts[k];
   __threadfence_system();
   __hip_atomic_store(host + kNlistSeq, (int)seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
@@ -119,14 +225,48 @@ __global__ void publish_stats_kernel(const int *__restrict__ counts, int *host, 
   if (threadIdx.x == 0) publish_stats(counts, host, seq);
 }
 
+// With rows.species set (the fixed-capacity bin path, where no bin scan
+// ran), the same workgroups also form the species-grouped row map of their
+// 256 atoms: each sums the per-block species counts of the blocks before it
+// and of all blocks, and the last one's totals go into the summary.
+struct CumRows {
+  const int *species = nullptr;
+  const int *bcount = nullptr;
+  int *row_of = nullptr, *atom_of_row = nullptr, *species_of_row = nullptr;
+};
+
 constexpr int kCumBlock = 256, kCumMaxBlocks = 1024, kCumStride = 32;
+static_assert(kCumBlock == kRowBlock, "the species blocks are the CSR blocks");
 __global__ void __launch_bounds__(kCumBlock)
 cum_blocks_kernel(int n, const int *__restrict__ count, const int *__restrict__ blk_sum,
                   int *__restrict__ cum, int *__restrict__ counts, int *host,
-                  unsigned seq) {
+                  unsigned seq, CumRows rows) {
   constexpr int kW = kCumBlock / kWave;
   __shared__ int before_w[kW], total_w[kW];
+  __shared__ int sp_before[kMaxSpecies], sp_total[kMaxSpecies];
   const int b = blockIdx.x, t = threadIdx.x, lane = t & (kWave - 1), wv = t / kWave;
+  if (rows.species) {
+    for (int sp = wv; sp < kMaxSpecies; sp += kW) {
+      int bef = 0, tot = 0;
+      for (int k = lane; k < (int)gridDim.x; k += kWave) {
+        const int v = rows.bcount[k * kMaxSpecies + sp];
+        tot += v;
+        if (k < b) bef += v;
+      }
+#pragma unroll
+      for (int off = kWave / 2; off > 0; off >>= 1) {
+        bef += __shfl_xor(bef, off, kWave);
+        tot += __shfl_xor(tot, off, kWave);
+      }
+      if (lane == 0) {
+        sp_before[sp] = bef;
+        sp_total[sp] = tot;
+      }
+    }
+    __syncthreads();
+    species_rows_body(n, rows.species, sp_before, sp_total, rows.row_of,
+                      rows.atom_of_row, rows.species_of_row);
+  }
   int s = 0;
   for (int k = t; k < b; k += kCumBlock) s += blk_sum[k * kCumStride];
 #pragma unroll
@@ -150,7 +290,9 @@ cum_blocks_kernel(int n, const int *__restrict__ count, const int *__restrict__ 
   if (b == gridDim.x - 1 && t == kCumBlock - 1) {
     cum[n] = ex + c;
     counts[kMaxSpecies + 2] = ex + c;
-    publish_stats(counts, host, seq);
+    if (rows.species)
+      for (int sp = 0; sp < kMaxSpecies; sp++) counts[sp] = sp_total[sp];
+    publish_stats(counts, host, seq, rows.species ? sp_total : nullptr);
   }
 }
 
@@ -390,6 +532,25 @@ __device__ inline bool image_test(const double *wi, const double *wj,
   return (dist < cutoff) && (dist != 0);
 }
 
+// Atoms grouped by bin: either a scanned layout (bin b's atoms at
+// [start[b], start[b + 1])) or fixed-capacity bins of cap slots (bin b's at
+// [b cap, b cap + min(cnt[b], cap)); a fuller bin makes the build fall back
+// to the scanned layout).
+struct BinLayout {
+  const int *start;
+  const int *cnt;
+  int cap;
+};
+__device__ inline void bin_range(const BinLayout &bl, int b, int &st, int &cs) {
+  if (bl.cap) {
+    st = b * bl.cap;
+    cs = min(bl.cnt[b], bl.cap);
+  } else {
+    st = bl.start[b];
+    cs = bl.start[b + 1] - st;
+  }
+}
+
 }  // namespace
 
 // MODE 0: count candidates per atom.  MODE 1: write them (unsorted) into
@@ -397,8 +558,7 @@ __device__ inline bool image_test(const double *wi, const double *wj,
 template <int MODE>
 __global__ void __launch_bounds__(256)
 nbr_enum_kernel(int n, WrapMats m, const double *__restrict__ wrapped,
-                const int *__restrict__ bin_of,
-                const int *__restrict__ bin_start,
+                const int *__restrict__ bin_of, BinLayout bl,
                 const int *__restrict__ bin_atoms, int *__restrict__ count,
                 const int *__restrict__ cum, int64_t *__restrict__ keys,
                 double *__restrict__ rel) {
@@ -426,12 +586,12 @@ nbr_enum_kernel(int n, WrapMats m, const double *__restrict__ wrapped,
     int cb = 0, s1 = 0, s2 = 0, s3 = 0, csize = 0;
     if (c < n_combo) {
       int ex = c / (ly * lz), ey = (c / lz) % ly, ez = c % lz;
-      int bx, by, bz;
+      int bx, by, bz, cst;
       axis_entry(m.nb[0], hx, sw, ex, bx, s1);
       axis_entry(m.nb[1], hy, sw, ey, by, s2);
       axis_entry(m.nb[2], hz, sw, ez, bz, s3);
       cb = (bx * m.nb[1] + by) * m.nb[2] + bz;
-      csize = bin_start[cb + 1] - bin_start[cb];
+      bin_range(bl, cb, cst, csize);
     }
     // inclusive scan of combo sizes across the wave
     int incl = csize;
@@ -468,8 +628,9 @@ nbr_enum_kernel(int n, WrapMats m, const double *__restrict__ wrapped,
         axis_entry(m.nb[0], hx, sw, ex, bx, t1);
         axis_entry(m.nb[1], hy, sw, ey, by, t2);
         axis_entry(m.nb[2], hz, sw, ez, bz, t3);
-        int b = (bx * m.nb[1] + by) * m.nb[2] + bz;
-        int j = bin_atoms[bin_start[b] + (t - before)];
+        int b = (bx * m.nb[1] + by) * m.nb[2] + bz, bst, bcs;
+        bin_range(bl, b, bst, bcs);
+        int j = bin_atoms[bst + (t - before)];
         const double wj[3] = {wrapped[3 * j], wrapped[3 * j + 1],
                               wrapped[3 * j + 2]};
         pass = image_test(wi, wj, m.cell, t1, t2, t3, m.cutoff, o);
@@ -505,8 +666,7 @@ nbr_enum_kernel(int n, WrapMats m, const double *__restrict__ wrapped,
 template <bool THIN>
 __device__ inline void enum_sort_atom(int i, const WrapMats &m,
                                       const double *__restrict__ wrapped,
-                                      const int *__restrict__ bin_of,
-                                      const int *__restrict__ bin_start,
+                                      const int *__restrict__ bin_of, const BinLayout &bl,
                                       const int *__restrict__ bin_atoms,
                                       const double *__restrict__ bin_pos,
                                       int *__restrict__ count,
@@ -536,7 +696,8 @@ __device__ inline void enum_sort_atom(int i, const WrapMats &m,
       axis_entry(m.nb[1], hy, sw, ey, by, s2);
       axis_entry(m.nb[2], hz, sw, ez, bz, s3);
       const int cb = (bx * m.nb[1] + by) * m.nb[2] + bz;
-      csize = bin_start[cb + 1] - bin_start[cb];
+      int cst;
+      bin_range(bl, cb, cst, csize);
     }
     int incl = csize;
 #pragma unroll
@@ -572,7 +733,9 @@ __device__ inline void enum_sort_atom(int i, const WrapMats &m,
         axis_entry(m.nb[1], hy, sw, ey, by, t2);
         axis_entry(m.nb[2], hz, sw, ez, bz, t3);
         const int b = (bx * m.nb[1] + by) * m.nb[2] + bz;
-        const int slot = bin_start[b] + (t - before);
+        int bst, bcs;
+        bin_range(bl, b, bst, bcs);
+        const int slot = bst + (t - before);
         const int j = bin_atoms[slot];
         const double wj[3] = {bin_pos[3 * slot], bin_pos[3 * slot + 1],
                               bin_pos[3 * slot + 2]};
@@ -615,8 +778,7 @@ __device__ inline void enum_sort_atom(int i, const WrapMats &m,
 template <bool THIN>
 __global__ void __launch_bounds__(256)
 nbr_enum_sort_kernel(int n, WrapMats m, const double *__restrict__ wrapped,
-                     const int *__restrict__ bin_of,
-                     const int *__restrict__ bin_start,
+                     const int *__restrict__ bin_of, BinLayout bl,
                      const int *__restrict__ bin_atoms,
                      const double *__restrict__ bin_pos, int *__restrict__ count,
                      int64_t *__restrict__ keys_pad, int *__restrict__ stat,
@@ -626,7 +788,7 @@ nbr_enum_sort_kernel(int n, WrapMats m, const double *__restrict__ wrapped,
   const int wv = threadIdx.x / kWave;
   const int i = blockIdx.x * 4 + wv;
   if (i >= n) return;  // whole wave exits together
-  enum_sort_atom<THIN>(i, m, wrapped, bin_of, bin_start, bin_atoms, bin_pos, count,
+  enum_sort_atom<THIN>(i, m, wrapped, bin_of, bl, bin_atoms, bin_pos, count,
                        keys_pad, stat, blk_sum, prefix_sh[wv], skey_sh[wv]);
 }
 
@@ -672,8 +834,7 @@ static_assert(kBinOffWave % 16 == 0 && kBinWaveBytes % 16 == 0 && (kNbrCap * 8) 
 __global__ void __launch_bounds__(256)
 nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
                          const double *__restrict__ wrapped,
-                         const int *__restrict__ bin_of,
-                         const int *__restrict__ bin_start,
+                         const int *__restrict__ bin_of, BinLayout bl,
                          const int *__restrict__ bin_atoms,
                          const double *__restrict__ bin_pos, int *__restrict__ count,
                          int64_t *__restrict__ keys_pad, int *__restrict__ stat,
@@ -682,7 +843,8 @@ nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
   __shared__ __align__(16) unsigned char lds[kBinLds];
   const int b = blockIdx.x;
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid / kWave;
-  const int a0 = bin_start[b], na = bin_start[b + 1] - a0;
+  int a0, na;
+  bin_range(bl, b, a0, na);
   if (na == 0) return;  // the whole workgroup
   const int nb1 = m.nb[1], nb2 = m.nb[2];
   if (wv == 0) {
@@ -694,8 +856,7 @@ nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
       axis_entry(nb1, hy, m.sweep, (lane / 3) % 3, by, s2);
       axis_entry(nb2, hz, m.sweep, lane % 3, bz, s3);
       const int cb = (bx * nb1 + by) * nb2 + bz;
-      st = bin_start[cb];
-      cs = bin_start[cb + 1] - st;
+      bin_range(bl, cb, st, cs);
       sh = ((s1 + 1) << 4) | ((s2 + 1) << 2) | (s3 + 1);
     }
     int incl = cs;
@@ -717,7 +878,7 @@ nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
     int *prefix = reinterpret_cast<int *>(lds) + wv * kWave;
     int64_t *skey = reinterpret_cast<int64_t *>(lds + 4 * kWave * 4) + wv * kNbrCap;
     for (int a = wv; a < na; a += 4)
-      enum_sort_atom<false>(bin_atoms[a0 + a], m, wrapped, bin_of, bin_start, bin_atoms,
+      enum_sort_atom<false>(bin_atoms[a0 + a], m, wrapped, bin_of, bl, bin_atoms,
                             bin_pos, count, keys_pad, stat, blk_sum, prefix, skey);
     return;
   }
@@ -863,7 +1024,7 @@ nbr_bin_enum_sort_kernel(WrapMats m, float cut_f2, int key32,
 // by binary search in the neighbor's sorted row.  The rel vector of
 // each entry is recomputed from its key by the enumeration's own test
 // (image_test on the same wrapped positions and shift: the same bits).  One
-// wavefront per atom; does nothing when *overflow is set.  It also zeroes
+// wavefront per atom; does nothing when a row or a bin overflowed.  It also zeroes
 // the build's counters z[0..nz) (the bin counts and fill cursors and the
 // per-block pair sums, whose last readers ran before it), so that the next
 // build of the same size needs no clearing launch.
@@ -873,7 +1034,7 @@ struct ZeroSpans {
 };
 
 __global__ void __launch_bounds__(256)
-nbr_compact_kernel(int n, WrapMats m, ZeroSpans zs, const int *__restrict__ overflow,
+nbr_compact_kernel(int n, WrapMats m, ZeroSpans zs, const int *__restrict__ flags,
                    const int *__restrict__ cum,
                    const int *__restrict__ count,
                    const int64_t *__restrict__ keys_pad,
@@ -887,9 +1048,11 @@ nbr_compact_kernel(int n, WrapMats m, ZeroSpans zs, const int *__restrict__ over
 #pragma unroll
   for (int z = 0; z < 3; z++)
     for (int q = gt; q < zs.len[z]; q += gn) zs.p[z][q] = 0;
-  // some atom has more than kNbrCap neighbors: the two-pass path rebuilds
-  // the arrays (the output capacity n * kNbrCap may then be too small)
-  if (i >= n || *overflow) return;
+  // some atom has more than kNbrCap neighbors (flags[0]): the two-pass path
+  // rebuilds the arrays (the output capacity n * kNbrCap may then be too
+  // small); a fixed-capacity bin overflowed (flags[2]): atoms left out of
+  // the bins have no count, and the build enumerates again
+  if (i >= n || flags[0] || flags[2]) return;
   const int sweep = m.sweep;
   const int b = cum[i], cnt = cum[i + 1] - b;
   const int64_t row = (int64_t)i * kNbrCap;
@@ -984,46 +1147,13 @@ __global__ void nbr_reverse_kernel(int64_t P, int sweep,
   rev[p] = (lo < cum[j + 1] && keys[lo] == want) ? lo : -1;
 }
 
-// Species-grouped row map (b2.cpp:104-176 regroup: atoms of species 0 in
-// original order, then species 1, ...): per-block species counts (in
-// wrap_bin_kernel or ext_species_kernel), their scan (species_block_scan),
-// and per-block ranks from wavefront ballots.
-// Block body (blockDim kRowBlock, every thread reaches it).
-__device__ inline void species_rows_body(int n, const int *__restrict__ species,
-                                         const int *__restrict__ boff,
-                                         const int *__restrict__ counts,
-                                         int *__restrict__ row_of,
-                                         int *__restrict__ atom_of_row,
-                                         int *__restrict__ species_of_row) {
-  __shared__ int wcnt[kRowBlock / 64][kMaxSpecies];
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const int a = blockIdx.x * kRowBlock + t;
-  const int s = a < n ? species[a] : -1;
-  const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
-  int my_rank = 0;
-  for (int q = 0; q < kMaxSpecies; q++) {
-    const uint64_t m = __ballot(s == q);
-    if (s == q) my_rank = __popcll(m & lt);
-    if (lane == 0) wcnt[wv][q] = __popcll(m);
-  }
-  __syncthreads();
-  if (s >= 0) {
-    int base = 0;
-    for (int q = 0; q < s; q++) base += counts[q];
-    for (int w = 0; w < wv; w++) my_rank += wcnt[w][s];
-    const int row = base + boff[blockIdx.x * kMaxSpecies + s] + my_rank;
-    row_of[a] = row;
-    atom_of_row[row] = a;
-    species_of_row[row] = s;
-  }
-}
-
 __global__ void __launch_bounds__(kRowBlock)
 species_rows_kernel(int n, const int *__restrict__ species,
                     const int *__restrict__ boff, const int *__restrict__ counts,
                     int *__restrict__ row_of, int *__restrict__ atom_of_row,
                     int *__restrict__ species_of_row) {
-  species_rows_body(n, species, boff, counts, row_of, atom_of_row, species_of_row);
+  species_rows_body(n, species, boff + blockIdx.x * kMaxSpecies, counts, row_of,
+                    atom_of_row, species_of_row);
 }
 
 // bin_atoms / bin_pos: atoms and their wrapped positions grouped by bin
@@ -1045,7 +1175,8 @@ bin_fill_rows_kernel(int n, const int *__restrict__ bin_of,
     bin_atoms[slot] = i;
     for (int c = 0; c < 3; c++) bin_pos[3 * slot + c] = wrapped[3 * i + c];
   }
-  species_rows_body(n, species, boff, counts, row_of, atom_of_row, species_of_row);
+  species_rows_body(n, species, boff + blockIdx.x * kMaxSpecies, counts, row_of,
+                    atom_of_row, species_of_row);
 }
 
 // exclusive prefix sum of in[0..n) into out[0..n], out[n] = total; in[n]
@@ -1165,6 +1296,88 @@ void nlist_wait_stats(hipStream_t st, const volatile int *stats_host, unsigned s
   std::atomic_thread_fence(std::memory_order_acquire);
 }
 
+// The scanned bin layout: bin_start from the counts in bin_count[1..nbins],
+// the species scan of the per-block counts (boff, counts[s]), atoms and
+// positions grouped by bin, and the species-grouped rows.
+static void scan_fill_bins(hipStream_t st, NListBuffers &b, int n, int nbins, int nblk,
+                           const int *bcount, int *boff, int *counts, const int *d_species) {
+  const int nblk_used = n > 0 ? nblk : 0;
+  if (nbins <= kScanMaxC * kScanChunk) {
+    const int chunks = (nbins + kScanChunk - 1) / kScanChunk;
+#define FLARE_SCAN(C)                                                                    \
+  hipLaunchKernelGGL(bin_scan_species_kernel<C>, dim3(1), dim3(kScanThreads), 0, st,      \
+                     nbins, b.bin_count.as<int>() + 1, b.bin_start.as<int>(), nblk_used, \
+                     bcount, boff, counts)
+    if (chunks <= 1)
+      FLARE_SCAN(1);
+    else if (chunks <= 2)
+      FLARE_SCAN(2);
+    else if (chunks <= 4)
+      FLARE_SCAN(4);
+    else
+      FLARE_SCAN(8);
+#undef FLARE_SCAN
+  } else {
+    exclusive_scan(st, b.scan_tmp, b.bin_count.as<int>() + 1, b.bin_start.as<int>(), nbins);
+    hipLaunchKernelGGL(species_scan_kernel, dim3(1), dim3(64 * kMaxSpecies), 0, st,
+                       nblk_used, bcount, boff, counts);
+  }
+  if (n > 0)
+    hipLaunchKernelGGL(bin_fill_rows_kernel, dim3((n + kRowBlock - 1) / kRowBlock),
+                       dim3(kRowBlock), 0, st, n, b.bin_of.as<int>(), b.bin_start.as<int>(),
+                       b.wrapped.as<double>(), b.bin_fill.as<int>(), b.bin_atoms.as<int>(),
+                       b.bin_pos.as<double>(), d_species, boff, counts, b.row_of.as<int>(),
+                       b.atom_of_row.as<int>(), b.species_of_row.as<int>());
+}
+
+// Per-atom neighbor rows (sorted keys in padded rows of kNbrCap, counts and
+// the per-block pair sums).
+static void enumerate_rows(hipStream_t st, NListBuffers &b, int n, const WrapHost &wh,
+                           const WrapMats &m, const BinLayout &bl, int *overflow,
+                           int *blk_sum) {
+  const int nbins = wh.nb[0] * wh.nb[1] * wh.nb[2];
+  const bool thin = wh.nb[0] < 3 || wh.nb[1] < 3 || wh.nb[2] < 3;
+  if (thin)
+    hipLaunchKernelGGL(nbr_enum_sort_kernel<true>, dim3((n + 3) / 4), dim3(256), 0, st, n,
+                       m, b.wrapped.as<double>(), b.bin_of.as<int>(), bl,
+                       b.bin_atoms.as<int>(), b.bin_pos.as<double>(), b.count.as<int>(),
+                       b.keys_pad.as<int64_t>(), overflow, blk_sum);
+  else
+    hipLaunchKernelGGL(nbr_bin_enum_sort_kernel, dim3(nbins), dim3(256), 0, st, m,
+                       prefilter_cut2(wh), n <= (1 << 26) ? 1 : 0, b.wrapped.as<double>(),
+                       b.bin_of.as<int>(), bl, b.bin_atoms.as<int>(),
+                       b.bin_pos.as<double>(), b.count.as<int>(), b.keys_pad.as<int64_t>(),
+                       overflow, blk_sum);
+}
+
+// CSR offsets, the pair total and the summary (with the species rows when
+// rows.species is set), then the compaction.
+static void cum_compact(hipStream_t st, NListBuffers &b, int n, int nbins, const WrapMats &m,
+                        int *blk_sum, int *boff, int *counts, int *stats_dev, unsigned seq,
+                        const CumRows &rows, const int *d_species) {
+  const int ncb = (n + kCumBlock - 1) / kCumBlock;
+  if (blk_sum) {
+    hipLaunchKernelGGL(cum_blocks_kernel, dim3(ncb), dim3(kCumBlock), 0, st, n,
+                       b.count.as<int>(), blk_sum, b.cum.as<int>(), counts, stats_dev, seq,
+                       rows);
+  } else {
+    count_scan_stats(st, b, n, 0, nullptr, boff, counts);
+    hipLaunchKernelGGL(publish_stats_kernel, dim3(1), dim3(64), 0, st, counts, stats_dev,
+                       seq);
+  }
+  if (n > 0) {
+    // every entry's reverse index is written by the partner with the
+    // smaller key (the predicate is symmetric bitwise, so both exist)
+    ZeroSpans zs{{b.bin_count.as<int>(), b.bin_fill.as<int>(), blk_sum},
+                 {nbins + 2, nbins + 1, blk_sum ? ncb * kCumStride : 0}};
+    hipLaunchKernelGGL(nbr_compact_kernel, dim3((n + 3) / 4), dim3(256), 0, st, n, m, zs,
+                       counts + kMaxSpecies + 1, b.cum.as<int>(), b.count.as<int>(),
+                       b.keys_pad.as<int64_t>(), b.wrapped.as<double>(), d_species,
+                       b.rel.as<double>(), b.idx.as<int>(), b.nspec.as<int>(),
+                       b.center.as<int>(), b.rev.as<int>());
+  }
+}
+
 void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_pos,
                        const int *d_species, const WrapHost &wh, int *stats_dev,
                        unsigned seq) {
@@ -1172,14 +1385,25 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
   const int nbins = wh.nb[0] * wh.nb[1] * wh.nb[2];
   const int nblk = std::max(1, (n + kRowBlock - 1) / kRowBlock);
   const size_t cap = (size_t)std::max(n, 1) * kNbrCap;
+  // cum from per-block pair sums (cum_blocks_kernel) up to kCumMaxBlocks
+  // blocks, else the single-workgroup / device-wide count scan
+  const int ncb = (n + kCumBlock - 1) / kCumBlock;
+  const bool cum_blocks = n > 0 && ncb <= kCumMaxBlocks;
+  // Fixed-capacity bins (placed by the wrap kernel, no bin scan or fill
+  // launch, rows formed by cum_blocks_kernel) where at least three bins lie
+  // along every axis and the slots stay within a few per atom.
+  const bool thin = wh.nb[0] < 3 || wh.nb[1] < 3 || wh.nb[2] < 3;
+  const bool fcb = cum_blocks && !thin && (int64_t)nbins * kBinSlots <= 8LL * n + 4096;
+  const size_t slots = std::max<size_t>(std::max(n, 1),
+                                        fcb ? (size_t)nbins * kBinSlots : (size_t)0);
 
   b.wrapped.ensure((size_t)std::max(n, 1) * 3 * sizeof(double));
   b.bin_of.ensure((size_t)std::max(n, 1) * sizeof(int));
   b.bin_count.ensure((size_t)(nbins + 2) * sizeof(int));
   b.bin_start.ensure((size_t)(nbins + 1) * sizeof(int));
   b.bin_fill.ensure((size_t)(nbins + 1) * sizeof(int));
-  b.bin_atoms.ensure((size_t)std::max(n, 1) * sizeof(int));
-  b.bin_pos.ensure((size_t)std::max(n, 1) * 3 * sizeof(double));
+  b.bin_atoms.ensure(slots * sizeof(int));
+  b.bin_pos.ensure(slots * 3 * sizeof(double));
   b.count.ensure((size_t)(n + 1) * sizeof(int));
   b.cum.ensure((size_t)(n + 1) * sizeof(int));
   b.keys_pad.ensure(cap * sizeof(int64_t));
@@ -1192,19 +1416,15 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
   b.atom_of_row.ensure((size_t)std::max(n, 1) * sizeof(int));
   b.species_of_row.ensure((size_t)std::max(n, 1) * sizeof(int));
   // counts[0..8): species counts, [8]: bad species code, [9]: largest
-  // over-capacity neighbor count, [10]: pair total.  Two blocks used in
-  // turn: a build zeroes the other block's accumulated ints [8], [9].
+  // over-capacity neighbor count, [10]: pair total, [11]: a fixed-capacity
+  // bin overflowed.  Two blocks used in turn: a build zeroes the other
+  // block's accumulated ints [8..11].
   b.species_count.ensure(2 * kNlistStats * sizeof(int));
   b.species_blk.ensure((size_t)2 * nblk * kMaxSpecies * sizeof(int));
   int *bcount = b.species_blk.as<int>(), *boff = bcount + nblk * kMaxSpecies;
   int *counts = b.species_count.as<int>() + b.parity * kNlistStats;
   int *next_counts = b.species_count.as<int>() + (1 - b.parity) * kNlistStats;
   int *overflow = counts + kMaxSpecies + 1;
-
-  // cum from per-block pair sums (cum_blocks_kernel) up to kCumMaxBlocks
-  // blocks, else the single-workgroup / device-wide count scan
-  const int ncb = (n + kCumBlock - 1) / kCumBlock;
-  const bool cum_blocks = n > 0 && ncb <= kCumMaxBlocks;
   b.blk_sum.ensure((size_t)std::max(ncb, 1) * kCumStride * sizeof(int));
   int *blk_sum = cum_blocks ? b.blk_sum.as<int>() : nullptr;
   // The counters start at zero: either the previous build's compaction left
@@ -1227,80 +1447,34 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
                        cum_blocks ? blk_sum : b.count.as<int>() + n,
                        cum_blocks ? ncb * kCumStride : 1);
   }
-  const int tb = 256;
-  const int g1 = (n + tb - 1) / tb;
-  const int gw = (n + 3) / 4;
-  if (n > 0)
-    hipLaunchKernelGGL(wrap_bin_kernel, dim3(g1), dim3(kRowBlock), 0, st, n, d_pos,
-                       m, d_species, b.wrapped.as<double>(), b.bin_of.as<int>(),
+  const int g1 = (n + kRowBlock - 1) / kRowBlock;
+  BinLayout bl{b.bin_start.as<int>(), nullptr, 0};
+  CumRows rows;
+  if (fcb) {
+    hipLaunchKernelGGL(wrap_place_kernel, dim3(g1), dim3(kRowBlock), 0, st, n, d_pos, m,
+                       d_species, b.wrapped.as<double>(), b.bin_of.as<int>(),
                        b.bin_count.as<int>(), bcount, counts + kMaxSpecies,
-                       next_counts + kMaxSpecies);
-  // bin_count[1..nbins] holds the counts; bin_count has one spare zero entry.
-  // The species scan of the per-block counts rides on the bin scan, so the
-  // species rows are written with the bin fill.
-  const int nblk_used = n > 0 ? nblk : 0;
-  if (nbins <= kScanMaxC * kScanChunk) {
-    const int chunks = (nbins + kScanChunk - 1) / kScanChunk;
-#define FLARE_SCAN(C)                                                                    \
-  hipLaunchKernelGGL(bin_scan_species_kernel<C>, dim3(1), dim3(kScanThreads), 0, st,      \
-                     nbins, b.bin_count.as<int>() + 1, b.bin_start.as<int>(), nblk_used, \
-                     bcount, boff, counts)
-    if (chunks <= 1)
-      FLARE_SCAN(1);
-    else if (chunks <= 2)
-      FLARE_SCAN(2);
-    else if (chunks <= 4)
-      FLARE_SCAN(4);
-    else
-      FLARE_SCAN(8);
-#undef FLARE_SCAN
+                       next_counts + kMaxSpecies, b.bin_atoms.as<int>(),
+                       b.bin_pos.as<double>(), counts + kMaxSpecies + 3);
+    bl = BinLayout{nullptr, b.bin_count.as<int>() + 1, kBinSlots};
+    rows.species = d_species;
+    rows.bcount = bcount;
+    rows.row_of = b.row_of.as<int>();
+    rows.atom_of_row = b.atom_of_row.as<int>();
+    rows.species_of_row = b.species_of_row.as<int>();
   } else {
-    exclusive_scan(st, b.scan_tmp, b.bin_count.as<int>() + 1, b.bin_start.as<int>(), nbins);
-    hipLaunchKernelGGL(species_scan_kernel, dim3(1), dim3(64 * kMaxSpecies), 0, st,
-                       nblk_used, bcount, boff, counts);
+    if (n > 0)
+      hipLaunchKernelGGL(wrap_bin_kernel, dim3(g1), dim3(kRowBlock), 0, st, n, d_pos, m,
+                         d_species, b.wrapped.as<double>(), b.bin_of.as<int>(),
+                         b.bin_count.as<int>(), bcount, counts + kMaxSpecies,
+                         next_counts + kMaxSpecies);
+    // bin_count[1..nbins] holds the counts; bin_count has one spare zero
+    // entry.  The species scan of the per-block counts rides on the bin
+    // scan, so the species rows are written with the bin fill.
+    scan_fill_bins(st, b, n, nbins, nblk, bcount, boff, counts, d_species);
   }
-  if (n > 0) {
-    hipLaunchKernelGGL(bin_fill_rows_kernel, dim3(g1), dim3(kRowBlock), 0, st, n,
-                       b.bin_of.as<int>(), b.bin_start.as<int>(),
-                       b.wrapped.as<double>(), b.bin_fill.as<int>(),
-                       b.bin_atoms.as<int>(), b.bin_pos.as<double>(), d_species, boff,
-                       counts, b.row_of.as<int>(), b.atom_of_row.as<int>(),
-                       b.species_of_row.as<int>());
-    const bool thin = wh.nb[0] < 3 || wh.nb[1] < 3 || wh.nb[2] < 3;
-    if (thin)
-      hipLaunchKernelGGL(nbr_enum_sort_kernel<true>, dim3(gw), dim3(256), 0, st,
-                         n, m, b.wrapped.as<double>(), b.bin_of.as<int>(),
-                         b.bin_start.as<int>(), b.bin_atoms.as<int>(),
-                         b.bin_pos.as<double>(), b.count.as<int>(),
-                         b.keys_pad.as<int64_t>(), overflow, blk_sum);
-    else
-      hipLaunchKernelGGL(nbr_bin_enum_sort_kernel, dim3(nbins), dim3(256), 0, st, m,
-                         prefilter_cut2(wh), n <= (1 << 26) ? 1 : 0,
-                         b.wrapped.as<double>(), b.bin_of.as<int>(),
-                         b.bin_start.as<int>(), b.bin_atoms.as<int>(),
-                         b.bin_pos.as<double>(), b.count.as<int>(),
-                         b.keys_pad.as<int64_t>(), overflow, blk_sum);
-  }
-  if (cum_blocks) {
-    hipLaunchKernelGGL(cum_blocks_kernel, dim3(ncb), dim3(kCumBlock), 0, st, n,
-                       b.count.as<int>(), blk_sum, b.cum.as<int>(), counts, stats_dev,
-                       seq);
-  } else {
-    count_scan_stats(st, b, n, 0, nullptr, boff, counts);
-    hipLaunchKernelGGL(publish_stats_kernel, dim3(1), dim3(64), 0, st, counts, stats_dev,
-                       seq);
-  }
-  if (n > 0) {
-    // every entry's reverse index is written by the partner with the
-    // smaller key (the predicate is symmetric bitwise, so both exist)
-    ZeroSpans zs{{b.bin_count.as<int>(), b.bin_fill.as<int>(), blk_sum},
-                 {nbins + 2, nbins + 1, cum_blocks ? ncb * kCumStride : 0}};
-    hipLaunchKernelGGL(nbr_compact_kernel, dim3(gw), dim3(256), 0, st, n, m, zs,
-                       overflow, b.cum.as<int>(), b.count.as<int>(),
-                       b.keys_pad.as<int64_t>(), b.wrapped.as<double>(),
-                       d_species, b.rel.as<double>(), b.idx.as<int>(),
-                       b.nspec.as<int>(), b.center.as<int>(), b.rev.as<int>());
-  }
+  if (n > 0) enumerate_rows(st, b, n, wh, m, bl, overflow, blk_sum);
+  cum_compact(st, b, n, nbins, m, blk_sum, boff, counts, stats_dev, seq, rows, d_species);
   FLARE_HIP_CHECK(hipGetLastError());
   // n > 0: the wrap kernel zeroed the other summary block's flags; with the
   // per-block sums the compaction zeroed this build's counters
@@ -1311,13 +1485,47 @@ void nlist_build_async(hipStream_t st, NListBuffers &b, int n, const double *d_p
   b.zr_ptr[1] = b.bin_fill.ptr;
   b.zr_ptr[2] = b.blk_sum.ptr;
   b.zr_ptr[3] = b.species_count.ptr;
+  b.fcb = fcb;
+  b.counts_block = counts;
   b.parity ^= 1;
 }
 
 int nlist_resolve(hipStream_t st, NListBuffers &b, int n, const int *d_species,
-                  const WrapHost &wh, const int *stats, int *host_counts) {
+                  const WrapHost &wh, const volatile int *stats_host, int *stats_dev,
+                  unsigned *seq, int *host_counts) {
+  int stats[kNlistStats];
+  for (int k = 0; k < kNlistStats; k++) stats[k] = stats_host[k];
   FLARE_REQUIRE(stats[kMaxSpecies] == 0,
                 "species codes must lie in [0, " + std::to_string(kMaxSpecies) + ")");
+  if (n > 0 && stats[kMaxSpecies + 3]) {
+    // A fixed-capacity bin held more atoms than its slots: group the atoms
+    // again in the scanned layout, from the build's wrapped positions and
+    // bins (the caller's positions are not read again), and enumerate again.
+    // The compaction left the bin counts, fill cursors and block sums zeroed.
+    const WrapMats m = wrap_mats(wh);
+    const int nbins = wh.nb[0] * wh.nb[1] * wh.nb[2];
+    const int nblk = std::max(1, (n + kRowBlock - 1) / kRowBlock);
+    int *counts = b.counts_block;
+    int *bcount = b.species_blk.as<int>(), *boff = bcount + nblk * kMaxSpecies;
+    int *blk_sum = b.blk_sum.as<int>();
+    // bin counts again; the summary's overflow, pair-total and bin-overflow
+    // ints cleared
+    hipLaunchKernelGGL(count_bins_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n,
+                       b.bin_of.as<int>(), b.bin_count.as<int>(), counts + kMaxSpecies + 1,
+                       3);
+    scan_fill_bins(st, b, n, nbins, nblk, bcount, boff, counts, d_species);
+    enumerate_rows(st, b, n, wh, m, BinLayout{b.bin_start.as<int>(), nullptr, 0},
+                   counts + kMaxSpecies + 1, blk_sum);
+    *seq += 1;
+    if (*seq == 0) *seq = 1;
+    cum_compact(st, b, n, nbins, m, blk_sum, boff, counts, stats_dev, *seq, CumRows(),
+                d_species);
+    FLARE_HIP_CHECK(hipGetLastError());
+    b.fcb = false;
+    nlist_wait_stats(st, stats_host, *seq);
+    for (int k = 0; k < kNlistStats; k++) stats[k] = stats_host[k];
+    FLARE_REQUIRE(stats[kMaxSpecies + 3] == 0, "neighbor build: bin regrouping failed");
+  }
   const int P = stats[kMaxSpecies + 2];
   b.P = P;
   if (n > 0 && stats[kMaxSpecies + 1] > kNbrCap) {
@@ -1333,11 +1541,12 @@ int nlist_resolve(hipStream_t st, NListBuffers &b, int n, const int *d_species,
     b.keys_tmp.ensure(Pa * sizeof(int64_t));
     b.rel_tmp.ensure(Pa * 4 * sizeof(double));
     const int gw = (n + 3) / 4;
+    const BinLayout bl = b.fcb ? BinLayout{nullptr, b.bin_count.as<int>() + 1, kBinSlots}
+                               : BinLayout{b.bin_start.as<int>(), nullptr, 0};
     hipLaunchKernelGGL(nbr_enum_kernel<1>, dim3(gw), dim3(256), 0, st, n, m,
-                       b.wrapped.as<double>(), b.bin_of.as<int>(),
-                       b.bin_start.as<int>(), b.bin_atoms.as<int>(),
-                       (int *)nullptr, b.cum.as<int>(), b.keys_tmp.as<int64_t>(),
-                       b.rel_tmp.as<double>());
+                       b.wrapped.as<double>(), b.bin_of.as<int>(), bl,
+                       b.bin_atoms.as<int>(), (int *)nullptr, b.cum.as<int>(),
+                       b.keys_tmp.as<int64_t>(), b.rel_tmp.as<double>());
     hipLaunchKernelGGL(nbr_sort_kernel, dim3(gw), dim3(256), 0, st, n,
                        wh.sweep, b.cum.as<int>(), b.keys_tmp.as<int64_t>(),
                        b.rel_tmp.as<double>(), d_species, b.keys.as<int64_t>(),

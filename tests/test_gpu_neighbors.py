@@ -173,3 +173,28 @@ def test_three_bins_per_axis(orc, shape):
     cell = np.diag([3 * rc + 0.4 + b for b in (0.0, 0.1, 0.2)]) * np.array(shape) / 3.0
     cell, species, pos = random_box(40 + shape[1], 0.04, None, cell)
     compare(orc, cell, species, pos, rc)
+
+
+def test_crowded_bin_regroups(orc):
+    """An fcc box (4 bins per axis) with 40 extra atoms packed into one
+    bin: more atoms than a fixed-capacity bin holds (32), so the build
+    groups the atoms again in the scanned layout and enumerates again.
+    Bitwise against the oracle."""
+    cell, species, pos = synth.fcc(5, 4.05, 0.05, [250, 250], 71)
+    rng = np.random.default_rng(72)
+    extra = []
+    while len(extra) < 40:
+        p = rng.uniform(0.6, 4.4, 3)
+        if all(np.linalg.norm(p - q) > 0.9 for q in extra):
+            extra.append(p)
+    pos = np.vstack([pos, np.array(extra)])
+    species = np.concatenate([species, rng.integers(0, 2, 40)])
+    frac = np.linalg.solve(cell.T, pos.T).T % 1.0
+    bins = np.floor(frac * 4).astype(int)
+    occupancy = np.bincount((bins[:, 0] * 4 + bins[:, 1]) * 4 + bins[:, 2], minlength=64)
+    assert occupancy.max() > 32
+    s, o = compare(orc, cell, species, pos, 5.0)
+    from flare_pp_amd.structure import d2h
+    rev = d2h(s._view.d_reverse_pair, s.n_neighbors, np.int32)
+    assert rev.min() >= 0
+    np.testing.assert_array_equal(o.rel[rev, 1:], -o.rel[:, 1:])
