@@ -100,7 +100,9 @@ const char *flare_last_error(void);
 /* ---- context: device, stream and BLAS handle ------------------------------*/
 int flare_ctx_create(int device, flare_ctx **out);
 int flare_ctx_destroy(flare_ctx *ctx);
-/* hipStream_t to enqueue on (NULL: the null stream) */
+/* hipStream_t to enqueue on (NULL: the null stream).  A neighbor list built
+ * on one stream and first used after the stream was changed is waited for
+ * (host synchronisation of the build's stream) before its use. */
 int flare_ctx_set_stream(flare_ctx *ctx, void *stream);
 int flare_ctx_synchronize(flare_ctx *ctx);
 /* blocking = 0: compute entry points return once their work is enqueued
@@ -129,7 +131,11 @@ int flare_ctx_profile_get(flare_ctx *ctx, int index, char *name, int name_len,
  * neighbor species and relative positions, in the reference's order. */
 int flare_nlist_create(flare_ctx *ctx, flare_nlist **out);
 int flare_nlist_destroy(flare_nlist *nl);
-/* cell: host 3x3, rows are lattice vectors. d_species in [0, 8). */
+/* cell: host 3x3, rows are lattice vectors. d_species in [0, 8).  With
+ * blocking off the build is only enqueued; its summary (species counts,
+ * pair total) is read on the list's first use, and d_positions / d_species
+ * are read by the build's kernels only (a regrouping of over-full bins
+ * works from the list's own wrapped positions). */
 int flare_nlist_build(flare_ctx *ctx, flare_nlist *nl, int n_atoms,
                       const double *cell, const double *d_positions,
                       const int *d_species, double cutoff);
