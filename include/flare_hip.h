@@ -133,9 +133,10 @@ int flare_nlist_create(flare_ctx *ctx, flare_nlist **out);
 int flare_nlist_destroy(flare_nlist *nl);
 /* cell: host 3x3, rows are lattice vectors. d_species in [0, 8).  With
  * blocking off the build is only enqueued; its summary (species counts,
- * pair total) is read on the list's first use, and d_positions / d_species
- * are read by the build's kernels only (a regrouping of over-full bins
- * works from the list's own wrapped positions). */
+ * pair total) is read on the list's first use.  d_positions is read by the
+ * build's own kernels only (a regrouping of over-full bins works from the
+ * list's wrapped positions); d_species must stay valid until that first use
+ * (the regrouping and the rebuild of rows over 128 neighbors read it). */
 int flare_nlist_build(flare_ctx *ctx, flare_nlist *nl, int n_atoms,
                       const double *cell, const double *d_positions,
                       const int *d_species, double cutoff);
