@@ -21,6 +21,8 @@
 //   A[i][k]: lane = i + 16 (k / 8), element k % 8
 //   B[k][j]: lane = j + 16 (k / 8), element k % 8
 //   D[i][j]: lane = j + 16 (i / 4), element i % 4
+#include <type_traits>
+
 #include "common.h"
 #include "flare_math.h"
 #include "mfma.h"
@@ -719,10 +721,15 @@ void gemm_f16x3_batched(hipStream_t st, int nb, const GemmProduct *prods,
 // 32,000-atom workload is 252 workgroups: a single round on 256 CUs, where
 // 128 x 256 tiles at two per CU need 1.5 rounds.
 //
-// Operands move global -> LDS without registers (global_load_lds_dwordx4:
-// lane l of an instruction fills bytes [16 l, 16 l + 16) of a 1 KB LDS run),
-// into two stages of 76 KB: the loads of step s + 1 are issued between the
-// MFMAs of step s's first columns, then one wait and one barrier end the step.  A stage
+// Operands move global -> LDS without registers (buffer_load_dwordx4 ... lds:
+// lane l of an instruction fills bytes [16 l, 16 l + 16) of a 1 KB LDS run;
+// a buffer resource per operand block with the K step as the scalar offset
+// keeps one 32-bit vector offset per run instead of a 64-bit address), into
+// two stages of 76 KB: the loads of step s + 1 are issued between the MFMAs
+// of step s's first columns; one wait and one barrier follow the first MFMA
+// pass of the step's last column, after which the next step's A-lo fragments
+// and first B column are read while the last column's other two passes run
+// (the A-hi fragments, still in use there, are read at the top of the step).  A stage
 // holds Ah, Al (256 rows each), Bh, Bl (352 rows), 64-byte rows whose four
 // 16-byte chunks are stored at chunk c ^ ((r >> 2) & 3), as swz() above.
 // Rows past M or N load a clamped in-range row: they only reach output
@@ -768,34 +775,23 @@ __device__ inline WideOffsets wide_offsets(int M, int N, int Kp, int m0, int n0,
   return o;
 }
 
-__device__ inline void lds_dma16(const unsigned char *src, unsigned char *dst) {
-  __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1))) *)src,
-                                   (void __attribute__((address_space(3))) *)dst, 16, 0,
-                                   0);
+// 16 bytes per lane from a buffer (byte offset voff + soff) into the LDS
+// run at dst (wave-uniform).
+__device__ inline void lds_buf16(__amdgpu_buffer_rsrc_t r, uint32_t voff, int soff,
+                                 unsigned char *dst) {
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(
+      r, (__attribute__((address_space(3))) void *)dst, 16, voff, soff, 0, 0);
 }
 
-// One stage (K columns k0 .. k0 + 31) into LDS.  Block bases are uniform
-// byte pointers already advanced to k0.
-__device__ inline void wide_stage_load(unsigned char *stage, const WideOffsets &o,
-                                       const unsigned char *Ah, const unsigned char *Al,
-                                       const unsigned char *Bh, const unsigned char *Bl,
-                                       int wv) {
-  unsigned char *sAh = stage, *sAl = stage + kWM * kWRowBytes;
-  unsigned char *sBh = stage + 2 * kWM * kWRowBytes, *sBl = sBh + kWN * kWRowBytes;
-#pragma unroll
-  for (int t = 0; t < kWPerA; t++) {
-    const int run = wv + kWaves * t;
-    lds_dma16(Ah + o.a[t], sAh + run * 1024);
-    lds_dma16(Al + o.a[t], sAl + run * 1024);
-  }
-#pragma unroll
-  for (int t = 0; t < kWPerB; t++) {
-    const int run = wv + kWaves * t;
-    if (run < kWRunsB) {
-      lds_dma16(Bh + o.b[t], sBh + run * 1024);
-      lds_dma16(Bl + o.b[t], sBl + run * 1024);
-    }
-  }
+// Raw buffer resources of the four operand blocks of one product.
+struct WideRsrc {
+  __amdgpu_buffer_rsrc_t ah, al, bh, bl;
+};
+
+__device__ inline __amdgpu_buffer_rsrc_t wide_rsrc(const void *base, int64_t bytes) {
+  // word 3 0x00020000: the raw-buffer format of the gfx9 family
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0,
+                                           (int)(uint32_t)bytes, 0x00020000);
 }
 
 // Piece j of the next stage's loads, issued between the MFMAs of column j:
@@ -805,9 +801,7 @@ __device__ inline void wide_stage_load(unsigned char *stage, const WideOffsets &
 // columns behind twenty queued loads per SIMD).
 constexpr int kWPiecesPerCol = 2;
 __device__ inline void wide_stage_piece(int j, unsigned char *stage, const WideOffsets &o,
-                                        const unsigned char *Ah, const unsigned char *Al,
-                                        const unsigned char *Bh, const unsigned char *Bl,
-                                        int wv) {
+                                        const WideRsrc &r, int kb, int wv) {
   unsigned char *sAh = stage, *sAl = stage + kWM * kWRowBytes;
   unsigned char *sBh = stage + 2 * kWM * kWRowBytes, *sBl = sBh + kWN * kWRowBytes;
 #pragma unroll
@@ -816,16 +810,16 @@ __device__ inline void wide_stage_piece(int j, unsigned char *stage, const WideO
     if (q < 2 * kWPerA) {
       const int t = q >> 1, run = wv + kWaves * t;
       if (q & 1)
-        lds_dma16(Al + o.a[t], sAl + run * 1024);
+        lds_buf16(r.al, o.a[t], kb, sAl + run * 1024);
       else
-        lds_dma16(Ah + o.a[t], sAh + run * 1024);
+        lds_buf16(r.ah, o.a[t], kb, sAh + run * 1024);
     } else if (q < 2 * kWPerA + 2 * kWPerB) {
       const int t = (q - 2 * kWPerA) >> 1, run = wv + kWaves * t;
       if (run < kWRunsB) {
         if (q & 1)
-          lds_dma16(Bl + o.b[t], sBl + run * 1024);
+          lds_buf16(r.bl, o.b[t], kb, sBl + run * 1024);
         else
-          lds_dma16(Bh + o.b[t], sBh + run * 1024);
+          lds_buf16(r.bh, o.b[t], kb, sBh + run * 1024);
       }
     }
   }
@@ -876,41 +870,64 @@ gemm_x3w_kernel(GemmBatch bt, const E *__restrict__ Ah0, const E *__restrict__ A
   // multiple of 4)
 
   const WideOffsets lo = wide_offsets(M, N, Kp, m0, n0, wv, lane);
-  const unsigned char *gAh = reinterpret_cast<const unsigned char *>(Ah);
-  const unsigned char *gAl = reinterpret_cast<const unsigned char *>(Al);
-  const unsigned char *gBh = reinterpret_cast<const unsigned char *>(Bh);
-  const unsigned char *gBl = reinterpret_cast<const unsigned char *>(Bl);
-  wide_stage_load(smem, lo, gAh, gAl, gBh, gBl, wv);
+  WideRsrc rs;
+  rs.ah = wide_rsrc(Ah, (int64_t)M * Kp * 2);
+  rs.al = wide_rsrc(Al, (int64_t)M * Kp * 2);
+  rs.bh = wide_rsrc(Bh, (int64_t)N * Kp * 2);
+  rs.bl = wide_rsrc(Bl, (int64_t)N * Kp * 2);
+#pragma unroll
+  for (int j = 0; j < kFragN; j++) wide_stage_piece(j, smem, lo, rs, 0, wv);
   __builtin_amdgcn_s_waitcnt(0);
   __syncthreads();
-  for (int s = 0; s < steps; s++) {
-    unsigned char *cur = smem + (s & 1) * kWStage;
+  V8 ah[kFragM], al[kFragM], bh, bl;
+  {
+    const unsigned char *sAl = smem + kWM * kWRowBytes, *sBh = smem + 2 * kWM * kWRowBytes;
+    const unsigned char *sBl = sBh + kWN * kWRowBytes;
+#pragma unroll
+    for (int i = 0; i < kFragM; i++)
+      al[i] = *reinterpret_cast<const V8 *>(sAl + offA + i * 16 * kWRowBytes);
+    bh = *reinterpret_cast<const V8 *>(sBh + offB);
+    bl = *reinterpret_cast<const V8 *>(sBl + offB);
+  }
+  // step s; `more`: a step s + 1 follows (a compile-time flag, so that the
+  // loop body has no branches on it)
+  auto step = [&](int s, auto more_c) {
+    constexpr bool more = decltype(more_c)::value;
+    const unsigned char *cur = smem + (s & 1) * kWStage;
     unsigned char *nst = smem + ((s + 1) & 1) * kWStage;
-    const bool more = s + 1 < steps;
-    const int kb = (more ? s + 1 : s) * kWK * 2;  // bytes
-    const unsigned char *sAh = cur, *sAl = cur + kWM * kWRowBytes;
+    const int kb = (s + 1) * kWK * 2;  // bytes
     const unsigned char *sBh = cur + 2 * kWM * kWRowBytes;
     const unsigned char *sBl = sBh + kWN * kWRowBytes;
-    V8 ah[kFragM], al[kFragM];
 #pragma unroll
-    for (int i = 0; i < kFragM; i++) {
-      ah[i] = *reinterpret_cast<const V8 *>(sAh + offA + i * 16 * kWRowBytes);
-      al[i] = *reinterpret_cast<const V8 *>(sAl + offA + i * 16 * kWRowBytes);
-    }
-    // column j's B fragments are read while column j - 1's MFMAs run
-    V8 bh = *reinterpret_cast<const V8 *>(sBh + offB);
-    V8 bl = *reinterpret_cast<const V8 *>(sBl + offB);
+    for (int i = 0; i < kFragM; i++)
+      ah[i] = *reinterpret_cast<const V8 *>(cur + offA + i * 16 * kWRowBytes);
 #pragma unroll
     for (int j = 0; j < kFragN; j++) {
+      // column j's B fragments are read while column j - 1's MFMAs run
       V8 nbh = bh, nbl = bl;
-      if (more) wide_stage_piece(j, nst, lo, gAh + kb, gAl + kb, gBh + kb, gBl + kb, wv);
+      if (more) wide_stage_piece(j, nst, lo, rs, kb, wv);
       if (j + 1 < kFragN) {
         nbh = *reinterpret_cast<const V8 *>(sBh + offB + (j + 1) * 16 * kWRowBytes);
         nbl = *reinterpret_cast<const V8 *>(sBl + offB + (j + 1) * 16 * kWRowBytes);
       }
-      // three independent passes over the accumulators of column j
+      // three passes over the accumulators of column j, in this order
 #pragma unroll
       for (int i = 0; i < kFragM; i++) acc[i][j] = mma32(al[i], bh, acc[i][j]);
+      if (j == kFragN - 1) {
+        // this wavefront's reads of the stage are done (its last B column is
+        // in registers) and its loads of the next stage have landed
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (more) {
+          const unsigned char *nAl = nst + kWM * kWRowBytes;
+          const unsigned char *nBh = nst + 2 * kWM * kWRowBytes, *nBl = nBh + kWN * kWRowBytes;
+#pragma unroll
+          for (int i = 0; i < kFragM; i++)
+            al[i] = *reinterpret_cast<const V8 *>(nAl + offA + i * 16 * kWRowBytes);
+          nbh = *reinterpret_cast<const V8 *>(nBh + offB);
+          nbl = *reinterpret_cast<const V8 *>(nBl + offB);
+        }
+      }
 #pragma unroll
       for (int i = 0; i < kFragM; i++) acc[i][j] = mma32(ah[i], bl, acc[i][j]);
 #pragma unroll
@@ -918,9 +935,9 @@ gemm_x3w_kernel(GemmBatch bt, const E *__restrict__ Ah0, const E *__restrict__ A
       bh = nbh;
       bl = nbl;
     }
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-  }
+  };
+  for (int s = 0; s + 1 < steps; s++) step(s, std::true_type{});
+  step(steps - 1, std::false_type{});
 
   // Z through LDS, 16 rows of the wavefront tile at a time, 16-byte stores
   float *stage = reinterpret_cast<float *>(smem) + wv * 16 * kWEpiStride;
@@ -970,7 +987,11 @@ void gemm_bf16x3_wide(hipStream_t st, int nb, const GemmProduct *prods, const vo
   int tiles = 0;
   for (int q = 0; q < nb; q++) {
     const GemmProduct &g = prods[q];
-    FLARE_REQUIRE(g.K % kWK == 0, "gemm_x3w: K must be padded to 32");
+    FLARE_REQUIRE(g.K >= kWK && g.K % kWK == 0, "gemm_x3w: K must be padded to 32");
+    // 32-bit buffer offsets and record counts
+    FLARE_REQUIRE((int64_t)g.M * g.K * 2 <= (int64_t)UINT32_MAX &&
+                      (int64_t)g.N * g.K * 2 <= (int64_t)UINT32_MAX,
+                  "gemm_x3w: an operand block of one product exceeds 4 GiB");
     FLARE_REQUIRE(!g.rs && !g.cs, "gemm_x3w: no row / column scales");
     if (g.M <= 0 || g.N <= 0) continue;
     bt.g[bt.nb] = g;
