@@ -118,6 +118,20 @@ def test_legacy_fixture_two_species_cosine(orc):
     check_forms(orc, pair, cell, species, pos, mixed_tol=5e-3)
 
 
+def test_wide_gemm_ragged_tiles(orc):
+    """The mixed form's wide-row GEMM (features > 256) with a feature count
+    that is neither a multiple of the 352-column tile nor of 4 (B2 {2, 7, 4},
+    F = 525: a full and a 173-column tile, scalar Z stores) and species of
+    300 and 200 atoms (a partial 256-row tile): every form against the
+    oracle, the low-precision forms to the stated bounds."""
+    cell, species, pos = synth.fcc(5, 4.05, 0.1, [300, 200], 57)
+    pair = make_pair(2, 7, 4, seed=58)
+    assert pair.n_descriptors == 525
+    e0, f0, v0 = check_forms(orc, pair, cell, species, pos)
+    ea0 = oracle_pair(orc, pair, cell, species, pos)[3]
+    check_low_precision(pair, cell, species, pos, e0, f0, v0, ea0)
+
+
 def test_empty_environment(orc):
     cell = np.eye(3) * 30.0
     pos = np.array([[1.0, 1.0, 1.0], [15.0, 15.0, 15.0], [17.0, 15.5, 15.0]])
