@@ -97,6 +97,7 @@ char *scratch_of(int device) {
 
 struct flare_ctx {
   int device = 0;
+  int n_cu = 0;  // compute units of the device
   hipStream_t stream = nullptr;
   rocblas_handle blas = nullptr;
   int blocking = 1;
@@ -1064,6 +1065,8 @@ int flare_ctx_create(int device, flare_ctx **out) {
     std::unique_ptr<flare_ctx> c(new flare_ctx);
     c->device = device;
     c->use();
+    FLARE_HIP_CHECK(
+        hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device));
     scratch_setup(device);
     BLAS_CHECK(rocblas_create_handle(&c->blas));
     BLAS_CHECK(rocblas_set_stream(c->blas, c->stream));
@@ -2228,8 +2231,14 @@ static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p
       g[s].rs = d_is + off[s];
       g[s].cs = gp->Phat_is.as<float>() + gp->m_off[s];
     }
-    flare::gemm_f16x3_batched(ctx->stream, p.S, g, dh, dl, gp->Phat_h.ptr,
-                              gp->Phat_l.ptr, Kt);
+    // 256 x 352 tiles when they make a single round of workgroups (32,000
+    // atoms, 250 sparse per species: 83 vs 96 us), 128 x 256 tiles otherwise
+    // (55,296 atoms, 667 per species: 367 vs 418 us)
+    if (flare::gemm_x3_wide_tiles(p.S, g) <= ctx->n_cu)
+      flare::gemm_f16x3_wide(ctx->stream, p.S, g, dh, dl, gp->Phat_h.ptr, gp->Phat_l.ptr, Kt);
+    else
+      flare::gemm_f16x3_batched(ctx->stream, p.S, g, dh, dl, gp->Phat_h.ptr,
+                                gp->Phat_l.ptr, Kt);
   } else if (mcore) {
     // t = d . p_hat with the kernel values, c = alpha t^(xi-1) and the
     // partial energies in the GEMM's epilogue
@@ -2264,6 +2273,8 @@ static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p
   }
   ctx->mark_end();
   ctx->mark_begin("gp_epilogue");
+  flare::GpEpiLp eps[flare::kMaxSpecies];
+  int neps = 0, packed_rows = 0;
   for (int s = 0; s < p.S; s++) {
     const int rows = off[s + 1] - off[s], Ms = gp->m_by_type[s], o = gp->m_off[s];
     if (!rows) continue;
@@ -2288,15 +2299,34 @@ static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p
     }
     if (mcore) continue;  // done in the GEMM
     // kval and (FP32 fallback) cf rows of ldk, or FP16 hi/lo rows of mp (mixed)
-    flare::launch_gp_epilogue_lp(
-        ctx->stream, rows, Ms, gp->mp[s], Kt + roff[s], st.fw.norm2 + off[s],
-        gp->pnorm.as<double>() + o, gp->alpha.as<double>() + o, gp->sig2,
-        gp->power, want_k ? st.kval + st.koff[s] : nullptr, gp->ldk[s],
-        mixed ? nullptr : cf + st.koff[s],
-        mixed ? (void *)((_Float16 *)ch + coff[s]) : nullptr,
-        mixed ? (void *)((_Float16 *)cl + coff[s]) : nullptr,
-        mixed ? c_is + off[s] : nullptr, e_part + off[s]);
+    flare::GpEpiLp &q = eps[neps++];
+    q.row0 = packed_rows;
+    q.rows = rows;
+    q.M = Ms;
+    q.Mp = gp->mp[s];
+    q.ldk = gp->ldk[s];
+    q.K = Kt + roff[s];
+    q.norm2_rows = st.fw.norm2 + off[s];
+    q.pnorm = gp->pnorm.as<double>() + o;
+    q.alpha = gp->alpha.as<double>() + o;
+    q.kval = want_k ? st.kval + st.koff[s] : nullptr;
+    q.cf = mixed ? nullptr : cf + st.koff[s];
+    q.ch = mixed ? (void *)((_Float16 *)ch + coff[s]) : nullptr;
+    q.cl = mixed ? (void *)((_Float16 *)cl + coff[s]) : nullptr;
+    q.c_is = mixed ? c_is + off[s] : nullptr;
+    q.e_row = e_part + off[s];
+    packed_rows += rows;
   }
+  // every species in one launch, rows in registers; rows wider than 1,024
+  // values one launch per species
+  if (neps > 0 && !flare::launch_gp_epilogue_lp_rows(ctx->stream, neps, eps, gp->sig2,
+                                                     gp->power))
+    for (int k = 0; k < neps; k++) {
+      const flare::GpEpiLp &q = eps[k];
+      flare::launch_gp_epilogue_lp(ctx->stream, q.rows, q.M, q.Mp, q.K, q.norm2_rows,
+                                   q.pnorm, q.alpha, gp->sig2, gp->power, q.kval, q.ldk,
+                                   q.cf, q.ch, q.cl, q.c_is, q.e_row);
+    }
   ctx->mark_end();
   ctx->mark_begin("gp_G");
   if (mixed) {
@@ -2307,8 +2337,13 @@ static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p
       g[s].rs = c_is + off[s];
       g[s].cs = gp->PhatT_is.as<float>() + (size_t)s * F;
     }
-    flare::gemm_f16x3_batched(ctx->stream, p.S, g, ch, cl, gp->PhatT_h.ptr,
-                              gp->PhatT_l.ptr, G);
+    // wide rows on 256 x 352 tiles (F = 680: 58 vs 85 us; F = 1,200: 391
+    // vs 436 us)
+    if (F > 256)
+      flare::gemm_f16x3_wide(ctx->stream, p.S, g, ch, cl, gp->PhatT_h.ptr, gp->PhatT_l.ptr, G);
+    else
+      flare::gemm_f16x3_batched(ctx->stream, p.S, g, ch, cl, gp->PhatT_h.ptr,
+                                gp->PhatT_l.ptr, G);
   } else {
     // G = c P_hat on the matrix cores: c rows of ldk against P_hat^T rows
     flare::GemmProduct g[flare::kMaxSpecies];

@@ -269,6 +269,82 @@ gp_epilogue_lp_kernel(int rows, int M, int Mp, const float *__restrict__ K,
   if (lane == 0) c_is[r] = ldexpf(1.0f, -ex);
 }
 
+// gp_epilogue_lp_kernel for every species in one launch, one wavefront per
+// row and KR values of the row per lane held in registers (one pass over
+// K): the same values, sums in the same order, the same bits.
+struct GpEpiLpBatch {
+  int nsp, rows;
+  GpEpiLp s[kMaxSpecies];
+};
+
+template <int KR>
+__global__ void __launch_bounds__(256)
+gp_epilogue_lp_rows_kernel(GpEpiLpBatch b, double sig2, double power) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int gr = blockIdx.x * 4 + (int)(threadIdx.x / kWave);
+  if (gr >= b.rows) return;
+  int sp = 0;
+  while (sp + 1 < b.nsp && gr >= b.s[sp + 1].row0) sp++;
+  const GpEpiLp &q = b.s[sp];
+  const int r = gr - q.row0;
+  const int M = q.M, U = max(M, q.ldk);
+  const bool empty = sqrt(q.norm2_rows[r]) < 1e-8;
+  float t[KR];
+  double pn[KR], al[KR];
+#pragma unroll
+  for (int j = 0; j < KR; j++) {
+    const int u = lane + kWave * j;
+    t[j] = 0.0f;
+    pn[j] = 0.0;
+    al[j] = 0.0;
+    if (u < M) {
+      t[j] = q.K[(int64_t)r * M + u];
+      pn[j] = q.pnorm[u];
+      al[j] = q.alpha[u];
+    }
+  }
+  double e = 0;
+  float cmax = 0.0f, cvs[KR];
+#pragma unroll
+  for (int j = 0; j < KR; j++) {
+    const int u = lane + kWave * j;
+    double kv = 0, cv = 0, ev = 0;
+    if (u < M && !empty && !(pn[j] < 1e-8)) {
+      const double tt = (double)t[j];
+      const double tx = powi(tt, power);
+      kv = sig2 * tx;
+      cv = al[j] * powi(tt, power - 1.0);
+      ev = al[j] * tx;
+    }
+    cvs[j] = (float)cv;
+    if (u < U) {
+      e += ev;
+      if (q.kval) q.kval[(int64_t)r * q.ldk + u] = kv;
+      if (q.cf) q.cf[(int64_t)r * q.ldk + u] = (float)cv;
+      cmax = fmaxf(cmax, fabsf((float)cv));
+    }
+  }
+  e = wave_sum(e);
+  if (lane == 0) q.e_row[r] = e;
+  if (!q.ch) return;
+  for (int off = 32; off > 0; off >>= 1) cmax = fmaxf(cmax, __shfl_xor(cmax, off));
+  int ex;
+  const bool nonzero = f16_row_scale(cmax, ex);
+  const float sc = ldexpf(1.0f, ex);
+  _Float16 *ch = reinterpret_cast<_Float16 *>(q.ch), *cl = reinterpret_cast<_Float16 *>(q.cl);
+#pragma unroll
+  for (int j = 0; j < KR; j++) {
+    const int u = lane + kWave * j;
+    if (u < q.Mp) {
+      const float x = nonzero ? cvs[j] * sc : 0.0f;
+      const _Float16 h = (_Float16)x;
+      ch[(int64_t)r * q.Mp + u] = h;
+      cl[(int64_t)r * q.Mp + u] = (_Float16)(x - (float)h);
+    }
+  }
+  if (lane == 0) q.c_is[r] = ldexpf(1.0f, -ex);
+}
+
 // w = sig2 xi / |B| (G - e d) with d = B/|B| the FP32 row and G = c P_hat.
 __global__ void __launch_bounds__(64)
 gp_w_lp_kernel(int rows, int F, const float *__restrict__ G,
@@ -526,6 +602,32 @@ void launch_gp_epilogue_lp(hipStream_t st, int rows, int M, int Mp,
                      M, Mp, K, norm2_rows, pnorm, alpha, sig2, power, kval, ldk, cf,
                      (_Float16 *)ch, (_Float16 *)cl, c_is, e_row);
   FLARE_HIP_CHECK(hipGetLastError());
+}
+
+bool launch_gp_epilogue_lp_rows(hipStream_t st, int nsp, const GpEpiLp *sp, double sig2,
+                                double power) {
+  FLARE_REQUIRE(nsp >= 0 && nsp <= kMaxSpecies, "gp epilogue: too many species");
+  GpEpiLpBatch b;
+  b.nsp = nsp;
+  b.rows = 0;
+  int width = 0;
+  for (int q = 0; q < nsp; q++) {
+    b.s[q] = sp[q];
+    FLARE_REQUIRE(sp[q].row0 == b.rows, "gp epilogue: species rows must be packed");
+    b.rows += sp[q].rows;
+    width = std::max(width, std::max(sp[q].M, std::max(sp[q].ldk, sp[q].Mp)));
+  }
+  if (width > 16 * kWave) return false;
+  if (b.rows == 0) return true;
+  const dim3 grid((b.rows + 3) / 4), block(4 * kWave);
+  if (width <= 4 * kWave)
+    hipLaunchKernelGGL(gp_epilogue_lp_rows_kernel<4>, grid, block, 0, st, b, sig2, power);
+  else if (width <= 8 * kWave)
+    hipLaunchKernelGGL(gp_epilogue_lp_rows_kernel<8>, grid, block, 0, st, b, sig2, power);
+  else
+    hipLaunchKernelGGL(gp_epilogue_lp_rows_kernel<16>, grid, block, 0, st, b, sig2, power);
+  FLARE_HIP_CHECK(hipGetLastError());
+  return true;
 }
 
 void launch_gp_w_lp(hipStream_t st, int rows, int F, const float *G,

@@ -297,6 +297,23 @@ void launch_gp_epilogue_lp(hipStream_t st, int rows, int M, int Mp,
                            const double *pnorm, const double *alpha, double sig2,
                            double power, double *kval, int ldk, float *cf, void *ch,
                            void *cl, float *c_is, double *e_row);
+// The same epilogue for every species in one launch: species q covers rows
+// row0 .. row0 + rows of a packed row space (species without rows or
+// sparse environments left out), four rows per workgroup, one wavefront per
+// row with the row's values in registers (single pass) when max(M, ldk, Mp)
+// <= 1,024; false (nothing launched) otherwise.
+struct GpEpiLp {
+  int row0, rows, M, Mp, ldk;
+  const float *K;
+  const double *norm2_rows, *pnorm, *alpha;
+  double *kval;
+  float *cf;
+  void *ch, *cl;
+  float *c_is;
+  double *e_row;
+};
+bool launch_gp_epilogue_lp_rows(hipStream_t st, int nsp, const GpEpiLp *sp, double sig2,
+                                double power);
 // w (FP32) = sig2 xi / |B| (G - e d), d the FP32 normalised rows; e as in
 // launch_gp_w
 void launch_gp_w_lp(hipStream_t st, int rows, int F, const float *G,

@@ -32,10 +32,15 @@ void gemm_bf16x3_batched(hipStream_t st, int nb, const GemmProduct *prods,
                          const void *Ah, const void *Al, const void *Bh,
                          const void *Bl, float *Z);
 // The same contraction with 256 x 352 workgroup tiles for wide rows (the
-// mapped Z = D beta, N = F); no row / column scales.
+// mapped Z = D beta, N = F), row / column scales as gemm_f16x3_batched's.
 void gemm_bf16x3_wide(hipStream_t st, int nb, const GemmProduct *prods,
                       const void *Ah, const void *Al, const void *Bh,
                       const void *Bl, float *Z);
+void gemm_f16x3_wide(hipStream_t st, int nb, const GemmProduct *prods,
+                     const void *Ah, const void *Al, const void *Bh,
+                     const void *Bl, float *Z);
+// Workgroup tiles of the wide-row products (one per CU at a time).
+int gemm_x3_wide_tiles(int nb, const GemmProduct *prods);
 // FP64 / FP32 products Z = A B^T on the matrix cores (row stride of A and
 // B = K, rows a multiple of 16 bytes; no hi/lo parts).
 void dgemm_batched(hipStream_t st, int nb, const GemmProduct *prods, const double *A,

@@ -943,6 +943,7 @@ gemm_x3w_kernel(GemmBatch bt, const E *__restrict__ Ah0, const E *__restrict__ A
   float *stage = reinterpret_cast<float *>(smem) + wv * 16 * kWEpiStride;
   const int dcol = lane & 15, drow = (lane >> 4) * 4;
   const bool vec4 = (N & 3) == 0 && (ldz & 3) == 0 && (gp.z_off & 3) == 0;
+  const bool scaled = gp.rs || gp.cs;
 #pragma unroll
   for (int i = 0; i < kFragM; i++) {
 #pragma unroll
@@ -958,8 +959,21 @@ gemm_x3w_kernel(GemmBatch bt, const E *__restrict__ Ah0, const E *__restrict__ A
       const int f = lane + 64 * t;
       const int r = f / kVec, c = (f - r * kVec) * 4;
       const int row = m0 + wm + 16 * i + r, col = n0 + wn + c;
-      const float4 v = *reinterpret_cast<const float4 *>(&stage[r * kWEpiStride + c]);
+      float4 v = *reinterpret_cast<const float4 *>(&stage[r * kWEpiStride + c]);
       if (row < M) {
+        if (scaled) {
+          // (acc x row scale) x column scale, as gemm_x3_kernel
+          const float rsc = gp.rs ? gp.rs[row] : 1.0f;
+          float cv[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+          if (gp.cs)
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+              if (col + e < N) cv[e] = gp.cs[col + e];
+          v.x = v.x * rsc * cv[0];
+          v.y = v.y * rsc * cv[1];
+          v.z = v.z * rsc * cv[2];
+          v.w = v.w * rsc * cv[3];
+        }
         float *dst = Z + (int64_t)row * ldz + col;
         if (vec4) {
           if (col < N) *reinterpret_cast<float4 *>(dst) = v;
@@ -979,8 +993,17 @@ gemm_x3w_kernel(GemmBatch bt, const E *__restrict__ Ah0, const E *__restrict__ A
 static_assert(16 * (kWaveN / 4) % 64 == 0, "epilogue rows split evenly over lanes");
 static_assert(kWaves * 16 * kWEpiStride * 4 <= 2 * kWStage, "epilogue staging fits the LDS");
 
-void gemm_bf16x3_wide(hipStream_t st, int nb, const GemmProduct *prods, const void *Ah,
-                      const void *Al, const void *Bh, const void *Bl, float *Z) {
+int gemm_x3_wide_tiles(int nb, const GemmProduct *prods) {
+  int tiles = 0;
+  for (int q = 0; q < nb; q++)
+    if (prods[q].M > 0 && prods[q].N > 0)
+      tiles += (prods[q].M + kWM - 1) / kWM * ((prods[q].N + kWN - 1) / kWN);
+  return tiles;
+}
+
+template <typename E>
+static void gemm_x3_wide(hipStream_t st, int nb, const GemmProduct *prods, const void *Ah,
+                         const void *Al, const void *Bh, const void *Bl, float *Z) {
   FLARE_REQUIRE(nb >= 0 && nb <= kGemmMaxBatch, "gemm_x3w: at most 8 products");
   GemmBatch bt;
   bt.nb = 0;
@@ -992,7 +1015,6 @@ void gemm_bf16x3_wide(hipStream_t st, int nb, const GemmProduct *prods, const vo
     FLARE_REQUIRE((int64_t)g.M * g.K * 2 <= (int64_t)UINT32_MAX &&
                       (int64_t)g.N * g.K * 2 <= (int64_t)UINT32_MAX,
                   "gemm_x3w: an operand block of one product exceeds 4 GiB");
-    FLARE_REQUIRE(!g.rs && !g.cs, "gemm_x3w: no row / column scales");
     if (g.M <= 0 || g.N <= 0) continue;
     bt.g[bt.nb] = g;
     bt.tile0[bt.nb] = tiles;
@@ -1002,10 +1024,19 @@ void gemm_bf16x3_wide(hipStream_t st, int nb, const GemmProduct *prods, const vo
   bt.tile0[bt.nb] = tiles;
   if (tiles == 0) return;
   const int grid = (tiles + 7) / 8 * 8;
-  hipLaunchKernelGGL(gemm_x3w_kernel<__bf16>, dim3(grid), dim3(64 * kWaves), 0, st, bt,
-                     (const __bf16 *)Ah, (const __bf16 *)Al, (const __bf16 *)Bh,
-                     (const __bf16 *)Bl, Z);
+  hipLaunchKernelGGL(gemm_x3w_kernel<E>, dim3(grid), dim3(64 * kWaves), 0, st, bt,
+                     (const E *)Ah, (const E *)Al, (const E *)Bh, (const E *)Bl, Z);
   FLARE_HIP_CHECK(hipGetLastError());
+}
+
+void gemm_bf16x3_wide(hipStream_t st, int nb, const GemmProduct *prods, const void *Ah,
+                      const void *Al, const void *Bh, const void *Bl, float *Z) {
+  gemm_x3_wide<__bf16>(st, nb, prods, Ah, Al, Bh, Bl, Z);
+}
+
+void gemm_f16x3_wide(hipStream_t st, int nb, const GemmProduct *prods, const void *Ah,
+                     const void *Al, const void *Bh, const void *Bl, float *Z) {
+  gemm_x3_wide<_Float16>(st, nb, prods, Ah, Al, Bh, Bl, Z);
 }
 
 // FP16 hi/lo split of rows with a power-of-two scale per row: the row's
