@@ -2215,30 +2215,51 @@ static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p
     ch = ctx->buf("gpl.ch", std::max<size_t>(ctot, 1) * 2);
     cl = ctx->buf("gpl.cl", std::max<size_t>(ctot, 1) * 2);
     c_is = ctx->buf_t<float>("gpl.c_is", std::max(n, 1));
-    Kt = ctx->buf_t<float>("gpl.K", std::max<size_t>(rtot, 1));
+    Kt = ctx->buf_t<float>("gpl.K", std::max<size_t>(2 * rtot, 1));  // room for a K split
   } else {
     cf = ctx->buf_t<float>("gpl.c", std::max<size_t>(ktot, 1));
     if (!mcore) Kt = ctx->buf_t<float>("gpl.K", std::max<size_t>(rtot, 1));
   }
+  int ksplit = 1;
   ctx->mark_begin("gp_K");
   if (mixed) {
     const int kp = gp->kp;
-    flare::GemmProduct g[flare::kMaxSpecies];
+    flare::GemmProduct g[2 * flare::kMaxSpecies];
+    int width = 0;
     for (int s = 0; s < p.S; s++) {
       g[s] = flare::GemmProduct{off[s + 1] - off[s], gp->m_by_type[s], kp,
                                 gp->m_by_type[s], (int64_t)off[s] * kp,
                                 (int64_t)gp->m_off[s] * kp, (int64_t)roff[s]};
       g[s].rs = d_is + off[s];
       g[s].cs = gp->Phat_is.as<float>() + gp->m_off[s];
+      width = std::max(width, std::max(gp->m_by_type[s], std::max(gp->ldk[s], gp->mp[s])));
     }
-    // 256 x 352 tiles when they make a single round of workgroups (32,000
-    // atoms, 250 sparse per species: 83 vs 96 us), 128 x 256 tiles otherwise
-    // (55,296 atoms, 667 per species: 367 vs 418 us)
-    if (flare::gemm_x3_wide_tiles(p.S, g) <= ctx->n_cu)
-      flare::gemm_f16x3_wide(ctx->stream, p.S, g, dh, dl, gp->Phat_h.ptr, gp->Phat_l.ptr, Kt);
-    else
+    // 256 x 352 tiles when they make a single round of workgroups, with K
+    // split in two halves (summed by the epilogue) when that still fits one
+    // round (32,000 atoms, 250 sparse per species: 96 us on 128 x 256 tiles,
+    // 83 us on 126 wide tiles), 128 x 256 tiles otherwise (55,296 atoms,
+    // 667 per species: 367 vs 418 us)
+    const int wt = flare::gemm_x3_wide_tiles(p.S, g);
+    if (wt <= ctx->n_cu) {
+      if (2 * wt <= ctx->n_cu && 2 * p.S <= flare::kGemmMaxBatch && kp % 64 == 0 &&
+          width <= 16 * flare::kWave) {
+        ksplit = 2;
+        for (int s = 0; s < p.S; s++) {
+          flare::GemmProduct &h0 = g[s], &h1 = g[p.S + s];
+          h0.K = kp / 2;
+          h0.ld_ab = kp;
+          h1 = h0;
+          h1.a_off += kp / 2;
+          h1.b_off += kp / 2;
+          h1.z_off += (int64_t)rtot;
+        }
+      }
+      flare::gemm_f16x3_wide(ctx->stream, ksplit * p.S, g, dh, dl, gp->Phat_h.ptr,
+                             gp->Phat_l.ptr, Kt);
+    } else {
       flare::gemm_f16x3_batched(ctx->stream, p.S, g, dh, dl, gp->Phat_h.ptr,
                                 gp->Phat_l.ptr, Kt);
+    }
   } else if (mcore) {
     // t = d . p_hat with the kernel values, c = alpha t^(xi-1) and the
     // partial energies in the GEMM's epilogue
@@ -2306,6 +2327,7 @@ static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p
     q.Mp = gp->mp[s];
     q.ldk = gp->ldk[s];
     q.K = Kt + roff[s];
+    q.K2 = ksplit == 2 ? Kt + rtot + roff[s] : nullptr;
     q.norm2_rows = st.fw.norm2 + off[s];
     q.pnorm = gp->pnorm.as<double>() + o;
     q.alpha = gp->alpha.as<double>() + o;
@@ -2320,13 +2342,15 @@ static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p
   // every species in one launch, rows in registers; rows wider than 1,024
   // values one launch per species
   if (neps > 0 && !flare::launch_gp_epilogue_lp_rows(ctx->stream, neps, eps, gp->sig2,
-                                                     gp->power))
+                                                     gp->power)) {
+    FLARE_REQUIRE(ksplit == 1, "gp epilogue: a K split needs the single-pass epilogue");
     for (int k = 0; k < neps; k++) {
       const flare::GpEpiLp &q = eps[k];
       flare::launch_gp_epilogue_lp(ctx->stream, q.rows, q.M, q.Mp, q.K, q.norm2_rows,
                                    q.pnorm, q.alpha, gp->sig2, gp->power, q.kval, q.ldk,
                                    q.cf, q.ch, q.cl, q.c_is, q.e_row);
     }
+  }
   ctx->mark_end();
   ctx->mark_begin("gp_G");
   if (mixed) {

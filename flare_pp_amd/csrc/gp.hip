@@ -12,6 +12,8 @@
 //  pack/unpack   mapping coefficient file convention: packed upper
 //                triangle, off-diagonals doubled (normalized_dot_product.cpp
 //                :736-753, pair_flare.cpp:378-395).
+#include <type_traits>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -271,7 +273,8 @@ gp_epilogue_lp_kernel(int rows, int M, int Mp, const float *__restrict__ K,
 
 // gp_epilogue_lp_kernel for every species in one launch, one wavefront per
 // row and KR values of the row per lane held in registers (one pass over
-// K): the same values, sums in the same order, the same bits.
+// K): the same values, sums in the same order, the same bits.  With K2, t is
+// the FP32 sum of the two halves of a K-split product.
 struct GpEpiLpBatch {
   int nsp, rows;
   GpEpiLp s[kMaxSpecies];
@@ -289,20 +292,32 @@ gp_epilogue_lp_rows_kernel(GpEpiLpBatch b, double sig2, double power) {
   const int r = gr - q.row0;
   const int M = q.M, U = max(M, q.ldk);
   const bool empty = sqrt(q.norm2_rows[r]) < 1e-8;
-  float t[KR];
+  float t[KR], t2[KR];
   double pn[KR], al[KR];
+  // every load of the row issued before any is used (the branch on K2 is
+  // outside the batch)
+  auto load = [&](auto two_c) {
+    constexpr bool two = decltype(two_c)::value;
 #pragma unroll
-  for (int j = 0; j < KR; j++) {
-    const int u = lane + kWave * j;
-    t[j] = 0.0f;
-    pn[j] = 0.0;
-    al[j] = 0.0;
-    if (u < M) {
-      t[j] = q.K[(int64_t)r * M + u];
-      pn[j] = q.pnorm[u];
-      al[j] = q.alpha[u];
+    for (int j = 0; j < KR; j++) {
+      const int u = lane + kWave * j;
+      t[j] = t2[j] = 0.0f;
+      pn[j] = al[j] = 0.0;
+      if (u < M) {
+        t[j] = q.K[(int64_t)r * M + u];
+        if (two) t2[j] = q.K2[(int64_t)r * M + u];
+        pn[j] = q.pnorm[u];
+        al[j] = q.alpha[u];
+      }
     }
-  }
+    if (two)
+#pragma unroll
+      for (int j = 0; j < KR; j++) t[j] += t2[j];
+  };
+  if (q.K2)
+    load(std::true_type{});
+  else
+    load(std::false_type{});
   double e = 0;
   float cmax = 0.0f, cvs[KR];
 #pragma unroll

@@ -304,7 +304,7 @@ void launch_gp_epilogue_lp(hipStream_t st, int rows, int M, int Mp,
 // <= 1,024; false (nothing launched) otherwise.
 struct GpEpiLp {
   int row0, rows, M, Mp, ldk;
-  const float *K;
+  const float *K, *K2;  // t = K (+ K2: the second half of a K split)
   const double *norm2_rows, *pnorm, *alpha;
   double *kval;
   float *cf;

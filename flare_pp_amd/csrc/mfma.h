@@ -20,10 +20,13 @@ constexpr int kGemmMaxBatch = 8;
 // column n.
 // rs / cs (optional, FP32, M and N values) multiply row i / column j of the
 // result: the inverse scales of pre-scaled operand rows.
+// ld_ab (wide-row products only; 0: K): row stride of A and B when the
+// product covers only K of their columns (a K split).
 struct GemmProduct {
   int M, N, K, ldz;
   int64_t a_off, b_off, z_off;
   const float *rs = nullptr, *cs = nullptr;
+  int ld_ab = 0;
 };
 
 // Up to kGemmMaxBatch products in one launch on BF16 MFMA with FP32

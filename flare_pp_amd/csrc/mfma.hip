@@ -581,6 +581,7 @@ static void dgemm_batched_t(hipStream_t st, int nb, const GemmProduct *prods,
   for (int q = 0; q < nb; q++) {
     const GemmProduct &g = prods[q];
     FLARE_REQUIRE((g.K * (int)sizeof(T)) % 16 == 0, "gemm: rows must be 16-byte multiples");
+    FLARE_REQUIRE(!g.ld_ab, "gemm: K splits only on the wide-row products");
     if (g.M <= 0 || g.N <= 0) continue;
     if (ep) {
       const DEpilogue &e = ep[q];
@@ -683,6 +684,7 @@ static void gemm_x3_batched(hipStream_t st, int nb, const GemmProduct *prods,
   for (int q = 0; q < nb; q++) {
     const GemmProduct &g = prods[q];
     FLARE_REQUIRE(g.K % kBK == 0, "gemm_x3: K must be padded to 32");
+    FLARE_REQUIRE(!g.ld_ab, "gemm_x3: K splits only on the wide-row products");
     if (g.M <= 0 || g.N <= 0) continue;
     bt.g[bt.nb] = g;
     bt.tile0[bt.nb] = tiles;
@@ -844,7 +846,8 @@ gemm_x3w_kernel(GemmBatch bt, const E *__restrict__ Ah0, const E *__restrict__ A
   int q = 0;
   while (T >= bt.tile0[q + 1]) q++;
   const GemmProduct &gp = bt.g[q];
-  const int M = gp.M, N = gp.N, Kp = gp.K, ldz = gp.ldz;
+  // K columns of A and B rows of stride Kp
+  const int M = gp.M, N = gp.N, K = gp.K, Kp = gp.ld_ab ? gp.ld_ab : gp.K, ldz = gp.ldz;
   const int nct = (N + kWN - 1) / kWN;
   static_assert(kWaves == 2 * (kWM / kWaveM) && 2 * kWaveN == kWN, "4 x 2 wavefronts");
   const int local = T - bt.tile0[q];
@@ -853,7 +856,7 @@ gemm_x3w_kernel(GemmBatch bt, const E *__restrict__ Ah0, const E *__restrict__ A
   const E *Bh = Bh0 + gp.b_off, *Bl = Bl0 + gp.b_off;
   float *Z = Z0 + gp.z_off;
   const int m0 = rt * kWM, n0 = ct * kWN;
-  const int steps = Kp / kWK;
+  const int steps = K / kWK;
 
   floatx4 acc[kFragM][kFragN];
 #pragma unroll
@@ -871,10 +874,11 @@ gemm_x3w_kernel(GemmBatch bt, const E *__restrict__ Ah0, const E *__restrict__ A
 
   const WideOffsets lo = wide_offsets(M, N, Kp, m0, n0, wv, lane);
   WideRsrc rs;
-  rs.ah = wide_rsrc(Ah, (int64_t)M * Kp * 2);
-  rs.al = wide_rsrc(Al, (int64_t)M * Kp * 2);
-  rs.bh = wide_rsrc(Bh, (int64_t)N * Kp * 2);
-  rs.bl = wide_rsrc(Bl, (int64_t)N * Kp * 2);
+  const int64_t bytes_a = ((int64_t)(M - 1) * Kp + K) * 2, bytes_b = ((int64_t)(N - 1) * Kp + K) * 2;
+  rs.ah = wide_rsrc(Ah, bytes_a);
+  rs.al = wide_rsrc(Al, bytes_a);
+  rs.bh = wide_rsrc(Bh, bytes_b);
+  rs.bl = wide_rsrc(Bl, bytes_b);
 #pragma unroll
   for (int j = 0; j < kFragN; j++) wide_stage_piece(j, smem, lo, rs, 0, wv);
   __builtin_amdgcn_s_waitcnt(0);
@@ -1011,9 +1015,11 @@ static void gemm_x3_wide(hipStream_t st, int nb, const GemmProduct *prods, const
   for (int q = 0; q < nb; q++) {
     const GemmProduct &g = prods[q];
     FLARE_REQUIRE(g.K >= kWK && g.K % kWK == 0, "gemm_x3w: K must be padded to 32");
+    const int64_t ld = g.ld_ab ? g.ld_ab : g.K;
+    FLARE_REQUIRE(ld >= g.K && ld % 8 == 0, "gemm_x3w: row stride below K or unaligned");
     // 32-bit buffer offsets and record counts
-    FLARE_REQUIRE((int64_t)g.M * g.K * 2 <= (int64_t)UINT32_MAX &&
-                      (int64_t)g.N * g.K * 2 <= (int64_t)UINT32_MAX,
+    FLARE_REQUIRE((int64_t)g.M * ld * 2 <= (int64_t)UINT32_MAX &&
+                      (int64_t)g.N * ld * 2 <= (int64_t)UINT32_MAX,
                   "gemm_x3w: an operand block of one product exceeds 4 GiB");
     if (g.M <= 0 || g.N <= 0) continue;
     bt.g[bt.nb] = g;
