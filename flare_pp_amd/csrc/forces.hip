@@ -200,6 +200,53 @@ __device__ inline void u_mfma_fixed(const T *A, const T *w, const T *nrm_lm, T *
   }
 }
 
+// u_mfma_fixed with A given in u2's layout, Au[(s H + lm) N + n] for channel
+// c = s N + n (N = 8), and u written over it: for each l every M tile's
+// product is formed (all reads of A's columns l^2 .. l^2 + 2l) before the
+// tile results are stored into those same columns, and a wavefront's LDS
+// accesses complete in issue order.  The W_l operand is read from w per
+// step instead of being held in registers.  Same products, same bits as
+// u_mfma_fixed.
+template <typename T, int L, int NR>
+__device__ inline void u_mfma_fixed_inplace(T *Au, const T *w, const T *nrm_lm) {
+  static_assert(NR % 8 == 0 && NR <= 24, "one or two M tiles of channels");
+  typedef typename Mma<T>::V V;
+  constexpr int L1 = L + 1, H = L1 * L1, N = 8, KS = NR / 4, MT = (NR + 15) / 16;
+  const int lane = threadIdx.x;
+  const int col = lane & 15, g = lane >> 4;
+#pragma unroll 1
+  for (int l = 0; l < L1; l++) {
+    const int nl = 2 * l + 1, l2 = l * l;
+    const int j = col;
+    const bool jok = j < nl;
+    V acc[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; mt++) {
+      const int c = mt * 16 + col;
+      const bool crow = c < NR;
+      acc[mt] = V{0, 0, 0, 0};
+#pragma unroll
+      for (int ks = 0; ks < KS; ks++) {
+        const int c2 = 4 * ks + g;
+        const T fac = c == c2 ? T(2) : T(1);
+        const T a = crow ? fac * w[b2_pair_index(c, c2, NR) * L1 + l] : T(0);
+        const T b = jok ? Au[((c2 >> 3) * H + l2 + j) * N + (c2 & 7)] : T(0);
+        acc[mt] = Mma<T>::run(a, b, acc[mt]);
+      }
+    }
+    if (jok) {
+      const T sc = nrm_lm[l2 + j];
+#pragma unroll
+      for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+          const int cr = mt * 16 + Mma<T>::row(g, q);
+          if (cr < NR) Au[((cr >> 3) * H + l2 + j) * N + (cr & 7)] = acc[mt][q] * sc;
+        }
+    }
+  }
+}
+
 namespace {
 
 // Adjoint contraction for one pair, everything recomputed in registers.
@@ -357,8 +404,15 @@ __device__ inline void adjoint_pair_fixed(const B2Dev &p, T r, T x, T y, T z,
 // One wavefront per atom.  Dynamic LDS (T): A[W], w[F], then u[W] sharing
 // max(W, F) values with the X row x[F] (x is read only before w is formed,
 // u written only after), n[H], then int ptab[nr * nr] unless the u
-// contraction's channel-pair offsets come from registers (force_lds_bytes).
-// At l_max 6 the LDS, not the registers, sets how many wavefronts a CU holds.
+// contraction's channel-pair offsets come from registers.  The l_max 6
+// instances (8, 16 or 24 channels) instead hold A in u's layout and
+// overwrite it with u, and keep the X row in registers (BF16 rows packed, 34
+// registers per lane up to 2,176 features, else read from memory on each of
+// its two uses): A/u[W], w[F], n[H], 13.3 KB in FP32 at F = 2,100, so that a
+// CU holds twelve wavefronts at 168 registers, FP64 six at 248 (eight and
+// four with a separate u and 18 / 36 KB of LDS).  The kernel's time follows
+// the resident wavefronts: 3.62 / 4.05 / 5.39 ms at eight / seven / six per
+// CU, 2.47 ms at twelve, for 262,144 atoms in the mixed form.
 // WM 3 is WM 1 with X given as BF16 hi + lo rows (the GEMM's operand).
 // WM selects where w = dE_i/dB_i comes from:
 //   0  precomputed (sparse-GP path);
@@ -371,9 +425,13 @@ __device__ inline void adjoint_pair_fixed(const B2Dev &p, T r, T x, T y, T z,
 // per SIMD, 124 used at l_max 4 without spills): the latency of the
 // prologue's loads is then hidden by more resident atoms (79 vs 94 us per
 // launch at 32,000 atoms, B2 {2,8,4}).  l_max 6 would spill ~50.
+constexpr int force_waves_per_eu(bool f32, int lfix) {
+  return !f32 ? (lfix == 6 ? 2 : 1) : lfix <= 4 ? 4 : lfix == 6 ? 3 : 1;
+}
+
 template <typename T, int MODE, int NMAX, int WM, int LFIX, bool EXACT>
 __global__ void __launch_bounds__(64)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && LFIX <= 4 ? 4 : 1)))
+__attribute__((amdgpu_waves_per_eu(force_waves_per_eu(sizeof(T) == 4, LFIX))))
 force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
              const int *__restrict__ row_of, const T *__restrict__ ynorm,
              const T *__restrict__ A_in, WSource<T> src,
@@ -383,18 +441,14 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
   const int i = a0 + blockIdx.x;
   const int lane = threadIdx.x;
   const int H = p.H, NH = p.NH, W = p.W, F = p.F;
-  // the fixed-size l_max 6 instances hold the X row in registers, kXPairs
-  // feature pairs per lane, when it fits (F <= 2,176; the launch's LDS size
-  // agrees): their LDS, not their registers, bounds the wavefronts per CU.
-  // (At l_max 3 and 4 the extra registers would spill.)
-  constexpr bool kFixedShape = MODE == 1 && LFIX == 6 && EXACT && NMAX == 8;
-  constexpr int kXPairs = 17;
-  const bool x_regs = kFixedShape && F <= 2 * kXPairs * kWave;
+  // the fixed-size l_max 6 instances (8, 16 or 24 channels, checked at
+  // launch): A staged in u's layout and overwritten by u
+  constexpr bool kOverlay = MODE == 1 && LFIX == 6 && EXACT && NMAX == 8;
   T *A = reinterpret_cast<T *>(smem);
   T *w = A + W;
-  T *u = w + F;
-  T *xs = u;  // X row (rebuilt from hi + lo when BF16) when not in registers
-  T *nrm_lm = u + (x_regs ? W : (W > F ? W : F));  // harmonic normalisation per lm
+  T *u = kOverlay ? A : w + F;
+  T *xs = u;  // X row (rebuilt from hi + lo when BF16), other instances
+  T *nrm_lm = kOverlay ? w + F : u + (W > F ? W : F);  // harmonic normalisation per lm
   int *ptab = reinterpret_cast<int *>(nrm_lm + H);  // nr x nr feature offsets
   const T *Ai = A_in + (int64_t)i * W;
   const int row = row_of[i];
@@ -405,7 +459,9 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
   // rather than once per element.  Waits on loads are in issue order, so in
   // the mixed form the x / z rows and the harmonic normalisation are issued
   // ahead of the A row and are in registers when its stores wait.
-  const bool mixed_fast = WM == 3 && (F & 3) == 0 && F <= 16 * kWave;
+  // (not in the l_max 6 instances, whose 168-register budget it would
+  // exceed: their X row path serves every F)
+  const bool mixed_fast = !kOverlay && WM == 3 && (F & 3) == 0 && F <= 16 * kWave;
   uint2 th[4], tl[4];
   float4 zv[4];
   double n2_row = 0;
@@ -441,7 +497,7 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
                                   (16 * kWave)
                             : 1;
   const int nva = W * (int)sizeof(T) / 16;
-  const bool a_batch = kFixed && ((W * (int)sizeof(T)) & 15) == 0 && nva <= KA * kWave &&
+  const bool a_batch = kFixed && !kOverlay && ((W * (int)sizeof(T)) & 15) == 0 && nva <= KA * kWave &&
                        ((reinterpret_cast<uintptr_t>(Ai) | reinterpret_cast<uintptr_t>(A)) & 15) == 0;
   uint4 ta[KA];
   if (kFixed && a_batch) {
@@ -461,7 +517,26 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
       const int c = q / p.nr, c2 = q - c * p.nr;
       ptab[q] = b2_pair_index(c, c2, p.nr) * (p.L + 1);
     }
-  if (kFixed && a_batch) {
+  if constexpr (kOverlay) {
+    // A[c][lm] into u's layout: (c / 8 H + lm) 8 + c % 8
+    constexpr int HF = (LFIX + 1) * (LFIX + 1);
+    for (int base = 0; base < W; base += 8 * kWave) {
+      T t[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int o = base + lane + j * kWave;
+        t[j] = o < W ? Ai[o] : T(0);
+      }
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const int o = base + lane + j * kWave;
+        if (o < W) {
+          const int c = o / HF, lm = o - c * HF;
+          A[((c >> 3) * HF + lm) * 8 + (c & 7)] = t[j];
+        }
+      }
+    }
+  } else if (kFixed && a_batch) {
     uint4 *d4 = reinterpret_cast<uint4 *>(A);
 #pragma unroll
     for (int j = 0; j < KA; j++) {
@@ -516,32 +591,42 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
     if (lane == 0) src.E_row[row] = E;
   } else {
     const double n2 = src.norm2[row];
-    // X row in registers: features 2 q and 2 q + 1, q = lane + j kWave
-    T xv[kFixedShape ? 2 * kXPairs : 1];
-    if constexpr (kFixedShape) {
+    // kOverlay: the X row in registers, features 2 q and 2 q + 1 for
+    // q = lane + j kWave, up to 2,176 features (loaded here, ahead of the z
+    // row); past that it is read from memory on each of its two uses
+    constexpr int kXPairs = 17;
+    const bool x_regs = kOverlay && F <= 2 * kXPairs * kWave;
+    // BF16 rows stay packed (hi and lo of a feature pair in two registers,
+    // decoded where used); T rows as values
+    constexpr bool kPacked = WM == 3;
+    uint32_t xh2[kOverlay && kPacked ? kXPairs : 1], xl2[kOverlay && kPacked ? kXPairs : 1];
+    T xv[kOverlay && !kPacked ? 2 * kXPairs : 1];
+    auto xpair = [&](int j, T &x0, T &x1) {
+      if constexpr (kPacked) {
+        x0 = (T)(__uint_as_float(xh2[j] << 16) + __uint_as_float(xl2[j] << 16));
+        x1 = (T)(__uint_as_float(xh2[j] & 0xffff0000u) + __uint_as_float(xl2[j] & 0xffff0000u));
+      } else {
+        x0 = xv[2 * j];
+        x1 = xv[2 * j + 1];
+      }
+    };
+    if constexpr (kOverlay) {
       if (x_regs) {
-        if (WM == 3) {
+        if constexpr (kPacked) {
           const uint32_t *xh =
               reinterpret_cast<const uint32_t *>(src.xh + (int64_t)row * src.kp);
           const uint32_t *xl =
               reinterpret_cast<const uint32_t *>(src.xl + (int64_t)row * src.kp);
           const int F2 = (F + 1) >> 1;
-          uint32_t th2[kXPairs], tl2[kXPairs];
 #pragma unroll
           for (int j = 0; j < kXPairs; j++) {
             const int q = lane + j * kWave;
-            th2[j] = 0;
-            tl2[j] = 0;
+            xh2[j] = 0;
+            xl2[j] = 0;
             if (q < F2) {
-              th2[j] = xh[q];
-              tl2[j] = xl[q];
+              xh2[j] = xh[q];
+              xl2[j] = xl[q];
             }
-          }
-#pragma unroll
-          for (int j = 0; j < kXPairs; j++) {
-            xv[2 * j] = (T)(__uint_as_float(th2[j] << 16) + __uint_as_float(tl2[j] << 16));
-            xv[2 * j + 1] = (T)(__uint_as_float(th2[j] & 0xffff0000u) +
-                                __uint_as_float(tl2[j] & 0xffff0000u));
           }
         } else {
           const T *x = src.X + (int64_t)row * F;
@@ -573,40 +658,79 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
         }
       }
     }
-    if (x_regs) {
-      // (held in xv)
-    } else if (WM == 3) {
-      // pairs of BF16 values: rows are padded to an even length (kp)
-      const uint32_t *xh = reinterpret_cast<const uint32_t *>(src.xh + (int64_t)row * src.kp);
-      const uint32_t *xl = reinterpret_cast<const uint32_t *>(src.xl + (int64_t)row * src.kp);
+    // the same features from memory, eight pairs at a time, passed to
+    // fn(f, x_f, x_f+1) in the same order as the registers' loop below
+    auto for_x = [&](auto &&fn) {
       const int F2 = (F + 1) >> 1;
       for (int base = 0; base < F2; base += 8 * kWave) {
-        uint32_t th[8], tl[8];
+        T v0[8], v1[8];
 #pragma unroll
         for (int j = 0; j < 8; j++) {
           const int q = base + lane + j * kWave;
-          th[j] = 0;
-          tl[j] = 0;
+          v0[j] = v1[j] = T(0);
           if (q < F2) {
-            th[j] = xh[q];
-            tl[j] = xl[q];
+            if (WM == 3) {
+              // pairs of BF16 values: rows are padded to an even length (kp)
+              const uint32_t th = reinterpret_cast<const uint32_t *>(
+                  src.xh + (int64_t)row * src.kp)[q];
+              const uint32_t tl = reinterpret_cast<const uint32_t *>(
+                  src.xl + (int64_t)row * src.kp)[q];
+              v0[j] = (T)(__uint_as_float(th << 16) + __uint_as_float(tl << 16));
+              v1[j] = (T)(__uint_as_float(th & 0xffff0000u) +
+                          __uint_as_float(tl & 0xffff0000u));
+            } else if ((F & 1) == 0) {
+              typedef typename std::conditional<sizeof(T) == 8, double2, float2>::type T2;
+              const T2 t2 = reinterpret_cast<const T2 *>(src.X + (int64_t)row * F)[q];
+              v0[j] = t2.x;
+              v1[j] = t2.y;
+            } else {
+              const T *x = src.X + (int64_t)row * F;
+              v0[j] = x[2 * q];
+              if (2 * q + 1 < F) v1[j] = x[2 * q + 1];
+            }
           }
         }
 #pragma unroll
         for (int j = 0; j < 8; j++) {
           const int q = base + lane + j * kWave;
-          if (q < F2) {
-            const float v0 = __uint_as_float(th[j] << 16) + __uint_as_float(tl[j] << 16);
-            const float v1 = __uint_as_float(th[j] & 0xffff0000u) +
-                             __uint_as_float(tl[j] & 0xffff0000u);
-            xs[2 * q] = (T)v0;
-            if (2 * q + 1 < F) xs[2 * q + 1] = (T)v1;
-          }
+          if (q < F2) fn(2 * q, v0[j], v1[j]);
         }
       }
-    } else {
-      const T *x = src.X + (int64_t)row * F;
-      stage_lds_vec<4>(F, x, xs);
+    };
+    if constexpr (!kOverlay) {
+      if (WM == 3) {
+        // pairs of BF16 values: rows are padded to an even length (kp)
+        const uint32_t *xh = reinterpret_cast<const uint32_t *>(src.xh + (int64_t)row * src.kp);
+        const uint32_t *xl = reinterpret_cast<const uint32_t *>(src.xl + (int64_t)row * src.kp);
+        const int F2 = (F + 1) >> 1;
+        for (int base = 0; base < F2; base += 8 * kWave) {
+          uint32_t th[8], tl[8];
+#pragma unroll
+          for (int j = 0; j < 8; j++) {
+            const int q = base + lane + j * kWave;
+            th[j] = 0;
+            tl[j] = 0;
+            if (q < F2) {
+              th[j] = xh[q];
+              tl[j] = xl[q];
+            }
+          }
+#pragma unroll
+          for (int j = 0; j < 8; j++) {
+            const int q = base + lane + j * kWave;
+            if (q < F2) {
+              const float v0 = __uint_as_float(th[j] << 16) + __uint_as_float(tl[j] << 16);
+              const float v1 = __uint_as_float(th[j] & 0xffff0000u) +
+                               __uint_as_float(tl[j] & 0xffff0000u);
+              xs[2 * q] = (T)v0;
+              if (2 * q + 1 < F) xs[2 * q + 1] = (T)v1;
+            }
+          }
+        }
+      } else {
+        const T *x = src.X + (int64_t)row * F;
+        stage_lds_vec<4>(F, x, xs);
+      }
     }
     // z (= beta X, or beta_s for power 1) staged into w, turned into w below
     const T *z = WM != 2 ? src.Z + (int64_t)row * F
@@ -619,18 +743,25 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
     } else {
       const double bn = sqrt(n2);
       double d = 0;
-      if constexpr (kFixedShape) {
+      if constexpr (kOverlay) {
         if (x_regs) {
 #pragma unroll
           for (int j = 0; j < kXPairs; j++) {
             const int f = 2 * (lane + j * kWave);
-            if (f < F) d += (double)xv[2 * j] * (double)w[f];
-            if (f + 1 < F) d += (double)xv[2 * j + 1] * (double)w[f + 1];
+            T x0, x1;
+            xpair(j, x0, x1);
+            if (f < F) d += (double)x0 * (double)w[f];
+            if (f + 1 < F) d += (double)x1 * (double)w[f + 1];
           }
+        } else {
+          for_x([&](int f, T x0, T x1) {
+            if (f < F) d += (double)x0 * (double)w[f];
+            if (f + 1 < F) d += (double)x1 * (double)w[f + 1];
+          });
         }
-      }
-      if (!x_regs)
+      } else {
         for (int f = lane; f < F; f += kWave) d += (double)xs[f] * (double)w[f];
+      }
       d = wave_sum(d);
       double E, sx, sz;
       if (WM != 2) {
@@ -642,19 +773,26 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
         sz = 1.0 / bn;
         sx = -E / bn * (src.normalized ? 1.0 : 1.0 / bn);
       }
-      if constexpr (kFixedShape) {
+      if constexpr (kOverlay) {
         if (x_regs) {
 #pragma unroll
           for (int j = 0; j < kXPairs; j++) {
             const int f = 2 * (lane + j * kWave);
-            if (f < F) w[f] = (T)(sz * (double)w[f] + sx * (double)xv[2 * j]);
-            if (f + 1 < F) w[f + 1] = (T)(sz * (double)w[f + 1] + sx * (double)xv[2 * j + 1]);
+            T x0, x1;
+            xpair(j, x0, x1);
+            if (f < F) w[f] = (T)(sz * (double)w[f] + sx * (double)x0);
+            if (f + 1 < F) w[f + 1] = (T)(sz * (double)w[f + 1] + sx * (double)x1);
           }
+        } else {
+          for_x([&](int f, T x0, T x1) {
+            if (f < F) w[f] = (T)(sz * (double)w[f] + sx * (double)x0);
+            if (f + 1 < F) w[f + 1] = (T)(sz * (double)w[f + 1] + sx * (double)x1);
+          });
         }
-      }
-      if (!x_regs)
+      } else {
         for (int f = lane; f < F; f += kWave)
           w[f] = (T)(sz * (double)w[f] + sx * (double)xs[f]);
+      }
       if (lane == 0) src.E_row[row] = E;
     }
   }
@@ -669,7 +807,16 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
     sj_next = nl.nspec[b + lane];
   }
   bool done = false;
-  if constexpr (kUFixed) {
+  if constexpr (kOverlay) {
+    // u over A (p.nr is 8, 16 or 24 here)
+    done = true;
+    if (p.nr == 24)
+      u_mfma_fixed_inplace<T, LFIX, 24>(A, w, nrm_lm);
+    else if (p.nr == 16)
+      u_mfma_fixed_inplace<T, LFIX, 16>(A, w, nrm_lm);
+    else
+      u_mfma_fixed_inplace<T, LFIX, 8>(A, w, nrm_lm);
+  } else if constexpr (kUFixed) {
     done = true;
     if (p.nr == 16)
       u_mfma_fixed<T, LFIX, 16>(A, w, nrm_lm, u);
@@ -934,11 +1081,14 @@ static void launch_forces_t(hipStream_t st, const B2Dev &p, const NListDev &nl,
   // contraction's channel-pair offsets in registers (no ptab)
   const bool fixed_shape = MODE == 1 && p.N == 8 && (p.L == 3 || p.L == 4 || p.L == 6);
   const bool fixed_u = fixed_shape && (p.nr == 8 || p.nr == 16 || p.nr == 24);
-  // and, at l_max 6, hold the X row in registers up to 2,176 features
-  const bool x_regs = MODE == 1 && p.N == 8 && p.L == 6 && p.F <= 2 * 17 * kWave;
-  const size_t lds = sizeof(T) * ((size_t)p.W + (size_t)p.F +
-                                  (size_t)(x_regs ? p.W : std::max(p.W, p.F)) + (size_t)p.H) +
-                     (fixed_u ? 0 : sizeof(int) * (size_t)p.nr * p.nr);
+  // and, at l_max 6, write u over A (LDS A/u, w, norms); l_max 6 with other
+  // channel counts takes the general instance
+  const bool overlay = fixed_u && p.L == 6;
+  const size_t lds =
+      overlay ? sizeof(T) * ((size_t)p.W + (size_t)p.F + (size_t)p.H)
+              : sizeof(T) * ((size_t)p.W + (size_t)p.F + (size_t)std::max(p.W, p.F) +
+                             (size_t)p.H) +
+                    (fixed_u ? 0 : sizeof(int) * (size_t)p.nr * p.nr);
   FLARE_REQUIRE(lds <= 64 * 1024, "descriptor settings too large for the force kernel");
   FLARE_REQUIRE(p.nr <= 32, "at most 32 radial channels (n_species * n_max)");
 #define FLARE_FORCE_LAUNCH(NM, LF, EX)                                        \
@@ -950,7 +1100,7 @@ static void launch_forces_t(hipStream_t st, const B2Dev &p, const NListDev &nl,
     FLARE_FORCE_LAUNCH(8, 3, true);
   else if (MODE == 1 && p.N == 8 && p.L == 4)
     FLARE_FORCE_LAUNCH(8, 4, true);
-  else if (MODE == 1 && p.N == 8 && p.L == 6)
+  else if (overlay)
     FLARE_FORCE_LAUNCH(8, 6, true);
   else if (p.N <= 8)
     FLARE_FORCE_LAUNCH(8, -1, false);
