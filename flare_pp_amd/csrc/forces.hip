@@ -407,8 +407,8 @@ __device__ inline void adjoint_pair_fixed(const B2Dev &p, T r, T x, T y, T z,
 // contraction's channel-pair offsets come from registers.  The l_max 6
 // instances (8, 16 or 24 channels) instead hold A in u's layout and
 // overwrite it with u, and keep the X row in registers (BF16 rows packed, 34
-// registers per lane up to 2,176 features, else read from memory on each of
-// its two uses): A/u[W], w[F], n[H], 13.3 KB in FP32 at F = 2,100, so that a
+// registers per lane; F <= 2,100 there): A/u[W], w[F], n[H], 13.3 KB in FP32
+// at F = 2,100, so that a
 // CU holds twelve wavefronts at 168 registers, FP64 six at 248 (eight and
 // four with a separate u and 18 / 36 KB of LDS).  The kernel's time follows
 // the resident wavefronts: 3.62 / 4.05 / 5.39 ms at eight / seven / six per
@@ -592,10 +592,10 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
   } else {
     const double n2 = src.norm2[row];
     // kOverlay: the X row in registers, features 2 q and 2 q + 1 for
-    // q = lane + j kWave, up to 2,176 features (loaded here, ahead of the z
-    // row); past that it is read from memory on each of its two uses
+    // q = lane + j kWave (loaded here, ahead of the z row): with 8, 16 or 24
+    // channels at l_max 6, F <= 2,100 <= 2 kXPairs kWave
     constexpr int kXPairs = 17;
-    const bool x_regs = kOverlay && F <= 2 * kXPairs * kWave;
+    constexpr bool x_regs = kOverlay;
     // BF16 rows stay packed (hi and lo of a feature pair in two registers,
     // decoded where used); T rows as values
     constexpr bool kPacked = WM == 3;
@@ -610,93 +610,52 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
         x1 = xv[2 * j + 1];
       }
     };
-    if constexpr (kOverlay) {
-      if (x_regs) {
-        if constexpr (kPacked) {
-          const uint32_t *xh =
-              reinterpret_cast<const uint32_t *>(src.xh + (int64_t)row * src.kp);
-          const uint32_t *xl =
-              reinterpret_cast<const uint32_t *>(src.xl + (int64_t)row * src.kp);
-          const int F2 = (F + 1) >> 1;
+    if constexpr (x_regs) {
+      if constexpr (kPacked) {
+        const uint32_t *xh =
+            reinterpret_cast<const uint32_t *>(src.xh + (int64_t)row * src.kp);
+        const uint32_t *xl =
+            reinterpret_cast<const uint32_t *>(src.xl + (int64_t)row * src.kp);
+        const int F2 = (F + 1) >> 1;
+#pragma unroll
+        for (int j = 0; j < kXPairs; j++) {
+          const int q = lane + j * kWave;
+          xh2[j] = 0;
+          xl2[j] = 0;
+          if (q < F2) {
+            xh2[j] = xh[q];
+            xl2[j] = xl[q];
+          }
+        }
+      } else {
+        const T *x = src.X + (int64_t)row * F;
+        if ((F & 1) == 0) {
+          // even rows: each lane's feature pairs as one 2-vector load
+          typedef typename std::conditional<sizeof(T) == 8, double2, float2>::type T2;
+          const T2 *x2 = reinterpret_cast<const T2 *>(x);
+          T2 t2[kXPairs];
 #pragma unroll
           for (int j = 0; j < kXPairs; j++) {
             const int q = lane + j * kWave;
-            xh2[j] = 0;
-            xl2[j] = 0;
-            if (q < F2) {
-              xh2[j] = xh[q];
-              xl2[j] = xl[q];
-            }
+            t2[j].x = T(0);
+            t2[j].y = T(0);
+            if (2 * q < F) t2[j] = x2[q];
+          }
+#pragma unroll
+          for (int j = 0; j < kXPairs; j++) {
+            xv[2 * j] = t2[j].x;
+            xv[2 * j + 1] = t2[j].y;
           }
         } else {
-          const T *x = src.X + (int64_t)row * F;
-          if ((F & 1) == 0) {
-            // even rows: each lane's feature pairs as one 2-vector load
-            typedef typename std::conditional<sizeof(T) == 8, double2, float2>::type T2;
-            const T2 *x2 = reinterpret_cast<const T2 *>(x);
-            T2 t2[kXPairs];
 #pragma unroll
-            for (int j = 0; j < kXPairs; j++) {
-              const int q = lane + j * kWave;
-              t2[j].x = T(0);
-              t2[j].y = T(0);
-              if (2 * q < F) t2[j] = x2[q];
-            }
-#pragma unroll
-            for (int j = 0; j < kXPairs; j++) {
-              xv[2 * j] = t2[j].x;
-              xv[2 * j + 1] = t2[j].y;
-            }
-          } else {
-#pragma unroll
-            for (int j = 0; j < kXPairs; j++) {
-              const int f = 2 * (lane + j * kWave);
-              xv[2 * j] = f < F ? x[f] : T(0);
-              xv[2 * j + 1] = f + 1 < F ? x[f + 1] : T(0);
-            }
+          for (int j = 0; j < kXPairs; j++) {
+            const int f = 2 * (lane + j * kWave);
+            xv[2 * j] = f < F ? x[f] : T(0);
+            xv[2 * j + 1] = f + 1 < F ? x[f + 1] : T(0);
           }
         }
       }
     }
-    // the same features from memory, eight pairs at a time, passed to
-    // fn(f, x_f, x_f+1) in the same order as the registers' loop below
-    auto for_x = [&](auto &&fn) {
-      const int F2 = (F + 1) >> 1;
-      for (int base = 0; base < F2; base += 8 * kWave) {
-        T v0[8], v1[8];
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-          const int q = base + lane + j * kWave;
-          v0[j] = v1[j] = T(0);
-          if (q < F2) {
-            if (WM == 3) {
-              // pairs of BF16 values: rows are padded to an even length (kp)
-              const uint32_t th = reinterpret_cast<const uint32_t *>(
-                  src.xh + (int64_t)row * src.kp)[q];
-              const uint32_t tl = reinterpret_cast<const uint32_t *>(
-                  src.xl + (int64_t)row * src.kp)[q];
-              v0[j] = (T)(__uint_as_float(th << 16) + __uint_as_float(tl << 16));
-              v1[j] = (T)(__uint_as_float(th & 0xffff0000u) +
-                          __uint_as_float(tl & 0xffff0000u));
-            } else if ((F & 1) == 0) {
-              typedef typename std::conditional<sizeof(T) == 8, double2, float2>::type T2;
-              const T2 t2 = reinterpret_cast<const T2 *>(src.X + (int64_t)row * F)[q];
-              v0[j] = t2.x;
-              v1[j] = t2.y;
-            } else {
-              const T *x = src.X + (int64_t)row * F;
-              v0[j] = x[2 * q];
-              if (2 * q + 1 < F) v1[j] = x[2 * q + 1];
-            }
-          }
-        }
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-          const int q = base + lane + j * kWave;
-          if (q < F2) fn(2 * q, v0[j], v1[j]);
-        }
-      }
-    };
     if constexpr (!kOverlay) {
       if (WM == 3) {
         // pairs of BF16 values: rows are padded to an even length (kp)
@@ -744,20 +703,13 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
       const double bn = sqrt(n2);
       double d = 0;
       if constexpr (kOverlay) {
-        if (x_regs) {
 #pragma unroll
-          for (int j = 0; j < kXPairs; j++) {
-            const int f = 2 * (lane + j * kWave);
-            T x0, x1;
-            xpair(j, x0, x1);
-            if (f < F) d += (double)x0 * (double)w[f];
-            if (f + 1 < F) d += (double)x1 * (double)w[f + 1];
-          }
-        } else {
-          for_x([&](int f, T x0, T x1) {
-            if (f < F) d += (double)x0 * (double)w[f];
-            if (f + 1 < F) d += (double)x1 * (double)w[f + 1];
-          });
+        for (int j = 0; j < kXPairs; j++) {
+          const int f = 2 * (lane + j * kWave);
+          T x0, x1;
+          xpair(j, x0, x1);
+          if (f < F) d += (double)x0 * (double)w[f];
+          if (f + 1 < F) d += (double)x1 * (double)w[f + 1];
         }
       } else {
         for (int f = lane; f < F; f += kWave) d += (double)xs[f] * (double)w[f];
@@ -774,20 +726,13 @@ force_kernel(B2Dev p, NListDev nl, const int *__restrict__ species,
         sx = -E / bn * (src.normalized ? 1.0 : 1.0 / bn);
       }
       if constexpr (kOverlay) {
-        if (x_regs) {
 #pragma unroll
-          for (int j = 0; j < kXPairs; j++) {
-            const int f = 2 * (lane + j * kWave);
-            T x0, x1;
-            xpair(j, x0, x1);
-            if (f < F) w[f] = (T)(sz * (double)w[f] + sx * (double)x0);
-            if (f + 1 < F) w[f + 1] = (T)(sz * (double)w[f + 1] + sx * (double)x1);
-          }
-        } else {
-          for_x([&](int f, T x0, T x1) {
-            if (f < F) w[f] = (T)(sz * (double)w[f] + sx * (double)x0);
-            if (f + 1 < F) w[f + 1] = (T)(sz * (double)w[f + 1] + sx * (double)x1);
-          });
+        for (int j = 0; j < kXPairs; j++) {
+          const int f = 2 * (lane + j * kWave);
+          T x0, x1;
+          xpair(j, x0, x1);
+          if (f < F) w[f] = (T)(sz * (double)w[f] + sx * (double)x0);
+          if (f + 1 < F) w[f + 1] = (T)(sz * (double)w[f + 1] + sx * (double)x1);
         }
       } else {
         for (int f = lane; f < F; f += kWave)

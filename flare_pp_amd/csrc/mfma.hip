@@ -1016,7 +1016,8 @@ static void gemm_x3_wide(hipStream_t st, int nb, const GemmProduct *prods, const
     const GemmProduct &g = prods[q];
     FLARE_REQUIRE(g.K >= kWK && g.K % kWK == 0, "gemm_x3w: K must be padded to 32");
     const int64_t ld = g.ld_ab ? g.ld_ab : g.K;
-    FLARE_REQUIRE(ld >= g.K && ld % 8 == 0, "gemm_x3w: row stride below K or unaligned");
+    FLARE_REQUIRE(ld >= g.K && ld % 8 == 0 && g.a_off % 8 == 0 && g.b_off % 8 == 0,
+                  "gemm_x3w: row stride below K, or operand rows not 16-byte aligned");
     // 32-bit buffer offsets and record counts
     FLARE_REQUIRE((int64_t)g.M * ld * 2 <= (int64_t)UINT32_MAX &&
                       (int64_t)g.N * ld * 2 <= (int64_t)UINT32_MAX,
