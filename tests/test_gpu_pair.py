@@ -180,6 +180,28 @@ def test_config5_settings_small_box(orc):
     check_low_precision(pair, cell, species, pos, e0, f0, v0, ea0)
 
 
+@pytest.mark.parametrize("S", [1, 2])
+def test_lmax6_fewer_channels(orc, S):
+    """The l_max 6 force kernel that writes u over A with 8 and 16 channels
+    (B2 {1, 8, 6}, F = 252; B2 {2, 8, 6}, F = 952) on a 300-atom
+    random-sequential-addition box: every form against the oracle."""
+    n = 300
+    box = (n / 0.06) ** (1.0 / 3.0)
+    pos = synth.rsa_positions(n, box, 2.2, seed=60 + S)
+    species = np.random.default_rng(62 + S).integers(0, S, n).astype(np.int32)
+    cell = np.eye(3) * box
+    F = (S * 8) * (S * 8 + 1) // 2 * 7
+    beta = synth.symmetric_beta(S, F, 1e-4, 64 + S)
+    pair = PairFLARE.from_coefficients(2, S, 8, 6, np.full((S, S), 5.5),
+                                       np.stack([pack_beta(b) for b in beta]))
+    e0, f0, v0, ea0 = oracle_pair(orc, pair, cell, species, pos)
+    for form in ("plain", "adjoint_fp64"):
+        e, f, v, ea = pair.compute(cell, species, pos, form=form, want_eatom=True)
+        assert abs(e - e0) <= 1e-10 * np.abs(ea0).sum(), form
+        assert nrel(f, f0) < 1e-10 and nrel(v, v0) < 1e-10 and nrel(ea, ea0) < 1e-10, form
+    check_low_precision(pair, cell, species, pos, e0, f0, v0, ea0)
+
+
 def test_dense_environments_mixed(orc):
     """About 190 neighbors per atom (rc 9 A in fcc at 4.05 A; the two-pass
     neighbor build): the benchmark's B2 {2, 8, 4} kernels in every form
