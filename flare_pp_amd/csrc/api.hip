@@ -1638,11 +1638,14 @@ static PairEval pair_eval(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
         flare::gemm_bf16x3_batched(ctx->stream, p.S, g, fw.xh, fw.xl, bh, bl,
                                    (float *)Z);
     }
-    // FP64 and FP32: the hand-written matrix-core GEMM where it was measured
-    // faster than rocBLAS (F = 680 at 32,000 atoms: FP64 0.575 vs 0.636 ms,
-    // FP32 0.296 vs 0.413 ms); rocBLAS for wide rows (F = 2,100 at 262,144
-    // atoms: FP64 35.3 vs 39.7 ms, FP32 18.6 vs 19.9 ms)
-    const bool mcore = !mixed && F <= 1024 &&
+    // FP64 and FP32: the hand-written matrix-core GEMMs where they were
+    // measured faster than rocBLAS (F = 680 at 32,000 atoms: FP64 0.575 vs
+    // 0.636 ms, FP32 0.25 on 256 x 352 tiles, 0.296 on 128 x 128, 0.413 in
+    // rocBLAS); FP32 wide rows on the 256 x 352 tiles at any F (F = 2,100 at
+    // 262,144 atoms: as fast as rocBLAS), FP64 rows past 1,024 on rocBLAS
+    // (35.3 vs 39.7 ms there)
+    const bool f32_wide = !mixed && prec == flare::kPrecF32 && F > 256 && F % 4 == 0;
+    const bool mcore = !mixed && (F <= 1024 || f32_wide) &&
                        (prec == flare::kPrecF64 ? F % 2 == 0 : F % 4 == 0);
     if (mcore) {  // all species in one launch
       flare::GemmProduct g[flare::kMaxSpecies];
@@ -1652,6 +1655,9 @@ static PairEval pair_eval(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
       if (prec == flare::kPrecF64)
         flare::dgemm_batched(ctx->stream, p.S, g, (const double *)X, (const double *)beta,
                              (double *)Z);
+      else if (f32_wide)
+        flare::sgemm_wide(ctx->stream, p.S, g, (const float *)X, (const float *)beta,
+                          (float *)Z);
       else
         flare::sgemm_batched(ctx->stream, p.S, g, (const float *)X, (const float *)beta,
                              (float *)Z);

@@ -48,6 +48,10 @@ int gemm_x3_wide_tiles(int nb, const GemmProduct *prods);
 // B = K, rows a multiple of 16 bytes; no hi/lo parts).
 void dgemm_batched(hipStream_t st, int nb, const GemmProduct *prods, const double *A,
                    const double *B, double *Z);
+// FP32 on 256 x 352 tiles for wide rows (K a multiple of 4, 16-byte aligned
+// rows of K values; no scales or strides)
+void sgemm_wide(hipStream_t st, int nb, const GemmProduct *prods, const float *A,
+                const float *B, float *Z);
 void sgemm_batched(hipStream_t st, int nb, const GemmProduct *prods, const float *A,
                    const float *B, float *Z);
 // Fused epilogues of the FP64 / FP32 matrix-core product Z = A B^T for the

@@ -1046,6 +1046,198 @@ void gemm_f16x3_wide(hipStream_t st, int nb, const GemmProduct *prods, const voi
   gemm_x3_wide<_Float16>(st, nb, prods, Ah, Al, Bh, Bl, Z);
 }
 
+// ---------------------------------------------------------------------------
+// FP32 wide-row GEMM Z = A B^T (the FP32 form's mapped contraction Z = X beta,
+// beta symmetric) on v_mfma_f32_16x16x4_f32: the wide kernel's tile (256 x
+// 352, eight wavefronts of 64 x 176, 176 accumulators each) with K steps of
+// 16 values (64-byte LDS rows, the same swizzle), two stages of 38 KB filled
+// by buffer loads (A 16 runs, B 22, up to five loads per wavefront per step,
+// issued one per column over the first five columns of the step before).
+// A lane group g holds chunk g (values 4 g .. 4 g + 3) of a K step, and MFMA t
+// of a block takes value 4 g + t of both operands, so each fragment is one
+// 16-byte read.  Rows are K values (K a multiple of 4, row starts 16-byte
+// aligned); the last step's chunks past K, which read the next row or past
+// the buffer (zeros), are zeroed in both operands' fragments.
+constexpr int kFK = 16;                               // K step (values)
+constexpr int kFStage = (kWM + kWN) * kFK * 4;        // 38,912
+constexpr int kFPerA = kWRunsA / kWaves, kFPerB = kWPerB;  // 2, 3 runs per wavefront
+constexpr int kFLoads = kFPerA + kFPerB;
+static_assert(kFLoads <= kFragN, "every load falls in a column of the step");
+
+struct F32Offsets {
+  uint32_t a[kFPerA], b[kFPerB];
+};
+
+__device__ inline F32Offsets f32w_offsets(int M, int N, int K, int m0, int n0, int wv,
+                                          int lane) {
+  F32Offsets o;
+#pragma unroll
+  for (int t = 0; t < kFPerA; t++) {
+    const int q = (wv + kWaves * t) * 64 + lane;
+    const int row = q >> 2, kc = (q & 3) ^ ((row >> 2) & 3);
+    o.a[t] = (uint32_t)(min(m0 + row, M - 1) * K + kc * 4) * 4u;
+  }
+#pragma unroll
+  for (int t = 0; t < kFPerB; t++) {
+    const int run = min(wv + kWaves * t, kWRunsB - 1);
+    const int q = run * 64 + lane;
+    const int row = q >> 2, kc = (q & 3) ^ ((row >> 2) & 3);
+    o.b[t] = (uint32_t)(min(n0 + row, N - 1) * K + kc * 4) * 4u;
+  }
+  return o;
+}
+
+// load q (0 .. kFLoads - 1) of this wavefront's share of a stage
+__device__ inline void f32w_piece(int q, unsigned char *stage, const F32Offsets &o,
+                                  __amdgpu_buffer_rsrc_t ra, __amdgpu_buffer_rsrc_t rb,
+                                  int kb, int wv) {
+  unsigned char *sA = stage, *sB = stage + kWM * kFK * 4;
+  if (q < kFPerA) {
+    const int run = wv + kWaves * q;
+    lds_buf16(ra, o.a[q], kb, sA + run * 1024);
+  } else if (q < kFLoads) {
+    const int t = q - kFPerA, run = wv + kWaves * t;
+    if (run < kWRunsB) lds_buf16(rb, o.b[t], kb, sB + run * 1024);
+  }
+}
+
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2)))
+gemm_f32w_kernel(GemmBatch bt, const float *__restrict__ A0, const float *__restrict__ B0,
+                 float *__restrict__ Z0) {
+  constexpr int kEpiBytes = kWaves * 16 * kWEpiStride * 4;
+  __shared__ __align__(16) unsigned char smem[2 * kFStage > kEpiBytes ? 2 * kFStage : kEpiBytes];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wm = (wv >> 1) * kWaveM, wn = (wv & 1) * kWaveN;
+  int T, dummy;
+  tile_of_block(bt.tile0[bt.nb], 1, T, dummy);
+  if (T >= bt.tile0[bt.nb]) return;
+  int q = 0;
+  while (T >= bt.tile0[q + 1]) q++;
+  const GemmProduct &gp = bt.g[q];
+  const int M = gp.M, N = gp.N, K = gp.K, ldz = gp.ldz;
+  const int nct = (N + kWN - 1) / kWN;
+  const int local = T - bt.tile0[q];
+  const int rt = local / nct, ct = local - rt * nct;
+  const float *A = A0 + gp.a_off, *B = B0 + gp.b_off;
+  float *Z = Z0 + gp.z_off;
+  const int m0 = rt * kWM, n0 = ct * kWN;
+  const int steps = (K + kFK - 1) / kFK;
+  const int tail = (K % kFK) / 4;  // chunks of the last step inside K (0: all four)
+  floatx4 acc[kFragM][kFragN];
+#pragma unroll
+  for (int i = 0; i < kFragM; i++)
+#pragma unroll
+    for (int j = 0; j < kFragN; j++) acc[i][j] = floatx4{0, 0, 0, 0};
+  const int fr = lane & 15, g = lane >> 4;
+  constexpr int kRow = kFK * 4;  // LDS row bytes (64)
+  const int offA = (wm + fr) * kRow + ((g ^ (((wm + fr) >> 2) & 3)) << 4);
+  const int offB = (wn + fr) * kRow + ((g ^ (((wn + fr) >> 2) & 3)) << 4);
+  const F32Offsets lo = f32w_offsets(M, N, K, m0, n0, wv, lane);
+  const __amdgpu_buffer_rsrc_t ra = wide_rsrc(A, (int64_t)M * K * 4);
+  const __amdgpu_buffer_rsrc_t rb = wide_rsrc(B, (int64_t)N * K * 4);
+#pragma unroll
+  for (int p = 0; p < kFLoads; p++) f32w_piece(p, smem, lo, ra, rb, 0, wv);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  for (int s = 0; s < steps; s++) {
+    const unsigned char *cur = smem + (s & 1) * kFStage;
+    unsigned char *nst = smem + ((s + 1) & 1) * kFStage;
+    const bool more = s + 1 < steps;
+    const int kb = (s + 1) * kRow;
+    const unsigned char *sA = cur, *sB = cur + kWM * kRow;
+    const bool zero = tail && s == steps - 1 && g >= tail;
+    floatx4 a[kFragM];
+#pragma unroll
+    for (int i = 0; i < kFragM; i++) {
+      a[i] = *reinterpret_cast<const floatx4 *>(sA + offA + i * 16 * kRow);
+      if (zero) a[i] = floatx4{0, 0, 0, 0};
+    }
+    floatx4 b = *reinterpret_cast<const floatx4 *>(sB + offB);
+    if (zero) b = floatx4{0, 0, 0, 0};
+#pragma unroll
+    for (int j = 0; j < kFragN; j++) {
+      floatx4 nb = b;
+      if (more && j < kFLoads) f32w_piece(j, nst, lo, ra, rb, kb, wv);
+      if (j + 1 < kFragN) {
+        nb = *reinterpret_cast<const floatx4 *>(sB + offB + (j + 1) * 16 * kRow);
+        if (zero) nb = floatx4{0, 0, 0, 0};
+      }
+#pragma unroll
+      for (int t = 0; t < 4; t++)
+#pragma unroll
+        for (int i = 0; i < kFragM; i++)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i][t], b[t], acc[i][j], 0, 0, 0);
+      b = nb;
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+  }
+  // Z through LDS, 16 rows of the wavefront tile at a time
+  float *stage = reinterpret_cast<float *>(smem) + wv * 16 * kWEpiStride;
+  const int dcol = lane & 15, drow = (lane >> 4) * 4;
+  const bool vec4 = (N & 3) == 0 && (ldz & 3) == 0 && (gp.z_off & 3) == 0;
+#pragma unroll
+  for (int i = 0; i < kFragM; i++) {
+#pragma unroll
+    for (int j = 0; j < kFragN; j++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) stage[(drow + e) * kWEpiStride + 16 * j + dcol] = acc[i][j][e];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    constexpr int kVec = kWaveN / 4;
+#pragma unroll
+    for (int t = 0; t < 16 * kVec / 64; t++) {
+      const int f = lane + 64 * t;
+      const int r = f / kVec, c = (f - r * kVec) * 4;
+      const int row = m0 + wm + 16 * i + r, col = n0 + wn + c;
+      const float4 v = *reinterpret_cast<const float4 *>(&stage[r * kWEpiStride + c]);
+      if (row < M) {
+        float *dst = Z + (int64_t)row * ldz + col;
+        if (vec4) {
+          if (col < N) *reinterpret_cast<float4 *>(dst) = v;
+        } else {
+          if (col < N) dst[0] = v.x;
+          if (col + 1 < N) dst[1] = v.y;
+          if (col + 2 < N) dst[2] = v.z;
+          if (col + 3 < N) dst[3] = v.w;
+        }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+void sgemm_wide(hipStream_t st, int nb, const GemmProduct *prods, const float *A,
+                const float *B, float *Z) {
+  FLARE_REQUIRE(nb >= 0 && nb <= kGemmMaxBatch, "sgemm_wide: at most 8 products");
+  GemmBatch bt;
+  bt.nb = 0;
+  int tiles = 0;
+  for (int q = 0; q < nb; q++) {
+    const GemmProduct &g = prods[q];
+    FLARE_REQUIRE(g.K >= 4 && g.K % 4 == 0 && g.a_off % 4 == 0 && g.b_off % 4 == 0,
+                  "sgemm_wide: rows of a multiple of 4 values, 16-byte aligned");
+    FLARE_REQUIRE(!g.ld_ab && !g.rs && !g.cs, "sgemm_wide: plain products only");
+    FLARE_REQUIRE((int64_t)g.M * g.K * 4 <= (int64_t)UINT32_MAX &&
+                      (int64_t)g.N * g.K * 4 <= (int64_t)UINT32_MAX,
+                  "sgemm_wide: an operand block of one product exceeds 4 GiB");
+    if (g.M <= 0 || g.N <= 0) continue;
+    bt.g[bt.nb] = g;
+    bt.tile0[bt.nb] = tiles;
+    tiles += (g.M + kWM - 1) / kWM * ((g.N + kWN - 1) / kWN);
+    bt.nb++;
+  }
+  bt.tile0[bt.nb] = tiles;
+  if (tiles == 0) return;
+  const int grid = (tiles + 7) / 8 * 8;
+  hipLaunchKernelGGL(gemm_f32w_kernel, dim3(grid), dim3(64 * kWaves), 0, st, bt, A, B, Z);
+  FLARE_HIP_CHECK(hipGetLastError());
+}
+
 // FP16 hi/lo split of rows with a power-of-two scale per row: the row's
 // largest |x| is brought into [0.5, 1); inv_scale[r] = 2^-e.  One wavefront
 // per row; X is FP64 or FP32 (row stride ldx), out rows of kp (zero padded).
