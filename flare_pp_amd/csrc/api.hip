@@ -219,6 +219,18 @@ struct flare_ctx {
   }
 };
 
+// The wide-row GEMMs pad N to 352-column tiles, the batched ones to `unit`
+// columns (256 for BF16/FP16 x 3, 128 for FP32). The wide tiles are taken
+// when they pad no more than `slack_pct` percent beyond what the batched
+// kernel pads: measured at F = 680 (704 vs 768 columns, wide faster in every
+// form) and F = 1,200 (1,408 vs 1,280: mixed still faster on wide tiles,
+// 391 vs 436 us; FP32 slower, 1.10 vs 0.94 ms).
+static bool wide_cols_fit(int F, int unit, int slack_pct) {
+  if (F <= 256) return false;
+  const int64_t wide = (F + 351) / 352 * 352, narrow = (F + unit - 1) / unit * unit;
+  return 100 * wide <= (100 + slack_pct) * narrow;
+}
+
 static void gemm_rm(flare_ctx *c, bool ta, bool tb, int M, int N, int K,
                     double alpha, const double *A, int lda, const double *B,
                     int ldb, double beta, double *C, int ldc) {
@@ -1632,7 +1644,7 @@ static PairEval pair_eval(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
                                   (int64_t)off[s] * kp, (int64_t)s * F * kp,
                                   (int64_t)off[s] * F};
       // wide rows: 256 x 352 tiles (one round of workgroups at F = 680)
-      if (F > 256)
+      if (wide_cols_fit(F, 256, 10))
         flare::gemm_bf16x3_wide(ctx->stream, p.S, g, fw.xh, fw.xl, bh, bl, (float *)Z);
       else
         flare::gemm_bf16x3_batched(ctx->stream, p.S, g, fw.xh, fw.xl, bh, bl,
@@ -1641,10 +1653,12 @@ static PairEval pair_eval(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
     // FP64 and FP32: the hand-written matrix-core GEMMs where they were
     // measured faster than rocBLAS (F = 680 at 32,000 atoms: FP64 0.575 vs
     // 0.636 ms, FP32 0.25 on 256 x 352 tiles, 0.296 on 128 x 128, 0.413 in
-    // rocBLAS); FP32 wide rows on the 256 x 352 tiles at any F (F = 2,100 at
-    // 262,144 atoms: as fast as rocBLAS), FP64 rows past 1,024 on rocBLAS
+    // rocBLAS); FP32 wide rows on the 256 x 352 tiles past 1,024 columns
+    // (F = 2,100 at 262,144 atoms: as fast as rocBLAS) and below where they
+    // pad no more than the 128 x 128 tiles, FP64 rows past 1,024 on rocBLAS
     // (35.3 vs 39.7 ms there)
-    const bool f32_wide = !mixed && prec == flare::kPrecF32 && F > 256 && F % 4 == 0;
+    const bool f32_wide = !mixed && prec == flare::kPrecF32 && F % 4 == 0 &&
+                          (F > 1024 || wide_cols_fit(F, 128, 0));
     const bool mcore = !mixed && (F <= 1024 || f32_wide) &&
                        (prec == flare::kPrecF64 ? F % 2 == 0 : F % 4 == 0);
     if (mcore) {  // all species in one launch
@@ -2369,22 +2383,20 @@ static GPState gp_mean_lowp(flare_ctx *ctx, flare_sgp *gp, const flare::B2Dev &p
     }
     // wide rows on 256 x 352 tiles (F = 680: 58 vs 85 us; F = 1,200: 391
     // vs 436 us)
-    if (F > 256)
+    if (wide_cols_fit(F, 256, 10))
       flare::gemm_f16x3_wide(ctx->stream, p.S, g, ch, cl, gp->PhatT_h.ptr, gp->PhatT_l.ptr, G);
     else
       flare::gemm_f16x3_batched(ctx->stream, p.S, g, ch, cl, gp->PhatT_h.ptr,
                                 gp->PhatT_l.ptr, G);
   } else {
     // G = c P_hat on the matrix cores: c rows of ldk against P_hat^T rows;
-    // 256 x 352 tiles where their column padding stays within 10 % (F = 680:
-    // 0.128 -> 0.111 ms at 32k atoms; F = 1200 pads to 1408 and is faster on
-    // 128-wide tiles)
+    // 256 x 352 tiles where they pad no more than 128 x 128 tiles (F = 680:
+    // 0.128 -> 0.111 ms at 32k atoms)
     flare::GemmProduct g[flare::kMaxSpecies];
     for (int s = 0; s < p.S; s++)
       g[s] = flare::GemmProduct{off[s + 1] - off[s], F, gp->ldk[s], F,
                                 (int64_t)st.koff[s], gp->toff[s], (int64_t)off[s] * F};
-    const int wide_cols = (F + 351) / 352 * 352;
-    if (F > 256 && F % 4 == 0 && 10 * wide_cols <= 11 * F)
+    if (F % 4 == 0 && wide_cols_fit(F, 128, 0))
       flare::sgemm_wide(ctx->stream, p.S, g, cf, gp->PhatT_f.as<float>(), G);
     else
       flare::sgemm_batched(ctx->stream, p.S, g, cf, gp->PhatT_f.as<float>(), G);
