@@ -1771,12 +1771,15 @@ int flare_pair_compute_noalloc(flare_ctx *ctx, flare_pair *pr, const flare_nlist
                   "(flare_nlist_get_view or flare_pair_compute)");
     }
     // rocBLAS manages device workspace of its own: only the forms whose
-    // contraction is a hand-written kernel are served here
+    // contraction is a hand-written kernel are served here (FP64 up to
+    // F = 1,024, FP32 with F a multiple of 4, mixed at any F)
     if (pr->power == 2 && form != FLARE_FORM_ADJOINT_MIXED) {
       const bool f64 = form == FLARE_FORM_PLAIN_FP64 || form == FLARE_FORM_ADJOINT_FP64;
-      REQUIRE(pr->F <= 1024 && pr->F % (f64 ? 2 : 4) == 0,
-              "the no-allocation entry point needs F <= 1024 and F a multiple of " +
-                  std::string(f64 ? "2" : "4") + " in this form (F = " +
+      REQUIRE(f64 ? pr->F <= 1024 && pr->F % 2 == 0 : pr->F % 4 == 0,
+              std::string(f64 ? "the no-allocation entry point needs F <= 1024 and F a "
+                                "multiple of 2 in this form (F = "
+                              : "the no-allocation entry point needs F a multiple of 4 "
+                                "in this form (F = ") +
                   std::to_string(pr->F) + ")");
     }
     flare_ctx::Arena arena;

@@ -207,7 +207,7 @@ int flare_pair_compute(flare_ctx *ctx, flare_pair *pr, const flare_nlist *nl,
  * process ends.  The call makes no hipMalloc / hipFree and no rocBLAS call;
  * it returns FLARE_EINVAL, before any device work, for
  *  - FP64 / FP32 power-2 forms whose contraction would need rocBLAS
- *    (F > 1024, or F not a multiple of 2 (FP64) / 4 (FP32));
+ *    (FP64 with F > 1024 or F odd; FP32 with F not a multiple of 4);
  *  - a pending neighbor list with atoms over the one-pass row capacity
  *    (complete it with flare_nlist_get_view first);
  * and, once the workspace outgrows the array, for inputs that are too large

@@ -91,3 +91,40 @@ def test_noalloc_refuses_input_beyond_scratch():
     out = run(pair, spc, "adjoint_mixed", noalloc=True)
     for a, b in zip(out, ref):
         assert np.array_equal(a, b)
+
+
+def test_noalloc_past_1024_columns():
+    """F = 1,200 (3 species, n_max 8, l_max 3): the FP32 and mixed forms run
+    their contractions on the hand-written kernels and are served; the FP64
+    forms would need rocBLAS and are refused before any device work."""
+    S, N, L = 3, 8, 3
+    F = (S * N) * (S * N + 1) // 2 * (L + 1)
+    beta = synth.symmetric_beta(S, F, 1e-3, seed=12)
+    packed = np.stack([pack_beta(b) for b in beta])
+    pair = PairFLARE.from_coefficients(2, S, N, L, np.full((S, S), 5.0), packed)
+    assert pair.n_descriptors == 1200
+    for n_cells, seed in ((3, 25), (2, 26)):
+        n = 4 * n_cells ** 3
+        cell, species, pos = synth.fcc(n_cells, 4.05, 0.08,
+                                       [n - 2 * (n // 3), n // 3, n // 3], seed)
+        pos = torch.tensor(pos, device=DEV)
+        spc = torch.tensor(species, dtype=torch.int32, device=DEV)
+        pair.build_neighbors(cell, pos, spc)
+        for form in ("adjoint_fp32", "adjoint_mixed"):
+            print(f"noalloc vs compute at F = {F}: {form}, {spc.numel()} atoms", flush=True)
+            ref = run(pair, spc, form, noalloc=False)
+            out = run(pair, spc, form, noalloc=True)
+            for a, b, name in zip(out, ref, ("energy", "forces", "virial", "eatom")):
+                assert np.all(np.isfinite(a)), (form, name)
+                assert np.array_equal(a, b), (form, name, np.abs(a - b).max())
+    n = spc.numel()
+    f = torch.zeros((n, 3), dtype=torch.float64, device=DEV)
+    e = torch.zeros(1, dtype=torch.float64, device=DEV)
+    rc = _capi.lib().flare_pair_compute_noalloc(
+        _capi.context(0).handle, pair._h, pair._nlist, _capi.dptr(spc),
+        _capi.FORMS["adjoint_fp64"], None, _capi.dptr(e), _capi.dptr(f), None, None)
+    msg = _capi.lib().flare_last_error().decode()
+    print("refusal:", rc, msg, flush=True)
+    assert rc == 1 and "F <= 1024" in msg
+    torch.cuda.synchronize()
+    assert float(f.abs().max()) == 0.0
