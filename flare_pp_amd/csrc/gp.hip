@@ -645,12 +645,51 @@ bool launch_gp_epilogue_lp_rows(hipStream_t st, int nsp, const GpEpiLp *sp, doub
   return true;
 }
 
+// gp_w_lp_kernel for rows of F % 4 == 0 floats on 16-byte boundaries: one
+// wavefront per row, four rows per workgroup, G, d and w moved as float4;
+// the same arithmetic per element, so the same bits.
+__global__ void __launch_bounds__(256)
+gp_w_lp4_kernel(int rows, int F, const float *__restrict__ G,
+                const double *__restrict__ e_part, int nparts, const float *__restrict__ d,
+                const double *__restrict__ norm2, double sig2, double power,
+                float *__restrict__ w, double *__restrict__ E_row) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int r = blockIdx.x * 4 + (int)(threadIdx.x / kWave);
+  if (r >= rows) return;
+  const double nb = sqrt(norm2[r]);
+  const bool empty = nb < 1e-8;
+  double e = 0;
+  for (int q = 0; q < nparts; q++) e += e_part[(int64_t)r * nparts + q];
+  const double s = empty ? 0.0 : sig2 * power / nb;
+  const int F4 = F / 4;
+  const float4 *g4 = reinterpret_cast<const float4 *>(G + (int64_t)r * F);
+  const float4 *d4 = reinterpret_cast<const float4 *>(d + (int64_t)r * F);
+  float4 *w4 = reinterpret_cast<float4 *>(w + (int64_t)r * F);
+  for (int f = lane; f < F4; f += kWave) {
+    float4 o = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (!empty) {
+      const float4 g = g4[f], x = d4[f];
+      o.x = (float)(s * ((double)g.x - e * (double)x.x));
+      o.y = (float)(s * ((double)g.y - e * (double)x.y));
+      o.z = (float)(s * ((double)g.z - e * (double)x.z));
+      o.w = (float)(s * ((double)g.w - e * (double)x.w));
+    }
+    w4[f] = o;
+  }
+  if (lane == 0) E_row[r] = empty ? 0.0 : sig2 * e;
+}
+
 void launch_gp_w_lp(hipStream_t st, int rows, int F, const float *G,
                     const double *e_part, int nparts, const float *d, const double *norm2,
                     double sig2, double power, float *w, double *E_row) {
   if (rows == 0) return;
-  hipLaunchKernelGGL(gp_w_lp_kernel, dim3(rows), dim3(kWave), 0, st, rows, F, G,
-                     e_part, nparts, d, norm2, sig2, power, w, E_row);
+  const bool v4 = F % 4 == 0 && ((uintptr_t)G | (uintptr_t)d | (uintptr_t)w) % 16 == 0;
+  if (v4)
+    hipLaunchKernelGGL(gp_w_lp4_kernel, dim3((rows + 3) / 4), dim3(4 * kWave), 0, st, rows,
+                       F, G, e_part, nparts, d, norm2, sig2, power, w, E_row);
+  else
+    hipLaunchKernelGGL(gp_w_lp_kernel, dim3(rows), dim3(kWave), 0, st, rows, F, G,
+                       e_part, nparts, d, norm2, sig2, power, w, E_row);
   FLARE_HIP_CHECK(hipGetLastError());
 }
 
