@@ -55,6 +55,38 @@ gp_epilogue_kernel(int rows, int M, int ld, int normalized, const double *__rest
   if (lane == 0 && e_row) e_row[r] = e;
 }
 
+// gp_w_kernel for rows of an even F on 16-byte boundaries: one wavefront per
+// row, four rows per workgroup, G, B and w moved as double2; the same
+// arithmetic per element.
+__global__ void __launch_bounds__(256)
+gp_w2_kernel(int rows, int F, const double *__restrict__ G,
+             const double *__restrict__ e_part, int nparts, const double *__restrict__ B,
+             const double *__restrict__ norm2, double sig2, double power,
+             double *__restrict__ w, double *__restrict__ E_row) {
+  const int lane = threadIdx.x & (kWave - 1);
+  const int r = blockIdx.x * 4 + (int)(threadIdx.x / kWave);
+  if (r >= rows) return;
+  const double nb = sqrt(norm2[r]);
+  const bool empty = nb < 1e-8;
+  double e = 0;
+  for (int q = 0; q < nparts; q++) e += e_part[(int64_t)r * nparts + q];
+  const double s = empty ? 0.0 : sig2 * power / nb;
+  const int F2 = F / 2;
+  const double2 *g2 = reinterpret_cast<const double2 *>(G + (int64_t)r * F);
+  const double2 *b2 = reinterpret_cast<const double2 *>(B + (int64_t)r * F);
+  double2 *w2 = reinterpret_cast<double2 *>(w + (int64_t)r * F);
+  for (int f = lane; f < F2; f += kWave) {
+    double2 o = {0.0, 0.0};
+    if (!empty) {
+      const double2 g = g2[f], x = b2[f];
+      o.x = s * (g.x - e * x.x / nb);
+      o.y = s * (g.y - e * x.y / nb);
+    }
+    w2[f] = o;
+  }
+  if (lane == 0) E_row[r] = empty ? 0.0 : sig2 * e;
+}
+
 __global__ void __launch_bounds__(64)
 gp_w_kernel(int rows, int F, const double *__restrict__ G,
             const double *__restrict__ e_part, int nparts, const double *__restrict__ B,
@@ -423,8 +455,13 @@ void launch_gp_w(hipStream_t st, int rows, int F, const double *G,
                  const double *e_part, int nparts, const double *B, const double *norm2,
                  double sig2, double power, double *w, double *E_row) {
   if (rows == 0) return;
-  hipLaunchKernelGGL(gp_w_kernel, dim3(rows), dim3(kWave), 0, st, rows, F, G,
-                     e_part, nparts, B, norm2, sig2, power, w, E_row);
+  const bool v2 = F % 2 == 0 && ((uintptr_t)G | (uintptr_t)B | (uintptr_t)w) % 16 == 0;
+  if (v2)
+    hipLaunchKernelGGL(gp_w2_kernel, dim3((rows + 3) / 4), dim3(4 * kWave), 0, st, rows, F,
+                       G, e_part, nparts, B, norm2, sig2, power, w, E_row);
+  else
+    hipLaunchKernelGGL(gp_w_kernel, dim3(rows), dim3(kWave), 0, st, rows, F, G,
+                       e_part, nparts, B, norm2, sig2, power, w, E_row);
   FLARE_HIP_CHECK(hipGetLastError());
 }
 
